@@ -1,0 +1,214 @@
+"""Deterministic synthetic LiDAR tiles (ctypes front end of csrc/synth.c).
+
+The workloads mirror the benchmark configurations of the OWM ground-seed
+pipeline: a 1M-point 500 m tile, a 20M-point 2 km tile with Gaussian clusters,
+a 120M-point 3 km urban tile with flat 1-cm-quantised roofs, a 512-tile uneven
+batch and a Poisson request stream.  Every point depends only on
+(seed, stream, index), so any subset can be regenerated independently.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class SynthSpec(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("stream", ctypes.c_uint64),
+        ("n", ctypes.c_uint64),
+        ("x0", ctypes.c_double),
+        ("y0", ctypes.c_double),
+        ("width", ctypes.c_double),
+        ("height", ctypes.c_double),
+        ("z0", ctypes.c_double),
+        ("veg_frac", ctypes.c_double),
+        ("veg_mean", ctypes.c_double),
+        ("cluster_frac", ctypes.c_double),
+        ("n_clusters", ctypes.c_uint32),
+        ("cluster_sigma", ctypes.c_double),
+        ("roof_frac", ctypes.c_double),
+        ("n_roofs", ctypes.c_uint32),
+        ("roof_side_min", ctypes.c_double),
+        ("roof_side_max", ctypes.c_double),
+        ("roof_h_min", ctypes.c_double),
+        ("roof_h_max", ctypes.c_double),
+    ]
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libowm_synth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} is missing: run __graft_entry__.build() (or fastowm_amd.build) first")
+        lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.owm_synth_generate_range.argtypes = [ctypes.POINTER(SynthSpec), ctypes.c_uint64,
+                                                 ctypes.c_uint64, dp, dp, dp]
+        lib.owm_synth_bbox.argtypes = [dp, dp, ctypes.c_uint64, dp]
+        lib.owm_synth_write_xyz.argtypes = [ctypes.c_char_p, dp, dp, dp, ctypes.c_uint64, dp]
+        lib.owm_synth_write_xyz.restype = ctypes.c_int
+        lib.owm_synth_ctr.argtypes = [ctypes.c_uint64] * 4 + [ctypes.c_uint32]
+        lib.owm_synth_ctr.restype = ctypes.c_uint64
+        _LIB = lib
+    return _LIB
+
+
+@dataclass
+class Tile:
+    """One synthetic tile plus the OWM parameters it is run with."""
+    seed: int = 0
+    stream: int = 0
+    n: int = 1_000_000
+    x0: float = 512_000.0
+    y0: float = 4_700_000.0
+    width: float = 500.0
+    height: float = 500.0
+    z0: float = 300.0
+    veg_frac: float = 0.4
+    veg_mean: float = 6.0
+    cluster_frac: float = 0.0
+    n_clusters: int = 0
+    cluster_sigma: float = 3.0
+    roof_frac: float = 0.0
+    n_roofs: int = 0
+    roof_side: tuple = (10.0, 60.0)
+    roof_h: tuple = (5.0, 40.0)
+    # OWM parameters
+    wsize: int = 12
+    bsize: int = 20
+    overlap: float = 0.8
+    leaf_size: int = 32
+    name: str = field(default="tile")
+
+    def spec(self, n=None) -> SynthSpec:
+        return SynthSpec(self.seed, self.stream, self.n if n is None else n, self.x0, self.y0,
+                         self.width, self.height, self.z0, self.veg_frac, self.veg_mean,
+                         self.cluster_frac, self.n_clusters, self.cluster_sigma,
+                         self.roof_frac, self.n_roofs, self.roof_side[0], self.roof_side[1],
+                         self.roof_h[0], self.roof_h[1])
+
+    def generate(self, begin: int = 0, end: int | None = None):
+        """Return float64 arrays (x, y, z) for points [begin, end)."""
+        end = self.n if end is None else end
+        m = max(end - begin, 0)
+        x = np.empty(m, np.float64)
+        y = np.empty(m, np.float64)
+        z = np.empty(m, np.float64)
+        if m:
+            dp = ctypes.POINTER(ctypes.c_double)
+            spec = self.spec()
+            _lib().owm_synth_generate_range(ctypes.byref(spec), begin, end,
+                                            x.ctypes.data_as(dp), y.ctypes.data_as(dp),
+                                            z.ctypes.data_as(dp))
+        return x, y, z
+
+
+def bbox(x: np.ndarray, y: np.ndarray):
+    """Exact fp64 (minx, maxx, miny, maxy), NaNs ignored (header order)."""
+    out = np.empty(4, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    _lib().owm_synth_bbox(x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(x), out.ctypes.data_as(dp))
+    return tuple(float(v) for v in out)
+
+
+def write_xyz(path: str, x, y, z, box=None):
+    """Write a headed .xyz file ("N minx maxx miny maxy" then "x y z" lines, %.17g)."""
+    box = bbox(x, y) if box is None else box
+    dp = ctypes.POINTER(ctypes.c_double)
+    b = np.asarray(box, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    rc = _lib().owm_synth_write_xyz(path.encode(), x.ctypes.data_as(dp), y.ctypes.data_as(dp),
+                                    z.ctypes.data_as(dp), len(x), b.ctypes.data_as(dp))
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def ctr(seed: int, stream: int, tag: int, i: int, k: int) -> int:
+    return int(_lib().owm_synth_ctr(seed, stream, tag, i, k))
+
+
+def u01(seed: int, stream: int, tag: int, i: int, k: int) -> float:
+    return (ctr(seed, stream, tag, i, k) >> 11) * (1.0 / 9007199254740992.0)
+
+
+# ------------------------------------------------------------ workloads ----
+
+def config_c1(seed: int = 0) -> Tile:
+    """Plain OWM, 1M points over a 500x500 m tile; Wsize 12, Overlap 0.8, Bsize 20, leaf 32."""
+    return Tile(seed=seed, n=1_000_000, width=500.0, height=500.0, name="c1_1M_500m")
+
+
+def config_c2(seed: int = 0) -> Tile:
+    """20M points, 2x2 km at 5 pts/m2, 10% in Gaussian clusters (sigma 3 m)."""
+    return Tile(seed=seed, n=20_000_000, width=2000.0, height=2000.0, cluster_frac=0.10,
+                n_clusters=1000, cluster_sigma=3.0, name="c2_20M_2km_clusters")
+
+
+def config_c3(seed: int = 0) -> Tile:
+    """120M points, 3x3 km, 30% on flat rectangular roofs with 1-cm z (exact ties)."""
+    return Tile(seed=seed, n=120_000_000, width=3000.0, height=3000.0, roof_frac=0.30,
+                n_roofs=3000, name="c3_120M_3km_roofs")
+
+
+def scaled(tile: Tile, n: int) -> Tile:
+    """The same generator at n points over an area scaled to keep the density."""
+    f = math.sqrt(n / tile.n)
+    return replace(tile, n=n, width=tile.width * f, height=tile.height * f,
+                   n_clusters=max(1, int(tile.n_clusters * n / tile.n)) if tile.n_clusters else 0,
+                   n_roofs=max(1, int(tile.n_roofs * n / tile.n)) if tile.n_roofs else 0,
+                   name=f"{tile.name}@{n}")
+
+
+_TAG_BATCH = 101
+_TAG_SERVE = 102
+
+
+def batch_tiles(n_tiles: int = 512, seed: int = 0, median: float = 400_000.0,
+                sigma: float = 0.87, lo: int = 20_000, hi: int = 6_000_000):
+    """The uneven batch: log-normal point counts (median 400k, clipped 20k-6M),
+    density U[1,20] pts/m2, Wsize from {8,12,16}, Overlap 0.8, Bsize 20."""
+    tiles = []
+    for t in range(n_tiles):
+        u1 = u01(seed, t, _TAG_BATCH, 0, 0)
+        u2 = u01(seed, t, _TAG_BATCH, 0, 1)
+        g = math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(2.0 * math.pi * u2)
+        n = int(min(max(median * math.exp(sigma * g), lo), hi))
+        dens = 1.0 + 19.0 * u01(seed, t, _TAG_BATCH, 0, 2)
+        side = math.sqrt(n / dens)
+        ws = (8, 12, 16)[min(2, int(3 * u01(seed, t, _TAG_BATCH, 0, 3)))]
+        tiles.append(Tile(seed=seed, stream=1000 + t, n=n, width=side, height=side,
+                          x0=400_000.0 + 10_000.0 * (t % 64), y0=4_600_000.0 + 10_000.0 * (t // 64),
+                          wsize=ws, bsize=20, overlap=0.8, name=f"batch{t}"))
+    return tiles
+
+
+def serve_requests(n_req: int, rate: float, seed: int = 0, lo: int = 10_000, hi: int = 8_000_000):
+    """Poisson request stream: (arrival_s, Tile).  N log-uniform in [lo, hi],
+    integer Wsize in [5,20], Bsize in [10,30], Overlap U[0.5,0.9]."""
+    out = []
+    t_arr = 0.0
+    for r in range(n_req):
+        t_arr += -math.log(1.0 - u01(seed, r, _TAG_SERVE, 0, 0)) / rate
+        n = int(lo * (hi / lo) ** u01(seed, r, _TAG_SERVE, 0, 1))
+        dens = 1.0 + 19.0 * u01(seed, r, _TAG_SERVE, 0, 2)
+        side = math.sqrt(n / dens)
+        ws = 5 + min(15, int(16 * u01(seed, r, _TAG_SERVE, 0, 3)))
+        bs = 10 + min(20, int(21 * u01(seed, r, _TAG_SERVE, 0, 4)))
+        ov = 0.5 + 0.4 * u01(seed, r, _TAG_SERVE, 0, 5)
+        out.append((t_arr, Tile(seed=seed, stream=5000 + r, n=n, width=side, height=side,
+                                wsize=ws, bsize=bs, overlap=ov, name=f"req{r}")))
+    return out

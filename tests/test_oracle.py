@@ -1,0 +1,270 @@
+"""Pins the CPU oracle (test infrastructure) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+closed instead of open window edges, a wrong window origin or step, a wrong
+tie rule, dropped NaN handling, a tree whose whole-node shortcut over-counts,
+a stage 2 that keeps single hits, a stage 3 that ignores LLPs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fastowm_amd import synth
+
+NONE = O.NONE
+
+
+def _tile(n, seed=0, **kw):
+    t = synth.scaled(synth.config_c1(seed), n)
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+def _py_stage1(x, y, z, p):
+    """Pure-Python restatement of one window scan (tiny inputs only)."""
+    out = []
+    h = float(p.h)
+    for jj in range(p.nrows):
+        for ii in range(p.ncols):
+            cx = p.initx + ii * p.displace
+            cy = p.inity + jj * p.displace
+            x0, y0, x1, y1 = cx - h, cy - h, cx + h, cy + h
+            cnt, best = 0, None
+            for i in range(len(x)):
+                if x[i] > x0 and y[i] > y0 and x[i] < x1 and y[i] < y1:
+                    cnt += 1
+                    if z[i] < 1.7976931348623157e308 and (best is None or z[i] < z[best]):
+                        best = i
+            out.append(best if (best is not None and cnt >= p.min_num_points) else NONE)
+    return np.array(out, np.uint32)
+
+
+def test_params_closed_form():
+    # 500 x 500 m, 1M points: Density 4, minNumPoints 0.5*4*144 = 288,
+    # Displace 2.4, nCols = round(519.2 / 2.4) - 1 = 215, nColsg = 26.
+    p = O.params(12, 20, 0.8, (500000.0, 500500.0, 4700000.0, 4700500.0), 1_000_000)
+    assert p.width == 500.0 and p.height == 500.0
+    assert p.density == 4.0
+    assert p.min_num_points == 288
+    assert p.displace == 2.4
+    assert (p.ncols, p.nrows) == (215, 215)
+    assert (p.ncolsg, p.nrowsg) == (26, 26)
+    assert p.h == 6 and p.hb == 10
+    assert p.initx == (500000.0 - 6) + 2.4
+    assert p.initxg == 500000.0 + 10
+
+
+def test_params_integer_semantics():
+    # odd Wsize: Wsize/2 is an integer division; Overlap 0 uses floor(W/Ws)+1
+    p = O.params(13, 21, 0.0, (0.0, 100.0, 0.0, 50.0), 5000)
+    assert p.h == 6 and p.hb == 10
+    assert p.displace == 13.0
+    assert (p.ncols, p.nrows) == (math.floor(100 / 13) + 1, math.floor(50 / 13) + 1)
+    # ushort truncation of minNumPoints: 0.5*D*Ws^2 >= 65536 wraps
+    p = O.params(20, 20, 0.8, (0.0, 10.0, 0.0, 10.0), 400_000)  # D = 4000/m2
+    assert p.min_num_points == int(0.5 * 4000 * 400) % 65536
+
+
+def test_params_rejected():
+    box = (0.0, 100.0, 0.0, 100.0)
+    assert O.params(12, 20, 1.0, box, 1000) is None  # Displace 0
+    assert O.params(0, 20, 0.8, box, 1000) is None
+    assert O.params(12, 20, 0.8, (0.0, 0.0, 0.0, 100.0), 1000) is None  # zero width
+
+
+def test_python_restatement_agrees_on_tiny_tile():
+    t = _tile(400, seed=3)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    p = O.params(5, 7, 0.6, box, len(x), leaf_size=4)
+    py = _py_stage1(x, y, z, p)
+    assert np.array_equal(O.stage1_brute(x, y, z, p), py)
+    assert np.array_equal(O.Tree(x, y, z, p).stage1(), py)
+
+
+CASES = [
+    # n, wsize, bsize, overlap, leaf
+    (3000, 12, 20, 0.8, 32),
+    (3000, 12, 20, 0.8, 1),
+    (5000, 13, 21, 0.8, 8),
+    (5000, 7, 11, 0.5, 16),
+    (4000, 10, 20, 0.9, 128),
+    (4000, 8, 15, 0.0, 32),
+    (6000, 16, 30, 0.75, 64),
+    (2500, 5, 10, 0.6, 3),
+]
+
+
+@pytest.mark.parametrize("n,ws,bs,ov,leaf", CASES)
+def test_tree_stages_match_bruteforce(n, ws, bs, ov, leaf):
+    t = _tile(n, seed=n + ws, cluster_frac=0.5, n_clusters=5, cluster_sigma=4.0)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    got = O.seeds_from_stages(x, y, z, ws, bs, ov, leaf, box)
+    ref = O.seeds_from_stages(x, y, z, ws, bs, ov, leaf, box, brute=True)
+    assert np.array_equal(got[1], ref[1]), "stage-1 window minima differ"
+    assert np.array_equal(got[2], ref[2]), "LLPs differ"
+    assert np.array_equal(np.sort(got[3]), np.sort(ref[3])), "stage-3 adds differ"
+    assert np.array_equal(got[0], ref[0])
+    seeds, _, _ = O.run(x, y, z, ws, bs, ov, leaf, box)
+    assert np.array_equal(seeds, got[0])
+
+
+def test_tree_structure():
+    t = _tile(20000, seed=1, cluster_frac=0.3, n_clusters=20)
+    x, y, z = t.generate()
+    p = O.params(12, 20, 0.8, synth.bbox(x, y), len(x), leaf_size=16)
+    tree = O.Tree(x, y, z, p)
+    assert tree.check() == 0
+    assert tree.nodes() > 1000
+
+
+def _edge_planted(ws=12, ov=0.8):
+    """Points planted exactly on window edges, on corners and at the window
+    centres, plus a background.  Returns x, y, z, params."""
+    rng = np.random.default_rng(7)
+    n = 1500
+    x = 1000.0 + 60.0 * rng.random(n)
+    y = 2000.0 + 60.0 * rng.random(n)
+    z = 10.0 + rng.random(n)
+    box = (1000.0, 1060.0, 2000.0, 2060.0)
+    p = O.params(ws, 20, ov, box, n)
+    # plant: for a few windows put low points exactly on their left/right/bottom/top edges
+    k = 0
+    for ii, jj in [(3, 3), (5, 4), (7, 9), (10, 2)]:
+        cx = p.initx + ii * p.displace
+        cy = p.inity + jj * p.displace
+        h = float(p.h)
+        for ex, ey in [(cx - h, cy), (cx + h, cy), (cx, cy - h), (cx, cy + h), (cx - h, cy - h)]:
+            # window (3, 3)'s own edge points are the lowest of the cloud
+            x[k], y[k], z[k] = ex, ey, (0.5 if k < 5 else 5.0) - 0.01 * k
+            k += 1
+    return x, y, z, p
+
+
+def test_edge_points_are_outside_open_windows():
+    x, y, z, p = _edge_planted()
+    brute = O.stage1_brute(x, y, z, p)
+    tree = O.Tree(x, y, z, p).stage1()
+    assert np.array_equal(tree, brute)
+    # the points planted on window (3, 3)'s edges (ids 0..4) are the lowest of
+    # the cloud but never its minimum; a closed-box (<=) restatement picks one
+    ii, jj = 3, 3
+    assert brute[jj * p.ncols + ii] >= 5
+    closed = _closed_box_min(x, y, z, p, ii, jj)
+    assert closed < 5, "planted data must be able to expose a closed-box mistake"
+
+
+def _closed_box_min(x, y, z, p, ii, jj):
+    cx = p.initx + ii * p.displace
+    cy = p.inity + jj * p.displace
+    h = float(p.h)
+    m = (x >= cx - h) & (x <= cx + h) & (y >= cy - h) & (y <= cy + h)
+    idx = np.nonzero(m)[0]
+    return int(idx[np.argmin(z[idx])])
+
+
+def test_ties_broken_by_lowest_index():
+    t = _tile(3000, seed=11)
+    x, y, z = t.generate()
+    z = np.round(z)  # 1 m steps: many exact ties
+    z[:] = np.minimum(z, 300.0)  # a large tied plateau at the minimum
+    box = synth.bbox(x, y)
+    p = O.params(10, 20, 0.8, box, len(x), leaf_size=8)
+    brute = O.stage1_brute(x, y, z, p)
+    tree = O.Tree(x, y, z, p).stage1()
+    assert np.array_equal(tree, brute)
+    # closed form: every valid window's minimum is the lowest index among the
+    # points at the window's minimum z
+    h = float(p.h)
+    for s in range(0, p.ncols * p.nrows, 37):
+        ii, jj = s % p.ncols, s // p.ncols
+        cx, cy = p.initx + ii * p.displace, p.inity + jj * p.displace
+        m = (x > cx - h) & (x < cx + h) & (y > cy - h) & (y < cy + h)
+        idx = np.nonzero(m)[0]
+        if len(idx) < p.min_num_points or len(idx) == 0:
+            assert brute[s] == NONE
+            continue
+        zmin = z[idx].min()
+        assert brute[s] == idx[z[idx] == zmin].min()
+
+
+def test_nan_and_inf_points():
+    t = _tile(3000, seed=5)
+    x, y, z = t.generate()
+    x[10], y[20], z[30], z[40] = np.nan, np.nan, np.nan, np.inf
+    # +0.0 and -0.0 at the same spot, lowest of the cloud: equal z, so the
+    # lower index (50) wins; an order on the raw bit pattern would pick 51
+    z[50], z[51] = 0.0, -0.0
+    x[51], y[51] = x[50], y[50]
+    box = synth.bbox(x, y)
+    for leaf in (4, 32):
+        p = O.params(12, 20, 0.8, box, len(x), leaf_size=leaf)
+        brute = O.stage1_brute(x, y, z, p)
+        tree = O.Tree(x, y, z, p)
+        assert tree.check() == 0
+        assert np.array_equal(tree.stage1(), brute)
+        assert not np.isin(brute, [10, 20, 30, 40]).any()
+    assert 50 in brute and 51 not in brute
+
+
+def test_coincident_points_beyond_leaf_size():
+    t = _tile(2000, seed=9)
+    x, y, z = t.generate()
+    x[100:400] = x[0]
+    y[100:400] = y[0]
+    box = synth.bbox(x, y)
+    p = O.params(12, 20, 0.8, box, len(x), leaf_size=8)
+    tree = O.Tree(x, y, z, p)
+    assert tree.check() == 0
+    assert np.array_equal(tree.stage1(), O.stage1_brute(x, y, z, p))
+
+
+def test_stage2_is_multiset_count():
+    rng = np.random.default_rng(1)
+    mins = rng.integers(0, 500, 4000).astype(np.uint32)
+    mins[rng.random(4000) < 0.3] = NONE
+    p = O.params(12, 20, 0.8, (0.0, 100.0, 0.0, 100.0), 1000)
+    got = O.stage2(p, mins)
+    vals, cnt = np.unique(mins[mins != NONE], return_counts=True)
+    assert np.array_equal(got, vals[cnt >= 2])
+    p0 = O.params(12, 20, 0.0, (0.0, 100.0, 0.0, 100.0), 1000)
+    assert np.array_equal(O.stage2(p0, mins), vals)
+
+
+def test_stage3_fills_empty_cells_only():
+    # dense cluster + sparse background: most background windows are invalid,
+    # so stage 3 has to add the cloud minimum of LLP-free cells
+    t = _tile(8000, seed=21, cluster_frac=0.8, n_clusters=3, cluster_sigma=6.0)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    seeds, mins, llp, adds, p = O.seeds_from_stages(x, y, z, 12, 20, 0.8, 32, box)
+    assert len(adds) > 0 and len(llp) > 0
+    assert np.array_equal(np.sort(adds), np.sort(O.stage3_brute(x, y, z, p, llp)))
+    # no add shares a cell with an LLP; every add is its cell's lowest point
+    hb = float(p.hb)
+    for a in adds:
+        ii = int(np.floor((x[a] - p.minx) / p.bsize))
+        jj = int(np.floor((y[a] - p.miny) / p.bsize))
+        cx, cy = p.initxg + ii * p.bsize, p.inityg + jj * p.bsize
+        inside = (x > cx - hb) & (x < cx + hb) & (y > cy - hb) & (y < cy + hb)
+        assert inside[a]
+        assert not inside[llp].any()
+        idx = np.nonzero(inside)[0]
+        assert a == idx[np.argmin(z[idx])]
+    assert len(np.intersect1d(adds, llp)) == 0
+
+
+def test_odd_bsize_leaves_gaps():
+    # Bsize 21 -> cells of width 2*10 = 20 spaced 21: a point in the 1 m gap
+    # belongs to no cell and is never added by stage 3
+    p = O.params(12, 21, 0.8, (0.0, 100.0, 0.0, 100.0), 100)
+    assert p.hb == 10
+    x = np.array([20.5, 5.0, 50.0])
+    y = np.array([5.0, 5.0, 50.0])
+    z = np.array([1.0, 2.0, 3.0])
+    adds = O.stage3_brute(x, y, z, p, np.zeros(0, np.uint32))
+    assert 0 not in adds and 1 in adds

@@ -1,0 +1,63 @@
+"""Oracle vs the reference's own binary (committed fixtures, and live when the
+reference binary was built in this container).
+
+tests/golden/ref_o1qtree.json holds seed lists written by the reference's exact
+OpenMP quadtree OWM (see tests/golden/make_golden.py).  On tie-free data its
+seeds are a function of the input, so the oracle must reproduce them exactly:
+stage-1 window geometry, minNumPoints, stage 2's >= 2 rule, stage 3's grid.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fastowm_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "ref_o1qtree.json")))
+
+
+def _tile(case):
+    t = synth.scaled(synth.config_c1(case["generator"].get("seed", 0)), case["n"])
+    for k, v in case["generator"].items():
+        setattr(t, k, v)
+    return t
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=[c["name"] for c in GOLD["cases"]])
+def test_oracle_matches_reference_binary(case):
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_golden import points_digest
+    x, y, z = _tile(case).generate()
+    assert points_digest(x, y, z) == case["points_sha256"], "generator drifted"
+    box = tuple(case["bbox"])
+    assert box == synth.bbox(x, y)
+    for leaf in (8, 32):
+        seeds, mins, llp, adds, p = O.seeds_from_stages(
+            x, y, z, case["wsize"], case["bsize"], case["overlap"], leaf, box)
+        if case["ref_llp"] is not None:
+            assert llp.tolist() == case["ref_llp"]
+        assert seeds.tolist() == case["ref_seeds"]
+
+
+def test_fixtures_cover_stage3_and_overlap0():
+    names = {c["name"]: c for c in GOLD["cases"]}
+    with_adds = [c for c in GOLD["cases"]
+                 if c["ref_llp"] is not None and len(c["ref_seeds"]) > len(c["ref_llp"])]
+    assert len(with_adds) >= 2
+    assert names["sparse_20k_w8_b15_o00"]["ref_llp"] is None
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "..", "oracle", "_ref", "o1qtree")),
+                    reason="reference binary not built here (needs /root/reference)")
+def test_live_reference_run_matches_fixture():
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_golden
+    c = make_golden.CASES[1]
+    got = make_golden.run_case(*c)
+    want = [g for g in GOLD["cases"] if g["name"] == c[0]][0]
+    assert got["ref_seeds"] == want["ref_seeds"]
