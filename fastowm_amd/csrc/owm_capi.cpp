@@ -1,0 +1,504 @@
+// C ABI entry points (include/owm.h).  Each catches every C++ exception and
+// returns a status; none exits the process.
+#include <cstring>
+#include <new>
+
+#include "owm_host.h"
+
+using namespace owm;
+
+struct owm_tree {
+    owm_ctx *ctx = nullptr;
+    DevBuf mem;
+    Prepared P;
+    TreeOut T{};
+    uint64_t n = 0;
+};
+
+#define GUARD_BEGIN try {
+#define GUARD_END(ctx)                                              \
+    }                                                               \
+    catch (const std::bad_alloc &) {                                \
+        set_err(ctx, "host allocation failed");                     \
+        return OWM_E_NOMEM;                                         \
+    }                                                               \
+    catch (const std::exception &e) {                               \
+        set_err(ctx, "internal error: %s", e.what());               \
+        return OWM_E_INTERNAL;                                      \
+    }
+
+static inline uint64_t al256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+extern "C" {
+
+const char *owm_status_str(owm_status s)
+{
+    switch (s) {
+    case OWM_OK: return "ok";
+    case OWM_E_ARG: return "invalid argument";
+    case OWM_E_NOMEM: return "out of memory";
+    case OWM_E_BUDGET: return "over the HBM budget";
+    case OWM_E_DEVICE: return "device error";
+    case OWM_E_CAPACITY: return "output buffer too small";
+    case OWM_E_CANCELLED: return "cancelled";
+    case OWM_E_NOTREADY: return "not ready";
+    case OWM_E_NOTFOUND: return "not found";
+    case OWM_E_INTERNAL: return "internal error";
+    }
+    return "unknown status";
+}
+
+const char *owm_last_error(const owm_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+owm_status owm_ctx_create(int device, uint64_t hbm_budget_bytes, owm_ctx **out)
+{
+    if (!out)
+        return OWM_E_ARG;
+    *out = nullptr;
+    GUARD_BEGIN
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return OWM_E_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return OWM_E_DEVICE;
+    owm_ctx *c = new owm_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return OWM_E_DEVICE;
+    }
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        delete c;
+        return OWM_E_DEVICE;
+    }
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(freeb * 0.9);
+    *out = c;
+    return OWM_OK;
+    GUARD_END(nullptr)
+}
+
+void owm_ctx_destroy(owm_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    owm_serve_stop(ctx);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+}
+
+uint64_t owm_ctx_device_bytes(const owm_ctx *ctx) { return ctx ? ctx->held_bytes() : 0; }
+
+owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_or_null,
+                      owm_derived *out)
+{
+    if (!params || !out)
+        return OWM_E_ARG;
+    const double *b = (params->flags & OWM_F_HAVE_BBOX) ? params->bbox : bbox_or_null;
+    if (!b)
+        return OWM_E_ARG;
+    Derived D;
+    std::string why;
+    owm_status st = derive_params(*params, n, b, D, why);
+    if (st != OWM_OK)
+        return st;
+    out->width = D.width;
+    out->height = D.height;
+    out->density = D.density;
+    out->displace = D.displace;
+    out->min_num_points = D.min_num_points;
+    out->ncols = D.ncols;
+    out->nrows = D.nrows;
+    out->ncolsg = D.ncolsg;
+    out->nrowsg = D.nrowsg;
+    for (int k = 0; k < 4; ++k)
+        out->bbox[k] = b[k];
+    return OWM_OK;
+}
+
+// Copies a host cloud into the context's upload buffer.
+static owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                               uint64_t n, double **dx, double **dy, double **dz)
+{
+    uint64_t b = al256(n * 8ull);
+    if (ctx->upload.bytes < 3 * b) {
+        if (ctx->budget && ctx->held_bytes() - ctx->upload.bytes + 3 * b > ctx->budget) {
+            set_err(ctx, "input upload needs %llu bytes over the budget", (unsigned long long)(3 * b));
+            return OWM_E_BUDGET;
+        }
+        (void)hipStreamSynchronize(ctx->stream);
+        hipError_t e = ctx->upload.reserve(3 * b);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "upload buffer");
+    }
+    uint8_t *base = (uint8_t *)ctx->upload.ptr;
+    *dx = (double *)base;
+    *dy = (double *)(base + b);
+    *dz = (double *)(base + 2 * b);
+    hipError_t e;
+    if (n) {
+        if ((e = hipMemcpyAsync(*dx, x, n * 8, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(*dy, y, n * 8, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(*dz, z, n * 8, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+            return hip_fail(ctx, e, "cloud H2D");
+    }
+    return OWM_OK;
+}
+
+static owm_status run_one(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                          uint64_t n, const owm_params *params, uint32_t *seeds, uint64_t cap,
+                          uint64_t *n_seeds)
+{
+    owm_tile t;
+    t.offset = 0;
+    t.n = n;
+    t.params = *params;
+    uint64_t off[2] = {0, 0};
+    owm_status st = run_batch(ctx, dx, dy, dz, &t, 1, 0, seeds, cap, off);
+    if (n_seeds)
+        *n_seeds = off[1];
+    return st;
+}
+
+owm_status owm_run_tile(owm_ctx *ctx, const double *x, const double *y, const double *z, uint64_t n,
+                        const owm_params *params, uint32_t *seeds, uint64_t cap, uint64_t *n_seeds)
+{
+    if (!ctx || !params || !n_seeds || (n && (!x || !y || !z)) || (cap && !seeds))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    double *dx, *dy, *dz;
+    owm_status st = upload_cloud(ctx, x, y, z, n, &dx, &dy, &dz);
+    if (st != OWM_OK)
+        return st;
+    return run_one(ctx, dx, dy, dz, n, params, seeds, cap, n_seeds);
+    GUARD_END(ctx)
+}
+
+owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                               const double *d_z, uint64_t n, const owm_params *params,
+                               uint32_t *seeds, uint64_t cap, uint64_t *n_seeds)
+{
+    if (!ctx || !params || !n_seeds || (n && (!d_x || !d_y || !d_z)) || (cap && !seeds))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    return run_one(ctx, d_x, d_y, d_z, n, params, seeds, cap, n_seeds);
+    GUARD_END(ctx)
+}
+
+owm_status owm_batch_run_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                const double *d_z, const owm_tile *tiles, uint32_t n_tiles,
+                                uint32_t flags, uint32_t *seeds, uint64_t cap, uint64_t *seed_off)
+{
+    if (!ctx || !tiles || !seed_off || (cap && !seeds))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    return run_batch(ctx, d_x, d_y, d_z, tiles, n_tiles, flags, seeds, cap, seed_off);
+    GUARD_END(ctx)
+}
+
+owm_status owm_batch_run(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                         const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
+                         uint64_t cap, uint64_t *seed_off)
+{
+    if (!ctx || !tiles || !seed_off || (cap && !seeds) || n_tiles == 0)
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    uint64_t n = tiles[n_tiles - 1].offset + tiles[n_tiles - 1].n;
+    double *dx, *dy, *dz;
+    owm_status st = upload_cloud(ctx, x, y, z, n, &dx, &dy, &dz);
+    if (st != OWM_OK)
+        return st;
+    return run_batch(ctx, dx, dy, dz, tiles, n_tiles, flags, seeds, cap, seed_off);
+    GUARD_END(ctx)
+}
+
+// ------------------------------------------------------------------ trees --
+
+owm_status owm_tree_build(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                          uint64_t n, int device_ptrs, uint32_t leaf_size, owm_tree **out)
+{
+    if (!ctx || !out || n == 0 || !x || !y || !z || leaf_size > 1024)
+        return OWM_E_ARG;
+    *out = nullptr;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    const double *dx = x, *dy = y, *dz = z;
+    if (!device_ptrs) {
+        double *ux, *uy, *uz;
+        owm_status st = upload_cloud(ctx, x, y, z, n, &ux, &uy, &uz);
+        if (st != OWM_OK)
+            return st;
+        dx = ux;
+        dy = uy;
+        dz = uz;
+    }
+    owm_tile tile{};
+    tile.offset = 0;
+    tile.n = n;
+    tile.params.leaf_size = leaf_size;
+    std::unique_ptr<owm_tree> tr(new owm_tree());
+    tr->ctx = ctx;
+    tr->n = n;
+    owm_status st = prepare_tree(ctx, dx, dy, &tile, 1, tr->P);
+    if (st != OWM_OK)
+        return st;
+    tr->P.Q.ntiles = 1;
+    tr->P.Q.win_off.assign(2, 0);
+    tr->P.Q.cell_off.assign(2, 0);
+    tr->P.Q.blk_off.assign(2, 0);
+    const uint64_t tb = tree_bytes(tr->P.L);
+    if (ctx->budget && ctx->held_bytes() + tb > ctx->budget) {
+        set_err(ctx, "tree needs %llu device bytes over the budget", (unsigned long long)tb);
+        return OWM_E_BUDGET;
+    }
+    hipError_t e = tr->mem.reserve(tb);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "tree allocation");
+    ctx->trees_bytes += tr->mem.bytes;
+    carve_tree((uint8_t *)tr->mem.ptr, tr->P.L, tr->T);
+    const int passes = (tr->P.L.tile_bits + tr->P.L.key_bits + 7) / 8;
+    st = ctx->arena.ensure(ctx, table_bytes(1) + sort_scratch_bytes(n, passes) + 8192);
+    if (st != OWM_OK) {
+        ctx->trees_bytes -= tr->mem.bytes;
+        return st;
+    }
+    uint32_t *guard = (uint32_t *)ctx->arena.take(256);
+    launch_fill_u32(ctx->stream, guard, 64, 0u);
+    BatchDev B;
+    st = upload_table(ctx, ctx->arena, tr->P, B);
+    if (st == OWM_OK)
+        st = build_tree(ctx, ctx->stream, tr->P.L, B, dx, dy, dz, tr->T, ctx->arena, guard);
+    uint32_t hg = 0;
+    if (st == OWM_OK) {
+        e = hipMemcpyAsync(&hg, guard, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess)
+            st = hip_fail(ctx, e, "tree build");
+        else if (hg) {
+            set_err(ctx, "tree build guard tripped (flags %u)", hg);
+            st = OWM_E_INTERNAL;
+        }
+    }
+    if (st != OWM_OK) {
+        ctx->trees_bytes -= tr->mem.bytes;
+        return st;
+    }
+    *out = tr.release();
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+void owm_tree_free(owm_tree *tree)
+{
+    if (!tree)
+        return;
+    if (tree->ctx) {
+        (void)hipStreamSynchronize(tree->ctx->stream);
+        tree->ctx->trees_bytes -= tree->mem.bytes;
+    }
+    delete tree;
+}
+
+uint64_t owm_tree_points(const owm_tree *tree) { return tree ? tree->n : 0; }
+
+owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_internal,
+                         uint64_t *root_count, double *root_bbox)
+{
+    if (!tree)
+        return OWM_E_ARG;
+    owm_ctx *ctx = tree->ctx;
+    GUARD_BEGIN
+    if (n_leaves)
+        *n_leaves = tree->P.L.nleaf;
+    if (n_internal)
+        *n_internal = tree->P.L.nnode;
+    uint32_t root = tree->P.td[0].root;
+    double4 box = make_double4(0, 0, 0, 0);
+    uint32_t cnt = 0;
+    if (root != OWM_NONE) {
+        const void *bsrc, *csrc;
+        if (root & OWM_LEAF_BIT) {
+            bsrc = tree->T.lf_box + (root & ~OWM_LEAF_BIT);
+            csrc = tree->T.lf_cnt + (root & ~OWM_LEAF_BIT);
+        } else {
+            bsrc = tree->T.nd_box + root;
+            csrc = tree->T.nd_cnt + root;
+        }
+        hipError_t e = hipMemcpy(&box, bsrc, sizeof box, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(&cnt, csrc, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "tree info");
+    }
+    if (root_count)
+        *root_count = cnt;
+    if (root_bbox) {
+        root_bbox[0] = box.x;
+        root_bbox[1] = box.z;
+        root_bbox[2] = box.y;
+        root_bbox[3] = box.w;
+    }
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+// Stages from..to on a resident tree.
+static owm_status tree_query(owm_ctx *ctx, owm_tree *tree, const owm_params *params, int from,
+                             int to, const uint32_t *h_in, uint64_t n_in, Prepared &P, QueryOut &qo)
+{
+    P = tree->P;
+    owm_status st = prepare_query(ctx, params, 0, P.box, P.L.np, P);
+    if (st != OWM_OK)
+        return st;
+    st = ctx->arena.ensure(ctx, table_bytes(1) + query_bytes(P.Q, P.L.np, n_in) + 4096);
+    if (st != OWM_OK)
+        return st;
+    BatchDev B;
+    st = upload_table(ctx, ctx->arena, P, B);
+    if (st != OWM_OK)
+        return st;
+    return enqueue_query(ctx, ctx->stream, B, tree_view(tree->T), P.Q, P.L.np, P.cfg,
+                         P.any_need1 ? 1u : 0u, ctx->arena, qo, from, to, h_in, n_in);
+}
+
+static owm_status fetch_counts(owm_ctx *ctx, const QueryOut &qo, uint32_t *hc)
+{
+    hipError_t e = hipMemcpyAsync(hc, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "counts D2H");
+    if (hc[3]) {
+        set_err(ctx, "device traversal stack overflow (flags %u)", hc[3]);
+        return OWM_E_INTERNAL;
+    }
+    return OWM_OK;
+}
+
+owm_status owm_stage1(owm_ctx *ctx, owm_tree *tree, const owm_params *params, uint32_t *min_ids,
+                      uint64_t cap, owm_derived *derived)
+{
+    if (!ctx || !tree || !params)
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    Prepared P;
+    QueryOut qo;
+    owm_status st = tree_query(ctx, tree, params, 1, 1, nullptr, 0, P, qo);
+    if (st != OWM_OK)
+        return st;
+    if (derived)
+        owm_derive(params, tree->n, P.box[0].data(), derived);
+    uint32_t hc[16];
+    st = fetch_counts(ctx, qo, hc);
+    if (st != OWM_OK)
+        return st;
+    if (P.Q.nwin > cap) {
+        set_err(ctx, "min_ids holds %llu, %llu windows", (unsigned long long)cap,
+                (unsigned long long)P.Q.nwin);
+        return OWM_E_CAPACITY;
+    }
+    hipError_t e = hipMemcpy(min_ids, qo.min_ids, P.Q.nwin * 4ull, hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "min_ids D2H");
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+owm_status owm_stage2(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
+                      const uint32_t *min_ids, uint64_t n_ids, uint32_t *llp, uint64_t cap,
+                      uint64_t *n_llp)
+{
+    if (!ctx || !tree || !params || !n_llp || (n_ids && !min_ids) || n_ids >= (1ull << 31))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    for (uint64_t i = 0; i < n_ids; ++i)
+        if (min_ids[i] != OWM_NONE && min_ids[i] >= tree->n) {
+            set_err(ctx, "min_ids[%llu] = %u is not a point index", (unsigned long long)i, min_ids[i]);
+            return OWM_E_ARG;
+        }
+    Prepared P;
+    QueryOut qo;
+    owm_status st = tree_query(ctx, tree, params, 2, 2, min_ids, n_ids, P, qo);
+    if (st != OWM_OK)
+        return st;
+    uint32_t hc[16];
+    st = fetch_counts(ctx, qo, hc);
+    if (st != OWM_OK)
+        return st;
+    *n_llp = hc[0];
+    if (hc[0] > cap) {
+        set_err(ctx, "llp holds %llu, %u needed", (unsigned long long)cap, hc[0]);
+        return OWM_E_CAPACITY;
+    }
+    if (hc[0]) {
+        hipError_t e = hipMemcpy(llp, qo.llp, hc[0] * 4ull, hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "llp D2H");
+    }
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+owm_status owm_stage3(owm_ctx *ctx, owm_tree *tree, const owm_params *params, const uint32_t *llp,
+                      uint64_t n_llp, uint32_t *adds, uint64_t cap, uint64_t *n_adds)
+{
+    if (!ctx || !tree || !params || !n_adds || (n_llp && !llp) || n_llp >= (1ull << 31))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    for (uint64_t i = 0; i < n_llp; ++i)
+        if (llp[i] >= tree->n) {
+            set_err(ctx, "llp[%llu] = %u is not a point index", (unsigned long long)i, llp[i]);
+            return OWM_E_ARG;
+        }
+    Prepared P;
+    QueryOut qo;
+    owm_status st = tree_query(ctx, tree, params, 3, 3, llp, n_llp, P, qo);
+    if (st != OWM_OK)
+        return st;
+    uint32_t hc[16];
+    st = fetch_counts(ctx, qo, hc);
+    if (st != OWM_OK)
+        return st;
+    *n_adds = hc[1];
+    if (hc[1] > cap) {
+        set_err(ctx, "adds holds %llu, %u needed", (unsigned long long)cap, hc[1]);
+        return OWM_E_CAPACITY;
+    }
+    if (hc[1]) {
+        hipError_t e = hipMemcpy(adds, qo.llp + hc[0], hc[1] * 4ull, hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "adds D2H");
+    }
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+owm_status owm_tree_run(owm_ctx *ctx, owm_tree *tree, const owm_params *params, uint32_t *seeds,
+                        uint64_t cap, uint64_t *n_seeds)
+{
+    if (!ctx || !tree || !params || !n_seeds || (cap && !seeds))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    Prepared P;
+    QueryOut qo;
+    owm_status st = tree_query(ctx, tree, params, 1, 4, nullptr, 0, P, qo);
+    if (st != OWM_OK)
+        return st;
+    uint64_t off[2];
+    st = read_seeds(ctx, qo, P, seeds, cap, off);
+    *n_seeds = off[1];
+    return st;
+    GUARD_END(ctx)
+}
+
+}  // extern "C"

@@ -1,0 +1,817 @@
+// Host orchestration of the OWM pipeline behind the C ABI of include/owm.h.
+//
+// A run is a batch of tiles (a single tile is a batch of one).  The host
+// derives each tile's parameters exactly as the reference driver does
+// (CPP-TBB-CPU/application/owm_optim3.cpp:92-132), lays every per-tile range
+// out in one concatenated index space, uploads the tile table, and enqueues
+// the device chain: Morton keys -> radix sort -> gather + leaves -> Karras ->
+// bottom-up -> stage 1 -> sort + stage 2 -> stage 3 -> merge + sort of the
+// seeds.  Only the bbox (when the caller has none) and the final seed count
+// come back to the host.
+#include "owm_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+namespace owm {
+
+// ------------------------------------------------------------ utilities --
+
+void set_err(owm_ctx *ctx, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx)
+        ctx->err = buf;
+}
+
+owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
+{
+    set_err(ctx, "%s: %s", what, hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory)
+        return OWM_E_NOMEM;
+    return OWM_E_DEVICE;
+}
+
+#define HIP_TRY(call, what)                        \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess)                      \
+            return hip_fail(ctx, e_, what);        \
+    } while (0)
+
+static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release()
+{
+    if (ptr)
+        (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+hipError_t DevBuf::reserve(uint64_t need)
+{
+    if (need <= bytes)
+        return hipSuccess;
+    release();
+    uint64_t want = need + need / 4;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        e = hipMalloc(&ptr, need);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        want = need;
+    }
+    bytes = want;
+    return hipSuccess;
+}
+
+// --------------------------------------------------------- parameters ----
+
+static double round2d(double z) { return std::round(z * 100.0) / 100.0; }
+
+// Parameter derivation of the reference driver (owm_optim3.cpp:92-132), with
+// the reference's integer types; values it would wrap or divide by zero on are
+// rejected with OWM_E_ARG.
+owm_status derive_params(const owm_params &P, uint64_t n, const double *bbox, Derived &D,
+                         std::string &why)
+{
+    if (P.wsize == 0 || P.wsize > 65535u) {
+        why = "wsize must be in [1, 65535]";
+        return OWM_E_ARG;
+    }
+    if (P.bsize > 65535u) {
+        why = "bsize must be <= 65535";
+        return OWM_E_ARG;
+    }
+    if (!(P.overlap >= 0.0 && P.overlap < 1.0)) {
+        why = "overlap must be in [0, 1)";
+        return OWM_E_ARG;
+    }
+    for (int k = 0; k < 4; ++k)
+        D.bbox[k] = bbox[k];
+    const double minx = bbox[0], maxx = bbox[1], miny = bbox[2], maxy = bbox[3];
+    const double width = round2d(maxx - minx), height = round2d(maxy - miny);
+    if (!(width > 0.0) || !(height > 0.0) || !std::isfinite(width) || !std::isfinite(height)) {
+        why = "bbox width and height (rounded to cm) must be positive and finite";
+        return OWM_E_ARG;
+    }
+    D.width = width;
+    D.height = height;
+    D.density = (double)n / (width * height);
+    const uint32_t Wsize = P.wsize;
+    const double mnp = 0.5 * D.density * Wsize * Wsize;
+    if (!(mnp < 4294967296.0)) {
+        why = "0.5*density*wsize^2 overflows the reference's u32";
+        return OWM_E_ARG;
+    }
+    D.min_num_points = (uint16_t)(uint32_t)mnp;
+    D.displace = round2d(Wsize * (1 - P.overlap));
+    if (!(D.displace > 0.0)) {
+        why = "window displacement round2d(wsize*(1-overlap)) is 0";
+        return OWM_E_ARG;
+    }
+    double c, r;
+    if (P.overlap > 0.0) {
+        c = std::round((width + 2 * Wsize * P.overlap) / D.displace) - 1;
+        r = std::round((height + 2 * Wsize * P.overlap) / D.displace) - 1;
+    } else {
+        c = std::floor(width / Wsize) + 1;
+        r = std::floor(height / Wsize) + 1;
+    }
+    if (!(c >= 1.0 && c <= 65535.0 && r >= 1.0 && r <= 65535.0)) {
+        why = "window grid must be 1..65535 columns and rows";
+        return OWM_E_ARG;
+    }
+    D.ncols = (uint32_t)c;
+    D.nrows = (uint32_t)r;
+    if (P.bsize > 0) {
+        const uint32_t Bsize = P.bsize;
+        double cg = std::floor(width / Bsize) + 1, rg = std::floor(height / Bsize) + 1;
+        if (!(cg >= 1.0 && cg <= 65535.0 && rg >= 1.0 && rg <= 65535.0)) {
+            why = "stage-3 grid must be 1..65535 columns and rows";
+            return OWM_E_ARG;
+        }
+        D.ncolsg = (uint32_t)cg;
+        D.nrowsg = (uint32_t)rg;
+    } else {
+        D.ncolsg = D.nrowsg = 0;
+    }
+    D.h = (int32_t)(P.wsize / 2);
+    D.hb = (int32_t)(P.bsize / 2);
+    D.initx = minx - D.h + D.displace;
+    D.inity = miny - D.h + D.displace;
+    D.initxg = minx + D.hb;
+    D.inityg = miny + D.hb;
+    return OWM_OK;
+}
+
+void fill_query(TileDev &T, const owm_params &P, const Derived &D)
+{
+    T.ncols = D.ncols;
+    T.nrows = D.nrows;
+    T.ncolsg = D.ncolsg;
+    T.nrowsg = D.nrowsg;
+    T.min_num_points = D.min_num_points;
+    T.stage2_need = P.overlap != 0.0 ? 2u : 1u;
+    T.h = D.h;
+    T.hb = D.hb;
+    T.bsize = P.bsize;
+    T.displace = D.displace;
+    T.initx = D.initx;
+    T.inity = D.inity;
+    T.initxg = D.initxg;
+    T.inityg = D.inityg;
+}
+
+int bit_length(uint64_t v)
+{
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+
+// ------------------------------------------------------------- planning --
+
+// Per-tile tree layout: leaves, internal nodes, Morton quantisation.
+void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
+                      const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
+                      TreeLayout &L)
+{
+    const uint32_t T = (uint32_t)tiles.size();
+    L.ntiles = T;
+    L.tile_bits = T > 1 ? bit_length(T - 1) : 0;
+    L.key_bits = (32 - L.tile_bits) & ~1;
+    L.pt_off.assign(T + 1, 0);
+    L.leaf_off.assign(T + 1, 0);
+    L.node_off.assign(T + 1, 0);
+    for (uint32_t t = 0; t < T; ++t) {
+        TileDev &D = tiles[t];
+        D.pt_off = L.pt_off[t];
+        D.n = (uint32_t)n[t];
+        D.leaf_size = leaf[t];
+        D.nleaf = (uint32_t)((n[t] + leaf[t] - 1) / leaf[t]);
+        D.leaf_off = L.leaf_off[t];
+        D.node_off = L.node_off[t];
+        D.root = D.nleaf == 0 ? OWM_NONE : (D.nleaf == 1 ? (OWM_LEAF_BIT | D.leaf_off) : D.node_off);
+        D.tile_key = L.key_bits >= 32 ? 0u : (t << L.key_bits);
+        const auto &b = qbox[t];
+        double side = std::max(b[1] - b[0], b[3] - b[2]);
+        if (!(side > 0.0) || !std::isfinite(side))
+            side = 1.0;
+        D.qminx = b[0];
+        D.qminy = b[2];
+        D.qscale = 4294967296.0 / (side * (1.0 + 1e-12));
+        L.pt_off[t + 1] = L.pt_off[t] + D.n;
+        L.leaf_off[t + 1] = L.leaf_off[t] + D.nleaf;
+        L.node_off[t + 1] = L.node_off[t] + (D.nleaf > 0 ? D.nleaf - 1 : 0);
+    }
+    L.np = L.pt_off[T];
+    L.nleaf = L.leaf_off[T];
+    L.nnode = L.node_off[T];
+}
+
+void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q)
+{
+    const uint32_t T = (uint32_t)tiles.size();
+    Q.win_off.assign(T + 1, 0);
+    Q.cell_off.assign(T + 1, 0);
+    Q.blk_off.assign(T + 1, 0);
+    for (uint32_t t = 0; t < T; ++t) {
+        TileDev &D = tiles[t];
+        D.win_off = Q.win_off[t];
+        D.cell_off = Q.cell_off[t];
+        D.blk_off = Q.blk_off[t];
+        D.nbx = (D.ncols + cfg.bx - 1) / cfg.bx;
+        uint32_t nby = (D.nrows + cfg.by - 1) / cfg.by;
+        Q.win_off[t + 1] = Q.win_off[t] + D.ncols * D.nrows;
+        Q.cell_off[t + 1] = Q.cell_off[t] + D.ncolsg * D.nrowsg;
+        Q.blk_off[t + 1] = Q.blk_off[t] + D.nbx * nby;
+    }
+    Q.nwin = Q.win_off[T];
+    Q.ncell = Q.cell_off[T];
+    Q.nblk = Q.blk_off[T];
+}
+
+// ------------------------------------------------------------- the tree --
+
+uint64_t tree_bytes(const TreeLayout &L)
+{
+    uint64_t b = 0;
+    b += align_up(L.np * 8ull, 256) * 3;                  // xs ys zs
+    b += align_up(L.np * 4ull, 256) * 2;                  // ids pos
+    b += align_up(L.nleaf * 32ull, 256);                  // lf_box
+    b += align_up(L.nleaf * 4ull, 256) * 6;               // cnt start npts zid parent key0
+    b += align_up(L.nleaf * 8ull, 256);                   // lf_zkey
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 32ull, 256);  // nd_box
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 4;  // cnt zid parent flag
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 2;  // zkey child
+    return b;
+}
+
+void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
+{
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        uint8_t *p = base + o;
+        o += align_up(bytes, 256);
+        return p;
+    };
+    const uint64_t nn = std::max<uint64_t>(L.nnode, 1);
+    T.xs = (double *)take(L.np * 8ull);
+    T.ys = (double *)take(L.np * 8ull);
+    T.zs = (double *)take(L.np * 8ull);
+    T.ids = (uint32_t *)take(L.np * 4ull);
+    T.pos = (uint32_t *)take(L.np * 4ull);
+    T.lf_box = (double4 *)take(L.nleaf * 32ull);
+    T.lf_cnt = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_start = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_npts = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_zid = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_parent = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_key0 = (uint32_t *)take(L.nleaf * 4ull);
+    T.lf_zkey = (unsigned long long *)take(L.nleaf * 8ull);
+    T.nd_box = (double4 *)take(nn * 32ull);
+    T.nd_cnt = (uint32_t *)take(nn * 4ull);
+    T.nd_zid = (uint32_t *)take(nn * 4ull);
+    T.nd_parent = (uint32_t *)take(nn * 4ull);
+    T.nd_flag = (uint32_t *)take(nn * 4ull);
+    T.nd_zkey = (unsigned long long *)take(nn * 8ull);
+    T.nd_child = (uint2 *)take(nn * 8ull);
+}
+
+TreeDev tree_view(const TreeOut &T)
+{
+    TreeDev V;
+    V.xs = T.xs;
+    V.ys = T.ys;
+    V.zs = T.zs;
+    V.ids = T.ids;
+    V.pos = T.pos;
+    V.lf_box = T.lf_box;
+    V.lf_cnt = T.lf_cnt;
+    V.lf_start = T.lf_start;
+    V.lf_npts = T.lf_npts;
+    V.lf_zid = T.lf_zid;
+    V.lf_zkey = T.lf_zkey;
+    V.nd_box = T.nd_box;
+    V.nd_cnt = T.nd_cnt;
+    V.nd_zid = T.nd_zid;
+    V.nd_zkey = T.nd_zkey;
+    V.nd_child = T.nd_child;
+    return V;
+}
+
+// Scratch of the tree build (keys, values, histograms, look-back flags).
+uint64_t sort_scratch_bytes(uint64_t n, int passes)
+{
+    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1));
+    return align_up(n * 4ull, 256) * 4 + align_up(4 * 256 * 4, 256) + align_up(64, 256) +
+           align_up((uint64_t)passes * tiles * 256 * 4, 256);
+}
+
+struct SortBufs {
+    uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
+};
+
+static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
+{
+    SortBufs S;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        uint8_t *p = base + o;
+        o += align_up(bytes, 256);
+        return p;
+    };
+    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1));
+    S.k0 = (uint32_t *)take(n * 4ull);
+    S.v0 = (uint32_t *)take(n * 4ull);
+    S.k1 = (uint32_t *)take(n * 4ull);
+    S.v1 = (uint32_t *)take(n * 4ull);
+    S.hist = (uint32_t *)take(4 * 256 * 4);
+    S.counters = (uint32_t *)take(64);  // per-pass tile counters
+    S.lookback = (uint32_t *)take((uint64_t)passes * tiles * 256 * 4);
+    return S;
+}
+
+// Runs the LSD passes; keys in S.k0 (values implicit iota when iota).  Returns
+// which buffer pair holds the result (0 -> k0/v0, 1 -> k1/v1).
+static int run_sort(hipStream_t s, SortBufs &S, uint32_t n_max, const uint32_t *d_n, int passes,
+                    bool keys_only, bool iota, bool hist_ready)
+{
+    if (!hist_ready) {
+        launch_fill_u32(s, S.hist, 4 * 256, 0u);
+        launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
+    }
+    launch_sort_scan(s, S.hist, passes, S.counters);
+    uint32_t tiles = sort_tiles(std::max<uint32_t>(n_max, 1));
+    launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
+    uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
+    for (int p = 0; p < passes; ++p) {
+        launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
+                        S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
+                        iota && p == 0, true, 0);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    return (passes & 1) ? 1 : 0;
+}
+
+// Builds the tree of a batch into T (device memory sized by tree_bytes).
+owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
+                             const BatchDev &B, const double *dx, const double *dy,
+                             const double *dz, const TreeOut &T, Arena &scratch, uint32_t *guard)
+{
+    if (L.np == 0)
+        return OWM_OK;
+    const int passes = (L.tile_bits + L.key_bits + 7) / 8;
+    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes));
+    if (!sb) {
+        set_err(ctx, "scratch arena exhausted (tree sort)");
+        return OWM_E_INTERNAL;
+    }
+    SortBufs S = carve_sort(sb, L.np, passes);
+    launch_fill_u32(s, S.hist, 4 * 256, 0u);
+    launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
+    int which = run_sort(s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
+    const uint32_t *skeys = which ? S.k1 : S.k0;
+    const uint32_t *svals = which ? S.v1 : S.v0;
+    launch_gather_leaves(s, B, svals, skeys, dx, dy, dz, T);
+    launch_karras(s, B, T);
+    launch_bottomup(s, B, T, guard);
+    HIP_TRY(hipGetLastError(), "tree build launch");
+    return OWM_OK;
+}
+
+// ------------------------------------------------------------- queries --
+
+uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
+{
+    int bits = std::max(bit_length(np), 1);
+    int passes = (bits + 7) / 8;
+    uint64_t nseed_max = std::max<uint64_t>(Q.nwin, n_in) + Q.ncell + 1;
+    uint64_t b = 0;
+    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes);
+    b += sort_scratch_bytes(nseed_max, passes);                           // seeds sort
+    b += align_up(std::max<uint64_t>(Q.ncell, 1) * 4ull, 256) * 2;      // cell flag / res
+    b += align_up((nseed_max / 4096 + 2) * 4ull, 256) * 2;               // compaction look-back
+    b += align_up(1024, 256);                                           // counters
+    b += align_up((Q.ntiles + 1) * 4ull, 256);                          // seed_off
+    return b;
+}
+
+// Enqueues stages `from`..`to` (1 = stage 1, 2 = sort + stage 2, 3 = stage 3,
+// 4 = merge of LLPs and stage-3 adds).  from == 2 takes the window minima
+// from h_in (n_in values), from == 3 takes the LLPs from h_in.  Results
+// (device): see QueryOut; cnt = [n_llp, n_add, n_seeds, error flags, ...].
+owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const TreeDev &TV,
+                         const QueryLayout &Q, uint64_t np, const Stage1Cfg &cfg,
+                         uint32_t any_need1, Arena &scratch, QueryOut &out, int from, int to,
+                         const uint32_t *h_in, uint64_t n_in)
+{
+    const int bits = std::max(bit_length(np), 1);
+    const int passes = (bits + 7) / 8;
+    const uint64_t nwin = from == 2 ? n_in : Q.nwin;
+    const uint64_t nw = std::max<uint64_t>(nwin, 1);
+    const uint64_t nseed_max = std::max<uint64_t>(nwin, from == 3 ? n_in : 0) + Q.ncell + 1;
+    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes));
+    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes));
+    uint32_t *cell_flag = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
+    uint32_t *cell_res = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
+    uint32_t *lb1 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
+    uint32_t *lb2 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
+    uint32_t *cnt = (uint32_t *)scratch.take(1024);
+    uint32_t *seed_off = (uint32_t *)scratch.take((Q.ntiles + 1) * 4ull);
+    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off) {
+        set_err(ctx, "scratch arena exhausted (query)");
+        return OWM_E_INTERNAL;
+    }
+    SortBufs W = carve_sort(p1, nw, passes);
+    SortBufs SD = carve_sort(p2, nseed_max, passes);
+    out.cnt = cnt;
+    out.seed_off = seed_off;
+    launch_fill_u32(s, cnt, 16, 0u);
+    if (from <= 1) {
+        launch_stage1(s, B, TV, cfg, W.k0, cnt + 3);
+        out.min_ids = W.k0;
+    }
+    if (to <= 1) {
+        HIP_TRY(hipGetLastError(), "stage 1 launch");
+        return OWM_OK;
+    }
+    if (from <= 2) {
+        if (from == 2 && nwin)
+            HIP_TRY(hipMemcpyAsync(W.k0, h_in, nwin * 4ull, hipMemcpyHostToDevice, s), "min_ids H2D");
+        int which = run_sort(s, W, (uint32_t)nwin, nullptr, passes, true, false, false);
+        const uint32_t *sorted = which ? W.k1 : W.k0;
+        out.sorted_min_ids = sorted;
+        launch_stage2_compact(s, B, sorted, (uint32_t)nwin, any_need1, SD.k0, cnt + 0, lb1, cnt + 4);
+    } else {
+        if (n_in)
+            HIP_TRY(hipMemcpyAsync(SD.k0, h_in, n_in * 4ull, hipMemcpyHostToDevice, s), "llp H2D");
+        uint32_t n32 = (uint32_t)n_in;
+        HIP_TRY(hipMemcpyAsync(cnt + 0, &n32, 4, hipMemcpyHostToDevice, s), "n_llp H2D");
+        HIP_TRY(hipStreamSynchronize(s), "n_llp H2D");
+    }
+    out.llp = SD.k0;
+    if (to <= 2) {
+        HIP_TRY(hipGetLastError(), "stage 2 launch");
+        return OWM_OK;
+    }
+    if (Q.ncell > 0) {
+        launch_stage3(s, B, TV, SD.k0, cnt + 0, (uint32_t)(nseed_max - Q.ncell), cell_flag,
+                      cell_res, cnt + 3);
+        launch_compact_valid(s, cell_res, (uint32_t)Q.ncell, SD.k0, cnt + 0, cnt + 1, lb2, cnt + 5);
+    }
+    out.cell_res = cell_res;
+    launch_add_u32(s, cnt + 0, cnt + 1, cnt + 2);
+    if (to <= 3) {
+        HIP_TRY(hipGetLastError(), "stage 3 launch");
+        return OWM_OK;
+    }
+    int which = run_sort(s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false, false);
+    out.seeds = which ? SD.k1 : SD.k0;
+    launch_seed_ranges(s, B, out.seeds, cnt + 2, seed_off);
+    HIP_TRY(hipGetLastError(), "query launch");
+    return OWM_OK;
+}
+
+// ------------------------------------------------------------- context --
+
+uint8_t *Arena::take(uint64_t bytes)
+{
+    bytes = align_up(std::max<uint64_t>(bytes, 1), 256);
+    if (used + bytes > buf.bytes)
+        return nullptr;
+    uint8_t *p = (uint8_t *)buf.ptr + used;
+    used += bytes;
+    return p;
+}
+
+owm_status Arena::ensure(owm_ctx *ctx, uint64_t bytes)
+{
+    used = 0;
+    if (bytes <= buf.bytes)
+        return OWM_OK;
+    if (ctx->budget && ctx->held_bytes() - buf.bytes + bytes > ctx->budget) {
+        set_err(ctx, "work needs %llu device bytes, budget is %llu", (unsigned long long)bytes,
+                (unsigned long long)ctx->budget);
+        return OWM_E_BUDGET;
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "sync before arena growth");
+    e = buf.reserve(bytes);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "arena allocation");
+    return OWM_OK;
+}
+
+
+// -------------------------------------------------------------- batches --
+
+static owm_status check_tiles(owm_ctx *ctx, const owm_tile *tiles, uint32_t n_tiles)
+{
+    if (n_tiles == 0) {
+        set_err(ctx, "n_tiles must be > 0");
+        return OWM_E_ARG;
+    }
+    uint64_t total = 0;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        total += tiles[t].n;
+        uint32_t L = tiles[t].params.leaf_size;
+        if (L > 1024) {
+            set_err(ctx, "tile %u: leaf_size must be <= 1024", t);
+            return OWM_E_ARG;
+        }
+    }
+    if (total >= (1ull << 30)) {
+        set_err(ctx, "a batch holds at most 2^30-1 points (got %llu)", (unsigned long long)total);
+        return OWM_E_ARG;
+    }
+    return OWM_OK;
+}
+
+uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested)
+{
+    if (requested)
+        return requested;
+    return ctx->tuner.leaf_for(n);
+}
+
+Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total)
+{
+    return ctx->tuner.stage1_for(np_total);
+}
+
+// Per-tile bbox (caller's, else an exact device reduction), leaf sizes and the
+// tree layout of a batch whose points are in device memory.
+owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const owm_tile *tiles,
+                        uint32_t n_tiles, Prepared &P)
+{
+    owm_status st = check_tiles(ctx, tiles, n_tiles);
+    if (st != OWM_OK)
+        return st;
+    hipStream_t s = ctx->stream;
+    P.td.assign(n_tiles, TileDev{});
+    P.n.resize(n_tiles);
+    P.leaf.resize(n_tiles);
+    P.box.resize(n_tiles);
+    bool need_bbox = false;
+    uint64_t np = 0;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        memset(&P.td[t], 0, sizeof(TileDev));
+        P.n[t] = tiles[t].n;
+        np += P.n[t];
+        P.leaf[t] = pick_leaf(ctx, P.n[t], tiles[t].params.leaf_size);
+        if (tiles[t].params.flags & OWM_F_HAVE_BBOX) {
+            for (int k = 0; k < 4; ++k)
+                P.box[t][k] = tiles[t].params.bbox[k];
+        } else {
+            need_bbox = true;
+        }
+        if (tiles[t].offset != (t ? tiles[t - 1].offset + tiles[t - 1].n : 0)) {
+            set_err(ctx, "tile %u: tiles must be consecutive slices of the cloud", t);
+            return OWM_E_ARG;
+        }
+    }
+    if (need_bbox) {
+        std::vector<std::array<double, 4>> unit(n_tiles, {0.0, 1.0, 0.0, 1.0});
+        plan_tree(P.td, P.n, P.leaf, unit, P.L);
+        uint64_t bytes = align_up((uint64_t)n_tiles * 32, 256) + align_up((n_tiles + 1) * 4ull, 256);
+        st = ctx->small.ensure(ctx, bytes);
+        if (st != OWM_OK)
+            return st;
+        unsigned long long *bk = (unsigned long long *)ctx->small.take(n_tiles * 32ull);
+        uint32_t *ptoff = (uint32_t *)ctx->small.take((n_tiles + 1) * 4ull);
+        HIP_TRY(hipMemcpyAsync(ptoff, P.L.pt_off.data(), (n_tiles + 1) * 4ull,
+                               hipMemcpyHostToDevice, s),
+                "upload offsets");
+        BatchDev B0{};
+        B0.ntiles = n_tiles;
+        B0.np = (uint32_t)np;
+        B0.pt_off = ptoff;
+        launch_bbox_init(s, bk, n_tiles);
+        if (np)
+            launch_bbox(s, B0, dx, dy, bk);
+        std::vector<unsigned long long> hk(4ull * n_tiles);
+        HIP_TRY(hipMemcpyAsync(hk.data(), bk, n_tiles * 32ull, hipMemcpyDeviceToHost, s), "bbox D2H");
+        HIP_TRY(hipStreamSynchronize(s), "bbox sync");
+        auto dec = [](unsigned long long k) {
+            uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+            double v;
+            memcpy(&v, &b, 8);
+            return v;
+        };
+        for (uint32_t t = 0; t < n_tiles; ++t)
+            if (!(tiles[t].params.flags & OWM_F_HAVE_BBOX)) {
+                if (P.n[t] == 0 || hk[4 * t] == ~0ull || hk[4 * t + 1] == ~0ull) {
+                    set_err(ctx, "tile %u: no finite points to take a bbox from", t);
+                    return OWM_E_ARG;
+                }
+                P.box[t] = {dec(hk[4 * t]), dec(hk[4 * t + 2]), dec(hk[4 * t + 1]),
+                            dec(hk[4 * t + 3])};
+            }
+    }
+    plan_tree(P.td, P.n, P.leaf, P.box, P.L);
+    return OWM_OK;
+}
+
+// Query fields of every tile (derived parameters) and the query layout.
+owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride_bytes,
+                         const std::vector<std::array<double, 4>> &box, uint64_t np_total,
+                         Prepared &P)
+{
+    const uint32_t n_tiles = (uint32_t)P.td.size();
+    P.any_need1 = false;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        const owm_params &pp = *(const owm_params *)((const uint8_t *)params + (size_t)t * stride_bytes);
+        Derived D;
+        std::string why;
+        const double *b = (pp.flags & OWM_F_HAVE_BBOX) ? pp.bbox : box[t].data();
+        owm_status st = derive_params(pp, P.td[t].n, b, D, why);
+        if (st != OWM_OK) {
+            set_err(ctx, "tile %u: %s", t, why.c_str());
+            return st;
+        }
+        fill_query(P.td[t], pp, D);
+        P.any_need1 |= pp.overlap == 0.0;
+    }
+    P.cfg = pick_stage1(ctx, np_total);
+    P.Q.ntiles = n_tiles;
+    plan_query(P.td, P.cfg, P.Q);
+    if (P.Q.nwin >= (1ull << 31) || P.Q.ncell >= (1ull << 31)) {
+        set_err(ctx, "too many windows or cells in one batch");
+        return OWM_E_ARG;
+    }
+    return OWM_OK;
+}
+
+// Uploads the tile table and offset arrays; fills B's pointers.
+owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t n_tiles = (uint32_t)P.td.size();
+    TileDev *d_tiles = (TileDev *)A.take(n_tiles * sizeof(TileDev));
+    uint32_t *d_off[6];
+    for (int k = 0; k < 6; ++k)
+        d_off[k] = (uint32_t *)A.take((n_tiles + 1) * 4ull);
+    if (!d_tiles || !d_off[5]) {
+        set_err(ctx, "scratch arena exhausted (table)");
+        return OWM_E_INTERNAL;
+    }
+    ctx->stage_table(P.td, P.L, P.Q);
+    HIP_TRY(hipMemcpyAsync(d_tiles, ctx->h_tiles, n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s),
+            "upload tiles");
+    for (int k = 0; k < 6; ++k)
+        HIP_TRY(hipMemcpyAsync(d_off[k], ctx->h_off[k], (n_tiles + 1) * 4ull, hipMemcpyHostToDevice, s),
+                "upload offsets");
+    B = BatchDev{};
+    B.ntiles = n_tiles;
+    B.np = (uint32_t)P.L.np;
+    B.nleaf = (uint32_t)P.L.nleaf;
+    B.nnode = (uint32_t)P.L.nnode;
+    B.nwin = (uint32_t)P.Q.nwin;
+    B.ncell = (uint32_t)P.Q.ncell;
+    B.nblk = (uint32_t)P.Q.nblk;
+    B.key_bits = (uint32_t)P.L.key_bits;
+    B.tiles = d_tiles;
+    B.pt_off = d_off[0];
+    B.leaf_off = d_off[1];
+    B.node_off = d_off[2];
+    B.win_off = d_off[3];
+    B.cell_off = d_off[4];
+    B.blk_off = d_off[5];
+    return OWM_OK;
+}
+
+uint64_t table_bytes(uint32_t n_tiles)
+{
+    return align_up((uint64_t)n_tiles * sizeof(TileDev), 256) + 6 * align_up((n_tiles + 1) * 4ull, 256);
+}
+
+// Reads the counts, checks the device guards, copies the seeds (tile-local).
+owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
+                      uint64_t cap, uint64_t *seed_off_host)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t n_tiles = (uint32_t)P.td.size();
+    uint32_t hc[16];
+    std::vector<uint32_t> hoff(n_tiles + 1);
+    HIP_TRY(hipMemcpyAsync(hc, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, s), "counts D2H");
+    HIP_TRY(hipMemcpyAsync(hoff.data(), qo.seed_off, (n_tiles + 1) * 4ull, hipMemcpyDeviceToHost, s),
+            "ranges D2H");
+    HIP_TRY(hipStreamSynchronize(s), "pipeline");
+    if (hc[3]) {
+        set_err(ctx, "device traversal stack overflow (flags %u)", hc[3]);
+        return OWM_E_INTERNAL;
+    }
+    const uint64_t ns = hc[2];
+    for (uint32_t t = 0; t <= n_tiles; ++t)
+        seed_off_host[t] = hoff[t];
+    if (ns > cap) {
+        set_err(ctx, "seed buffer holds %llu, %llu needed", (unsigned long long)cap,
+                (unsigned long long)ns);
+        return OWM_E_CAPACITY;
+    }
+    if (ns)
+        HIP_TRY(hipMemcpy(seeds, qo.seeds, ns * 4ull, hipMemcpyDeviceToHost), "seeds D2H");
+    for (uint32_t t = 0; t < n_tiles; ++t)
+        for (uint64_t i = hoff[t]; i < hoff[t + 1]; ++i)
+            seeds[i] -= P.L.pt_off[t];
+    return OWM_OK;
+}
+
+// Runs the whole pipeline on a batch whose points are in device memory
+// (concatenated SoA).  seeds/seed_off_host receive tile-local ids.
+owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                     const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
+                     uint64_t cap, uint64_t *seed_off_host)
+{
+    (void)flags;
+    Prepared P;
+    owm_status st = prepare_tree(ctx, dx, dy, tiles, n_tiles, P);
+    if (st != OWM_OK)
+        return st;
+    st = prepare_query(ctx, &tiles[0].params, sizeof(owm_tile), P.box, P.L.np, P);
+    if (st != OWM_OK)
+        return st;
+    const int tpasses = (P.L.tile_bits + P.L.key_bits + 7) / 8;
+    const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) +
+                          sort_scratch_bytes(P.L.np, tpasses) + query_bytes(P.Q, P.L.np, 0) + 8192;
+    st = ctx->arena.ensure(ctx, need);
+    if (st != OWM_OK)
+        return st;
+    Arena &A = ctx->arena;
+    uint32_t *guard = (uint32_t *)A.take(256);
+    launch_fill_u32(ctx->stream, guard, 64, 0u);
+    BatchDev B;
+    st = upload_table(ctx, A, P, B);
+    if (st != OWM_OK)
+        return st;
+    TreeOut T;
+    carve_tree(A.take(tree_bytes(P.L)), P.L, T);
+    st = build_tree(ctx, ctx->stream, P.L, B, dx, dy, dz, T, A, guard);
+    if (st != OWM_OK)
+        return st;
+    QueryOut qo;
+    st = enqueue_query(ctx, ctx->stream, B, tree_view(T), P.Q, P.L.np, P.cfg,
+                       P.any_need1 ? 1u : 0u, A, qo, 1, 4, nullptr, 0);
+    if (st != OWM_OK)
+        return st;
+    launch_add_u32(ctx->stream, qo.cnt + 3, guard, qo.cnt + 3);
+    return read_seeds(ctx, qo, P, seeds, cap, seed_off_host);
+}
+
+}  // namespace owm
+
+using owm::TreeLayout;
+using owm::QueryLayout;
+
+void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, const QueryLayout &Q)
+{
+    const uint32_t T = (uint32_t)td.size();
+    if (h_cap < T + 1) {
+        if (h_tiles)
+            (void)hipHostFree(h_tiles);
+        for (auto &p : h_off)
+            if (p)
+                (void)hipHostFree(p);
+        h_cap = T + 1 + 64;
+        (void)hipHostMalloc((void **)&h_tiles, h_cap * sizeof(TileDev), 0);
+        for (auto &p : h_off)
+            (void)hipHostMalloc((void **)&p, h_cap * 4, 0);
+    }
+    // the previous upload from these buffers must have completed
+    (void)hipStreamSynchronize(stream);
+    memcpy(h_tiles, td.data(), T * sizeof(TileDev));
+    const std::vector<uint32_t> *src[6] = {&L.pt_off, &L.leaf_off, &L.node_off,
+                                           &Q.win_off, &Q.cell_off, &Q.blk_off};
+    for (int k = 0; k < 6; ++k)
+        memcpy(h_off[k], src[k]->data(), (T + 1) * 4);
+}
+
+uint64_t owm_ctx::held_bytes() const
+{
+    uint64_t b = arena.buf.bytes + small.buf.bytes + upload.bytes;
+    b += trees_bytes.load();
+    return b;
+}
+
+

@@ -1,0 +1,166 @@
+// Host-side state of the OWM library (C++; never crosses the C ABI).
+#pragma once
+#include <array>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/owm.h"
+#include "owm_kernels.h"
+
+struct owm_ctx;
+
+namespace owm {
+
+struct DevBuf {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    void release();
+    hipError_t reserve(uint64_t need);
+};
+
+// Bump allocator over one device buffer, reset at the start of every run.
+struct Arena {
+    DevBuf buf;
+    uint64_t used = 0;
+    owm_status ensure(owm_ctx *ctx, uint64_t bytes);
+    uint8_t *take(uint64_t bytes);
+};
+
+struct Derived {
+    double bbox[4];
+    double width, height, density, displace;
+    uint32_t min_num_points;
+    uint32_t ncols, nrows, ncolsg, nrowsg;
+    int32_t h, hb;
+    double initx, inity, initxg, inityg;
+};
+
+struct TreeLayout {
+    uint32_t ntiles = 0;
+    int tile_bits = 0, key_bits = 32;
+    uint64_t np = 0, nleaf = 0, nnode = 0;
+    std::vector<uint32_t> pt_off, leaf_off, node_off;
+};
+
+struct QueryLayout {
+    uint32_t ntiles = 0;
+    uint64_t nwin = 0, ncell = 0, nblk = 0;
+    std::vector<uint32_t> win_off, cell_off, blk_off;
+};
+
+struct QueryOut {
+    const uint32_t *min_ids = nullptr;         // stage-1 minima (unsorted, per window)
+    const uint32_t *sorted_min_ids = nullptr;  // after the stage-2 sort
+    const uint32_t *llp = nullptr;             // LLPs (then stage-3 adds) before the merge sort
+    const uint32_t *cell_res = nullptr;        // stage-3 result per cell
+    const uint32_t *seeds = nullptr;           // merged, sorted
+    uint32_t *cnt = nullptr;                   // [0] n_llp [1] n_add [2] n_seeds [3] error flags
+    uint32_t *seed_off = nullptr;
+};
+
+// Size-class tuning table: leaf size and stage-1 window block per point count.
+struct Tuner {
+    struct Entry {
+        uint64_t max_n;
+        uint32_t leaf;
+        Stage1Cfg s1;
+    };
+    std::vector<Entry> table{{~0ull, 32, {8, 8}}};
+    uint32_t leaf_for(uint64_t n) const
+    {
+        for (const auto &e : table)
+            if (n <= e.max_n)
+                return e.leaf;
+        return 32;
+    }
+    Stage1Cfg stage1_for(uint64_t n) const
+    {
+        for (const auto &e : table)
+            if (n <= e.max_n)
+                return e.s1;
+        return {8, 8};
+    }
+};
+
+struct Prepared {
+    std::vector<TileDev> td;
+    std::vector<uint64_t> n;
+    std::vector<uint32_t> leaf;
+    std::vector<std::array<double, 4>> box;
+    TreeLayout L;
+    QueryLayout Q;
+    Stage1Cfg cfg{8, 8};
+    bool any_need1 = false;
+};
+
+owm_status derive_params(const owm_params &P, uint64_t n, const double *bbox, Derived &D,
+                         std::string &why);
+void fill_query(TileDev &T, const owm_params &P, const Derived &D);
+int bit_length(uint64_t v);
+void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
+               const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
+               TreeLayout &L);
+void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
+void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
+uint64_t sort_scratch_bytes(uint64_t n, int passes);
+uint64_t table_bytes(uint32_t n_tiles);
+owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const BatchDev &B,
+                      const double *dx, const double *dy, const double *dz, const TreeOut &T,
+                      Arena &scratch, uint32_t *guard);
+owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const TreeDev &TV,
+                         const QueryLayout &Q, uint64_t np, const Stage1Cfg &cfg,
+                         uint32_t any_need1, Arena &scratch, QueryOut &out, int from, int to,
+                         const uint32_t *h_in, uint64_t n_in);
+owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const owm_tile *tiles,
+                        uint32_t n_tiles, Prepared &P);
+owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride_bytes,
+                         const std::vector<std::array<double, 4>> &box, uint64_t np_total,
+                         Prepared &P);
+owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B);
+owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
+                      uint64_t cap, uint64_t *seed_off_host);
+uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested);
+Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total);
+void set_err(owm_ctx *ctx, const char *fmt, ...);
+owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what);
+owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                     const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
+                     uint64_t cap, uint64_t *seed_off_host);
+uint64_t tree_bytes(const TreeLayout &L);
+uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in);
+TreeDev tree_view(const TreeOut &T);
+
+struct ServeState;
+
+}  // namespace owm
+
+struct owm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::string err;
+    owm::Arena arena;   // tree + scratch of the current run
+    owm::Arena small;   // bbox pass
+    owm::DevBuf upload; // host inputs staged to the device
+    owm::Tuner tuner;
+    std::atomic<uint64_t> trees_bytes{0};
+    ::TileDev *h_tiles = nullptr;
+    uint32_t *h_off[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t h_cap = 0;
+    owm::ServeState *serve = nullptr;  // owned; released by owm_serve_stop
+    void stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
+                     const owm::QueryLayout &Q);
+    uint64_t held_bytes() const;
+    ~owm_ctx();
+};

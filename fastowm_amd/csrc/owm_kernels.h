@@ -1,0 +1,66 @@
+// Host-side launchers of the OWM kernels (one translation unit per kernel
+// family; every launcher enqueues on the given stream and never synchronises).
+#pragma once
+#include "owm_dev.h"
+
+namespace owm {
+
+// ---- radix sort (sort.hip) ----
+uint32_t sort_tiles(uint32_t n);
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+void launch_sort_hist(hipStream_t s, const uint32_t *keys, const uint32_t *d_n, uint32_t n_max,
+                      int passes, uint32_t *hist);
+void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *counters);
+void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
+                     uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n, uint32_t n_max,
+                     int shift, const uint32_t *digit_off, uint32_t *lookback,
+                     uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
+                     uint32_t grid_cap);
+
+// ---- tree (tree.hip) ----
+struct TreeOut {
+    double *xs, *ys, *zs;
+    uint32_t *ids, *pos;
+    double4 *lf_box;
+    uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid, *lf_parent, *lf_key0;
+    unsigned long long *lf_zkey;
+    double4 *nd_box;
+    uint32_t *nd_cnt, *nd_zid, *nd_parent, *nd_flag;
+    unsigned long long *nd_zkey;
+    uint2 *nd_child;
+};
+
+// bbox per tile as order-preserving u64 keys [minx, miny, maxx(inv), maxy(inv)]
+void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                 unsigned long long *bbox_keys);
+void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
+void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                   uint32_t *keys, uint32_t *hist, int passes);
+void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *sorted_vals,
+                          const uint32_t *sorted_keys, const double *x, const double *y,
+                          const double *z, const TreeOut &T);
+void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
+void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *err);
+
+// ---- stages (stages.hip) ----
+struct Stage1Cfg {
+    uint32_t bx, by;  // window block shape
+};
+void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                   uint32_t *min_ids, uint32_t *err);
+// sorted window minima -> LLP list (ascending) + count
+void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
+                           uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,
+                           uint32_t *tile_counter);
+void launch_stage3(hipStream_t s, const BatchDev &B, const TreeDev &T, const uint32_t *llp,
+                   const uint32_t *n_llp, uint32_t n_llp_max, uint32_t *cell_flag,
+                   uint32_t *cell_res, uint32_t *err);
+// cell_res (NONE = no add) -> compacted list appended after the LLPs
+void launch_compact_valid(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out,
+                          const uint32_t *d_out_base, uint32_t *d_count, uint32_t *lookback,
+                          uint32_t *tile_counter);
+void launch_seed_ranges(hipStream_t s, const BatchDev &B, const uint32_t *seeds,
+                        const uint32_t *d_nseeds, uint32_t *seed_off);
+void launch_add_u32(hipStream_t s, const uint32_t *a, const uint32_t *b, uint32_t *out);
+
+}  // namespace owm

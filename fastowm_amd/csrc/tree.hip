@@ -1,0 +1,440 @@
+// Spatial index of the OWM pipeline on gfx950: bbox reduction, 2D Morton
+// keys, gather into Morton order with per-leaf aggregates, Karras LBVH over the
+// leaves and an atomic-flag bottom-up pass that propagates each node's tight
+// bbox, point count and lowest (z, index).
+//
+// Reference counterparts: ordered/cuda/ordered_builder.inl (get_morton_code
+// :33-58, order_points :60-74, init_leafs_size :78-126, init_nodes5 :129-205,
+// create_aabb_size :212-254) and the CPU quadtree + storeMinAndNumPoints of
+// CPP-TBB-CPU/src/optim3_func.cpp:218-307.  Differences by design: 64-bit
+// Morton codes (32 bits per axis) over the square bbox, whose top bits form the
+// sort key; Karras splits over the leaves' Morton keys (spatial splits) rather
+// than over leaf indices; tight point bboxes (not cell geometry), so a node
+// whose bbox lies strictly inside a window can be taken whole, exactly; the
+// (z, index) minimum replaces the reference's 1e-10 tolerance rules.
+#include "owm_kernels.h"
+
+namespace owm {
+
+constexpr int TB = 256;
+constexpr uint32_t CHUNK = 8192;
+
+__device__ __forceinline__ unsigned long long dkey(double v)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double wave_min_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// lexicographic (key, id) minimum over the wave
+__device__ __forceinline__ void wave_min_kid(unsigned long long &k, uint32_t &id)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long k2 = __shfl_xor(k, o, 64);
+        uint32_t i2 = __shfl_xor(id, o, 64);
+        if (owm_less(k2, i2, k, id)) {
+            k = k2;
+            id = i2;
+        }
+    }
+}
+
+__global__ void k_bbox_init(unsigned long long *bk, uint32_t ntiles)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntiles; i += gridDim.x * blockDim.x) {
+        bk[4 * i + 0] = ~0ull;  // min x
+        bk[4 * i + 1] = ~0ull;  // min y
+        bk[4 * i + 2] = 0ull;   // max x
+        bk[4 * i + 3] = 0ull;   // max y
+    }
+}
+
+__device__ __forceinline__ void bbox_flush(unsigned long long *bk, uint32_t t, double mnx, double mny,
+                                           double mxx, double mxy)
+{
+    if (mnx <= mxx) {
+        atomicMin(&bk[4 * t + 0], dkey(mnx));
+        atomicMax(&bk[4 * t + 2], dkey(mxx));
+    }
+    if (mny <= mxy) {
+        atomicMin(&bk[4 * t + 1], dkey(mny));
+        atomicMax(&bk[4 * t + 3], dkey(mxy));
+    }
+}
+
+// Exact fp64 min/max of x and y per tile (NaNs ignored).
+__global__ __launch_bounds__(TB) void k_bbox(BatchDev B, const double *__restrict__ X,
+                                              const double *__restrict__ Y, unsigned long long *bk)
+{
+    __shared__ double red[4][TB / 64];
+    const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
+        uint32_t t0 = owm_find_tile(B.pt_off, B.ntiles, c0);
+        uint32_t t1 = owm_find_tile(B.pt_off, B.ntiles, c1 - 1);
+        double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+        uint32_t t = t0;
+        for (uint32_t i = c0 + tid; i < c1; i += TB) {
+            while (i >= B.pt_off[t + 1]) {
+                bbox_flush(bk, t, mnx, mny, mxx, mxy);
+                mnx = mny = INFINITY;
+                mxx = mxy = -INFINITY;
+                ++t;
+            }
+            double x = X[i], y = Y[i];
+            mnx = fmin(mnx, x);
+            mxx = fmax(mxx, x);
+            mny = fmin(mny, y);
+            mxy = fmax(mxy, y);
+        }
+        if (t0 != t1) {
+            bbox_flush(bk, t, mnx, mny, mxx, mxy);
+            continue;
+        }
+        mnx = wave_min_d(mnx);
+        mny = wave_min_d(mny);
+        mxx = wave_max_d(mxx);
+        mxy = wave_max_d(mxy);
+        if (lane == 0) {
+            red[0][w] = mnx;
+            red[1][w] = mny;
+            red[2][w] = mxx;
+            red[3][w] = mxy;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = 1; k < TB / 64; ++k) {
+                mnx = fmin(mnx, red[0][k]);
+                mny = fmin(mny, red[1][k]);
+                mxx = fmax(mxx, red[2][k]);
+                mxy = fmax(mxy, red[3][k]);
+            }
+            bbox_flush(bk, t0, mnx, mny, mxx, mxy);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ unsigned long long spread2(uint32_t v)
+{
+    unsigned long long x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quantise(double v, double vmin, double scale)
+{
+    double q = __dmul_rn(__dsub_rn(v, vmin), scale);
+    if (!(q > 0.0))
+        return 0u;
+    if (q >= 4294967295.0)
+        return 0xFFFFFFFFu;
+    return (uint32_t)q;
+}
+
+// 64-bit 2D Morton code: 32 bits per axis over the tile's square bbox,
+// x in the even bits, y in the odd bits.
+__device__ __forceinline__ unsigned long long morton64(const TileDev &T, double x, double y)
+{
+    uint32_t qx = quantise(x, T.qminx, T.qscale);
+    uint32_t qy = quantise(y, T.qminy, T.qscale);
+    return spread2(qx) | (spread2(qy) << 1);
+}
+
+// Sort keys (tile bits above the top key_bits of the Morton code) plus the
+// digit histograms of every radix pass.
+__global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restrict__ X,
+                                                const double *__restrict__ Y,
+                                                uint32_t *__restrict__ keys, uint32_t *hist,
+                                                int passes)
+{
+    __shared__ uint32_t h[4][256];
+    for (int i = threadIdx.x; i < 4 * 256; i += TB)
+        (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
+    const uint32_t shift = 64 - B.key_bits;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
+        uint32_t t = owm_find_tile(B.pt_off, B.ntiles, c0);
+        for (uint32_t i = c0 + threadIdx.x; i < c1; i += TB) {
+            while (i >= B.pt_off[t + 1])
+                ++t;
+            const TileDev &T = B.tiles[t];
+            unsigned long long m = morton64(T, X[i], Y[i]);
+            uint32_t k = T.tile_key | (uint32_t)(m >> shift);
+            keys[i] = k;
+            for (int p = 0; p < passes; ++p)
+                atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * 256; i += TB) {
+        uint32_t c = (&h[0][0])[i];
+        if (c)
+            atomicAdd(&hist[i], c);
+    }
+}
+
+// One wave per leaf: gather the leaf's points into Morton order and reduce
+// its tight bbox, count (non-NaN x and y) and lowest (z, index).
+__global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t *__restrict__ vals,
+                                                       const uint32_t *__restrict__ skeys,
+                                                       const double *__restrict__ X,
+                                                       const double *__restrict__ Y,
+                                                       const double *__restrict__ Z, TreeOut T)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = gridDim.x * (TB / 64);
+    for (uint32_t leaf = blockIdx.x * (TB / 64) + (threadIdx.x >> 6); leaf < B.nleaf; leaf += nw) {
+        uint32_t t = owm_find_tile(B.leaf_off, B.ntiles, leaf);
+        const TileDev &TD = B.tiles[t];
+        uint32_t j = leaf - TD.leaf_off;
+        uint32_t start = TD.pt_off + j * TD.leaf_size;
+        uint32_t npts = min(TD.leaf_size, TD.n - j * TD.leaf_size);
+        double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+        uint32_t cnt = 0, bid = OWM_NONE;
+        unsigned long long bk = OWM_ZKEY_NONE;
+        for (uint32_t k = lane; k < npts; k += 64) {
+            uint32_t p = start + k;
+            uint32_t v = vals[p];
+            double x = X[v], y = Y[v], z = Z[v];
+            T.xs[p] = x;
+            T.ys[p] = y;
+            T.zs[p] = z;
+            T.ids[p] = v;
+            T.pos[v] = p;
+            if (!isnan(x) && !isnan(y)) {
+                ++cnt;
+                mnx = fmin(mnx, x);
+                mxx = fmax(mxx, x);
+                mny = fmin(mny, y);
+                mxy = fmax(mxy, y);
+                if (owm_eligible(z)) {
+                    unsigned long long zk = owm_zkey(z);
+                    if (owm_less(zk, v, bk, bid)) {
+                        bk = zk;
+                        bid = v;
+                    }
+                }
+            }
+        }
+        cnt = wave_sum_u(cnt);
+        mnx = wave_min_d(mnx);
+        mny = wave_min_d(mny);
+        mxx = wave_max_d(mxx);
+        mxy = wave_max_d(mxy);
+        wave_min_kid(bk, bid);
+        if (lane == 0) {
+            T.lf_box[leaf] = make_double4(mnx, mny, mxx, mxy);
+            T.lf_cnt[leaf] = cnt;
+            T.lf_start[leaf] = start;
+            T.lf_npts[leaf] = npts;
+            T.lf_zkey[leaf] = bk;
+            T.lf_zid[leaf] = bid;
+            T.lf_key0[leaf] = skeys[start];
+            T.lf_parent[leaf] = OWM_NONE;
+        }
+    }
+}
+
+__device__ __forceinline__ int kdelta(const uint32_t *key0, uint32_t nl, int a, int b)
+{
+    if (b < 0 || b >= (int)nl)
+        return -1;
+    uint32_t ka = key0[a], kb = key0[b];
+    if (ka == kb)
+        return 32 + __clz((uint32_t)a ^ (uint32_t)b);
+    return __clz(ka ^ kb);
+}
+
+// Karras (2012) binary radix tree over the leaves of each tile, keys = Morton
+// key of each leaf's first point, ties broken by leaf index.
+__global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T)
+{
+    for (uint32_t g = blockIdx.x * TB + threadIdx.x; g < B.nnode; g += gridDim.x * TB) {
+        uint32_t t = owm_find_tile(B.node_off, B.ntiles, g);
+        const TileDev &TD = B.tiles[t];
+        const uint32_t nl = TD.nleaf;
+        const uint32_t *key0 = T.lf_key0 + TD.leaf_off;
+        const int i = (int)(g - TD.node_off);
+        int d = (kdelta(key0, nl, i, i + 1) - kdelta(key0, nl, i, i - 1)) > 0 ? 1 : -1;
+        int dmin = kdelta(key0, nl, i, i - d);
+        int lmax = 2;
+        while (kdelta(key0, nl, i, i + lmax * d) > dmin)
+            lmax <<= 1;
+        int l = 0;
+        for (int s = lmax >> 1; s >= 1; s >>= 1)
+            if (kdelta(key0, nl, i, i + (l + s) * d) > dmin)
+                l += s;
+        int j = i + l * d;
+        int dnode = kdelta(key0, nl, i, j);
+        int s = 0;
+        for (int div = 2;; div <<= 1) {
+            int step = (l + div - 1) / div;
+            if (kdelta(key0, nl, i, i + (s + step) * d) > dnode)
+                s += step;
+            if (step <= 1)
+                break;
+        }
+        int gamma = i + s * d + (d < 0 ? -1 : 0);
+        int lo = min(i, j), hi = max(i, j);
+        uint32_t left, right;
+        if (lo == gamma) {
+            left = OWM_LEAF_BIT | (TD.leaf_off + gamma);
+            T.lf_parent[TD.leaf_off + gamma] = g;
+        } else {
+            left = TD.node_off + gamma;
+            T.nd_parent[TD.node_off + gamma] = g;
+        }
+        if (hi == gamma + 1) {
+            right = OWM_LEAF_BIT | (TD.leaf_off + gamma + 1);
+            T.lf_parent[TD.leaf_off + gamma + 1] = g;
+        } else {
+            right = TD.node_off + gamma + 1;
+            T.nd_parent[TD.node_off + gamma + 1] = g;
+        }
+        T.nd_child[g] = make_uint2(left, right);
+        if (i == 0)
+            T.nd_parent[g] = OWM_NONE;
+    }
+}
+
+struct Agg {
+    double4 box;
+    uint32_t cnt, zid;
+    unsigned long long zk;
+};
+
+__device__ __forceinline__ Agg load_agg(const TreeOut &T, uint32_t ref)
+{
+    Agg a;
+    if (ref & OWM_LEAF_BIT) {
+        uint32_t l = ref & ~OWM_LEAF_BIT;
+        a.box = T.lf_box[l];
+        a.cnt = T.lf_cnt[l];
+        a.zk = T.lf_zkey[l];
+        a.zid = T.lf_zid[l];
+    } else {
+        a.box = T.nd_box[ref];
+        a.cnt = T.nd_cnt[ref];
+        a.zk = T.nd_zkey[ref];
+        a.zid = T.nd_zid[ref];
+    }
+    return a;
+}
+
+// Bottom-up: the second child to arrive at a node merges both children
+// (atomic arrival counter per node, release/acquire fences around it).
+__global__ __launch_bounds__(TB) void k_bottomup(BatchDev B, TreeOut T, uint32_t *err)
+{
+    for (uint32_t l = blockIdx.x * TB + threadIdx.x; l < B.nleaf; l += gridDim.x * TB) {
+        uint32_t p = T.lf_parent[l];
+        for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
+            if (depth > 128) {  // deeper than any tree over < 2^30 leaves: malformed
+                atomicOr(err, 4u);
+                break;
+            }
+            __threadfence();
+            uint32_t old = atomicAdd(&T.nd_flag[p], 1u);
+            if (old == 0)
+                break;
+            __threadfence();
+            uint2 c = T.nd_child[p];
+            Agg a = load_agg(T, c.x), b = load_agg(T, c.y);
+            double4 bx = make_double4(fmin(a.box.x, b.box.x), fmin(a.box.y, b.box.y),
+                                      fmax(a.box.z, b.box.z), fmax(a.box.w, b.box.w));
+            T.nd_box[p] = bx;
+            T.nd_cnt[p] = a.cnt + b.cnt;
+            if (owm_less(b.zk, b.zid, a.zk, a.zid)) {
+                a.zk = b.zk;
+                a.zid = b.zid;
+            }
+            T.nd_zkey[p] = a.zk;
+            T.nd_zid[p] = a.zid;
+            p = T.nd_parent[p];
+        }
+    }
+}
+
+static uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap)
+{
+    uint64_t g = (work + per_block - 1) / per_block;
+    if (g < 1)
+        g = 1;
+    if (g > cap)
+        g = cap;
+    return (uint32_t)g;
+}
+
+void launch_bbox_init(hipStream_t s, unsigned long long *bk, uint32_t ntiles)
+{
+    hipLaunchKernelGGL(k_bbox_init, dim3(grid_for(ntiles, TB, 64)), dim3(TB), 0, s, bk, ntiles);
+}
+
+void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                 unsigned long long *bk)
+{
+    hipLaunchKernelGGL(k_bbox, dim3(grid_for(B.np, CHUNK, 4096)), dim3(TB), 0, s, B, x, y, bk);
+}
+
+void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                   uint32_t *keys, uint32_t *hist, int passes)
+{
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, keys,
+                       hist, passes);
+}
+
+void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *vals,
+                          const uint32_t *skeys, const double *x, const double *y, const double *z,
+                          const TreeOut &T)
+{
+    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.nleaf, TB / 64, 8192)), dim3(TB), 0, s, B,
+                       vals, skeys, x, y, z, T);
+}
+
+void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
+{
+    if (B.nnode == 0)
+        return;
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(B.nnode, TB, 8192)), dim3(TB), 0, s, B, T);
+}
+
+void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *err)
+{
+    if (B.nnode == 0)
+        return;
+    launch_fill_u32(s, T.nd_flag, B.nnode, 0u);
+    hipLaunchKernelGGL(k_bottomup, dim3(grid_for(B.nleaf, TB, 8192)), dim3(TB), 0, s, B, T, err);
+}
+
+}  // namespace owm

@@ -1,0 +1,322 @@
+"""Python host interface of the MI355X OWM ground-seed pipeline (ctypes over
+libfastowm.so, the C ABI of include/owm.h).
+
+The names mirror the reference's C++ interface for this path
+(CPP-TBB-CPU/include/optim3_func.hpp:85-93): ``parallel_qtree`` builds the
+spatial index (there: quadtree + storeMinAndNumPoints; here: a device-resident
+LBVH), ``stage1`` / ``stage2`` / ``stage3`` run the three OWM stages on it, and
+``run_tile`` is the reference driver's timed loop (owm_optim3.cpp:143-230).
+Point ids are 0-based indices into the caller's arrays (the reference's 1-based
+id minus one).  Errors raise OWMError carrying the library's status and message;
+nothing falls back to a CPU path: without the HIP library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfastowm.so")
+NONE = 0xFFFFFFFF
+F_HAVE_BBOX = 1
+B_GRAPH = 1
+
+STATUS = {0: "OK", 1: "E_ARG", 2: "E_NOMEM", 3: "E_BUDGET", 4: "E_DEVICE", 5: "E_CAPACITY",
+          6: "E_CANCELLED", 7: "E_NOTREADY", 8: "E_NOTFOUND", 9: "E_INTERNAL"}
+
+
+class OWMError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("wsize", ctypes.c_uint32), ("bsize", ctypes.c_uint32),
+                ("overlap", ctypes.c_double), ("leaf_size", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("bbox", ctypes.c_double * 4)]
+
+
+class Derived(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_double), ("height", ctypes.c_double),
+                ("density", ctypes.c_double), ("displace", ctypes.c_double),
+                ("min_num_points", ctypes.c_uint32), ("ncols", ctypes.c_uint32),
+                ("nrows", ctypes.c_uint32), ("ncolsg", ctypes.c_uint32),
+                ("nrowsg", ctypes.c_uint32), ("bbox", ctypes.c_double * 4)]
+
+
+class TileDesc(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("params", Params)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # One HIP runtime per process: when torch is installed its bundled
+        # libamdhip64.so.7 must be the one loaded first, so that this library's
+        # dependency of the same soname binds to it instead of a second copy.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, u32, dp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p
+        P = ctypes.POINTER
+        sig = {
+            "owm_ctx_create": ([ctypes.c_int, u64, P(vp)], ctypes.c_int),
+            "owm_ctx_destroy": ([vp], None),
+            "owm_last_error": ([vp], ctypes.c_char_p),
+            "owm_status_str": ([ctypes.c_int], ctypes.c_char_p),
+            "owm_ctx_device_bytes": ([vp], u64),
+            "owm_derive": ([P(Params), u64, dp, P(Derived)], ctypes.c_int),
+            "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
+            "owm_run_tile_device": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
+            "owm_batch_run": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp], ctypes.c_int),
+            "owm_batch_run_device": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp],
+                                     ctypes.c_int),
+            "owm_tree_build": ([vp, dp, dp, dp, u64, ctypes.c_int, u32, P(vp)], ctypes.c_int),
+            "owm_tree_free": ([vp], None),
+            "owm_tree_points": ([vp], u64),
+            "owm_tree_info": ([vp, P(u64), P(u64), P(u64), dp], ctypes.c_int),
+            "owm_stage1": ([vp, vp, P(Params), dp, u64, P(Derived)], ctypes.c_int),
+            "owm_stage2": ([vp, vp, P(Params), dp, u64, dp, u64, P(u64)], ctypes.c_int),
+            "owm_stage3": ([vp, vp, P(Params), dp, u64, dp, u64, P(u64)], ctypes.c_int),
+            "owm_tree_run": ([vp, vp, P(Params), dp, u64, P(u64)], ctypes.c_int),
+            "owm_serve_start": ([vp, u32, u32], ctypes.c_int),
+            "owm_serve_submit": ([vp, dp, dp, dp, u64, P(Params), u64, P(u64)], ctypes.c_int),
+            "owm_serve_poll": ([vp, u64, dp, u64, P(u64), P(ctypes.c_double)], ctypes.c_int),
+            "owm_serve_wait": ([vp, u64, dp, u64, P(u64), P(ctypes.c_double)], ctypes.c_int),
+            "owm_serve_cancel": ([vp, u64], ctypes.c_int),
+            "owm_serve_evict": ([vp, u64], ctypes.c_int),
+            "owm_serve_stop": ([vp], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def make_params(wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None) -> Params:
+    p = Params()
+    p.wsize, p.bsize, p.overlap, p.leaf_size = int(wsize), int(bsize), float(overlap), int(leaf_size)
+    p.flags = 0
+    if bbox is not None:
+        p.flags |= F_HAVE_BBOX
+        for k in range(4):
+            p.bbox[k] = float(bbox[k])
+    return p
+
+
+def derive(wsize, bsize, overlap, n, bbox) -> Derived:
+    """Derived parameters (owm_optim3.cpp:92-132) without device work."""
+    d = Derived()
+    p = make_params(wsize, bsize, overlap, 0, bbox)
+    st = lib().owm_derive(ctypes.byref(p), int(n), None, ctypes.byref(d))
+    if st:
+        raise OWMError(st, "parameters rejected")
+    return d
+
+
+def _host(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data
+
+
+def _is_device(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+def _dev_ptr(t):
+    import torch
+    if t.dtype != torch.float64 or not t.is_contiguous():
+        raise TypeError("device arrays must be contiguous float64 torch tensors")
+    # the library runs on its own stream: make the producer's work visible first
+    torch.cuda.current_stream(t.device).synchronize()
+    return t.data_ptr()
+
+
+class Tree:
+    """A device-resident spatial index of one cloud (parallel_qtree +
+    storeMinAndNumPoints of the reference)."""
+
+    def __init__(self, ctx: "Context", handle, n: int):
+        self.ctx, self.h, self.n = ctx, handle, n
+
+    def info(self):
+        nl, ni, rc = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        box = (ctypes.c_double * 4)()
+        self.ctx._check(lib().owm_tree_info(self.h, ctypes.byref(nl), ctypes.byref(ni),
+                                            ctypes.byref(rc), box))
+        return {"leaves": nl.value, "internal": ni.value, "root_count": rc.value,
+                "root_bbox": tuple(box)}
+
+    def free(self):
+        if self.h and getattr(self.ctx, "h", None):
+            lib().owm_tree_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device, one stream, reusable device arenas."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        st = lib().owm_ctx_create(device, hbm_budget, ctypes.byref(h))
+        if st:
+            raise OWMError(st, f"cannot create a context on device {device}")
+        self.h = h
+        self._trees = weakref.WeakSet()
+
+    def close(self):
+        if getattr(self, "h", None):
+            for t in list(self._trees):
+                t.free()
+            lib().owm_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        return lib().owm_last_error(self.h).decode()
+
+    def device_bytes(self) -> int:
+        return int(lib().owm_ctx_device_bytes(self.h))
+
+    def _check(self, st):
+        if st:
+            raise OWMError(st, self.last_error())
+
+    # -- whole pipeline ----------------------------------------------------
+    def run_tile(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None,
+                 cap=None):
+        """Seeds (sorted 0-based ids) of one tile; x/y/z are numpy arrays or
+        contiguous float64 torch tensors already on the GPU."""
+        p = make_params(wsize, bsize, overlap, leaf_size, bbox)
+        n = len(x)
+        cap = n + 1 if cap is None else cap
+        out = np.empty(max(cap, 1), np.uint32)
+        ns = ctypes.c_uint64()
+        if _is_device(x):
+            st = lib().owm_run_tile_device(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), n,
+                                           ctypes.byref(p), out.ctypes.data, cap, ctypes.byref(ns))
+        else:
+            (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
+            st = lib().owm_run_tile(self.h, xp, yp, zp, n, ctypes.byref(p), out.ctypes.data, cap,
+                                    ctypes.byref(ns))
+        self._check(st)
+        return out[:ns.value].copy()
+
+    def batch_run(self, x, y, z, tiles, graph=False):
+        """tiles: list of (n, dict(wsize=, bsize=, overlap=, leaf_size=, bbox=)) in
+        cloud order.  Returns a list of seed arrays (tile-local ids)."""
+        descs = (TileDesc * len(tiles))()
+        off = 0
+        for i, (n, kw) in enumerate(tiles):
+            descs[i].offset = off
+            descs[i].n = n
+            descs[i].params = make_params(**kw)
+            off += n
+        cap = off + len(tiles) + 1
+        out = np.empty(cap, np.uint32)
+        so = np.zeros(len(tiles) + 1, np.uint64)
+        flags = B_GRAPH if graph else 0
+        if _is_device(x):
+            st = lib().owm_batch_run_device(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), descs,
+                                            len(tiles), flags, out.ctypes.data, cap, so.ctypes.data)
+        else:
+            (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
+            st = lib().owm_batch_run(self.h, xp, yp, zp, descs, len(tiles), flags,
+                                     out.ctypes.data, cap, so.ctypes.data)
+        self._check(st)
+        return [out[int(so[i]):int(so[i + 1])].copy() for i in range(len(tiles))]
+
+    # -- stage-level interface (reference names) ----------------------------
+    def parallel_qtree(self, x, y, z, leaf_size=32) -> Tree:
+        n = len(x)
+        h = ctypes.c_void_p()
+        if _is_device(x):
+            st = lib().owm_tree_build(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), n, 1,
+                                      leaf_size, ctypes.byref(h))
+        else:
+            (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
+            st = lib().owm_tree_build(self.h, xp, yp, zp, n, 0, leaf_size, ctypes.byref(h))
+        self._check(st)
+        tree = Tree(self, h, n)
+        self._trees.add(tree)
+        return tree
+
+    def stage1(self, tree: Tree, wsize, overlap, bsize=20, bbox=None):
+        """Per-window minima, min_ids[jj*ncols + ii] (NONE = invalid window)."""
+        p = make_params(wsize, bsize, overlap, 0, bbox)
+        d = Derived()
+        cap = 1 << 20
+        while True:
+            out = np.empty(cap, np.uint32)
+            st = lib().owm_stage1(self.h, tree.h, ctypes.byref(p), out.ctypes.data, cap,
+                                  ctypes.byref(d))
+            if st == 5 and d.ncols:
+                cap = d.ncols * d.nrows
+                continue
+            self._check(st)
+            return out[:d.ncols * d.nrows].copy(), d
+
+    def stage2(self, tree: Tree, min_ids, wsize, overlap, bsize=20, bbox=None):
+        p = make_params(wsize, bsize, overlap, 0, bbox)
+        m = np.ascontiguousarray(min_ids, np.uint32)
+        out = np.empty(max(len(m), 1), np.uint32)
+        n = ctypes.c_uint64()
+        self._check(lib().owm_stage2(self.h, tree.h, ctypes.byref(p), m.ctypes.data, len(m),
+                                     out.ctypes.data, len(out), ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    def stage3(self, tree: Tree, llp, wsize, overlap, bsize=20, bbox=None):
+        p = make_params(wsize, bsize, overlap, 0, bbox)
+        l_ = np.ascontiguousarray(llp, np.uint32)
+        cap = 1 << 16
+        while True:
+            out = np.empty(cap, np.uint32)
+            n = ctypes.c_uint64()
+            st = lib().owm_stage3(self.h, tree.h, ctypes.byref(p), l_.ctypes.data, len(l_),
+                                  out.ctypes.data, cap, ctypes.byref(n))
+            if st == 5:
+                cap = int(n.value)
+                continue
+            self._check(st)
+            return out[:n.value].copy()
+
+    def tree_run(self, tree: Tree, wsize, bsize, overlap, bbox=None):
+        p = make_params(wsize, bsize, overlap, 0, bbox)
+        cap = tree.n + 1
+        out = np.empty(cap, np.uint32)
+        n = ctypes.c_uint64()
+        self._check(lib().owm_tree_run(self.h, tree.h, ctypes.byref(p), out.ctypes.data, cap,
+                                       ctypes.byref(n)))
+        return out[:n.value].copy()

@@ -1,0 +1,175 @@
+/*
+ * owm.h -- C ABI of the MI355X-native OWM (Overlapping Window Minimum)
+ * ground-seed pipeline.
+ *
+ * A LiDAR cloud (fp64 x, y, z arrays) and the OWM parameters go in; the
+ * indices of the ground seed points come out, sorted ascending, 0-based into
+ * the caller's arrays (the reference's 1-based id minus one; the reference's
+ * sentinel slot 0 does not exist here).
+ *
+ * Plain C types only: no C++ or torch types cross this boundary, no call ever
+ * exits the process, no exception crosses it.  Every entry point returns an
+ * owm_status; owm_last_error() gives the message of the last failure on a
+ * context.  One context may be used by one host thread at a time (the serving
+ * loop owns its own thread and locks its own queue).
+ */
+#ifndef OWM_H
+#define OWM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum owm_status {
+    OWM_OK = 0,
+    OWM_E_ARG = 1,       /* invalid parameters or inputs */
+    OWM_E_NOMEM = 2,     /* device or host allocation failed */
+    OWM_E_BUDGET = 3,    /* the work does not fit the context's HBM budget */
+    OWM_E_DEVICE = 4,    /* HIP runtime error */
+    OWM_E_CAPACITY = 5,  /* output buffer too small; the size needed is returned */
+    OWM_E_CANCELLED = 6, /* served request was cancelled */
+    OWM_E_NOTREADY = 7,  /* served request not finished yet */
+    OWM_E_NOTFOUND = 8,  /* unknown request id or tree handle */
+    OWM_E_INTERNAL = 9   /* a device-side capacity guard tripped (see message) */
+} owm_status;
+
+/* owm_params.flags */
+#define OWM_F_HAVE_BBOX 1u /* bbox[] holds the header bbox; else computed from the data */
+
+/* Parameters of one OWM run.  Field meanings follow the reference CLI
+ * (CPP-TBB-CPU/application/owm_optim3.cpp:14-40): -W Wsize, -B Bsize,
+ * -O Overlap, -s maxNumber.  bbox is the header bbox of readXYZfile
+ * (CPP-TBB-CPU/src/optim3_func.cpp:681-700), in header order. */
+typedef struct owm_params {
+    uint32_t wsize;     /* sliding-window side in metres (u32, as the reference) */
+    uint32_t bsize;     /* stage-3 grid cell side in metres; 0 skips stage 3 */
+    double overlap;     /* window overlap ratio, 0 <= overlap < 1 */
+    uint32_t leaf_size; /* maxNumber points per tree leaf; 0 = autotuned */
+    uint32_t flags;     /* OWM_F_* */
+    double bbox[4];     /* minx, maxx, miny, maxy when OWM_F_HAVE_BBOX */
+} owm_params;
+
+/* Derived parameters (application/owm_optim3.cpp:92-132), for callers that
+ * size their buffers: windows = ncols*nrows, grid cells = ncolsg*nrowsg. */
+typedef struct owm_derived {
+    double width, height, density, displace;
+    uint32_t min_num_points; /* after the reference's ushort truncation */
+    uint32_t ncols, nrows, ncolsg, nrowsg;
+    double bbox[4];
+} owm_derived;
+
+typedef struct owm_ctx owm_ctx;
+typedef struct owm_tree owm_tree;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device`.  hbm_budget_bytes caps the device
+ * memory the context may hold (0 = 90% of the device's free memory). */
+owm_status owm_ctx_create(int device, uint64_t hbm_budget_bytes, owm_ctx **out);
+void owm_ctx_destroy(owm_ctx *ctx);
+const char *owm_last_error(const owm_ctx *ctx);
+const char *owm_status_str(owm_status s);
+/* Device bytes currently held by the context's arenas. */
+uint64_t owm_ctx_device_bytes(const owm_ctx *ctx);
+
+/* Parameter derivation only (no device work): owm_optim3.cpp:92-132. */
+owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_or_null,
+                      owm_derived *out);
+
+/* ---- whole pipeline, one tile -------------------------------------------
+ * Replaces the timed loop of the reference driver: tree construction
+ * (parallel_qtree + storeMinAndNumPoints, owm_optim3.cpp:77-89), stage 1,
+ * sort + stage 2, stage 3 and the final merge (owm_optim3.cpp:143-230).
+ * seeds: caller-owned, capacity `cap`; on OWM_E_CAPACITY *n_seeds holds the
+ * size needed and nothing is written. */
+owm_status owm_run_tile(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                        uint64_t n, const owm_params *params, uint32_t *seeds, uint64_t cap,
+                        uint64_t *n_seeds);
+/* Same, with x/y/z already in device memory (HBM); seeds go to host memory. */
+owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                               const double *d_z, uint64_t n, const owm_params *params,
+                               uint32_t *seeds, uint64_t cap, uint64_t *n_seeds);
+
+/* ---- batch: many independent tiles in one launch chain ------------------
+ * The tiles are slices [offset, offset+n) of one concatenated SoA cloud.
+ * Seeds of tile t are written to seeds[seed_off[t] .. seed_off[t+1]) with
+ * tile-local indices; seed_off has n_tiles+1 entries. */
+typedef struct owm_tile {
+    uint64_t offset;
+    uint64_t n;
+    owm_params params;
+} owm_tile;
+
+/* flags of owm_batch_run*: */
+#define OWM_B_GRAPH 1u /* replay the device chain from a captured hipGraph */
+
+owm_status owm_batch_run(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                         const owm_tile *tiles, uint32_t n_tiles, uint32_t flags,
+                         uint32_t *seeds, uint64_t cap, uint64_t *seed_off);
+owm_status owm_batch_run_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                const double *d_z, const owm_tile *tiles, uint32_t n_tiles,
+                                uint32_t flags, uint32_t *seeds, uint64_t cap,
+                                uint64_t *seed_off);
+
+/* ---- stage-level API on a device-resident tree ---------------------------
+ * Mirrors the reference's internal boundary (include/optim3_func.hpp:85-93):
+ * parallel_qtree + storeMinAndNumPoints -> owm_tree_build, stage1 ->
+ * owm_stage1, sort + stage2 -> owm_stage2, stage3 -> owm_stage3.  The tree
+ * depends only on the cloud and the leaf size, so one tree serves any number
+ * of parameter re-queries. */
+owm_status owm_tree_build(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                          uint64_t n, int device_ptrs, uint32_t leaf_size, owm_tree **out);
+void owm_tree_free(owm_tree *tree);
+uint64_t owm_tree_points(const owm_tree *tree);
+/* Tree statistics for checks: counts of leaves and internal nodes, the root's
+ * point count and bbox (minx, maxx, miny, maxy). */
+owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_internal,
+                         uint64_t *root_count, double *root_bbox);
+
+/* Stage 1: min_ids[jj*ncols + ii] = index of the lowest point of window
+ * (ii, jj) if it holds at least minNumPoints points, else 0xFFFFFFFF
+ * (the reference's -1).  cap >= ncols*nrows. */
+owm_status owm_stage1(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
+                      uint32_t *min_ids, uint64_t cap, owm_derived *derived);
+/* Stage 2 (sort + run length): ids that are the minimum of >= 2 windows
+ * (all valid minima when overlap == 0), ascending. */
+owm_status owm_stage2(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
+                      const uint32_t *min_ids, uint64_t n_ids, uint32_t *llp, uint64_t cap,
+                      uint64_t *n_llp);
+/* Stage 3: for every Bsize cell with no LLP strictly inside, the lowest
+ * point strictly inside it, if any (cell order). */
+owm_status owm_stage3(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
+                      const uint32_t *llp, uint64_t n_llp, uint32_t *adds, uint64_t cap,
+                      uint64_t *n_adds);
+/* Stages 1-3 on a resident tree; seeds as owm_run_tile. */
+owm_status owm_tree_run(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
+                        uint32_t *seeds, uint64_t cap, uint64_t *n_seeds);
+
+/* ---- serving: continuous batching with HBM-budget admission --------------
+ * A background thread admits submitted tiles while device memory and
+ * in-flight slots allow, runs each admitted group as one batch on its own
+ * stream (groups overlap on the device), and keeps trees of tiles submitted
+ * with a nonzero tile_key resident, so re-queries of the same key with other
+ * parameters skip the upload and the tree build.  Input arrays must stay
+ * valid until the request completes. */
+owm_status owm_serve_start(owm_ctx *ctx, uint32_t max_inflight, uint32_t max_batch_tiles);
+owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                            uint64_t n, const owm_params *params, uint64_t tile_key,
+                            uint64_t *req_id);
+/* OWM_E_NOTREADY while running; on completion copies the seeds and returns
+ * the request's own status.  latency_s: submit -> seeds ready. */
+owm_status owm_serve_poll(owm_ctx *ctx, uint64_t req_id, uint32_t *seeds, uint64_t cap,
+                          uint64_t *n_seeds, double *latency_s);
+owm_status owm_serve_wait(owm_ctx *ctx, uint64_t req_id, uint32_t *seeds, uint64_t cap,
+                          uint64_t *n_seeds, double *latency_s);
+owm_status owm_serve_cancel(owm_ctx *ctx, uint64_t req_id);
+/* Drops the resident tree of tile_key (if any). */
+owm_status owm_serve_evict(owm_ctx *ctx, uint64_t tile_key);
+owm_status owm_serve_stop(owm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OWM_H */

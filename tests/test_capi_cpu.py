@@ -1,0 +1,83 @@
+"""CPU-side checks of the C ABI: the HIP library loads, exports every entry point
+include/owm.h declares, and its host-only parameter derivation equals the
+oracle's (no device call is made here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "owm.h")
+LIB = os.path.join(ROOT, "fastowm_amd", "libfastowm.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(owm_[a-z0-9_]+)\s*\(", text)))
+
+
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="libfastowm.so not built")
+
+
+def test_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) >= 25
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_header_is_plain_c():
+    raw = open(HEADER).read()
+    assert 'extern "C"' in raw
+    text = re.sub(r"/\*.*?\*/", "", raw, flags=re.S)
+    for bad in ("std::", "torch", "at::", "template", "class "):
+        assert bad not in text
+
+
+def test_status_strings():
+    from fastowm_amd import owm
+    L = owm.lib()
+    assert L.owm_status_str(0) == b"ok"
+    assert L.owm_status_str(5) == b"output buffer too small"
+
+
+@pytest.mark.parametrize("ws,bs,ov,box,n", [
+    (12, 20, 0.8, (500000.0, 500500.0, 4700000.0, 4700500.0), 1_000_000),
+    (13, 21, 0.8, (512000.123, 512731.4, 4700001.7, 4700200.05), 300_000),
+    (7, 11, 0.5, (0.0, 70.7, 0.0, 70.7), 20_000),
+    (8, 15, 0.0, (10.0, 1010.0, 20.0, 520.0), 250_000),
+    (20, 30, 0.9, (1.0, 2.0, 3.0, 4.0), 3),
+])
+def test_derive_matches_oracle(ws, bs, ov, box, n):
+    from fastowm_amd import owm
+    d = owm.derive(ws, bs, ov, n, box)
+    p = O.params(ws, bs, ov, box, n)
+    assert (d.width, d.height, d.density, d.displace) == (p.width, p.height, p.density, p.displace)
+    assert (d.ncols, d.nrows, d.ncolsg, d.nrowsg) == (p.ncols, p.nrows, p.ncolsg, p.nrowsg)
+    assert d.min_num_points == p.min_num_points
+
+
+@pytest.mark.parametrize("ws,ov,box", [
+    (12, 1.0, (0.0, 10.0, 0.0, 10.0)),
+    (0, 0.8, (0.0, 10.0, 0.0, 10.0)),
+    (12, 0.8, (0.0, 0.001, 0.0, 10.0)),
+    (12, -0.1, (0.0, 10.0, 0.0, 10.0)),
+    (70000, 0.8, (0.0, 10.0, 0.0, 10.0)),
+])
+def test_derive_rejects_like_oracle(ws, ov, box):
+    from fastowm_amd import owm
+    with pytest.raises(owm.OWMError) as e:
+        owm.derive(ws, 20, ov, 100, box)
+    assert e.value.name == "E_ARG"
+    assert O.params(ws, 20, ov, box, 100) is None
+
+
+def test_python_api_has_reference_names():
+    from fastowm_amd import owm
+    for name in ("parallel_qtree", "stage1", "stage2", "stage3", "run_tile", "batch_run"):
+        assert hasattr(owm.Context, name)

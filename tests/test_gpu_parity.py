@@ -1,0 +1,189 @@
+"""HIP path vs the CPU oracle (bit-exact seed sets and per-window minima).
+
+Every case prints its name before touching the device, so a failing run names it.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fastowm_amd import owm, synth
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = owm.Context(0)
+    yield c
+    c.close()
+
+
+def _tile(n, seed=0, **kw):
+    t = synth.scaled(synth.config_c1(seed), n)
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+SMALL = [
+    # n, wsize, bsize, overlap, leaf, generator overrides
+    (3000, 12, 20, 0.8, 32, {}),
+    (3000, 12, 20, 0.8, 1, {}),
+    (5000, 13, 21, 0.8, 8, {"cluster_frac": 0.5, "n_clusters": 5}),
+    (5000, 7, 11, 0.5, 16, {}),
+    (4000, 10, 20, 0.9, 128, {"cluster_frac": 0.3, "n_clusters": 3}),
+    (4000, 8, 15, 0.0, 32, {}),
+    (6000, 16, 30, 0.75, 64, {"roof_frac": 0.3, "n_roofs": 4}),
+    (2500, 5, 10, 0.6, 3, {}),
+    (50_000, 12, 20, 0.8, 32, {"cluster_frac": 0.7, "n_clusters": 6, "cluster_sigma": 8.0}),
+    (200_000, 12, 20, 0.8, 0, {"cluster_frac": 0.1, "n_clusters": 20}),
+]
+
+
+@pytest.mark.parametrize("case", SMALL, ids=[f"n{c[0]}_w{c[1]}_b{c[2]}_o{c[3]}_l{c[4]}" for c in SMALL])
+def test_small_tiles_match_oracle(ctx, case):
+    n, ws, bs, ov, leaf, over = case
+    say("small", case)
+    x, y, z = _tile(n, seed=n + ws, **over).generate()
+    want, _, _ = O.run(x, y, z, ws, bs, ov, leaf or 32)
+    got = ctx.run_tile(x, y, z, ws, bs, ov, leaf)
+    assert np.array_equal(got, want)
+
+
+def test_stage_api_matches_oracle(ctx):
+    say("stage api 30k")
+    x, y, z = _tile(30_000, seed=4, cluster_frac=0.6, n_clusters=4).generate()
+    box = synth.bbox(x, y)
+    seeds, mins, llp, adds, p = O.seeds_from_stages(x, y, z, 12, 20, 0.8, 16, box)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=16)
+    info = tree.info()
+    assert info["root_count"] == len(x)
+    assert info["root_bbox"] == box
+    assert info["leaves"] == (len(x) + 15) // 16 and info["internal"] == info["leaves"] - 1
+    gmins, d = ctx.stage1(tree, 12, 0.8, 20)
+    assert (d.ncols, d.nrows) == (p.ncols, p.nrows)
+    assert np.array_equal(gmins, mins), "stage-1 window minima differ"
+    gllp = ctx.stage2(tree, gmins, 12, 0.8, 20)
+    assert np.array_equal(gllp, llp)
+    gadds = ctx.stage3(tree, gllp, 12, 0.8, 20)
+    assert np.array_equal(gadds, adds), "stage-3 additions differ (cell order)"
+    assert np.array_equal(ctx.tree_run(tree, 12, 20, 0.8), seeds)
+    # re-query the resident tree with other parameters
+    for ws, bs, ov in [(8, 15, 0.5), (16, 21, 0.9), (13, 30, 0.0)]:
+        want, _, _ = O.run(x, y, z, ws, bs, ov, 16, box)
+        assert np.array_equal(ctx.tree_run(tree, ws, bs, ov), want)
+    tree.free()
+
+
+def test_golden_reference_fixtures(ctx):
+    gold = json.load(open(os.path.join(HERE, "golden", "ref_o1qtree.json")))
+    for c in gold["cases"]:
+        say("golden", c["name"])
+        t = synth.scaled(synth.config_c1(c["generator"].get("seed", 0)), c["n"])
+        for k, v in c["generator"].items():
+            setattr(t, k, v)
+        x, y, z = t.generate()
+        got = ctx.run_tile(x, y, z, c["wsize"], c["bsize"], c["overlap"], 32, bbox=c["bbox"])
+        assert got.tolist() == c["ref_seeds"], c["name"]
+
+
+def test_edge_planted_and_ties(ctx):
+    sys.path.insert(0, HERE)
+    from test_oracle import _edge_planted
+    say("edge planted")
+    x, y, z, p = _edge_planted()
+    box = (p.minx, p.maxx, p.miny, p.maxy)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=8)
+    gm, _ = ctx.stage1(tree, 12, 0.8, 20, bbox=box)
+    assert np.array_equal(gm, O.stage1_brute(x, y, z, p))
+    tree.free()
+    say("ties")
+    x, y, z = _tile(20_000, seed=11).generate()
+    z = np.minimum(np.round(z), 300.0)
+    want, _, _ = O.run(x, y, z, 10, 20, 0.8, 8)
+    assert np.array_equal(ctx.run_tile(x, y, z, 10, 20, 0.8, 8), want)
+
+
+def test_nan_inf_signed_zero(ctx):
+    say("nan/inf/-0")
+    x, y, z = _tile(3000, seed=5).generate()
+    x[10], y[20], z[30], z[40] = np.nan, np.nan, np.nan, np.inf
+    z[50], z[51] = 0.0, -0.0
+    x[51], y[51] = x[50], y[50]
+    box = synth.bbox(x, y)
+    p = O.params(12, 20, 0.8, box, len(x), 4)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=4)
+    gm, _ = ctx.stage1(tree, 12, 0.8, 20)
+    assert np.array_equal(gm, O.stage1_brute(x, y, z, p))
+    assert 50 in gm and 51 not in gm
+    tree.free()
+
+
+def test_duplicates_beyond_leaf(ctx):
+    say("coincident points")
+    x, y, z = _tile(5000, seed=9).generate()
+    x[100:1400] = x[0]
+    y[100:1400] = y[0]
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 8)
+    assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 8), want)
+
+
+def test_batch_matches_per_tile_oracle(ctx):
+    say("batch 24 tiles")
+    tiles = synth.batch_tiles(24, seed=3, median=20_000, lo=2_000, hi=120_000)
+    xs, ys, zs, desc, wants = [], [], [], [], []
+    for i, t in enumerate(tiles):
+        x, y, z = t.generate()
+        ov = (0.8, 0.5, 0.0, 0.9)[i % 4]
+        bs = (20, 21, 0, 15)[i % 4]
+        xs.append(x)
+        ys.append(y)
+        zs.append(z)
+        desc.append((t.n, dict(wsize=t.wsize, bsize=bs, overlap=ov, leaf_size=(0, 16, 64)[i % 3])))
+        wants.append(O.run(x, y, z, t.wsize, bs, ov, 32)[0])
+    got = ctx.batch_run(np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), desc)
+    for i, (g, w) in enumerate(zip(got, wants)):
+        assert np.array_equal(g, w), f"tile {i}"
+
+
+def test_device_resident_input(ctx):
+    import torch
+    say("device input c1")
+    t = synth.config_c1()
+    x, y, z = t.generate()
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    dx, dy, dz = (torch.from_numpy(a).cuda() for a in (x, y, z))
+    got = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, 32)
+    assert np.array_equal(got, want)
+
+
+def test_c2_full_size(ctx):
+    say("c2 20M")
+    t = synth.config_c2()
+    x, y, z = t.generate()
+    want, _, p = O.run(x, y, z, 12, 20, 0.8, 32)
+    got = ctx.run_tile(x, y, z, 12, 20, 0.8, 32)
+    assert np.array_equal(got, want)
+
+
+def test_errors(ctx):
+    say("errors")
+    x, y, z = _tile(2000).generate()
+    with pytest.raises(owm.OWMError) as e:
+        ctx.run_tile(x, y, z, 12, 20, 1.0)
+    assert e.value.name == "E_ARG"
+    with pytest.raises(owm.OWMError) as e:
+        ctx.run_tile(x, y, z, 12, 20, 0.8, cap=1)
+    assert e.value.name == "E_CAPACITY"
+    # the context stays usable after errors
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 32), want)
