@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""OWM ground-seed benchmark on one MI355X.
+
+Default workload: the 20M-point 2x2 km tile (5 pts/m2, 10% of points in
+Gaussian clusters), Wsize 12, Overlap 0.8, Bsize 20, leaf 32, fp64 coordinates,
+synthetic data (fastowm_amd.synth, seed 0).  One step = the whole pipeline on
+that tile with the cloud already resident in HBM: spatial tree, stage 1,
+sort + stage 2, stage 3, merge, seeds copied back to the host.
+
+Prints ONE JSON line: Mpoints/s (higher is better), the roofline of the
+dominant kernel family (HIP-event time on the library's stream, algorithmic
+bytes), the CPU baseline (the oracle timed on this host's cores on the same
+tile) and, unless --no-batch, the 512-tile uneven batch (plain one-tile-per-
+launch-chain vs batched) on the same device.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(x, y, z, want_seeds, reps):
+    """The oracle (a C restatement of the reference TBB algorithm, OpenMP) on
+    the same tile; median of `reps` runs of tree + stages 1-3."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    cores = cpu_cores()
+    pyoracle.set_threads(cores)
+    times, parity = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        seeds, _, _ = pyoracle.run(x, y, z, 12, 20, 0.8, 32)
+        times.append(time.perf_counter() - t0)
+        parity = bool(np.array_equal(seeds, want_seeds))
+    med = statistics.median(times)
+    return {"value": round(len(x) / med / 1e6, 3), "unit": "Mpoints/s", "cores": cores,
+            "kind": "port",
+            "sample": f"full 20M-point tile, median of {reps} runs (tree + stages 1-3), "
+                      f"{cores} OpenMP threads"}, parity
+
+
+def run_batch_bench(ctx, torch, synth, n_tiles, steps):
+    """Plain (one tile per launch chain) vs batched (all tiles in one chain)."""
+    import numpy as np
+    tiles = synth.batch_tiles(n_tiles, seed=0)
+    arrs = [t.generate() for t in tiles]
+    n_total = sum(t.n for t in tiles)
+    dev = [torch.from_numpy(np.concatenate([a[k] for a in arrs])).to("cuda:0") for k in range(3)]
+    desc = [(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0)) for t in tiles]
+    torch.cuda.synchronize()
+    # plain: one run_tile per tile on device-resident slices
+    offs = np.cumsum([0] + [t.n for t in tiles])
+    views = [[d[offs[i]:offs[i + 1]] for d in dev] for i in range(len(tiles))]
+    for i in range(min(4, len(tiles))):
+        ctx.run_tile(*views[i], tiles[i].wsize, tiles[i].bsize, tiles[i].overlap)
+    t0 = time.perf_counter()
+    plain = []
+    for i, t in enumerate(tiles):
+        plain.append(ctx.run_tile(*views[i], t.wsize, t.bsize, t.overlap))
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    ctx.batch_run(*dev, desc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = ctx.batch_run(*dev, desc)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    same = all(np.array_equal(a, b) for a, b in zip(plain, got))
+    return {"tiles": len(tiles), "points": int(n_total),
+            "plain_mpts": round(n_total / t_plain / 1e6, 1),
+            "batched_mpts": round(n_total / t_batch / 1e6, 1),
+            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+            "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-tile batch leg")
+    ap.add_argument("--batch-tiles", type=int, default=512)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the batch and serving paths "
+                                   "shard tiles across GPUs by design but no multi-GPU run is "
+                                   "measured here", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from fastowm_amd import owm, synth
+
+    tile = synth.config_c2(seed=0)
+    log("generating", tile.name)
+    x, y, z = tile.generate()
+    dx, dy, dz = (torch.from_numpy(a).to("cuda:0") for a in (x, y, z))
+    torch.cuda.synchronize()
+    ctx = owm.Context(0)
+    for _ in range(args.warmup):
+        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, 32)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, 32)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ms = (t1 - t0) / args.steps * 1e3
+    value = tile.n / (ms / 1e3) / 1e6
+
+    phases = {}
+    for name, p in prof.items():
+        if p["calls"] == 0:
+            continue
+        phases[name] = {"ms_per_step": round(p["ms"] / args.steps, 4),
+                        "calls_per_step": p["calls"] // args.steps,
+                        "gbs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1) if p["ms"] > 0 else None}
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    dname, dp = dom
+    per_launch_bytes = dp["bytes"] / dp["calls"]
+    per_launch_s = dp["ms"] * 1e-3 / dp["calls"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "launch_ms": round(per_launch_s * 1e3, 4),
+                "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+
+    line = {"metric": "Mpoints/s end-to-end OWM (tree + stages 1-3), plain GPU, 20M-point tile",
+            "value": round(value, 1), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "c2: 20M points, 2x2 km, 5 pts/m2, 10% in Gaussian clusters "
+                                   "(sigma 3 m), Wsize 12, Overlap 0.8, Bsize 20, leaf 32",
+                       "points": tile.n, "seeds": int(len(seeds))},
+            "roofline": roofline, "phases": phases}
+    if not args.no_batch:
+        log("batch leg")
+        line["batch"] = run_batch_bench(ctx, torch, synth, args.batch_tiles, max(1, args.steps // 4))
+    if not args.no_cpu:
+        log("cpu baseline leg")
+        cb, parity = cpu_baseline(x, y, z, seeds, args.cpu_reps)
+        line["cpu_baseline"] = cb
+        line["parity_vs_oracle"] = parity
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -261,8 +261,7 @@ owm_status owm_tree_build(owm_ctx *ctx, const double *x, const double *y, const 
         return hip_fail(ctx, e, "tree allocation");
     ctx->trees_bytes += tr->mem.bytes;
     carve_tree((uint8_t *)tr->mem.ptr, tr->P.L, tr->T);
-    const int passes = (tr->P.L.tile_bits + tr->P.L.key_bits + 7) / 8;
-    st = ctx->arena.ensure(ctx, table_bytes(1) + sort_scratch_bytes(n, passes) + 8192);
+    st = ctx->arena.ensure(ctx, table_bytes(1) + build_scratch_bytes(tr->P.L) + 8192);
     if (st != OWM_OK) {
         ctx->trees_bytes -= tr->mem.bytes;
         return st;
@@ -278,6 +277,7 @@ owm_status owm_tree_build(owm_ctx *ctx, const double *x, const double *y, const 
         e = hipMemcpyAsync(&hg, guard, 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess)
             e = hipStreamSynchronize(ctx->stream);
+        ctx->prof.collect();
         if (e != hipSuccess)
             st = hip_fail(ctx, e, "tree build");
         else if (hg) {
@@ -372,6 +372,7 @@ static owm_status fetch_counts(owm_ctx *ctx, const QueryOut &qo, uint32_t *hc)
     hipError_t e = hipMemcpyAsync(hc, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
+    ctx->prof.collect();
     if (e != hipSuccess)
         return hip_fail(ctx, e, "counts D2H");
     if (hc[3]) {
@@ -498,6 +499,41 @@ owm_status owm_tree_run(owm_ctx *ctx, owm_tree *tree, const owm_params *params, 
     st = read_seeds(ctx, qo, P, seeds, cap, off);
     *n_seeds = off[1];
     return st;
+    GUARD_END(ctx)
+}
+
+owm_status owm_profile_enable(owm_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->prof.collect();
+    ctx->prof.reset();
+    ctx->prof.on = enable != 0;
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
+owm_status owm_profile_read(owm_ctx *ctx, owm_phase_stat *out, uint32_t cap, uint32_t *n)
+{
+    if (!ctx || !n || (cap && !out))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->prof.collect();
+    *n = PH_COUNT;
+    if (cap < PH_COUNT)
+        return OWM_E_CAPACITY;
+    for (int k = 0; k < PH_COUNT; ++k) {
+        memset(out[k].name, 0, sizeof out[k].name);
+        strncpy(out[k].name, phase_name(k), sizeof out[k].name - 1);
+        out[k].ms = ctx->prof.ms[k];
+        out[k].calls = ctx->prof.calls[k];
+        out[k].bytes = ctx->prof.bytes[k];
+        out[k].items = ctx->prof.items[k];
+    }
+    return OWM_OK;
     GUARD_END(ctx)
 }
 

@@ -23,6 +23,8 @@ struct TileDev {
     uint32_t node_off, leaf_size;
     uint32_t root;      // child reference of the root (OWM_LEAF_BIT | leaf) or internal
     uint32_t tile_key;  // high Morton key bits identifying the tile in a batch sort
+    uint32_t grp_off;   // first leaf group (64/leaf_size leaves per wave) of the tile
+    uint32_t pad_t;
     double qminx, qminy, qscale;  // quantisation: q = (x - qminx) * qscale in [0, 2^32)
     // query (stages 1-3)
     uint32_t ncols, nrows, ncolsg, nrowsg;
@@ -36,7 +38,7 @@ struct TileDev {
 // Offsets of every per-tile range, one array per kind, for binary searches.
 struct BatchDev {
     uint32_t ntiles;
-    uint32_t np, nleaf, nnode, nwin, ncell, nblk;
+    uint32_t np, nleaf, nnode, nwin, ncell, nblk, ngrp;
     uint32_t key_bits;     // Morton bits in the sort key (below the tile bits)
     const TileDev *tiles;
     const uint32_t *pt_off;    // [ntiles+1]
@@ -45,6 +47,7 @@ struct BatchDev {
     const uint32_t *win_off;   // [ntiles+1]
     const uint32_t *cell_off;  // [ntiles+1]
     const uint32_t *blk_off;   // [ntiles+1]
+    const uint32_t *grp_off;   // [ntiles+1]
 };
 
 // Tree arrays of a batch.

@@ -49,6 +49,74 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
 
 static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
+static const char *PHASE_NAMES[PH_COUNT] = {
+    "bbox", "morton", "sort_tree_pass", "gather_leaves", "karras", "bottomup", "stage1",
+    "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds"};
+
+const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
+
+int Profiler::begin(hipStream_t s, int ph)
+{
+    if (!on)
+        return -1;
+    hipEvent_t ev[2];
+    for (auto &e : ev) {
+        if (pool.empty()) {
+            if (hipEventCreate(&e) != hipSuccess)
+                return -1;
+        } else {
+            e = pool.back();
+            pool.pop_back();
+        }
+    }
+    (void)hipEventRecord(ev[0], s);
+    pending.push_back({ph, ev[0], ev[1], 0, 0});
+    return (int)pending.size() - 1;
+}
+
+void Profiler::end(hipStream_t s, int idx, uint64_t b, uint64_t it)
+{
+    if (idx < 0 || idx >= (int)pending.size())
+        return;
+    (void)hipEventRecord(pending[idx].b, s);
+    pending[idx].bytes = b;
+    pending[idx].items = it;
+}
+
+void Profiler::collect()
+{
+    for (auto &r : pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            ms[r.ph] += t;
+            calls[r.ph] += 1;
+            bytes[r.ph] += r.bytes;
+            items[r.ph] += r.items;
+        }
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+
+void Profiler::reset()
+{
+    for (int k = 0; k < PH_COUNT; ++k) {
+        ms[k] = 0;
+        calls[k] = bytes[k] = items[k] = 0;
+    }
+}
+
+Profiler::~Profiler()
+{
+    for (auto &r : pending) {
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    for (auto e : pool)
+        (void)hipEventDestroy(e);
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()
@@ -201,6 +269,7 @@ void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
     L.pt_off.assign(T + 1, 0);
     L.leaf_off.assign(T + 1, 0);
     L.node_off.assign(T + 1, 0);
+    L.grp_off.assign(T + 1, 0);
     for (uint32_t t = 0; t < T; ++t) {
         TileDev &D = tiles[t];
         D.pt_off = L.pt_off[t];
@@ -221,7 +290,11 @@ void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
         L.pt_off[t + 1] = L.pt_off[t] + D.n;
         L.leaf_off[t + 1] = L.leaf_off[t] + D.nleaf;
         L.node_off[t + 1] = L.node_off[t] + (D.nleaf > 0 ? D.nleaf - 1 : 0);
+        const uint32_t per = leaf[t] <= 64 ? 64 / leaf[t] : 1;
+        D.grp_off = L.grp_off[t];
+        L.grp_off[t + 1] = L.grp_off[t] + (D.nleaf + per - 1) / per;
     }
+    L.ngrp = L.grp_off[T];
     L.np = L.pt_off[T];
     L.nleaf = L.leaf_off[T];
     L.nnode = L.node_off[T];
@@ -261,7 +334,7 @@ uint64_t tree_bytes(const TreeLayout &L)
     b += align_up(L.nleaf * 8ull, 256);                   // lf_zkey
     b += align_up(std::max<uint64_t>(L.nnode, 1) * 32ull, 256);  // nd_box
     b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 4;  // cnt zid parent flag
-    b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 2;  // zkey child
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 3;  // zkey child range
     return b;
 }
 
@@ -294,6 +367,7 @@ void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
     T.nd_flag = (uint32_t *)take(nn * 4ull);
     T.nd_zkey = (unsigned long long *)take(nn * 8ull);
     T.nd_child = (uint2 *)take(nn * 8ull);
+    T.nd_range = (uint2 *)take(nn * 8ull);
 }
 
 TreeDev tree_view(const TreeOut &T)
@@ -330,6 +404,14 @@ struct SortBufs {
     uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
 };
 
+// Device scratch of one tree build: the sort buffers and the bottom-up work list.
+uint64_t build_scratch_bytes(const TreeLayout &L)
+{
+    const int passes = (L.tile_bits + L.key_bits + 7) / 8;
+    return sort_scratch_bytes(L.np, passes) +
+           align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
+}
+
 static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
 {
     SortBufs S;
@@ -352,8 +434,8 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
 
 // Runs the LSD passes; keys in S.k0 (values implicit iota when iota).  Returns
 // which buffer pair holds the result (0 -> k0/v0, 1 -> k1/v1).
-static int run_sort(hipStream_t s, SortBufs &S, uint32_t n_max, const uint32_t *d_n, int passes,
-                    bool keys_only, bool iota, bool hist_ready)
+static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
+                    const uint32_t *d_n, int passes, bool keys_only, bool iota, bool hist_ready)
 {
     if (!hist_ready) {
         launch_fill_u32(s, S.hist, 4 * 256, 0u);
@@ -364,9 +446,13 @@ static int run_sort(hipStream_t s, SortBufs &S, uint32_t n_max, const uint32_t *
     launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
     for (int p = 0; p < passes; ++p) {
+        int pi = ctx->prof.begin(s, phase);
         launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
                         S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
                         iota && p == 0, true, 0);
+        // per key: read key (+ value unless generated), write key (+ value)
+        uint64_t per = keys_only ? 8 : ((iota && p == 0) ? 12 : 16);
+        ctx->prof.end(s, pi, per * n_max, n_max);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -387,14 +473,29 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         return OWM_E_INTERNAL;
     }
     SortBufs S = carve_sort(sb, L.np, passes);
+    const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
+    uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
+    if (!bu_work) {
+        set_err(ctx, "scratch arena exhausted (bottom-up)");
+        return OWM_E_INTERNAL;
+    }
     launch_fill_u32(s, S.hist, 4 * 256, 0u);
+    int pi = ctx->prof.begin(s, PH_MORTON);
     launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
-    int which = run_sort(s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
+    ctx->prof.end(s, pi, 20ull * L.np, L.np);  // read x, y; write key
+    int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
     const uint32_t *skeys = which ? S.k1 : S.k0;
     const uint32_t *svals = which ? S.v1 : S.v0;
+    pi = ctx->prof.begin(s, PH_GATHER);
     launch_gather_leaves(s, B, svals, skeys, dx, dy, dz, T);
+    // per point: read index + x,y,z, write x,y,z + index + inverse; per leaf: key + record
+    ctx->prof.end(s, pi, 60ull * L.np + 68ull * L.nleaf, L.np);
+    pi = ctx->prof.begin(s, PH_KARRAS);
     launch_karras(s, B, T);
-    launch_bottomup(s, B, T, guard);
+    ctx->prof.end(s, pi, 24ull * L.nnode, L.nnode);
+    pi = ctx->prof.begin(s, PH_BOTTOMUP);
+    launch_bottomup(s, B, T, bu_work, bu_words, guard);
+    ctx->prof.end(s, pi, 4ull * L.nleaf + 160ull * L.nnode, L.nnode);
     HIP_TRY(hipGetLastError(), "tree build launch");
     return OWM_OK;
 }
@@ -448,7 +549,10 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     out.seed_off = seed_off;
     launch_fill_u32(s, cnt, 16, 0u);
     if (from <= 1) {
+        int pi = ctx->prof.begin(s, PH_STAGE1);
         launch_stage1(s, B, TV, cfg, W.k0, cnt + 3);
+        // every point read once (x, y, z, index) and one minimum written per window
+        ctx->prof.end(s, pi, 28ull * np + 4ull * Q.nwin, np);
         out.min_ids = W.k0;
     }
     if (to <= 1) {
@@ -458,10 +562,12 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     if (from <= 2) {
         if (from == 2 && nwin)
             HIP_TRY(hipMemcpyAsync(W.k0, h_in, nwin * 4ull, hipMemcpyHostToDevice, s), "min_ids H2D");
-        int which = run_sort(s, W, (uint32_t)nwin, nullptr, passes, true, false, false);
+        int which = run_sort(ctx, PH_SORT_WIN, s, W, (uint32_t)nwin, nullptr, passes, true, false, false);
         const uint32_t *sorted = which ? W.k1 : W.k0;
         out.sorted_min_ids = sorted;
+        int pi = ctx->prof.begin(s, PH_STAGE2);
         launch_stage2_compact(s, B, sorted, (uint32_t)nwin, any_need1, SD.k0, cnt + 0, lb1, cnt + 4);
+        ctx->prof.end(s, pi, 4ull * nwin, nwin);
     } else {
         if (n_in)
             HIP_TRY(hipMemcpyAsync(SD.k0, h_in, n_in * 4ull, hipMemcpyHostToDevice, s), "llp H2D");
@@ -475,9 +581,11 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         return OWM_OK;
     }
     if (Q.ncell > 0) {
+        int pi = ctx->prof.begin(s, PH_STAGE3);
         launch_stage3(s, B, TV, SD.k0, cnt + 0, (uint32_t)(nseed_max - Q.ncell), cell_flag,
                       cell_res, cnt + 3);
         launch_compact_valid(s, cell_res, (uint32_t)Q.ncell, SD.k0, cnt + 0, cnt + 1, lb2, cnt + 5);
+        ctx->prof.end(s, pi, 12ull * Q.ncell, Q.ncell);
     }
     out.cell_res = cell_res;
     launch_add_u32(s, cnt + 0, cnt + 1, cnt + 2);
@@ -485,7 +593,7 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         HIP_TRY(hipGetLastError(), "stage 3 launch");
         return OWM_OK;
     }
-    int which = run_sort(s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false, false);
+    int which = run_sort(ctx, PH_MERGE, s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false, false);
     out.seeds = which ? SD.k1 : SD.k0;
     launch_seed_ranges(s, B, out.seeds, cnt + 2, seed_off);
     HIP_TRY(hipGetLastError(), "query launch");
@@ -608,11 +716,14 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
         B0.np = (uint32_t)np;
         B0.pt_off = ptoff;
         launch_bbox_init(s, bk, n_tiles);
+        int pi = ctx->prof.begin(s, PH_BBOX);
         if (np)
             launch_bbox(s, B0, dx, dy, bk);
+        ctx->prof.end(s, pi, 16ull * np, np);
         std::vector<unsigned long long> hk(4ull * n_tiles);
         HIP_TRY(hipMemcpyAsync(hk.data(), bk, n_tiles * 32ull, hipMemcpyDeviceToHost, s), "bbox D2H");
         HIP_TRY(hipStreamSynchronize(s), "bbox sync");
+        ctx->prof.collect();
         auto dec = [](unsigned long long k) {
             uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
             double v;
@@ -669,17 +780,17 @@ owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
     TileDev *d_tiles = (TileDev *)A.take(n_tiles * sizeof(TileDev));
-    uint32_t *d_off[6];
-    for (int k = 0; k < 6; ++k)
+    uint32_t *d_off[7];
+    for (int k = 0; k < 7; ++k)
         d_off[k] = (uint32_t *)A.take((n_tiles + 1) * 4ull);
-    if (!d_tiles || !d_off[5]) {
+    if (!d_tiles || !d_off[6]) {
         set_err(ctx, "scratch arena exhausted (table)");
         return OWM_E_INTERNAL;
     }
     ctx->stage_table(P.td, P.L, P.Q);
     HIP_TRY(hipMemcpyAsync(d_tiles, ctx->h_tiles, n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s),
             "upload tiles");
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < 7; ++k)
         HIP_TRY(hipMemcpyAsync(d_off[k], ctx->h_off[k], (n_tiles + 1) * 4ull, hipMemcpyHostToDevice, s),
                 "upload offsets");
     B = BatchDev{};
@@ -698,12 +809,14 @@ owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
     B.win_off = d_off[3];
     B.cell_off = d_off[4];
     B.blk_off = d_off[5];
+    B.grp_off = d_off[6];
+    B.ngrp = (uint32_t)P.L.ngrp;
     return OWM_OK;
 }
 
 uint64_t table_bytes(uint32_t n_tiles)
 {
-    return align_up((uint64_t)n_tiles * sizeof(TileDev), 256) + 6 * align_up((n_tiles + 1) * 4ull, 256);
+    return align_up((uint64_t)n_tiles * sizeof(TileDev), 256) + 7 * align_up((n_tiles + 1) * 4ull, 256);
 }
 
 // Reads the counts, checks the device guards, copies the seeds (tile-local).
@@ -718,6 +831,7 @@ owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint3
     HIP_TRY(hipMemcpyAsync(hoff.data(), qo.seed_off, (n_tiles + 1) * 4ull, hipMemcpyDeviceToHost, s),
             "ranges D2H");
     HIP_TRY(hipStreamSynchronize(s), "pipeline");
+    ctx->prof.collect();
     if (hc[3]) {
         set_err(ctx, "device traversal stack overflow (flags %u)", hc[3]);
         return OWM_E_INTERNAL;
@@ -752,9 +866,8 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     st = prepare_query(ctx, &tiles[0].params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
-    const int tpasses = (P.L.tile_bits + P.L.key_bits + 7) / 8;
-    const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) +
-                          sort_scratch_bytes(P.L.np, tpasses) + query_bytes(P.Q, P.L.np, 0) + 8192;
+    const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) + build_scratch_bytes(P.L) +
+                          query_bytes(P.Q, P.L.np, 0) + 8192;
     st = ctx->arena.ensure(ctx, need);
     if (st != OWM_OK)
         return st;
@@ -801,9 +914,9 @@ void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, c
     // the previous upload from these buffers must have completed
     (void)hipStreamSynchronize(stream);
     memcpy(h_tiles, td.data(), T * sizeof(TileDev));
-    const std::vector<uint32_t> *src[6] = {&L.pt_off, &L.leaf_off, &L.node_off,
-                                           &Q.win_off, &Q.cell_off, &Q.blk_off};
-    for (int k = 0; k < 6; ++k)
+    const std::vector<uint32_t> *src[7] = {&L.pt_off, &L.leaf_off, &L.node_off, &Q.win_off,
+                                           &Q.cell_off, &Q.blk_off, &L.grp_off};
+    for (int k = 0; k < 7; ++k)
         memcpy(h_off[k], src[k]->data(), (T + 1) * 4);
 }
 

@@ -49,8 +49,8 @@ struct Derived {
 struct TreeLayout {
     uint32_t ntiles = 0;
     int tile_bits = 0, key_bits = 32;
-    uint64_t np = 0, nleaf = 0, nnode = 0;
-    std::vector<uint32_t> pt_off, leaf_off, node_off;
+    uint64_t np = 0, nleaf = 0, nnode = 0, ngrp = 0;
+    std::vector<uint32_t> pt_off, leaf_off, node_off, grp_off;
 };
 
 struct QueryLayout {
@@ -114,6 +114,7 @@ void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
 void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
 uint64_t sort_scratch_bytes(uint64_t n, int passes);
+uint64_t build_scratch_bytes(const TreeLayout &L);
 uint64_t table_bytes(uint32_t n_tiles);
 owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const BatchDev &B,
                       const double *dx, const double *dy, const double *dz, const TreeOut &T,
@@ -143,6 +144,32 @@ TreeDev tree_view(const TreeOut &T);
 
 struct ServeState;
 
+enum Phase {
+    PH_BBOX, PH_MORTON, PH_SORT_TREE, PH_GATHER, PH_KARRAS, PH_BOTTOMUP, PH_STAGE1,
+    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_COUNT
+};
+
+const char *phase_name(int ph);
+
+// HIP-event timing of the kernel families on the context's stream.
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int ph;
+        hipEvent_t a, b;
+        uint64_t bytes, items;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[PH_COUNT] = {};
+    uint64_t calls[PH_COUNT] = {}, bytes[PH_COUNT] = {}, items[PH_COUNT] = {};
+    int begin(hipStream_t s, int ph);
+    void end(hipStream_t s, int idx, uint64_t bytes, uint64_t items);
+    void collect();  // after the stream has been synchronised
+    void reset();
+    ~Profiler();
+};
+
 }  // namespace owm
 
 struct owm_ctx {
@@ -156,9 +183,10 @@ struct owm_ctx {
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};
     ::TileDev *h_tiles = nullptr;
-    uint32_t *h_off[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t *h_off[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t h_cap = 0;
-    owm::ServeState *serve = nullptr;  // owned; released by owm_serve_stop
+    owm::ServeState *serve = nullptr;
+    owm::Profiler prof;  // owned; released by owm_serve_stop
     void stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
                      const owm::QueryLayout &Q);
     uint64_t held_bytes() const;

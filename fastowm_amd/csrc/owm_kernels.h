@@ -27,7 +27,7 @@ struct TreeOut {
     double4 *nd_box;
     uint32_t *nd_cnt, *nd_zid, *nd_parent, *nd_flag;
     unsigned long long *nd_zkey;
-    uint2 *nd_child;
+    uint2 *nd_child, *nd_range;  // children; leaf range [lo, hi] (tile-local leaf indices)
 };
 
 // bbox per tile as order-preserving u64 keys [minx, miny, maxx(inv), maxy(inv)]
@@ -40,7 +40,12 @@ void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *sort
                           const uint32_t *sorted_keys, const double *x, const double *y,
                           const double *z, const TreeOut &T);
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
-void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *err);
+// Bottom-up aggregation: chunk-local climb with LDS arrival flags, then a
+// global climb over the nodes whose leaf range crosses a chunk.  `work` holds
+// bottomup_work_words(nleaf, ntiles) u32 of scratch.
+uint64_t bottomup_work_words(uint64_t nleaf, uint32_t ntiles);
+void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *work,
+                     uint64_t work_words, uint32_t *err);
 
 // ---- stages (stages.hip) ----
 struct Stage1Cfg {

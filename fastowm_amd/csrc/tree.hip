@@ -205,8 +205,22 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
     }
 }
 
-// One wave per leaf: gather the leaf's points into Morton order and reduce
-// its tight bbox, count (non-NaN x and y) and lowest (z, index).
+// Segmented reductions over contiguous lane segments of length ls (a segment
+// never spans more than one leaf): after the loop the first lane of each
+// segment holds the segment's result.
+template <typename F>
+__device__ __forceinline__ void seg_reduce(uint32_t off, uint32_t ls, F &&step)
+{
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1)
+        step(o, off + o < ls);
+}
+
+// Gather into Morton order and per-leaf aggregates.  A wave owns a group of
+// 64/leaf_size consecutive leaves of one tile (one leaf when leaf_size > 64,
+// its points strided over the lanes): every lane gathers one point per step
+// and segmented shuffles reduce each leaf's tight bbox, count (non-NaN x and
+// y) and lowest (z, index).
 __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t *__restrict__ vals,
                                                        const uint32_t *__restrict__ skeys,
                                                        const double *__restrict__ X,
@@ -215,16 +229,20 @@ __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = gridDim.x * (TB / 64);
-    for (uint32_t leaf = blockIdx.x * (TB / 64) + (threadIdx.x >> 6); leaf < B.nleaf; leaf += nw) {
-        uint32_t t = owm_find_tile(B.leaf_off, B.ntiles, leaf);
-        const TileDev &TD = B.tiles[t];
-        uint32_t j = leaf - TD.leaf_off;
-        uint32_t start = TD.pt_off + j * TD.leaf_size;
-        uint32_t npts = min(TD.leaf_size, TD.n - j * TD.leaf_size);
+    for (uint32_t g = blockIdx.x * (TB / 64) + (threadIdx.x >> 6); g < B.ngrp; g += nw) {
+        const TileDev &TD = B.tiles[owm_find_tile(B.grp_off, B.ntiles, g)];
+        const uint32_t L = TD.leaf_size;
+        const uint32_t nl_grp = L <= 64 ? 64 / L : 1;
+        const uint32_t ls = L <= 64 ? L : 64;
+        const uint32_t sub = lane / ls, off = lane - sub * ls;
+        const uint32_t jl = (g - TD.grp_off) * nl_grp + sub;
+        const bool active = sub < nl_grp && jl < TD.nleaf;
+        const uint32_t start = TD.pt_off + jl * L;
+        const uint32_t npts = active ? min(L, TD.n - jl * L) : 0u;
         double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
         uint32_t cnt = 0, bid = OWM_NONE;
         unsigned long long bk = OWM_ZKEY_NONE;
-        for (uint32_t k = lane; k < npts; k += 64) {
+        for (uint32_t k = off; k < npts; k += ls) {
             uint32_t p = start + k;
             uint32_t v = vals[p];
             double x = X[v], y = Y[v], z = Z[v];
@@ -248,13 +266,26 @@ __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t
                 }
             }
         }
-        cnt = wave_sum_u(cnt);
-        mnx = wave_min_d(mnx);
-        mny = wave_min_d(mny);
-        mxx = wave_max_d(mxx);
-        mxy = wave_max_d(mxy);
-        wave_min_kid(bk, bid);
-        if (lane == 0) {
+        seg_reduce(off, ls, [&](uint32_t o, bool take) {
+            uint32_t c2 = __shfl_down(cnt, o, 64);
+            double a = __shfl_down(mnx, o, 64), b = __shfl_down(mny, o, 64);
+            double c = __shfl_down(mxx, o, 64), d = __shfl_down(mxy, o, 64);
+            unsigned long long k2 = __shfl_down(bk, o, 64);
+            uint32_t i2 = __shfl_down(bid, o, 64);
+            if (take) {
+                cnt += c2;
+                mnx = fmin(mnx, a);
+                mny = fmin(mny, b);
+                mxx = fmax(mxx, c);
+                mxy = fmax(mxy, d);
+                if (owm_less(k2, i2, bk, bid)) {
+                    bk = k2;
+                    bid = i2;
+                }
+            }
+        });
+        if (active && off == 0) {
+            const uint32_t leaf = TD.leaf_off + jl;
             T.lf_box[leaf] = make_double4(mnx, mny, mxx, mxy);
             T.lf_cnt[leaf] = cnt;
             T.lf_start[leaf] = start;
@@ -324,6 +355,7 @@ __global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T)
             T.nd_parent[TD.node_off + gamma + 1] = g;
         }
         T.nd_child[g] = make_uint2(left, right);
+        T.nd_range[g] = make_uint2((uint32_t)lo, (uint32_t)hi);
         if (i == 0)
             T.nd_parent[g] = OWM_NONE;
     }
@@ -335,32 +367,127 @@ struct Agg {
     unsigned long long zk;
 };
 
+// Aggregates are read with device-coherent loads: the writer may be another
+// workgroup (global climb) or another wave of this workgroup (chunk climb).
+__device__ __forceinline__ double ld_d(const double *p)
+{
+    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
 __device__ __forceinline__ Agg load_agg(const TreeOut &T, uint32_t ref)
 {
     Agg a;
+    const double *bp;
     if (ref & OWM_LEAF_BIT) {
         uint32_t l = ref & ~OWM_LEAF_BIT;
-        a.box = T.lf_box[l];
-        a.cnt = T.lf_cnt[l];
-        a.zk = T.lf_zkey[l];
-        a.zid = T.lf_zid[l];
+        bp = (const double *)(T.lf_box + l);
+        a.cnt = __hip_atomic_load(T.lf_cnt + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.zk = __hip_atomic_load(T.lf_zkey + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.zid = __hip_atomic_load(T.lf_zid + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-        a.box = T.nd_box[ref];
-        a.cnt = T.nd_cnt[ref];
-        a.zk = T.nd_zkey[ref];
-        a.zid = T.nd_zid[ref];
+        bp = (const double *)(T.nd_box + ref);
+        a.cnt = __hip_atomic_load(T.nd_cnt + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.zk = __hip_atomic_load(T.nd_zkey + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.zid = __hip_atomic_load(T.nd_zid + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    a.box = make_double4(ld_d(bp), ld_d(bp + 1), ld_d(bp + 2), ld_d(bp + 3));
     return a;
 }
 
-// Bottom-up: the second child to arrive at a node merges both children
-// (atomic arrival counter per node, release/acquire fences around it).
-__global__ __launch_bounds__(TB) void k_bottomup(BatchDev B, TreeOut T, uint32_t *err)
+__device__ __forceinline__ Agg merge_agg(const Agg &a, const Agg &b)
 {
-    for (uint32_t l = blockIdx.x * TB + threadIdx.x; l < B.nleaf; l += gridDim.x * TB) {
-        uint32_t p = T.lf_parent[l];
+    Agg r;
+    r.box = make_double4(fmin(a.box.x, b.box.x), fmin(a.box.y, b.box.y), fmax(a.box.z, b.box.z),
+                         fmax(a.box.w, b.box.w));
+    r.cnt = a.cnt + b.cnt;
+    if (owm_less(b.zk, b.zid, a.zk, a.zid)) {
+        r.zk = b.zk;
+        r.zid = b.zid;
+    } else {
+        r.zk = a.zk;
+        r.zid = a.zid;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void store_agg(const TreeOut &T, uint32_t p, const Agg &a)
+{
+    T.nd_box[p] = a.box;
+    T.nd_cnt[p] = a.cnt;
+    T.nd_zkey[p] = a.zk;
+    T.nd_zid[p] = a.zid;
+}
+
+constexpr uint32_t BU_CHUNK = 2048;  // leaves per workgroup in the chunk climb
+
+// Chunk climb: a workgroup owns leaves [A, E] of the batch; a node whose
+// leaf range lies inside [A, E] has both children completed by this
+// workgroup, so its arrival counter lives in LDS and needs no device-wide
+// fence.  A completed node whose parent's range leaves the chunk is exported
+// for the global climb.
+__global__ __launch_bounds__(TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exported,
+                                                        uint32_t *n_exported, uint32_t cap,
+                                                        uint32_t *err)
+{
+    __shared__ uint32_t sFlag[BU_CHUNK];
+    const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
+    for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const uint32_t A = ch * BU_CHUNK, E = min(A + BU_CHUNK, B.nleaf) - 1;
+        for (uint32_t k = threadIdx.x; k < BU_CHUNK; k += TB)
+            sFlag[k] = 0;
+        __syncthreads();
+        for (uint32_t l = A + threadIdx.x; l <= E; l += TB) {
+            const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
+            uint32_t c = OWM_LEAF_BIT | l;
+            Agg mine = load_agg(T, c);
+            uint32_t p = T.lf_parent[l];
+            for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
+                if (depth > 128) {
+                    atomicOr(err, 4u);
+                    break;
+                }
+                uint2 rg = T.nd_range[p];
+                uint32_t glo = TD.leaf_off + rg.x, ghi = TD.leaf_off + rg.y;
+                if (glo < A || ghi > E) {
+                    uint32_t k = atomicAdd(n_exported, 1u);
+                    if (k < cap)
+                        exported[k] = c;
+                    else
+                        atomicOr(err, 8u);
+                    break;
+                }
+                uint32_t slot = TD.leaf_off + (p - TD.node_off) - A;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                uint32_t old = atomicAdd(&sFlag[slot], 1u);
+                if (old == 0)
+                    break;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                uint2 chd = T.nd_child[p];
+                Agg other = load_agg(T, chd.x == c ? chd.y : chd.x);
+                mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
+                store_agg(T, p, mine);
+                c = p;
+                p = T.nd_parent[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Global climb from the exported nodes: arrival counters in global memory
+// with device-scope release/acquire fences (only chunk-crossing nodes).
+__global__ __launch_bounds__(TB) void k_bottomup_global(TreeOut T, const uint32_t *exported,
+                                                         const uint32_t *n_exported, uint32_t cap,
+                                                         uint32_t *err)
+{
+    const uint32_t ne = min(*n_exported, cap);
+    for (uint32_t k = blockIdx.x * TB + threadIdx.x; k < ne; k += gridDim.x * TB) {
+        uint32_t c = exported[k];
+        uint32_t p = (c & OWM_LEAF_BIT) ? T.lf_parent[c & ~OWM_LEAF_BIT] : T.nd_parent[c];
         for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
-            if (depth > 128) {  // deeper than any tree over < 2^30 leaves: malformed
+            if (depth > 128) {
                 atomicOr(err, 4u);
                 break;
             }
@@ -369,18 +496,10 @@ __global__ __launch_bounds__(TB) void k_bottomup(BatchDev B, TreeOut T, uint32_t
             if (old == 0)
                 break;
             __threadfence();
-            uint2 c = T.nd_child[p];
-            Agg a = load_agg(T, c.x), b = load_agg(T, c.y);
-            double4 bx = make_double4(fmin(a.box.x, b.box.x), fmin(a.box.y, b.box.y),
-                                      fmax(a.box.z, b.box.z), fmax(a.box.w, b.box.w));
-            T.nd_box[p] = bx;
-            T.nd_cnt[p] = a.cnt + b.cnt;
-            if (owm_less(b.zk, b.zid, a.zk, a.zid)) {
-                a.zk = b.zk;
-                a.zid = b.zid;
-            }
-            T.nd_zkey[p] = a.zk;
-            T.nd_zid[p] = a.zid;
+            uint2 chd = T.nd_child[p];
+            Agg a = load_agg(T, chd.x), b = load_agg(T, chd.y);
+            store_agg(T, p, merge_agg(a, b));
+            c = p;
             p = T.nd_parent[p];
         }
     }
@@ -418,7 +537,7 @@ void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *vals
                           const uint32_t *skeys, const double *x, const double *y, const double *z,
                           const TreeOut &T)
 {
-    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.nleaf, TB / 64, 8192)), dim3(TB), 0, s, B,
+    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.ngrp, TB / 64, 16384)), dim3(TB), 0, s, B,
                        vals, skeys, x, y, z, T);
 }
 
@@ -429,12 +548,25 @@ void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
     hipLaunchKernelGGL(k_karras, dim3(grid_for(B.nnode, TB, 8192)), dim3(TB), 0, s, B, T);
 }
 
-void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *err)
+uint64_t bottomup_work_words(uint64_t nleaf, uint32_t ntiles)
+{
+    uint64_t chunks = (nleaf + BU_CHUNK - 1) / BU_CHUNK;
+    return 1 + 160 * (chunks + ntiles) + 1024;
+}
+
+void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *work,
+                     uint64_t work_words, uint32_t *err)
 {
     if (B.nnode == 0)
         return;
+    const uint32_t cap = (uint32_t)(work_words - 1);
     launch_fill_u32(s, T.nd_flag, B.nnode, 0u);
-    hipLaunchKernelGGL(k_bottomup, dim3(grid_for(B.nleaf, TB, 8192)), dim3(TB), 0, s, B, T, err);
+    launch_fill_u32(s, work, 1, 0u);
+    const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
+    hipLaunchKernelGGL(k_bottomup_local, dim3(nch < 1 ? 1 : (nch > 16384 ? 16384 : nch)), dim3(TB),
+                       0, s, B, T, work + 1, work, cap, err);
+    hipLaunchKernelGGL(k_bottomup_global, dim3(64), dim3(TB), 0, s, T, (const uint32_t *)(work + 1),
+                       (const uint32_t *)work, cap, err);
 }
 
 }  // namespace owm

@@ -49,6 +49,11 @@ class Derived(ctypes.Structure):
                 ("nrowsg", ctypes.c_uint32), ("bbox", ctypes.c_double * 4)]
 
 
+class PhaseStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("calls", ctypes.c_uint64),
+                ("bytes", ctypes.c_uint64), ("items", ctypes.c_uint64)]
+
+
 class TileDesc(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("params", Params)]
 
@@ -99,6 +104,8 @@ def lib():
             "owm_serve_cancel": ([vp, u64], ctypes.c_int),
             "owm_serve_evict": ([vp, u64], ctypes.c_int),
             "owm_serve_stop": ([vp], ctypes.c_int),
+            "owm_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
+            "owm_profile_read": ([vp, P(PhaseStat), u32, P(u32)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -213,6 +220,19 @@ class Context:
     def _check(self, st):
         if st:
             raise OWMError(st, self.last_error())
+
+    def profile(self, enable: bool = True):
+        """Start (zeroing) or stop per-phase HIP-event timing."""
+        self._check(lib().owm_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """{phase: {ms, calls, bytes, items}} summed since profile(True)."""
+        arr = (PhaseStat * 32)()
+        n = ctypes.c_uint32()
+        self._check(lib().owm_profile_read(self.h, arr, 32, ctypes.byref(n)))
+        return {arr[i].name.decode(): {"ms": arr[i].ms, "calls": arr[i].calls,
+                                       "bytes": arr[i].bytes, "items": arr[i].items}
+                for i in range(n.value)}
 
     # -- whole pipeline ----------------------------------------------------
     def run_tile(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None,

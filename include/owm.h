@@ -74,6 +74,23 @@ const char *owm_status_str(owm_status s);
 /* Device bytes currently held by the context's arenas. */
 uint64_t owm_ctx_device_bytes(const owm_ctx *ctx);
 
+/* Per-phase device timing, measured with HIP events recorded on the
+ * context's own stream around each kernel family.  `bytes` is the phase's
+ * algorithmic HBM traffic (what the work must read and write once), so
+ * bytes / ms is the achieved bandwidth against the roofline.
+ * No counterpart in the reference (its -DDEBUG timers, ordered_builder.inl:433-540,
+ * print per-kernel times to stdout). */
+typedef struct owm_phase_stat {
+    char name[32];
+    double ms;       /* summed device time */
+    uint64_t calls;  /* launches of the phase */
+    uint64_t bytes;  /* summed algorithmic bytes */
+    uint64_t items;  /* summed work items (points, windows, keys ...) */
+} owm_phase_stat;
+/* enable != 0 starts recording and zeroes the counters; 0 stops. */
+owm_status owm_profile_enable(owm_ctx *ctx, int enable);
+owm_status owm_profile_read(owm_ctx *ctx, owm_phase_stat *out, uint32_t cap, uint32_t *n);
+
 /* Parameter derivation only (no device work): owm_optim3.cpp:92-132. */
 owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_or_null,
                       owm_derived *out);
