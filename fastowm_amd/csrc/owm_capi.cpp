@@ -350,10 +350,16 @@ owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_i
 
 // Stages from..to on a resident tree.
 static owm_status tree_query(owm_ctx *ctx, owm_tree *tree, const owm_params *params, int from,
-                             int to, const uint32_t *h_in, uint64_t n_in, Prepared &P, QueryOut &qo)
+                             int to, const uint32_t *h_in, uint64_t n_in, Prepared &P, QueryOut &qo,
+                             uint64_t s1_cap = 0)
 {
     P = tree->P;
     owm_status st = prepare_query(ctx, params, 0, P.box, P.L.np, P);
+    if (st != OWM_OK)
+        return st;
+    if (s1_cap)
+        P.Q.s1_cap = s1_cap;
+    st = ensure_s1_list(ctx, P.Q.s1_cap);
     if (st != OWM_OK)
         return st;
     st = ctx->arena.ensure(ctx, table_bytes(1) + query_bytes(P.Q, P.L.np, n_in) + 4096);
@@ -367,7 +373,7 @@ static owm_status tree_query(owm_ctx *ctx, owm_tree *tree, const owm_params *par
                          P.any_need1 ? 1u : 0u, ctx->arena, qo, from, to, h_in, n_in);
 }
 
-static owm_status fetch_counts(owm_ctx *ctx, const QueryOut &qo, uint32_t *hc)
+static owm_status fetch_counts(owm_ctx *ctx, const QueryOut &qo, uint32_t *hc, uint64_t *s1_needed)
 {
     hipError_t e = hipMemcpyAsync(hc, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
@@ -375,11 +381,29 @@ static owm_status fetch_counts(owm_ctx *ctx, const QueryOut &qo, uint32_t *hc)
     ctx->prof.collect();
     if (e != hipSuccess)
         return hip_fail(ctx, e, "counts D2H");
-    if (hc[3]) {
-        set_err(ctx, "device traversal stack overflow (flags %u)", hc[3]);
-        return OWM_E_INTERNAL;
+    return check_guards(ctx, hc, s1_needed);
+}
+
+// tree_query + counts, re-running once with the exact stage-1 join size when
+// the default list was too small.
+static owm_status tree_query_sync(owm_ctx *ctx, owm_tree *tree, const owm_params *params, int from,
+                                  int to, const uint32_t *h_in, uint64_t n_in, Prepared &P,
+                                  QueryOut &qo, uint32_t *hc)
+{
+    uint64_t need = 0;
+    owm_status st = tree_query(ctx, tree, params, from, to, h_in, n_in, P, qo);
+    if (st == OWM_OK)
+        st = fetch_counts(ctx, qo, hc, &need);
+    if (st == OWM_OK && need) {
+        st = tree_query(ctx, tree, params, from, to, h_in, n_in, P, qo, need + need / 8 + 1024);
+        if (st == OWM_OK)
+            st = fetch_counts(ctx, qo, hc, &need);
+        if (st == OWM_OK && need) {
+            set_err(ctx, "stage-1 join list overflow persisted");
+            st = OWM_E_INTERNAL;
+        }
     }
-    return OWM_OK;
+    return st;
 }
 
 owm_status owm_stage1(owm_ctx *ctx, owm_tree *tree, const owm_params *params, uint32_t *min_ids,
@@ -391,13 +415,10 @@ owm_status owm_stage1(owm_ctx *ctx, owm_tree *tree, const owm_params *params, ui
     (void)hipSetDevice(ctx->device);
     Prepared P;
     QueryOut qo;
-    owm_status st = tree_query(ctx, tree, params, 1, 1, nullptr, 0, P, qo);
-    if (st != OWM_OK)
-        return st;
-    if (derived)
-        owm_derive(params, tree->n, P.box[0].data(), derived);
     uint32_t hc[16];
-    st = fetch_counts(ctx, qo, hc);
+    owm_status st = tree_query_sync(ctx, tree, params, 1, 1, nullptr, 0, P, qo, hc);
+    if (derived && (st == OWM_OK || st == OWM_E_CAPACITY))
+        owm_derive(params, tree->n, P.box[0].data(), derived);
     if (st != OWM_OK)
         return st;
     if (P.Q.nwin > cap) {
@@ -427,11 +448,8 @@ owm_status owm_stage2(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
         }
     Prepared P;
     QueryOut qo;
-    owm_status st = tree_query(ctx, tree, params, 2, 2, min_ids, n_ids, P, qo);
-    if (st != OWM_OK)
-        return st;
     uint32_t hc[16];
-    st = fetch_counts(ctx, qo, hc);
+    owm_status st = tree_query_sync(ctx, tree, params, 2, 2, min_ids, n_ids, P, qo, hc);
     if (st != OWM_OK)
         return st;
     *n_llp = hc[0];
@@ -462,11 +480,8 @@ owm_status owm_stage3(owm_ctx *ctx, owm_tree *tree, const owm_params *params, co
         }
     Prepared P;
     QueryOut qo;
-    owm_status st = tree_query(ctx, tree, params, 3, 3, llp, n_llp, P, qo);
-    if (st != OWM_OK)
-        return st;
     uint32_t hc[16];
-    st = fetch_counts(ctx, qo, hc);
+    owm_status st = tree_query_sync(ctx, tree, params, 3, 3, llp, n_llp, P, qo, hc);
     if (st != OWM_OK)
         return st;
     *n_adds = hc[1];
@@ -492,11 +507,20 @@ owm_status owm_tree_run(owm_ctx *ctx, owm_tree *tree, const owm_params *params, 
     (void)hipSetDevice(ctx->device);
     Prepared P;
     QueryOut qo;
+    uint64_t off[2] = {0, 0};
+    uint64_t need = 0;
     owm_status st = tree_query(ctx, tree, params, 1, 4, nullptr, 0, P, qo);
-    if (st != OWM_OK)
-        return st;
-    uint64_t off[2];
-    st = read_seeds(ctx, qo, P, seeds, cap, off);
+    if (st == OWM_OK)
+        st = read_seeds(ctx, qo, P, seeds, cap, off, &need);
+    if (st == OWM_OK && need) {
+        st = tree_query(ctx, tree, params, 1, 4, nullptr, 0, P, qo, need + need / 8 + 1024);
+        if (st == OWM_OK)
+            st = read_seeds(ctx, qo, P, seeds, cap, off, &need);
+        if (st == OWM_OK && need) {
+            set_err(ctx, "stage-1 join list overflow persisted");
+            st = OWM_E_INTERNAL;
+        }
+    }
     *n_seeds = off[1];
     return st;
     GUARD_END(ctx)

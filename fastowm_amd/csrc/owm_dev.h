@@ -54,7 +54,8 @@ struct BatchDev {
 struct TreeDev {
     const double *xs, *ys, *zs;
     const uint32_t *ids;  // sorted position -> original (batch-global) index
-    const uint32_t *pos;  // original index -> sorted position
+    const uint32_t *pos;  // original index -> sorted position (resident trees)
+    const double *X, *Y;  // original cloud when still available (else pos is used)
     const double4 *lf_box;
     const uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid;
     const unsigned long long *lf_zkey;

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -389,6 +390,7 @@ TreeDev tree_view(const TreeOut &T)
     V.nd_zid = T.nd_zid;
     V.nd_zkey = T.nd_zkey;
     V.nd_child = T.nd_child;
+    V.X = V.Y = nullptr;
     return V;
 }
 
@@ -408,7 +410,7 @@ struct SortBufs {
 uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    return sort_scratch_bytes(L.np, passes) +
+    return sort_scratch_bytes(L.np, passes) + align_up(L.np * 32ull, 256) +
            align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
 }
 
@@ -475,21 +477,23 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     SortBufs S = carve_sort(sb, L.np, passes);
     const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
     uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
-    if (!bu_work) {
+    double4 *aos = (double4 *)scratch.take(L.np * 32ull);
+    if (!bu_work || !aos) {
         set_err(ctx, "scratch arena exhausted (bottom-up)");
         return OWM_E_INTERNAL;
     }
     launch_fill_u32(s, S.hist, 4 * 256, 0u);
     int pi = ctx->prof.begin(s, PH_MORTON);
-    launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
-    ctx->prof.end(s, pi, 20ull * L.np, L.np);  // read x, y; write key
+    launch_morton(s, B, dx, dy, dz, S.k0, aos, S.hist, passes);
+    ctx->prof.end(s, pi, 60ull * L.np, L.np);  // read x, y, z; write key + 32-byte record
     int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
     const uint32_t *skeys = which ? S.k1 : S.k0;
     const uint32_t *svals = which ? S.v1 : S.v0;
     pi = ctx->prof.begin(s, PH_GATHER);
-    launch_gather_leaves(s, B, svals, skeys, dx, dy, dz, T);
-    // per point: read index + x,y,z, write x,y,z + index + inverse; per leaf: key + record
-    ctx->prof.end(s, pi, 60ull * L.np + 68ull * L.nleaf, L.np);
+    launch_gather_leaves(s, B, svals, skeys, aos, T);
+    // per point: read index + 32-byte record, write x,y,z + index (+ inverse);
+    // per leaf: key + record
+    ctx->prof.end(s, pi, (T.pos ? 68ull : 64ull) * L.np + 68ull * L.nleaf, L.np);
     pi = ctx->prof.begin(s, PH_KARRAS);
     launch_karras(s, B, T);
     ctx->prof.end(s, pi, 24ull * L.nnode, L.nnode);
@@ -513,6 +517,7 @@ uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
     b += align_up(std::max<uint64_t>(Q.ncell, 1) * 4ull, 256) * 2;      // cell flag / res
     b += align_up((nseed_max / 4096 + 2) * 4ull, 256) * 2;               // compaction look-back
     b += align_up(1024, 256);                                           // counters
+    b += align_up(stage1_work_words(Q.nblk) * 4ull, 256);              // stage-1 join
     b += align_up((Q.ntiles + 1) * 4ull, 256);                          // seed_off
     return b;
 }
@@ -539,7 +544,8 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     uint32_t *lb2 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
     uint32_t *cnt = (uint32_t *)scratch.take(1024);
     uint32_t *seed_off = (uint32_t *)scratch.take((Q.ntiles + 1) * 4ull);
-    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off) {
+    uint32_t *s1work = (uint32_t *)scratch.take(stage1_work_words(Q.nblk) * 4ull);
+    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off || !s1work) {
         set_err(ctx, "scratch arena exhausted (query)");
         return OWM_E_INTERNAL;
     }
@@ -550,7 +556,9 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     launch_fill_u32(s, cnt, 16, 0u);
     if (from <= 1) {
         int pi = ctx->prof.begin(s, PH_STAGE1);
-        launch_stage1(s, B, TV, cfg, W.k0, cnt + 3);
+        launch_stage1(s, B, TV, cfg, s1work, (uint32_t *)ctx->s1_list.ptr,
+                      (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4), W.k0, cnt + 3,
+                      cnt + 6);
         // every point read once (x, y, z, index) and one minimum written per window
         ctx->prof.end(s, pi, 28ull * np + 4ull * Q.nwin, np);
         out.min_ids = W.k0;
@@ -767,6 +775,24 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
     P.cfg = pick_stage1(ctx, np_total);
     P.Q.ntiles = n_tiles;
     plan_query(P.td, P.cfg, P.Q);
+    // expected size of the leaf/block join: a leaf of side ~sqrt(L/density)
+    // meets about (side + 2h + 2D) / (block side) + 1 blocks per axis
+    double est = 4096.0;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        const TileDev &D = P.td[t];
+        if (D.nleaf == 0 || D.ncols == 0)
+            continue;
+        const double dens = std::max(P.td[t].n / std::max(1e-9, (double)D.ncols * D.nrows *
+                                                                  D.displace * D.displace), 1e-9);
+        const double side = std::sqrt(D.leaf_size / dens);
+        const double ext = side + 2.0 * D.h + 2.0 * D.displace;
+        const double fx = ext / (P.cfg.bx * D.displace) + 1.0, fy = ext / (P.cfg.by * D.displace) + 1.0;
+        est += 1.5 * D.nleaf * fx * fy + 64;
+    }
+    P.Q.s1_cap = (uint64_t)std::min(est, 4.0e9);
+    // OWM_S1_LIST_CAP overrides the first guess (used to exercise the re-run)
+    if (const char *ov = getenv("OWM_S1_LIST_CAP"))
+        P.Q.s1_cap = std::max<uint64_t>(1, strtoull(ov, nullptr, 10));
     if (P.Q.nwin >= (1ull << 31) || P.Q.ncell >= (1ull << 31)) {
         set_err(ctx, "too many windows or cells in one batch");
         return OWM_E_ARG;
@@ -820,8 +846,36 @@ uint64_t table_bytes(uint32_t n_tiles)
 }
 
 // Reads the counts, checks the device guards, copies the seeds (tile-local).
+owm_status ensure_s1_list(owm_ctx *ctx, uint64_t cap)
+{
+    if (cap * 4 <= ctx->s1_list.bytes)
+        return OWM_OK;
+    if (ctx->budget && ctx->held_bytes() - ctx->s1_list.bytes + cap * 4 > ctx->budget) {
+        set_err(ctx, "stage-1 join list needs %llu bytes over the budget", (unsigned long long)(cap * 4));
+        return OWM_E_BUDGET;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync before list growth");
+    HIP_TRY(ctx->s1_list.reserve(cap * 4), "stage-1 join list");
+    return OWM_OK;
+}
+
+owm_status check_guards(owm_ctx *ctx, const uint32_t *hc, uint64_t *s1_needed)
+{
+    *s1_needed = 0;
+    uint32_t f = hc[3];
+    if (f & 16u) {
+        *s1_needed = hc[6];
+        f &= ~16u;
+    }
+    if (f) {
+        set_err(ctx, "device capacity guard tripped (flags %u)", f);
+        return OWM_E_INTERNAL;
+    }
+    return OWM_OK;
+}
+
 owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
-                      uint64_t cap, uint64_t *seed_off_host)
+                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed)
 {
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
@@ -832,10 +886,9 @@ owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint3
             "ranges D2H");
     HIP_TRY(hipStreamSynchronize(s), "pipeline");
     ctx->prof.collect();
-    if (hc[3]) {
-        set_err(ctx, "device traversal stack overflow (flags %u)", hc[3]);
-        return OWM_E_INTERNAL;
-    }
+    owm_status st = check_guards(ctx, hc, s1_needed);
+    if (st != OWM_OK || *s1_needed)
+        return st;
     const uint64_t ns = hc[2];
     for (uint32_t t = 0; t <= n_tiles; ++t)
         seed_off_host[t] = hoff[t];
@@ -880,16 +933,35 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
         return st;
     TreeOut T;
     carve_tree(A.take(tree_bytes(P.L)), P.L, T);
+    T.pos = nullptr;  // the cloud stays available: stage 3 reads LLP coordinates from it
     st = build_tree(ctx, ctx->stream, P.L, B, dx, dy, dz, T, A, guard);
     if (st != OWM_OK)
         return st;
-    QueryOut qo;
-    st = enqueue_query(ctx, ctx->stream, B, tree_view(T), P.Q, P.L.np, P.cfg,
-                       P.any_need1 ? 1u : 0u, A, qo, 1, 4, nullptr, 0);
-    if (st != OWM_OK)
-        return st;
-    launch_add_u32(ctx->stream, qo.cnt + 3, guard, qo.cnt + 3);
-    return read_seeds(ctx, qo, P, seeds, cap, seed_off_host);
+    TreeDev TV = tree_view(T);
+    TV.X = dx;
+    TV.Y = dy;
+    const uint64_t mark = A.used;
+    for (int attempt = 0;; ++attempt) {
+        st = ensure_s1_list(ctx, P.Q.s1_cap);
+        if (st != OWM_OK)
+            return st;
+        QueryOut qo;
+        st = enqueue_query(ctx, ctx->stream, B, TV, P.Q, P.L.np, P.cfg, P.any_need1 ? 1u : 0u, A,
+                           qo, 1, 4, nullptr, 0);
+        if (st != OWM_OK)
+            return st;
+        launch_add_u32(ctx->stream, qo.cnt + 3, guard, qo.cnt + 3);
+        uint64_t need = 0;
+        st = read_seeds(ctx, qo, P, seeds, cap, seed_off_host, &need);
+        if (st != OWM_OK || need == 0)
+            return st;
+        if (attempt > 0) {
+            set_err(ctx, "stage-1 join list overflow persisted");
+            return OWM_E_INTERNAL;
+        }
+        P.Q.s1_cap = need + need / 8 + 1024;  // the join's exact size, then run the query again
+        A.used = mark;
+    }
 }
 
 }  // namespace owm
@@ -922,7 +994,7 @@ void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, c
 
 uint64_t owm_ctx::held_bytes() const
 {
-    uint64_t b = arena.buf.bytes + small.buf.bytes + upload.bytes;
+    uint64_t b = arena.buf.bytes + small.buf.bytes + upload.bytes + s1_list.bytes;
     b += trees_bytes.load();
     return b;
 }

@@ -32,7 +32,7 @@ struct DevBuf {
 // Bump allocator over one device buffer, reset at the start of every run.
 struct Arena {
     DevBuf buf;
-    uint64_t used = 0;
+    uint64_t used = 0;  // bump offset; callers may save and restore it to re-run a phase
     owm_status ensure(owm_ctx *ctx, uint64_t bytes);
     uint8_t *take(uint64_t bytes);
 };
@@ -56,6 +56,7 @@ struct TreeLayout {
 struct QueryLayout {
     uint32_t ntiles = 0;
     uint64_t nwin = 0, ncell = 0, nblk = 0;
+    uint64_t s1_cap = 0;  // capacity of the stage-1 leaf/block join list
     std::vector<uint32_t> win_off, cell_off, blk_off;
 };
 
@@ -65,7 +66,7 @@ struct QueryOut {
     const uint32_t *llp = nullptr;             // LLPs (then stage-3 adds) before the merge sort
     const uint32_t *cell_res = nullptr;        // stage-3 result per cell
     const uint32_t *seeds = nullptr;           // merged, sorted
-    uint32_t *cnt = nullptr;                   // [0] n_llp [1] n_add [2] n_seeds [3] error flags
+    uint32_t *cnt = nullptr;  // [0] n_llp [1] n_add [2] n_seeds [3] error flags [6] s1 list need
     uint32_t *seed_off = nullptr;
 };
 
@@ -130,7 +131,12 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
                          Prepared &P);
 owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B);
 owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
-                      uint64_t cap, uint64_t *seed_off_host);
+                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed);
+// Makes ctx->s1_list hold at least cap entries.
+owm_status ensure_s1_list(owm_ctx *ctx, uint64_t cap);
+// Device-guard flags of a finished query: OWM_OK, or OWM_E_INTERNAL with a
+// message; a stage-1 list overflow alone sets *s1_needed and returns OWM_OK.
+owm_status check_guards(owm_ctx *ctx, const uint32_t *hc, uint64_t *s1_needed);
 uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested);
 Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total);
 void set_err(owm_ctx *ctx, const char *fmt, ...);
@@ -180,6 +186,7 @@ struct owm_ctx {
     owm::Arena arena;   // tree + scratch of the current run
     owm::Arena small;   // bbox pass
     owm::DevBuf upload; // host inputs staged to the device
+    owm::DevBuf s1_list; // stage-1 leaf/block join list (grows on demand)
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};
     ::TileDev *h_tiles = nullptr;

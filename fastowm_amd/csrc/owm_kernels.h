@@ -35,10 +35,10 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
                  unsigned long long *bbox_keys);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   uint32_t *keys, uint32_t *hist, int passes);
+                   const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes);
+// T.pos may be null (no inverse permutation is written then)
 void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *sorted_vals,
-                          const uint32_t *sorted_keys, const double *x, const double *y,
-                          const double *z, const TreeOut &T);
+                          const uint32_t *sorted_keys, const double4 *aos, const TreeOut &T);
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
 // Bottom-up aggregation: chunk-local climb with LDS arrival flags, then a
 // global climb over the nodes whose leaf range crosses a chunk.  `work` holds
@@ -51,8 +51,13 @@ void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_
 struct Stage1Cfg {
     uint32_t bx, by;  // window block shape
 };
+// work: stage1_work_words(nblk) u32; list: leaf ids per window block, capacity
+// cap.  *total_out receives the number of list entries needed; when it
+// exceeds cap, err gets bit 16 and the call must be repeated with a larger list.
+uint64_t stage1_work_words(uint64_t nblk);
 void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   uint32_t *min_ids, uint32_t *err);
+                   uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *min_ids, uint32_t *err,
+                   uint32_t *total_out);
 // sorted window minima -> LLP list (ascending) + count
 void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
                            uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,

@@ -1,0 +1,461 @@
+// Stage 1 of OWM on gfx950: the exact minimum (and point count) of every
+// overlapping window.  Reference: stage1 (CPP-TBB-CPU/src/optim3_func.cpp:541-561,
+// the exact -DNOMEMOA build) and the GPU query2D (SYCL-CUDA-GPU/basic/traverse.h:686-721).
+//
+// Windows are processed in blocks of BX x BY windows, one workgroup per block.
+// The block's window edges split x into "classes": class(x) =
+// #{edges < x} + #{edges <= x} over the multiset of the block's left and right
+// x edges (likewise y), so that points of one class cell lie in exactly the
+// same windows and window (i, j) is the class rectangle
+// [2#{e <= L_i}, 2#{e < R_i}] x [2#{e <= B_j}, 2#{e < T_j}].  Each point strictly
+// inside the block is binned into an LDS class cell (count, lowest (z, index));
+// a leaf whose tight bbox lies inside one class cell is binned whole from its
+// aggregate without reading its points.  Class cells are then reduced to
+// windows separably (along x per class row, then along y per window): every
+// window re-uses the cells it shares with its neighbours, which is the
+// reference's memoisation of the previous window's minimum taken to its limit.
+//
+// The leaves each block must read come from a spatial join of leaf bboxes
+// against block extents (count, scan, fill), so no tree walk is on the
+// critical path of stage 1.
+#include "owm_kernels.h"
+
+namespace owm {
+
+constexpr int S1_THREADS = 256;
+constexpr int S1_K = 4;
+constexpr uint32_t S1_LEAFTAG = 0x80000000u;
+
+// #{k : E[k] < x} and #{k : E[k] <= x} over a strictly increasing edge list
+// E[0..n) spaced ~1/inv apart: estimate, then correct with exact compares.
+__device__ __forceinline__ uint32_t cnt_lt(const double *E, uint32_t n, double x, double inv)
+{
+    double f = (x - E[0]) * inv;
+    int k = f >= 0.0 ? (f < (double)n ? (int)f + 1 : (int)n) : 0;
+    while (k > 0 && !(E[k - 1] < x))
+        --k;
+    while (k < (int)n && E[k] < x)
+        ++k;
+    return (uint32_t)k;
+}
+
+__device__ __forceinline__ uint32_t cnt_le(const double *E, uint32_t n, double x, double inv)
+{
+    double f = (x - E[0]) * inv;
+    int k = f >= 0.0 ? (f < (double)n ? (int)f + 1 : (int)n) : 0;
+    while (k > 0 && E[k - 1] > x)
+        --k;
+    while (k < (int)n && E[k] <= x)
+        ++k;
+    return (uint32_t)k;
+}
+
+// #{E < x} + #{E <= x}: the "<=" count only differs when x equals the first
+// edge not below it.
+__device__ __forceinline__ uint32_t cnt_2(const double *E, uint32_t n, double x, double inv)
+{
+    uint32_t k = cnt_lt(E, n, x, inv);
+    return 2 * k + (k < n && E[k] == x ? 1u : 0u);
+}
+
+// class of a coordinate against left edges L and right edges R (n each)
+__device__ __forceinline__ uint32_t cls(const double *L, const double *R, uint32_t n, double x,
+                                        double inv)
+{
+    return cnt_2(L, n, x, inv) + cnt_2(R, n, x, inv);
+}
+
+// Windows [a, b] whose open extent (c_i - h, c_i + h) can hold a point of
+// the closed range [lo, hi]: R_i > lo and L_i < hi, with the exact edges.
+__device__ __forceinline__ bool win_span(double lo, double hi, double init, double disp, double hd,
+                                         uint32_t n, uint32_t &a, uint32_t &b)
+{
+    double fa = floor((lo - init - hd) / disp), fb = ceil((hi - init + hd) / disp);
+    if (!(fb >= 0.0) || !(fa <= (double)n) || !(fa <= fb + 2.0))
+        return false;
+    int ia = fa < 0.0 ? 0 : (int)fa, ib = fb > (double)n - 1.0 ? (int)n - 1 : (int)fb;
+    // first i with R_i > lo
+    while (ia > 0 && __dadd_rn(owm_wcenter(init, disp, ia - 1), hd) > lo)
+        --ia;
+    while (ia < (int)n && !(__dadd_rn(owm_wcenter(init, disp, ia), hd) > lo))
+        ++ia;
+    // last i with L_i < hi
+    while (ib < (int)n - 1 && __dsub_rn(owm_wcenter(init, disp, ib + 1), hd) < hi)
+        ++ib;
+    while (ib >= 0 && !(__dsub_rn(owm_wcenter(init, disp, ib), hd) < hi))
+        --ib;
+    if (ia > ib)
+        return false;
+    a = (uint32_t)ia;
+    b = (uint32_t)ib;
+    return true;
+}
+
+// Leaf -> window-block join.  mode 0 counts, mode 1 fills the lists.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t bx, uint32_t by,
+                                                uint32_t *__restrict__ cnt,
+                                                const uint32_t *__restrict__ start,
+                                                uint32_t *__restrict__ list, uint32_t cap,
+                                                uint32_t *err)
+{
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < B.nleaf;
+         l += gridDim.x * blockDim.x) {
+        if (T.lf_cnt[l] == 0)
+            continue;
+        const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
+        if (TD.ncols == 0)
+            continue;
+        const double4 box = T.lf_box[l];
+        const double hd = (double)(float)TD.h;
+        uint32_t a, b, c, d;
+        if (!win_span(box.x, box.z, TD.initx, TD.displace, hd, TD.ncols, a, b) ||
+            !win_span(box.y, box.w, TD.inity, TD.displace, hd, TD.nrows, c, d))
+            continue;
+        for (uint32_t j = c / by; j <= d / by; ++j)
+            for (uint32_t i = a / bx; i <= b / bx; ++i) {
+                uint32_t blk = TD.blk_off + j * TD.nbx + i;
+                if (MODE == 0) {
+                    atomicAdd(&cnt[blk], 1u);
+                } else {
+                    uint32_t k = start[blk] + atomicAdd(&cnt[blk], 1u);
+                    if (k < cap)
+                        list[k] = l;
+                    else
+                        atomicOr(err, 16u);
+                }
+            }
+    }
+}
+
+template <int BX, int BY>
+struct S1Shape {
+    static constexpr int NXC = 4 * BX + 1;
+    static constexpr int NYC = 4 * BY + 1;
+    static constexpr int NCELL = NXC * NYC;
+};
+
+template <int BX, int BY>
+__global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
+                                                        const uint32_t *__restrict__ blk_start,
+                                                        const uint32_t *__restrict__ list,
+                                                        uint32_t cap,
+                                                        uint32_t *__restrict__ min_ids)
+{
+    using S = S1Shape<BX, BY>;
+    constexpr int NXC = S::NXC;
+    __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
+    __shared__ uint32_t sCxl[BX], sCxh[BX], sCyl[BY], sCyh[BY];
+    __shared__ uint32_t sCnt[S::NCELL];
+    __shared__ unsigned long long sZ[S::NCELL];
+    __shared__ uint32_t sId[S::NCELL];
+    __shared__ uint8_t sDirty[S::NCELL];
+    __shared__ uint32_t m1cnt[S::NYC * BX], m1id[S::NYC * BX];
+    __shared__ unsigned long long m1z[S::NYC * BX];
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (uint32_t blk = blockIdx.x; blk < B.nblk; blk += gridDim.x) {
+        const TileDev &TD = B.tiles[owm_find_tile(B.blk_off, B.ntiles, blk)];
+        const uint32_t b = blk - TD.blk_off;
+        const uint32_t i0 = (b % TD.nbx) * BX, j0 = (b / TD.nbx) * BY;
+        const uint32_t ni = min((uint32_t)BX, TD.ncols - i0), nj = min((uint32_t)BY, TD.nrows - j0);
+        const double hd = (double)(float)TD.h;
+        const double inv = 1.0 / TD.displace;
+
+        if (tid < ni) {
+            double c = owm_wcenter(TD.initx, TD.displace, i0 + tid);
+            sLx[tid] = __dsub_rn(c, hd);
+            sRx[tid] = __dadd_rn(c, hd);
+        }
+        if (tid >= 64 && tid < 64 + nj) {
+            double c = owm_wcenter(TD.inity, TD.displace, j0 + tid - 64);
+            sLy[tid - 64] = __dsub_rn(c, hd);
+            sRy[tid - 64] = __dadd_rn(c, hd);
+        }
+        for (int c = tid; c < S::NCELL; c += S1_THREADS) {
+            sCnt[c] = 0;
+            sZ[c] = OWM_ZKEY_NONE;
+            sId[c] = OWM_NONE;
+            sDirty[c] = 0;
+        }
+        __syncthreads();
+        // window (i, j) = classes [2#{e <= L_i}, 2#{e < R_i}] (likewise y)
+        if (tid < ni) {
+            double l = sLx[tid], r = sRx[tid];
+            sCxl[tid] = 2 * (cnt_le(sLx, ni, l, inv) + cnt_le(sRx, ni, l, inv));
+            sCxh[tid] = 2 * (cnt_lt(sLx, ni, r, inv) + cnt_lt(sRx, ni, r, inv));
+        } else if (tid >= 64 && tid < 64 + nj) {
+            double l = sLy[tid - 64], r = sRy[tid - 64];
+            sCyl[tid - 64] = 2 * (cnt_le(sLy, nj, l, inv) + cnt_le(sRy, nj, l, inv));
+            sCyh[tid - 64] = 2 * (cnt_lt(sLy, nj, r, inv) + cnt_lt(sRy, nj, r, inv));
+        }
+        const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
+        // the class ranges are read by other threads after the rounds, and a
+        // block without leaves runs no round: publish them here
+        __syncthreads();
+
+        const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
+        const uint32_t nl = s1 - s0;
+        const uint32_t L = TD.leaf_size;
+        const uint32_t nl_grp = L <= 64 ? 64 / L : 1;
+        const uint32_t ls = L <= 64 ? L : 64;
+        const uint32_t nseg = L <= 64 ? 1 : (L + 63) / 64;
+        const uint32_t sub = lane / ls, off = lane - sub * ls;
+        const uint32_t nunits = (nl + nl_grp - 1) / nl_grp * nseg;
+        const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
+
+        for (uint32_t r = 0; r < rounds; ++r) {
+            uint32_t cell[S1_K], pid[S1_K];
+            unsigned long long zk[S1_K];
+#pragma unroll
+            for (int k = 0; k < S1_K; ++k) {
+                pid[k] = OWM_NONE;
+                const uint32_t u = (r * S1_K + k) * 4 + wave;
+                if (u >= nunits)
+                    continue;
+                const uint32_t g = u / nseg, seg = u - g * nseg;
+                const uint32_t li = g * nl_grp + sub;
+                if (sub >= nl_grp || li >= nl)
+                    continue;
+                const uint32_t leaf = list[s0 + li];
+                const uint32_t o = seg * 64 + off;
+                if (nseg == 1) {
+                    // whole leaf inside one class cell: bin its aggregate (a class
+                    // cell is never wider than the window step)
+                    const double4 bx = T.lf_box[leaf];
+                    if (bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace && bx.x > BL &&
+                        bx.z < BR && bx.y > BB && bx.w < BT) {
+                        uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
+                        if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
+                            if (o == 0) {
+                                uint32_t c = cy * NXC + cx;
+                                atomicAdd(&sCnt[c], T.lf_cnt[leaf]);
+                                unsigned long long key = T.lf_zkey[leaf];
+                                if (key != OWM_ZKEY_NONE) {
+                                    unsigned long long old = atomicMin(&sZ[c], key);
+                                    if (key < old)
+                                        sDirty[c] = 1;
+                                    cell[k] = c;
+                                    zk[k] = key;
+                                    pid[k] = S1_LEAFTAG | leaf;
+                                }
+                            }
+                            continue;
+                        }
+                    }
+                }
+                if (o >= T.lf_npts[leaf])
+                    continue;
+                const uint32_t p = T.lf_start[leaf] + o;
+                const double x = T.xs[p], y = T.ys[p];
+                if (!(x > BL && x < BR && y > BB && y < BT))
+                    continue;
+                const uint32_t c = cls(sLy, sRy, nj, y, inv) * NXC + cls(sLx, sRx, ni, x, inv);
+                atomicAdd(&sCnt[c], 1u);
+                const double z = T.zs[p];
+                if (owm_eligible(z)) {
+                    unsigned long long key = owm_zkey(z);
+                    unsigned long long old = atomicMin(&sZ[c], key);
+                    if (key < old)
+                        sDirty[c] = 1;
+                    cell[k] = c;
+                    zk[k] = key;
+                    pid[k] = p;
+                }
+            }
+            __syncthreads();
+            // a cell whose minimum z dropped in this round drops its old index
+#pragma unroll
+            for (int k = 0; k < S1_K; ++k)
+                if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k] && sDirty[cell[k]])
+                    sId[cell[k]] = OWM_NONE;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < S1_K; ++k)
+                if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k]) {
+                    uint32_t id = (pid[k] & S1_LEAFTAG) ? T.lf_zid[pid[k] & ~S1_LEAFTAG] : T.ids[pid[k]];
+                    atomicMin(&sId[cell[k]], id);
+                    sDirty[cell[k]] = 0;
+                }
+            __syncthreads();
+        }
+
+        // separable reduction: x-pass per (class row, window column) ...
+        const uint32_t nyc = 4 * nj + 1;
+        for (uint32_t q = tid; q < nyc * ni; q += S1_THREADS) {
+            uint32_t rr = q / ni, w = q - rr * ni;
+            uint32_t cnt = 0, id = OWM_NONE;
+            unsigned long long z = OWM_ZKEY_NONE;
+            const uint32_t chi = min(sCxh[w], 4 * ni);
+            for (uint32_t c = sCxl[w]; c <= chi; ++c) {
+                uint32_t cc = rr * NXC + c;
+                cnt += sCnt[cc];
+                if (owm_less(sZ[cc], sId[cc], z, id)) {
+                    z = sZ[cc];
+                    id = sId[cc];
+                }
+            }
+            m1cnt[rr * BX + w] = cnt;
+            m1id[rr * BX + w] = id;
+            m1z[rr * BX + w] = z;
+        }
+        __syncthreads();
+        // ... then y-pass per window
+        for (uint32_t q = tid; q < ni * nj; q += S1_THREADS) {
+            uint32_t v = q / ni, w = q - v * ni;
+            uint32_t cnt = 0, id = OWM_NONE;
+            unsigned long long z = OWM_ZKEY_NONE;
+            const uint32_t rhi = min(sCyh[v], 4 * nj);
+            for (uint32_t rr = sCyl[v]; rr <= rhi; ++rr) {
+                uint32_t k = rr * BX + w;
+                cnt += m1cnt[k];
+                if (owm_less(m1z[k], m1id[k], z, id)) {
+                    z = m1z[k];
+                    id = m1id[k];
+                }
+            }
+            bool valid = cnt >= TD.min_num_points && z != OWM_ZKEY_NONE;
+            min_ids[TD.win_off + (j0 + v) * TD.ncols + (i0 + w)] = valid ? id : OWM_NONE;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------ exclusive scan ----
+
+constexpr int SC_BLOCK = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_BLOCK * SC_ITEMS;
+constexpr uint32_t LB_AGG = 1u << 30, LB_INC = 2u << 30, LB_CNT = (1u << 30) - 1;
+
+// out[i] = sum(in[0..i)), out[n] = total; single pass, decoupled look-back.
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t *__restrict__ in, uint32_t n,
+                                                         uint32_t *__restrict__ out,
+                                                         uint32_t *lookback, uint32_t *tile_counter)
+{
+    __shared__ uint32_t s_tile, s_tmp[SC_BLOCK / 64], s_prefix;
+    const uint32_t ntiles = (n + SC_TILE - 1) / SC_TILE;
+    if (n == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            out[0] = 0;
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_tile = atomicAdd(tile_counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= ntiles)
+            break;
+        const uint32_t i0 = tile * SC_TILE + threadIdx.x * SC_ITEMS;
+        uint32_t v[SC_ITEMS], sum = 0;
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; ++k) {
+            v[k] = i0 + k < n ? in[i0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t inc = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t u = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o)
+                inc += u;
+        }
+        if (lane == 63)
+            s_tmp[w] = inc;
+        __syncthreads();
+        uint32_t add = 0, total = 0;
+        for (uint32_t k = 0; k < SC_BLOCK / 64; ++k) {
+            if (k < w)
+                add += s_tmp[k];
+            total += s_tmp[k];
+        }
+        if (threadIdx.x == 0) {
+            uint32_t *flag = lookback + tile;
+            uint32_t prefix = 0;
+            if (tile == 0) {
+                __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int32_t j = (int32_t)tile - 1;
+                for (;;) {
+                    uint32_t f = __hip_atomic_load(lookback + j, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    uint32_t st = f & ~LB_CNT;
+                    if (st == 0)
+                        continue;
+                    prefix += f & LB_CNT;
+                    if (st == LB_INC)
+                        break;
+                    --j;
+                }
+                __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_prefix = prefix;
+            if (tile == ntiles - 1)
+                out[n] = prefix + total;
+        }
+        __syncthreads();
+        uint32_t run = s_prefix + add + inc - sum;
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; ++k) {
+            if (i0 + k < n)
+                out[i0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------- launch --
+
+
+uint64_t stage1_work_words(uint64_t nblk) { return 2 * (nblk + 1) + (nblk / SC_TILE + 2) + 8; }
+
+template <int BX, int BY>
+static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, const uint32_t *start,
+                         const uint32_t *list, uint32_t cap, uint32_t *min_ids)
+{
+    uint32_t grid = B.nblk < 1 ? 1 : (B.nblk > 65536 ? 65536 : B.nblk);
+    hipLaunchKernelGGL((k_stage1<BX, BY>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, start, list,
+                       cap, min_ids);
+}
+
+void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                   uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *min_ids, uint32_t *err,
+                   uint32_t *total_out)
+{
+    if (B.nblk == 0)
+        return;
+    // work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter
+    uint32_t *cnt = work;
+    uint32_t *start = work + (B.nblk + 1);
+    uint32_t *lookback = start + (B.nblk + 1);
+    const uint32_t nt = B.nblk / SC_TILE + 2;
+    uint32_t *counter = lookback + nt;
+    launch_fill_u32(s, work, stage1_work_words(B.nblk), 0u);
+    uint32_t g = (B.nleaf + 255) / 256;
+    g = g < 1 ? 1 : (g > 8192 ? 8192 : g);
+    hipLaunchKernelGGL(k_s1_bin<0>, dim3(g), dim3(256), 0, s, B, T, cfg.bx, cfg.by, cnt,
+                       (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u, err);
+    uint32_t gs = (B.nblk + SC_TILE - 1) / SC_TILE;
+    hipLaunchKernelGGL(k_scan_excl, dim3(gs < 1 ? 1 : gs), dim3(SC_BLOCK), 0, s, cnt, B.nblk, start,
+                       lookback, counter);
+    launch_fill_u32(s, cnt, B.nblk, 0u);
+    hipLaunchKernelGGL(k_s1_bin<1>, dim3(g), dim3(256), 0, s, B, T, cfg.bx, cfg.by, cnt,
+                       (const uint32_t *)start, list, cap, err);
+    launch_add_u32(s, start + B.nblk, counter + 1, total_out);
+    if (cfg.bx == 16 && cfg.by == 8)
+        stage1_shape<16, 8>(s, B, T, start, list, cap, min_ids);
+    else if (cfg.bx == 16 && cfg.by == 16)
+        stage1_shape<16, 16>(s, B, T, start, list, cap, min_ids);
+    else if (cfg.bx == 32 && cfg.by == 8)
+        stage1_shape<32, 8>(s, B, T, start, list, cap, min_ids);
+    else
+        stage1_shape<8, 8>(s, B, T, start, list, cap, min_ids);
+}
+
+}  // namespace owm

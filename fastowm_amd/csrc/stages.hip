@@ -1,17 +1,6 @@
 // OWM stages 1-3 on gfx950.
 //
-// Stage 1 (reference: stage1, CPP-TBB-CPU/src/optim3_func.cpp:541-561, and the
-// GPU query2D, SYCL-CUDA-GPU/basic/traverse.h:686-721): one workgroup per
-// block of BX x BY windows.  The block's window edges split x (and y) into
-// "classes": class(x) = #{edges < x} + #{edges <= x} over the multiset of the
-// block's window edges, so that points of one class lie in exactly the same
-// windows and window (i, j) is the rectangle of classes
-// [2#{e <= L_i}, 2#{e < R_i}] x [2#{e <= B_j}, 2#{e < T_j}].  The workgroup
-// walks the LBVH for the leaves that overlap the block, bins each point
-// strictly inside the block into an LDS class cell (count, lowest (z, index)),
-// then reduces class cells to windows separably: along x per class row, then
-// along y per window.  Every comparison is the reference's strict open-box
-// test on the exact fp64 edges, so counts and minima are exact.
+// OWM stages 2 and 3 on gfx950 (stage 1 is in stage1.hip).
 //
 // Stage 2 (sort + stage2, owm_optim3.cpp:158-188, optim3_func.cpp:610-625):
 // radix-sorted window minima, run-length test, ordered compaction.
@@ -22,38 +11,6 @@
 
 namespace owm {
 
-constexpr int S1_THREADS = 256;
-constexpr int S1_STACK = 2048;
-constexpr int S1_LBUF = 512;
-constexpr int S1_K = 4;
-
-__device__ __forceinline__ uint32_t cnt_less(const double *e, uint32_t m, double x)
-{
-    // #{e[k] < x} over a sorted array
-    uint32_t lo = 0, hi = m;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (e[mid] < x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t cnt_leq(const double *e, uint32_t m, double x)
-{
-    uint32_t lo = 0, hi = m;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (e[mid] <= x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ void node_box(const TreeDev &T, uint32_t ref, double4 &box, uint32_t &cnt)
 {
     if (ref & OWM_LEAF_BIT) {
@@ -63,269 +20,6 @@ __device__ __forceinline__ void node_box(const TreeDev &T, uint32_t ref, double4
     } else {
         box = T.nd_box[ref];
         cnt = T.nd_cnt[ref];
-    }
-}
-
-template <int BX, int BY>
-struct S1Shape {
-    static constexpr int NXC = 4 * BX + 1;
-    static constexpr int NYC = 4 * BY + 1;
-    static constexpr int NCELL = NXC * NYC;
-    // traversal scratch and the x-pass result share one region
-    static constexpr int TRAV_BYTES = (S1_STACK + S1_LBUF + S1_LBUF + 1) * 4;
-    static constexpr int M1_BYTES = NYC * BX * 16;
-    static constexpr int SCRATCH = TRAV_BYTES > M1_BYTES ? TRAV_BYTES : M1_BYTES;
-};
-
-template <int BX, int BY>
-__global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
-                                                        uint32_t *__restrict__ min_ids,
-                                                        uint32_t *err)
-{
-    using S = S1Shape<BX, BY>;
-    constexpr int NXC = S::NXC;
-    __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
-    __shared__ double sEx[2 * BX], sEy[2 * BY];
-    __shared__ uint32_t sCxl[BX], sCxh[BX], sCyl[BY], sCyh[BY];
-    __shared__ uint32_t sCnt[S::NCELL];
-    __shared__ unsigned long long sZ[S::NCELL];
-    __shared__ uint32_t sId[S::NCELL];
-    __shared__ uint8_t sDirty[S::NCELL];
-    __shared__ __attribute__((aligned(16))) uint8_t sScratch[S::SCRATCH];
-    __shared__ uint32_t sNStack, sNLeaf, sTotal;
-
-    uint32_t *sStack = (uint32_t *)sScratch;
-    uint32_t *sLeaf = sStack + S1_STACK;
-    uint32_t *sPre = sLeaf + S1_LBUF;
-    uint32_t *m1cnt = (uint32_t *)sScratch;
-    uint32_t *m1id = m1cnt + S::NYC * BX;
-    unsigned long long *m1z = (unsigned long long *)(m1id + S::NYC * BX);
-
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-
-    for (uint32_t blk = blockIdx.x; blk < B.nblk; blk += gridDim.x) {
-        const uint32_t t = owm_find_tile(B.blk_off, B.ntiles, blk);
-        const TileDev &TD = B.tiles[t];
-        const uint32_t b = blk - TD.blk_off;
-        const uint32_t i0 = (b % TD.nbx) * BX, j0 = (b / TD.nbx) * BY;
-        const uint32_t ni = min((uint32_t)BX, TD.ncols - i0), nj = min((uint32_t)BY, TD.nrows - j0);
-        const double hd = (double)(float)TD.h;
-
-        if (tid < ni) {
-            double c = owm_wcenter(TD.initx, TD.displace, i0 + tid);
-            sLx[tid] = __dsub_rn(c, hd);
-            sRx[tid] = __dadd_rn(c, hd);
-        }
-        if (tid >= 64 && tid < 64 + nj) {
-            double c = owm_wcenter(TD.inity, TD.displace, j0 + tid - 64);
-            sLy[tid - 64] = __dsub_rn(c, hd);
-            sRy[tid - 64] = __dadd_rn(c, hd);
-        }
-        for (int c = tid; c < S::NCELL; c += S1_THREADS) {
-            sCnt[c] = 0;
-            sZ[c] = OWM_ZKEY_NONE;
-            sId[c] = OWM_NONE;
-            sDirty[c] = 0;
-        }
-        __syncthreads();
-        // stable merge of the two sorted edge lists (left edges first on ties)
-        if (tid < ni) {
-            double a = sLx[tid];
-            uint32_t pos = tid;
-            for (uint32_t m = 0; m < ni; ++m)
-                pos += sRx[m] < a;
-            sEx[pos] = a;
-        } else if (tid >= BX && tid < BX + ni) {
-            uint32_t k = tid - BX;
-            double r = sRx[k];
-            uint32_t pos = k;
-            for (uint32_t m = 0; m < ni; ++m)
-                pos += sLx[m] <= r;
-            sEx[pos] = r;
-        } else if (tid >= 128 && tid < 128 + nj) {
-            uint32_t k = tid - 128;
-            double a = sLy[k];
-            uint32_t pos = k;
-            for (uint32_t m = 0; m < nj; ++m)
-                pos += sRy[m] < a;
-            sEy[pos] = a;
-        } else if (tid >= 128 + BY && tid < 128 + BY + nj) {
-            uint32_t k = tid - 128 - BY;
-            double r = sRy[k];
-            uint32_t pos = k;
-            for (uint32_t m = 0; m < nj; ++m)
-                pos += sLy[m] <= r;
-            sEy[pos] = r;
-        }
-        if (tid == 0) {
-            sNStack = 0;
-            if (TD.n > 0 && TD.root != OWM_NONE) {
-                sStack[0] = TD.root;
-                sNStack = 1;
-            }
-        }
-        __syncthreads();
-        const uint32_t mx = 2 * ni, my = 2 * nj;
-        if (tid < ni) {
-            sCxl[tid] = 2 * cnt_leq(sEx, mx, sLx[tid]);
-            sCxh[tid] = 2 * cnt_less(sEx, mx, sRx[tid]);
-        } else if (tid >= 64 && tid < 64 + nj) {
-            sCyl[tid - 64] = 2 * cnt_leq(sEy, my, sLy[tid - 64]);
-            sCyh[tid - 64] = 2 * cnt_less(sEy, my, sRy[tid - 64]);
-        }
-        const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
-
-        for (;;) {
-            if (wave == 0) {
-                uint32_t nstack = sNStack, nleaf = 0;
-                while (nstack > 0 && nleaf + 64 <= S1_LBUF) {
-                    uint32_t room = S1_STACK - nstack;
-                    uint32_t m = min(min(64u, nstack), room);
-                    if (m == 0) {
-                        if (lane == 0)
-                            atomicOr(err, 1u);
-                        nstack = 0;
-                        break;
-                    }
-                    uint32_t node = lane < m ? sStack[nstack - 1 - lane] : 0u;
-                    nstack -= m;
-                    bool ov = false, leaf = false;
-                    if (lane < m) {
-                        double4 bx;
-                        uint32_t c;
-                        node_box(T, node, bx, c);
-                        ov = c > 0 && !(bx.z <= BL || bx.x >= BR || bx.w <= BB || bx.y >= BT);
-                        leaf = (node & OWM_LEAF_BIT) != 0;
-                    }
-                    bool emit = ov && leaf, push = ov && !leaf;
-                    unsigned long long em = __ballot(emit), pm = __ballot(push);
-                    if (emit)
-                        sLeaf[nleaf + __popcll(em & lt_mask)] = node & ~OWM_LEAF_BIT;
-                    if (push) {
-                        uint2 ch = T.nd_child[node];
-                        uint32_t base = nstack + 2 * __popcll(pm & lt_mask);
-                        sStack[base] = ch.y;
-                        sStack[base + 1] = ch.x;
-                    }
-                    nleaf += __popcll(em);
-                    nstack += 2 * __popcll(pm);
-                }
-                // point-count prefix over the buffered leaves
-                uint32_t run = 0;
-                for (uint32_t k0 = 0; k0 < nleaf; k0 += 64) {
-                    uint32_t k = k0 + lane;
-                    uint32_t c = k < nleaf ? T.lf_npts[sLeaf[k]] : 0u;
-                    uint32_t inc = c;
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        uint32_t u = __shfl_up(inc, o, 64);
-                        if (lane >= (uint32_t)o)
-                            inc += u;
-                    }
-                    if (k < nleaf)
-                        sPre[k] = run + inc - c;
-                    run += __shfl(inc, 63, 64);
-                }
-                if (lane == 0) {
-                    sPre[nleaf] = run;
-                    sNStack = nstack;
-                    sNLeaf = nleaf;
-                    sTotal = run;
-                }
-            }
-            __syncthreads();
-            const uint32_t nleaf = sNLeaf, total = sTotal;
-            if (nleaf == 0)
-                break;
-            for (uint32_t base = 0; base < total; base += S1_THREADS * S1_K) {
-                uint32_t cell[S1_K], pid[S1_K];
-                unsigned long long zk[S1_K];
-#pragma unroll
-                for (int k = 0; k < S1_K; ++k) {
-                    pid[k] = OWM_NONE;
-                    uint32_t f = base + k * S1_THREADS + tid;
-                    if (f >= total)
-                        continue;
-                    uint32_t lo = 0, hi = nleaf;  // last leaf with sPre <= f
-                    while (hi - lo > 1) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (sPre[mid] <= f)
-                            lo = mid;
-                        else
-                            hi = mid;
-                    }
-                    uint32_t p = T.lf_start[sLeaf[lo]] + (f - sPre[lo]);
-                    double x = T.xs[p], y = T.ys[p];
-                    if (!(x > BL && x < BR && y > BB && y < BT))
-                        continue;
-                    uint32_t cx = cnt_less(sEx, mx, x) + cnt_leq(sEx, mx, x);
-                    uint32_t cy = cnt_less(sEy, my, y) + cnt_leq(sEy, my, y);
-                    uint32_t c = cy * NXC + cx;
-                    atomicAdd(&sCnt[c], 1u);
-                    double z = T.zs[p];
-                    if (owm_eligible(z)) {
-                        unsigned long long key = owm_zkey(z);
-                        unsigned long long old = atomicMin(&sZ[c], key);
-                        if (key < old)
-                            sDirty[c] = 1;
-                        cell[k] = c;
-                        zk[k] = key;
-                        pid[k] = p;
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < S1_K; ++k)
-                    if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k] && sDirty[cell[k]])
-                        sId[cell[k]] = OWM_NONE;
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < S1_K; ++k)
-                    if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k]) {
-                        atomicMin(&sId[cell[k]], T.ids[pid[k]]);
-                        sDirty[cell[k]] = 0;
-                    }
-                __syncthreads();
-            }
-        }
-
-        // separable reduction: x-pass per (class row, window column) ...
-        const uint32_t nyc = 2 * my + 1;
-        for (uint32_t q = tid; q < nyc * ni; q += S1_THREADS) {
-            uint32_t r = q / ni, w = q - r * ni;
-            uint32_t cnt = 0, id = OWM_NONE;
-            unsigned long long z = OWM_ZKEY_NONE;
-            for (uint32_t c = sCxl[w]; c <= sCxh[w]; ++c) {
-                uint32_t cc = r * NXC + c;
-                cnt += sCnt[cc];
-                if (owm_less(sZ[cc], sId[cc], z, id)) {
-                    z = sZ[cc];
-                    id = sId[cc];
-                }
-            }
-            m1cnt[r * BX + w] = cnt;
-            m1id[r * BX + w] = id;
-            m1z[r * BX + w] = z;
-        }
-        __syncthreads();
-        // ... then y-pass per window
-        for (uint32_t q = tid; q < ni * nj; q += S1_THREADS) {
-            uint32_t v = q / ni, w = q - v * ni;
-            uint32_t cnt = 0, id = OWM_NONE;
-            unsigned long long z = OWM_ZKEY_NONE;
-            for (uint32_t r = sCyl[v]; r <= sCyh[v]; ++r) {
-                uint32_t rr = r * BX + w;
-                cnt += m1cnt[rr];
-                if (owm_less(m1z[rr], m1id[rr], z, id)) {
-                    z = m1z[rr];
-                    id = m1id[rr];
-                }
-            }
-            bool valid = cnt >= TD.min_num_points && z != OWM_ZKEY_NONE;
-            min_ids[TD.win_off + (j0 + v) * TD.ncols + (i0 + w)] = valid ? id : OWM_NONE;
-        }
-        __syncthreads();
     }
 }
 
@@ -458,8 +152,15 @@ __global__ void k_s3_mark(BatchDev B, TreeDev T, const uint32_t *__restrict__ ll
         const TileDev &TD = B.tiles[owm_find_tile(B.pt_off, B.ntiles, id)];
         if (TD.bsize == 0)
             continue;
-        const uint32_t sp = T.pos[id];
-        double x = T.xs[sp], y = T.ys[sp];
+        double x, y;
+        if (T.X) {
+            x = T.X[id];
+            y = T.Y[id];
+        } else {
+            const uint32_t sp = T.pos[id];
+            x = T.xs[sp];
+            y = T.ys[sp];
+        }
         if (isnan(x) || isnan(y))
             continue;
         const double hb = (double)(float)TD.hb;
@@ -623,31 +324,6 @@ __global__ void k_seed_ranges(BatchDev B, const uint32_t *__restrict__ seeds, co
 __global__ void k_add_u32(const uint32_t *a, const uint32_t *b, uint32_t *out) { *out = *a + *b; }
 
 // ----------------------------------------------------------------- launch --
-
-template <int BX, int BY>
-static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, uint32_t *min_ids,
-                         uint32_t *err)
-{
-    uint32_t grid = B.nblk < 1 ? 1 : B.nblk;
-    if (grid > 65536)
-        grid = 65536;
-    hipLaunchKernelGGL((k_stage1<BX, BY>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, min_ids, err);
-}
-
-void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   uint32_t *min_ids, uint32_t *err)
-{
-    if (B.nblk == 0)
-        return;
-    if (cfg.bx == 16 && cfg.by == 8)
-        stage1_shape<16, 8>(s, B, T, min_ids, err);
-    else if (cfg.bx == 16 && cfg.by == 16)
-        stage1_shape<16, 16>(s, B, T, min_ids, err);
-    else if (cfg.bx == 32 && cfg.by == 8)
-        stage1_shape<32, 8>(s, B, T, min_ids, err);
-    else
-        stage1_shape<8, 8>(s, B, T, min_ids, err);
-}
 
 static uint32_t cp_grid(uint32_t n)
 {

@@ -98,13 +98,14 @@ __global__ __launch_bounds__(TB) void k_bbox(BatchDev B, const double *__restric
         uint32_t t0 = owm_find_tile(B.pt_off, B.ntiles, c0);
         uint32_t t1 = owm_find_tile(B.pt_off, B.ntiles, c1 - 1);
         double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-        uint32_t t = t0;
+        uint32_t t = t0, next = B.pt_off[t0 + 1];
         for (uint32_t i = c0 + tid; i < c1; i += TB) {
-            while (i >= B.pt_off[t + 1]) {
+            while (i >= next) {
                 bbox_flush(bk, t, mnx, mny, mxx, mxy);
                 mnx = mny = INFINITY;
                 mxx = mxy = -INFINITY;
                 ++t;
+                next = B.pt_off[t + 1];
             }
             double x = X[i], y = Y[i];
             mnx = fmin(mnx, x);
@@ -171,11 +172,14 @@ __device__ __forceinline__ unsigned long long morton64(const TileDev &T, double 
 }
 
 // Sort keys (tile bits above the top key_bits of the Morton code) plus the
-// digit histograms of every radix pass.
+// digit histograms of every radix pass.  The cloud is also packed into 32-byte
+// (x, y, z, 0) records, so that the gather after the sort reads one aligned
+// sector per point instead of three scattered 8-byte words.
 __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restrict__ X,
                                                 const double *__restrict__ Y,
-                                                uint32_t *__restrict__ keys, uint32_t *hist,
-                                                int passes)
+                                                const double *__restrict__ Z,
+                                                uint32_t *__restrict__ keys, double4 *__restrict__ aos,
+                                                uint32_t *hist, int passes)
 {
     __shared__ uint32_t h[4][256];
     for (int i = threadIdx.x; i < 4 * 256; i += TB)
@@ -186,13 +190,16 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
         uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
         uint32_t t = owm_find_tile(B.pt_off, B.ntiles, c0);
+        uint32_t next = B.pt_off[t + 1];
         for (uint32_t i = c0 + threadIdx.x; i < c1; i += TB) {
-            while (i >= B.pt_off[t + 1])
-                ++t;
+            while (i >= next)
+                next = B.pt_off[++t + 1];
             const TileDev &T = B.tiles[t];
-            unsigned long long m = morton64(T, X[i], Y[i]);
+            const double x = X[i], y = Y[i];
+            unsigned long long m = morton64(T, x, y);
             uint32_t k = T.tile_key | (uint32_t)(m >> shift);
             keys[i] = k;
+            aos[i] = make_double4(x, y, Z[i], 0.0);
             for (int p = 0; p < passes; ++p)
                 atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
         }
@@ -223,9 +230,7 @@ __device__ __forceinline__ void seg_reduce(uint32_t off, uint32_t ls, F &&step)
 // y) and lowest (z, index).
 __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t *__restrict__ vals,
                                                        const uint32_t *__restrict__ skeys,
-                                                       const double *__restrict__ X,
-                                                       const double *__restrict__ Y,
-                                                       const double *__restrict__ Z, TreeOut T)
+                                                       const double4 *__restrict__ aos, TreeOut T)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = gridDim.x * (TB / 64);
@@ -245,12 +250,14 @@ __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t
         for (uint32_t k = off; k < npts; k += ls) {
             uint32_t p = start + k;
             uint32_t v = vals[p];
-            double x = X[v], y = Y[v], z = Z[v];
+            const double4 r = aos[v];
+            const double x = r.x, y = r.y, z = r.z;
             T.xs[p] = x;
             T.ys[p] = y;
             T.zs[p] = z;
             T.ids[p] = v;
-            T.pos[v] = p;
+            if (T.pos)
+                T.pos[v] = p;
             if (!isnan(x) && !isnan(y)) {
                 ++cnt;
                 mnx = fmin(mnx, x);
@@ -527,18 +534,17 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 }
 
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   uint32_t *keys, uint32_t *hist, int passes)
+                   const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes)
 {
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, keys,
-                       hist, passes);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, z, keys,
+                       aos, hist, passes);
 }
 
 void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *vals,
-                          const uint32_t *skeys, const double *x, const double *y, const double *z,
-                          const TreeOut &T)
+                          const uint32_t *skeys, const double4 *aos, const TreeOut &T)
 {
     hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.ngrp, TB / 64, 16384)), dim3(TB), 0, s, B,
-                       vals, skeys, x, y, z, T);
+                       vals, skeys, aos, T);
 }
 
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)

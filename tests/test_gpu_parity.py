@@ -155,6 +155,30 @@ def test_batch_matches_per_tile_oracle(ctx):
         assert np.array_equal(g, w), f"tile {i}"
 
 
+def test_stage1_join_rerun_and_empty_blocks(ctx):
+    """A leaf/block join list too small for the data forces the re-run path;
+    blocks left without leaves must still publish their window class ranges.
+    (Regression: Overlap 0.9 tile whose join exceeded the first guess.)"""
+    tiles = synth.batch_tiles(24, seed=3, median=20_000, lo=2_000, hi=120_000)
+    t = tiles[3]
+    x, y, z = t.generate()
+    want, _, _ = O.run(x, y, z, t.wsize, 15, 0.9, 32)
+    say("overlap 0.9 tile, default join size")
+    assert np.array_equal(ctx.run_tile(x, y, z, t.wsize, 15, 0.9, 0), want)
+    say("overlap 0.9 tile, join list forced to 64 entries")
+    os.environ["OWM_S1_LIST_CAP"] = "64"
+    try:
+        got = ctx.run_tile(x, y, z, t.wsize, 15, 0.9, 0)
+        tree = ctx.parallel_qtree(x, y, z, leaf_size=32)
+        gm, _ = ctx.stage1(tree, t.wsize, 0.9, 15)
+        tree.free()
+    finally:
+        del os.environ["OWM_S1_LIST_CAP"]
+    assert np.array_equal(got, want)
+    p = O.params(t.wsize, 15, 0.9, synth.bbox(x, y), len(x))
+    assert np.array_equal(gm, O.Tree(x, y, z, p).stage1())
+
+
 def test_device_resident_input(ctx):
     import torch
     say("device input c1")
