@@ -1,0 +1,195 @@
+// owm: command-line driver of the MI355X OWM pipeline.
+//
+// Mirrors the reference CPU driver's flags (CPP-TBB-CPU/application/owm_optim3.cpp:14-40):
+//   -i <input without .xyz>  -W <Wsize>  -B <Bsize>  -O <Overlap>  -s <maxNumber>
+//   -l <loops>  --results <csv>  (-n, -r and -L are accepted and ignored: the
+//   GPU path has no thread count, minRadius policy or parallel tree level)
+// plus --device <id> and -o <output .xyz of the seeds>.
+//
+// Input: .xyz text, "x y z" per line; an optional 5-number header
+// "N minx maxx miny maxy" (the reference's *H.xyz convention,
+// src/optim3_func.cpp:681-700) gives the bbox used for the OWM parameters.
+// The timed region is the pipeline only (tree + stages 1-3), like the
+// reference's (owm_optim3.cpp:77-89,145-206); file reading is not timed.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "owm.h"
+
+namespace {
+
+bool read_xyz(const std::string &path, std::vector<double> &x, std::vector<double> &y,
+              std::vector<double> &z, bool &have_box, double box[4])
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f)
+        return false;
+    std::vector<char> buf;
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    buf.resize((size_t)sz + 1);
+    size_t got = fread(buf.data(), 1, (size_t)sz, f);
+    fclose(f);
+    buf[got] = 0;
+    char *p = buf.data();
+    char *end = p + got;
+    // header: first line holds a single number
+    have_box = false;
+    {
+        char *eol = (char *)memchr(p, '\n', (size_t)(end - p));
+        if (eol) {
+            char *q;
+            strtod(p, &q);
+            while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r'))
+                ++q;
+            if (q == eol) {
+                unsigned long long n = strtoull(p, &q, 10);
+                p = eol + 1;
+                double h[4];
+                for (int k = 0; k < 4; ++k)
+                    h[k] = strtod(p, &p);
+                box[0] = h[0];  // minx
+                box[1] = h[1];  // maxx
+                box[2] = h[2];  // miny
+                box[3] = h[3];  // maxy
+                have_box = true;
+                x.reserve(n);
+                y.reserve(n);
+                z.reserve(n);
+            }
+        }
+    }
+    while (p < end) {
+        char *q;
+        double a = strtod(p, &q);
+        if (q == p) {
+            ++p;
+            continue;
+        }
+        double b = strtod(q, &q);
+        double c = strtod(q, &q);
+        x.push_back(a);
+        y.push_back(b);
+        z.push_back(c);
+        char *eol = (char *)memchr(q, '\n', (size_t)(end - q));
+        p = eol ? eol + 1 : end;
+    }
+    return true;
+}
+
+void usage()
+{
+    fprintf(stderr,
+            "usage: owm -i <input without .xyz> [-W 10] [-B 20] [-O 0.80] [-s 32] [-l 1]\n"
+            "           [--results o3_memoization.csv] [-o seeds.xyz] [--device 0]\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    std::string input = "data/INAER_2011_Alcoy", results = "o3_memoization.csv", out;
+    owm_params P{};
+    P.wsize = 10;
+    P.bsize = 20;
+    P.overlap = 0.80;
+    P.leaf_size = 32;
+    int loops = 1, device = 0;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto val = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                usage();
+                exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-i" || a == "--input") input = val();
+        else if (a == "-W" || a == "--Wsize") P.wsize = (uint32_t)strtoul(val(), nullptr, 10);
+        else if (a == "-B" || a == "--Bsize") P.bsize = (uint32_t)strtoul(val(), nullptr, 10);
+        else if (a == "-O" || a == "--Overlap") P.overlap = strtod(val(), nullptr);
+        else if (a == "-s" || a == "--size") P.leaf_size = (uint32_t)strtoul(val(), nullptr, 10);
+        else if (a == "-l" || a == "--loop") loops = atoi(val());
+        else if (a == "--results") results = val();
+        else if (a == "-o") out = val();
+        else if (a == "--device") device = atoi(val());
+        else if (a == "-n" || a == "-r" || a == "-L" || a == "--num_threads" || a == "--radius" ||
+                 a == "--level") (void)val();
+        else if (a == "-h" || a == "--help") {
+            usage();
+            return 0;
+        } else {
+            usage();
+            return 2;
+        }
+    }
+    std::vector<double> x, y, z;
+    double box[4];
+    bool have_box = false;
+    std::string path = input + ".xyz";
+    if (!read_xyz(path, x, y, z, have_box, box)) {
+        fprintf(stderr, "Unable to read %s\n", path.c_str());
+        return 1;
+    }
+    if (have_box) {
+        P.flags |= OWM_F_HAVE_BBOX;
+        memcpy(P.bbox, box, sizeof box);
+    }
+    printf("Input: %s (%zu points) on device %d\n", path.c_str(), x.size(), device);
+    owm_ctx *ctx = nullptr;
+    owm_status st = owm_ctx_create(device, 0, &ctx);
+    if (st != OWM_OK) {
+        fprintf(stderr, "owm_ctx_create: %s\n", owm_status_str(st));
+        return 1;
+    }
+    std::vector<uint32_t> seeds(x.size() + 1);
+    uint64_t ns = 0;
+    std::vector<double> times;
+    for (int r = 0; r < loops; ++r) {
+        auto t0 = std::chrono::steady_clock::now();
+        st = owm_run_tile(ctx, x.data(), y.data(), z.data(), x.size(), &P, seeds.data(),
+                          seeds.size(), &ns);
+        auto t1 = std::chrono::steady_clock::now();
+        if (st != OWM_OK) {
+            fprintf(stderr, "owm_run_tile: %s: %s\n", owm_status_str(st), owm_last_error(ctx));
+            owm_ctx_destroy(ctx);
+            return 1;
+        }
+        double s = std::chrono::duration<double>(t1 - t0).count();
+        times.push_back(s);
+        printf("TOTAL time elapsed:     %.6f s\n", s);
+        printf("Output ground seed-point cloud with %llu points, %llu fewer points than input cloud\n",
+               (unsigned long long)ns, (unsigned long long)(x.size() - ns));
+    }
+    double avg = 0;
+    for (double t : times)
+        avg += t;
+    avg /= times.size();
+    printf("Average OWM: %.6lf\n", avg);
+    printf("Mpoints/s (including host transfers): %.1f\n", x.size() / avg / 1e6);
+    if (!results.empty()) {
+        FILE *rf = fopen(results.c_str(), "a+");
+        if (rf) {
+            fseek(rf, 0, SEEK_END);
+            if (ftell(rf) == 0)
+                fprintf(rf, "ExecName InputFile NumThreads MinRadius MaxNumber Level TreeTime OwmTime Correctness\n");
+            fprintf(rf, "%s %s 0 0 %u 0 %.6f %.6f -1\n", argv[0], path.c_str(), P.leaf_size, 0.0, avg);
+            fclose(rf);
+        }
+    }
+    if (!out.empty()) {
+        FILE *of = fopen(out.c_str(), "w");
+        if (of) {
+            for (uint64_t i = 0; i < ns; ++i)
+                fprintf(of, "%.2f %.2f %.2f\n", x[seeds[i]], y[seeds[i]], z[seeds[i]]);
+            fclose(of);
+        }
+    }
+    owm_ctx_destroy(ctx);
+    return 0;
+}
