@@ -95,6 +95,59 @@ def run_batch_bench(ctx, torch, synth, n_tiles, steps):
             "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same}
 
 
+def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
+    """Poisson request streams against the serving loop: N log-uniform
+    10k-8M points, Wsize 5-20, Bsize 10-30, Overlap 0.5-0.9; one request in
+    four re-queries a resident tile (same key, new parameters)."""
+    import threading
+    import numpy as np
+    pool = []
+    for _, t in synth.serve_requests(pool_size, rate=1.0, seed=11):
+        pool.append((t, t.generate()))
+    out = []
+    with owm.Context(0) as ctx:
+        srv = owm.Server(ctx, max_inflight=slots, max_batch=32)
+        # warm the slots once (allocation of arenas)
+        for t, (x, y, z) in pool[:slots * 2]:
+            srv.wait(srv.submit(x, y, z, t.wsize, t.bsize, t.overlap))
+        for rate in rates:
+            reqs = synth.serve_requests(n_req, rate=rate, seed=int(rate))
+            ids = [None] * n_req
+            npts = [0] * n_req
+            t0 = time.perf_counter()
+
+            def submitter():
+                for i, (t_arr, tr) in enumerate(reqs):
+                    now = time.perf_counter() - t0
+                    if t_arr > now:
+                        time.sleep(t_arr - now)
+                    k = i % pool_size
+                    t, (x, y, z) = pool[k]
+                    key = k + 1 if i % 4 == 3 else 0
+                    ids[i] = srv.submit(x, y, z, tr.wsize, tr.bsize, tr.overlap, key=key)
+                    npts[i] = t.n
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            lats = []
+            for i in range(n_req):
+                while ids[i] is None:
+                    time.sleep(0.0002)
+                _, lat = srv.wait(ids[i])
+                lats.append(lat)
+            th.join()
+            wall = time.perf_counter() - t0
+            lats = np.array(lats) * 1e3
+            out.append({"rate_tiles_s": rate, "requests": n_req,
+                        "p50_ms": round(float(np.percentile(lats, 50)), 2),
+                        "p99_ms": round(float(np.percentile(lats, 99)), 2),
+                        "mpts": round(sum(npts) / wall / 1e6, 1),
+                        "offered_mpts": round(sum(npts) / reqs[-1][0] / 1e6, 1)})
+            log("serve", out[-1])
+        srv.stop()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +157,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-tile batch leg")
     ap.add_argument("--batch-tiles", type=int, default=512)
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rates", default="50,500,2000")
+    ap.add_argument("--serve-requests", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the batch and serving paths "
@@ -164,6 +220,10 @@ def main():
     if not args.no_batch:
         log("batch leg")
         line["batch"] = run_batch_bench(ctx, torch, synth, args.batch_tiles, max(1, args.steps // 4))
+    if not args.no_serve:
+        log("serve leg")
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        line["serve"] = run_serve_bench(owm, synth, rates, args.serve_requests, 48, 3)
     if not args.no_cpu:
         log("cpu baseline leg")
         cb, parity = cpu_baseline(x, y, z, seeds, args.cpu_reps)

@@ -7,14 +7,6 @@
 
 using namespace owm;
 
-struct owm_tree {
-    owm_ctx *ctx = nullptr;
-    DevBuf mem;
-    Prepared P;
-    TreeOut T{};
-    uint64_t n = 0;
-};
-
 #define GUARD_BEGIN try {
 #define GUARD_END(ctx)                                              \
     }                                                               \
@@ -128,7 +120,7 @@ static owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, c
             return OWM_E_BUDGET;
         }
         (void)hipStreamSynchronize(ctx->stream);
-        hipError_t e = ctx->upload.reserve(3 * b);
+        hipError_t e = ctx->upload.reserve(3 * b, ctx->room_for(ctx->upload));
         if (e != hipSuccess)
             return hip_fail(ctx, e, "upload buffer");
     }
@@ -256,7 +248,7 @@ owm_status owm_tree_build(owm_ctx *ctx, const double *x, const double *y, const 
         set_err(ctx, "tree needs %llu device bytes over the budget", (unsigned long long)tb);
         return OWM_E_BUDGET;
     }
-    hipError_t e = tr->mem.reserve(tb);
+    hipError_t e = tr->mem.reserve(tb, tb);
     if (e != hipSuccess)
         return hip_fail(ctx, e, "tree allocation");
     ctx->trees_bytes += tr->mem.bytes;

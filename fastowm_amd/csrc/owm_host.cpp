@@ -128,12 +128,12 @@ void DevBuf::release()
     bytes = 0;
 }
 
-hipError_t DevBuf::reserve(uint64_t need)
+hipError_t DevBuf::reserve(uint64_t need, uint64_t cap)
 {
     if (need <= bytes)
         return hipSuccess;
     release();
-    uint64_t want = need + need / 4;
+    uint64_t want = std::max(need, std::min(need + need / 4, cap));
     hipError_t e = hipMalloc(&ptr, want);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -633,7 +633,7 @@ owm_status Arena::ensure(owm_ctx *ctx, uint64_t bytes)
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess)
         return hip_fail(ctx, e, "sync before arena growth");
-    e = buf.reserve(bytes);
+    e = buf.reserve(bytes, ctx->room_for(buf));
     if (e != hipSuccess)
         return hip_fail(ctx, e, "arena allocation");
     return OWM_OK;
@@ -855,7 +855,7 @@ owm_status ensure_s1_list(owm_ctx *ctx, uint64_t cap)
         return OWM_E_BUDGET;
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "sync before list growth");
-    HIP_TRY(ctx->s1_list.reserve(cap * 4), "stage-1 join list");
+    HIP_TRY(ctx->s1_list.reserve(cap * 4, ctx->room_for(ctx->s1_list)), "stage-1 join list");
     return OWM_OK;
 }
 
@@ -990,6 +990,14 @@ void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, c
                                            &Q.cell_off, &Q.blk_off, &L.grp_off};
     for (int k = 0; k < 7; ++k)
         memcpy(h_off[k], src[k]->data(), (T + 1) * 4);
+}
+
+uint64_t owm_ctx::room_for(const owm::DevBuf &b) const
+{
+    if (!budget)
+        return ~0ull;
+    uint64_t others = held_bytes() - b.bytes;
+    return budget > others ? budget - others : 0;
 }
 
 uint64_t owm_ctx::held_bytes() const

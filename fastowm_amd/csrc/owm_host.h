@@ -26,7 +26,8 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf();
     void release();
-    hipError_t reserve(uint64_t need);
+    // Grows to at least `need` bytes (with 25% headroom, never above `cap`).
+    hipError_t reserve(uint64_t need, uint64_t cap = ~0ull);
 };
 
 // Bump allocator over one device buffer, reset at the start of every run.
@@ -197,5 +198,18 @@ struct owm_ctx {
     void stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
                      const owm::QueryLayout &Q);
     uint64_t held_bytes() const;
+    // Largest size buffer `b` may grow to without exceeding the budget.
+    uint64_t room_for(const owm::DevBuf &b) const;
     ~owm_ctx();
+};
+
+// A device-resident tree (owm_tree_build): its own device buffer, the host copy
+// of its layout, and the context whose stream built it (null once handed to
+// the serving cache, which then owns it).
+struct owm_tree {
+    owm_ctx *ctx = nullptr;
+    owm::DevBuf mem;
+    owm::Prepared P;
+    owm::TreeOut T{};
+    uint64_t n = 0;
 };

@@ -340,3 +340,57 @@ class Context:
         self._check(lib().owm_tree_run(self.h, tree.h, ctypes.byref(p), out.ctypes.data, cap,
                                        ctypes.byref(n)))
         return out[:n.value].copy()
+
+
+class Server:
+    """Continuous-batching serving loop on a context (owm_serve_*).
+
+    Requests are admitted as slots and device memory free up; groups of
+    different slots overlap on the device.  ``key`` != 0 keeps the tile's tree
+    resident so that re-queries with other parameters skip the tree build.
+    Input arrays are referenced until their request completes.
+    """
+
+    def __init__(self, ctx: Context, max_inflight: int = 2, max_batch: int = 64):
+        self.ctx = ctx
+        ctx._check(lib().owm_serve_start(ctx.h, max_inflight, max_batch))
+        self._keep = {}
+
+    def submit(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None, key=0):
+        p = make_params(wsize, bsize, overlap, leaf_size, bbox)
+        (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
+        rid = ctypes.c_uint64()
+        self.ctx._check(lib().owm_serve_submit(self.ctx.h, xp, yp, zp, len(xa), ctypes.byref(p),
+                                               int(key), ctypes.byref(rid)))
+        self._keep[rid.value] = (xa, ya, za)
+        return rid.value
+
+    def _result(self, rid, wait):
+        n = len(self._keep[rid][0]) if rid in self._keep else 1 << 20
+        out = np.empty(max(n + 1, 1), np.uint32)
+        ns, lat = ctypes.c_uint64(), ctypes.c_double()
+        f = lib().owm_serve_wait if wait else lib().owm_serve_poll
+        st = f(self.ctx.h, rid, out.ctypes.data, len(out), ctypes.byref(ns), ctypes.byref(lat))
+        if st == 7:  # not ready
+            return None
+        self._keep.pop(rid, None)
+        self.ctx._check(st)
+        return out[:ns.value].copy(), lat.value
+
+    def poll(self, rid):
+        """(seeds, latency_s) when done, None while running."""
+        return self._result(rid, False)
+
+    def wait(self, rid):
+        return self._result(rid, True)
+
+    def cancel(self, rid):
+        self.ctx._check(lib().owm_serve_cancel(self.ctx.h, rid))
+
+    def evict(self, key):
+        self.ctx._check(lib().owm_serve_evict(self.ctx.h, int(key)))
+
+    def stop(self):
+        if self.ctx.h:
+            lib().owm_serve_stop(self.ctx.h)
+        self._keep.clear()
