@@ -160,7 +160,20 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times "
+                         "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        a = torch.ones(n // 4, dtype=torch.int32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the batch and serving paths "
                                    "shard tiles across GPUs by design but no multi-GPU run is "

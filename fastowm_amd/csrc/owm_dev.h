@@ -33,6 +33,7 @@ struct TileDev {
     int32_t h, hb;
     uint32_t bsize, pad0;
     double displace, initx, inity, initxg, inityg;
+    double inv_disp;  // 1 / displace (estimates only; exact compares follow)
 };
 
 // Offsets of every per-tile range, one array per kind, for binary searches.

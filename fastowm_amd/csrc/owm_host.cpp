@@ -240,6 +240,7 @@ void fill_query(TileDev &T, const owm_params &P, const Derived &D)
     T.hb = D.hb;
     T.bsize = P.bsize;
     T.displace = D.displace;
+    T.inv_disp = 1.0 / D.displace;
     T.initx = D.initx;
     T.inity = D.inity;
     T.initxg = D.initxg;
@@ -673,7 +674,14 @@ uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested)
 
 Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total)
 {
-    return ctx->tuner.stage1_for(np_total);
+    Stage1Cfg c = ctx->tuner.stage1_for(np_total);
+    // "bx,by,k" forces one window-block shape (for experiments)
+    if (const char *ov = getenv("OWM_S1_SHAPE")) {
+        Stage1Cfg f{};
+        if (sscanf(ov, "%u,%u,%u", &f.bx, &f.by, &f.k) == 3 && stage1_shape_ok(f))
+            c = f;
+    }
+    return c;
 }
 
 // Per-tile bbox (caller's, else an exact device reduction), leaf sizes and the
@@ -710,11 +718,12 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
     if (need_bbox) {
         std::vector<std::array<double, 4>> unit(n_tiles, {0.0, 1.0, 0.0, 1.0});
         plan_tree(P.td, P.n, P.leaf, unit, P.L);
-        uint64_t bytes = align_up((uint64_t)n_tiles * 32, 256) + align_up((n_tiles + 1) * 4ull, 256);
+        const uint64_t bk_bytes = bbox_key_words(n_tiles) * 8;
+        uint64_t bytes = align_up(bk_bytes, 256) + align_up((n_tiles + 1) * 4ull, 256);
         st = ctx->small.ensure(ctx, bytes);
         if (st != OWM_OK)
             return st;
-        unsigned long long *bk = (unsigned long long *)ctx->small.take(n_tiles * 32ull);
+        unsigned long long *bk = (unsigned long long *)ctx->small.take(bk_bytes);
         uint32_t *ptoff = (uint32_t *)ctx->small.take((n_tiles + 1) * 4ull);
         HIP_TRY(hipMemcpyAsync(ptoff, P.L.pt_off.data(), (n_tiles + 1) * 4ull,
                                hipMemcpyHostToDevice, s),

@@ -78,7 +78,7 @@ struct Tuner {
         uint32_t leaf;
         Stage1Cfg s1;
     };
-    std::vector<Entry> table{{~0ull, 32, {8, 8}}};
+    std::vector<Entry> table{{~0ull, 32, {8, 8, 4}}};
     uint32_t leaf_for(uint64_t n) const
     {
         for (const auto &e : table)
@@ -91,7 +91,7 @@ struct Tuner {
         for (const auto &e : table)
             if (n <= e.max_n)
                 return e.s1;
-        return {8, 8};
+        return {8, 8, 4};
     }
 };
 
@@ -102,7 +102,7 @@ struct Prepared {
     std::vector<std::array<double, 4>> box;
     TreeLayout L;
     QueryLayout Q;
-    Stage1Cfg cfg{8, 8};
+    Stage1Cfg cfg{8, 8, 4};
     bool any_need1 = false;
 };
 

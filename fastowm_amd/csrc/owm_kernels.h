@@ -33,6 +33,9 @@ struct TreeOut {
 // bbox per tile as order-preserving u64 keys [minx, miny, maxx(inv), maxy(inv)]
 void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                  unsigned long long *bbox_keys);
+// bbox_keys holds bbox_key_words(ntiles) words; the result is in the first 4*ntiles
+// (min x, min y, max x, max y as order keys per tile).
+uint64_t bbox_key_words(uint32_t ntiles);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                    const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes);
@@ -50,7 +53,9 @@ void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_
 // ---- stages (stages.hip) ----
 struct Stage1Cfg {
     uint32_t bx, by;  // window block shape
+    uint32_t k;       // points per lane per binning round
 };
+bool stage1_shape_ok(const Stage1Cfg &c);
 // work: stage1_work_words(nblk) u32; list: leaf ids per window block, capacity
 // cap.  *total_out receives the number of list entries needed; when it
 // exceeds cap, err gets bit 16 and the call must be repeated with a larger list.

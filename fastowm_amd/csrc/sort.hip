@@ -18,8 +18,12 @@ constexpr int RB = 8;
 constexpr int RADIX = 1 << RB;
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
-constexpr int ITEMS = 16;
-constexpr int TILE = BLOCK * ITEMS;
+// Keys per thread: every tile costs one same-address atomic (the tile
+// counter, which serialises device-wide at roughly 12 ns per increment) and
+// one look-back, so large sorts use 8192-key tiles; small sorts keep 4096-key
+// tiles for more workgroups.
+constexpr uint32_t BIG_SORT = 1u << 21;
+static inline int sort_items(uint32_t n) { return n >= BIG_SORT ? 32 : 16; }
 constexpr uint32_t LB_AGG = 1u << 30;
 constexpr uint32_t LB_INC = 2u << 30;
 constexpr uint32_t LB_CNT = (1u << 30) - 1;
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(uint32_t *hist, int passes,
         counters[threadIdx.x] = 0;
 }
 
-template <bool KEYS_ONLY, bool IOTA>
+template <bool KEYS_ONLY, bool IOTA, int ITEMS_T>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__ keys_in,
                                                      const uint32_t *__restrict__ vals_in,
                                                      uint32_t *__restrict__ keys_out,
@@ -96,16 +100,17 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
                                                      uint32_t *lookback, uint32_t *tile_counter,
                                                      int write_keys)
 {
+    constexpr int TILE_T = BLOCK * ITEMS_T;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_whist[WAVES][RADIX];
     __shared__ uint32_t s_excl[RADIX];
     __shared__ uint32_t s_glob[RADIX];
     __shared__ uint32_t s_tmp[WAVES];
-    __shared__ uint32_t s_keys[TILE];
-    __shared__ uint32_t s_vals[KEYS_ONLY ? 1 : TILE];
+    __shared__ uint32_t s_keys[TILE_T];
+    __shared__ uint32_t s_vals[KEYS_ONLY ? 1 : TILE_T];
 
     const uint32_t n = d_n ? *d_n : n_host;
-    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    const uint32_t ntiles = (n + TILE_T - 1) / TILE_T;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
 
@@ -118,20 +123,20 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
         const uint32_t tile = s_tile;
         if (tile >= ntiles)
             break;
-        const uint32_t base = tile * TILE;
+        const uint32_t base = tile * TILE_T;
 
-        uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+        uint32_t key[ITEMS_T], val[ITEMS_T], rank[ITEMS_T];
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            uint32_t idx = base + w * (64 * ITEMS) + k * 64 + lane;
+        for (int k = 0; k < ITEMS_T; ++k) {
+            uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
             bool valid = idx < n;
             key[k] = valid ? keys_in[idx] : 0u;
             if (!KEYS_ONLY)
                 val[k] = valid ? (IOTA ? idx : vals_in[idx]) : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            uint32_t idx = base + w * (64 * ITEMS) + k * 64 + lane;
+        for (int k = 0; k < ITEMS_T; ++k) {
+            uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
             bool valid = idx < n;
             uint32_t d = (key[k] >> shift) & (RADIX - 1);
             unsigned long long m = __ballot(valid);
@@ -184,8 +189,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
         __syncthreads();
 
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            uint32_t idx = base + w * (64 * ITEMS) + k * 64 + lane;
+        for (int k = 0; k < ITEMS_T; ++k) {
+            uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
             if (idx < n) {
                 uint32_t dd = (key[k] >> shift) & (RADIX - 1);
                 uint32_t slot = s_excl[dd] + s_whist[w][dd] + rank[k];
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
             }
         }
         __syncthreads();
-        const uint32_t cnt = min((uint32_t)TILE, n - base);
+        const uint32_t cnt = min((uint32_t)TILE_T, n - base);
         for (uint32_t s = tid; s < cnt; s += BLOCK) {
             uint32_t k = s_keys[s];
             uint32_t dd = (k >> shift) & (RADIX - 1);
@@ -216,7 +221,11 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
-uint32_t sort_tiles(uint32_t n) { return (n + TILE - 1) / TILE; }
+uint32_t sort_tiles(uint32_t n)
+{
+    const uint32_t tile = (uint32_t)(BLOCK * sort_items(n));
+    return (n + tile - 1) / tile;
+}
 
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v)
 {
@@ -242,29 +251,44 @@ void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *count
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(BLOCK), 0, s, hist, passes, counters);
 }
 
+template <int IT>
+static void onesweep_items(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
+                           uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n,
+                           uint32_t n_max, int shift, const uint32_t *digit_off, uint32_t *lookback,
+                           uint32_t *tile_counter, bool keys_only, bool iota, int wk, uint32_t grid)
+{
+    if (keys_only)
+        hipLaunchKernelGGL((k_onesweep<true, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
+                           tile_counter, wk);
+    else if (iota)
+        hipLaunchKernelGGL((k_onesweep<false, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
+                           tile_counter, wk);
+    else
+        hipLaunchKernelGGL((k_onesweep<false, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
+                           tile_counter, wk);
+}
+
 void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
                      uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n, uint32_t n_max,
                      int shift, const uint32_t *digit_off, uint32_t *lookback,
                      uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
                      uint32_t grid_cap)
 {
+    // the tile size follows n_max; a device count d_n <= n_max uses the same
     uint32_t tiles = sort_tiles(n_max);
     uint32_t grid = tiles < 1 ? 1 : tiles;
     if (grid_cap && grid > grid_cap)
         grid = grid_cap;
     int wk = write_keys ? 1 : 0;
-    if (keys_only)
-        hipLaunchKernelGGL((k_onesweep<true, false>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
-    else if (iota)
-        hipLaunchKernelGGL((k_onesweep<false, true>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
+    if (sort_items(n_max) == 32)
+        onesweep_items<32>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
+                           lookback, tile_counter, keys_only, iota, wk, grid);
     else
-        hipLaunchKernelGGL((k_onesweep<false, false>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
+        onesweep_items<16>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
+                           lookback, tile_counter, keys_only, iota, wk, grid);
 }
 
 }  // namespace owm

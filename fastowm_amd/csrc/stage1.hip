@@ -23,7 +23,7 @@
 namespace owm {
 
 constexpr int S1_THREADS = 256;
-constexpr int S1_K = 4;
+constexpr int S1_CHUNK = S1_THREADS;  // leaves whose metadata is staged at once
 constexpr uint32_t S1_LEAFTAG = 0x80000000u;
 
 // #{k : E[k] < x} and #{k : E[k] <= x} over a strictly increasing edge list
@@ -67,10 +67,10 @@ __device__ __forceinline__ uint32_t cls(const double *L, const double *R, uint32
 
 // Windows [a, b] whose open extent (c_i - h, c_i + h) can hold a point of
 // the closed range [lo, hi]: R_i > lo and L_i < hi, with the exact edges.
-__device__ __forceinline__ bool win_span(double lo, double hi, double init, double disp, double hd,
-                                         uint32_t n, uint32_t &a, uint32_t &b)
+__device__ __forceinline__ bool win_span(double lo, double hi, double init, double disp, double inv,
+                                         double hd, uint32_t n, uint32_t &a, uint32_t &b)
 {
-    double fa = floor((lo - init - hd) / disp), fb = ceil((hi - init + hd) / disp);
+    double fa = floor((lo - init - hd) * inv), fb = ceil((hi - init + hd) * inv);
     if (!(fb >= 0.0) || !(fa <= (double)n) || !(fa <= fb + 2.0))
         return false;
     int ia = fa < 0.0 ? 0 : (int)fa, ib = fb > (double)n - 1.0 ? (int)n - 1 : (int)fb;
@@ -91,7 +91,10 @@ __device__ __forceinline__ bool win_span(double lo, double hi, double init, doub
     return true;
 }
 
-// Leaf -> window-block join.  mode 0 counts, mode 1 fills the lists.
+// Leaf -> window-block join.  mode 0 counts, mode 1 fills the lists.  A lane
+// owns a leaf; consecutive leaves are Morton neighbours and mostly share their
+// blocks, so the lanes of a wave that name the same block combine into one
+// atomic per block (match by ballot), issued by the lowest such lane.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t bx, uint32_t by,
                                                 uint32_t *__restrict__ cnt,
@@ -99,32 +102,55 @@ __global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t 
                                                 uint32_t *__restrict__ list, uint32_t cap,
                                                 uint32_t *err)
 {
-    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < B.nleaf;
-         l += gridDim.x * blockDim.x) {
-        if (T.lf_cnt[l] == 0)
-            continue;
-        const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
-        if (TD.ncols == 0)
-            continue;
-        const double4 box = T.lf_box[l];
-        const double hd = (double)(float)TD.h;
-        uint32_t a, b, c, d;
-        if (!win_span(box.x, box.z, TD.initx, TD.displace, hd, TD.ncols, a, b) ||
-            !win_span(box.y, box.w, TD.inity, TD.displace, hd, TD.nrows, c, d))
-            continue;
-        for (uint32_t j = c / by; j <= d / by; ++j)
-            for (uint32_t i = a / bx; i <= b / bx; ++i) {
-                uint32_t blk = TD.blk_off + j * TD.nbx + i;
-                if (MODE == 0) {
-                    atomicAdd(&cnt[blk], 1u);
-                } else {
-                    uint32_t k = start[blk] + atomicAdd(&cnt[blk], 1u);
-                    if (k < cap)
-                        list[k] = l;
-                    else
-                        atomicOr(err, 16u);
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); l0 < B.nleaf; l0 += stride) {
+        const uint32_t l = l0 + lane;
+        uint32_t i0 = 0, j0 = 0, wi = 0, nb = 0, boff = 0, nbx = 0;
+        if (l < B.nleaf && T.lf_cnt[l] != 0) {
+            const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
+            if (TD.ncols != 0) {
+                const double4 box = T.lf_box[l];
+                const double hd = (double)(float)TD.h;
+                uint32_t a, b, c, d;
+                if (win_span(box.x, box.z, TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols, a, b) &&
+                    win_span(box.y, box.w, TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows, c, d)) {
+                    i0 = a / bx;
+                    j0 = c / by;
+                    wi = b / bx - i0 + 1;
+                    nb = wi * (d / by - j0 + 1);
+                    boff = TD.blk_off;
+                    nbx = TD.nbx;
                 }
             }
+        }
+        for (uint32_t k = 0; __ballot(k < nb) != 0ull; ++k) {
+            const uint32_t key = k < nb ? boff + (j0 + k / wi) * nbx + (i0 + k % wi) : OWM_NONE;
+            unsigned long long pending = __ballot(key != OWM_NONE);
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const uint32_t lk = __builtin_amdgcn_readlane(key, leader);
+                const unsigned long long peers = __ballot(key == lk);
+                if (MODE == 0) {
+                    if ((int)lane == leader)
+                        atomicAdd(&cnt[lk], (uint32_t)__popcll(peers));
+                } else {
+                    uint32_t b0 = 0;
+                    if ((int)lane == leader)
+                        b0 = atomicAdd(&cnt[lk], (uint32_t)__popcll(peers));
+                    b0 = __builtin_amdgcn_readlane(b0, leader);
+                    if (key == lk) {
+                        const uint32_t e = start[lk] + b0 + (uint32_t)__popcll(peers & below);
+                        if (e < cap)
+                            list[e] = l;
+                        else
+                            atomicOr(err, 16u);
+                    }
+                }
+                pending &= ~peers;
+            }
+        }
     }
 }
 
@@ -133,9 +159,14 @@ struct S1Shape {
     static constexpr int NXC = 4 * BX + 1;
     static constexpr int NYC = 4 * BY + 1;
     static constexpr int NCELL = NXC * NYC;
+    // the window pass arrays and the leaf chunk metadata are never live together
+    static constexpr int M1 = NYC * BX;
+    static constexpr int META_BYTES = 2 * S1_CHUNK * 4;
+    static constexpr int M1_BYTES = M1 * 16;
+    static constexpr int UNION_BYTES = M1_BYTES > META_BYTES ? M1_BYTES : META_BYTES;
 };
 
-template <int BX, int BY>
+template <int BX, int BY, int S1_K>
 __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
                                                         const uint32_t *__restrict__ blk_start,
                                                         const uint32_t *__restrict__ list,
@@ -149,9 +180,15 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
     __shared__ uint32_t sCnt[S::NCELL];
     __shared__ unsigned long long sZ[S::NCELL];
     __shared__ uint32_t sId[S::NCELL];
-    __shared__ uint8_t sDirty[S::NCELL];
-    __shared__ uint32_t m1cnt[S::NYC * BX], m1id[S::NYC * BX];
-    __shared__ unsigned long long m1z[S::NYC * BX];
+    __shared__ uint16_t sTag[S::NCELL];  // round in which the cell's minimum z last dropped
+    __shared__ __align__(16) unsigned char sUnion[S::UNION_BYTES];
+    // leaf chunk: first sorted position (or leaf index) and point count (or
+    // S1_LEAFTAG | class cell when the leaf is binned whole from its aggregate)
+    uint32_t *sLfA = reinterpret_cast<uint32_t *>(sUnion);
+    uint32_t *sLfB = sLfA + S1_CHUNK;
+    unsigned long long *m1z = reinterpret_cast<unsigned long long *>(sUnion);
+    uint32_t *m1cnt = reinterpret_cast<uint32_t *>(m1z + S::M1);
+    uint32_t *m1id = m1cnt + S::M1;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -177,8 +214,9 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
             sCnt[c] = 0;
             sZ[c] = OWM_ZKEY_NONE;
             sId[c] = OWM_NONE;
-            sDirty[c] = 0;
+            sTag[c] = 0;
         }
+        uint32_t rt = 0;  // round tag
         __syncthreads();
         // window (i, j) = classes [2#{e <= L_i}, 2#{e < R_i}] (likewise y)
         if (tid < ni) {
@@ -191,9 +229,6 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
             sCyh[tid - 64] = 2 * (cnt_lt(sLy, nj, r, inv) + cnt_lt(sRy, nj, r, inv));
         }
         const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
-        // the class ranges are read by other threads after the rounds, and a
-        // block without leaves runs no round: publish them here
-        __syncthreads();
 
         const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
         const uint32_t nl = s1 - s0;
@@ -202,84 +237,117 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
         const uint32_t ls = L <= 64 ? L : 64;
         const uint32_t nseg = L <= 64 ? 1 : (L + 63) / 64;
         const uint32_t sub = lane / ls, off = lane - sub * ls;
-        const uint32_t nunits = (nl + nl_grp - 1) / nl_grp * nseg;
-        const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
 
-        for (uint32_t r = 0; r < rounds; ++r) {
-            uint32_t cell[S1_K], pid[S1_K];
-            unsigned long long zk[S1_K];
-#pragma unroll
-            for (int k = 0; k < S1_K; ++k) {
-                pid[k] = OWM_NONE;
-                const uint32_t u = (r * S1_K + k) * 4 + wave;
-                if (u >= nunits)
-                    continue;
-                const uint32_t g = u / nseg, seg = u - g * nseg;
-                const uint32_t li = g * nl_grp + sub;
-                if (sub >= nl_grp || li >= nl)
-                    continue;
-                const uint32_t leaf = list[s0 + li];
-                const uint32_t o = seg * 64 + off;
-                if (nseg == 1) {
-                    // whole leaf inside one class cell: bin its aggregate (a class
-                    // cell is never wider than the window step)
-                    const double4 bx = T.lf_box[leaf];
-                    if (bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace && bx.x > BL &&
-                        bx.z < BR && bx.y > BB && bx.w < BT) {
-                        uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
-                        if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
-                            if (o == 0) {
-                                uint32_t c = cy * NXC + cx;
-                                atomicAdd(&sCnt[c], T.lf_cnt[leaf]);
-                                unsigned long long key = T.lf_zkey[leaf];
-                                if (key != OWM_ZKEY_NONE) {
-                                    unsigned long long old = atomicMin(&sZ[c], key);
-                                    if (key < old)
-                                        sDirty[c] = 1;
-                                    cell[k] = c;
-                                    zk[k] = key;
-                                    pid[k] = S1_LEAFTAG | leaf;
-                                }
-                            }
-                            continue;
-                        }
+        for (uint32_t ch = 0; ch < nl; ch += S1_CHUNK) {
+            const uint32_t m = min((uint32_t)S1_CHUNK, nl - ch);
+            // the class ranges (first chunk) and the previous chunk's rounds
+            // are complete before the metadata is overwritten
+            __syncthreads();
+            if (tid < m) {
+                const uint32_t leaf = list[s0 + ch + tid];
+                const uint32_t np = T.lf_npts[leaf], st = T.lf_start[leaf];
+                const double4 bx = T.lf_box[leaf];
+                uint32_t a = st, n = np;
+                // whole leaf inside one class cell: bin its aggregate (a class
+                // cell is never wider than the window step)
+                if (nseg == 1 && np > 0 && bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace &&
+                    bx.x > BL && bx.z < BR && bx.y > BB && bx.w < BT) {
+                    uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
+                    if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
+                        a = leaf;
+                        n = S1_LEAFTAG | (cy * NXC + cx);
                     }
                 }
-                if (o >= T.lf_npts[leaf])
-                    continue;
-                const uint32_t p = T.lf_start[leaf] + o;
-                const double x = T.xs[p], y = T.ys[p];
-                if (!(x > BL && x < BR && y > BB && y < BT))
-                    continue;
-                const uint32_t c = cls(sLy, sRy, nj, y, inv) * NXC + cls(sLx, sRx, ni, x, inv);
-                atomicAdd(&sCnt[c], 1u);
-                const double z = T.zs[p];
-                if (owm_eligible(z)) {
-                    unsigned long long key = owm_zkey(z);
-                    unsigned long long old = atomicMin(&sZ[c], key);
-                    if (key < old)
-                        sDirty[c] = 1;
-                    cell[k] = c;
-                    zk[k] = key;
-                    pid[k] = p;
-                }
+                sLfA[tid] = a;
+                sLfB[tid] = n;
             }
             __syncthreads();
-            // a cell whose minimum z dropped in this round drops its old index
-#pragma unroll
-            for (int k = 0; k < S1_K; ++k)
-                if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k] && sDirty[cell[k]])
-                    sId[cell[k]] = OWM_NONE;
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < S1_K; ++k)
-                if (pid[k] != OWM_NONE && sZ[cell[k]] == zk[k]) {
-                    uint32_t id = (pid[k] & S1_LEAFTAG) ? T.lf_zid[pid[k] & ~S1_LEAFTAG] : T.ids[pid[k]];
-                    atomicMin(&sId[cell[k]], id);
-                    sDirty[cell[k]] = 0;
+            const uint32_t nunits = (m + nl_grp - 1) / nl_grp * nseg;
+            const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
+
+            for (uint32_t r = 0; r < rounds; ++r) {
+                if (++rt == 0x10000u) {
+                    __syncthreads();
+                    for (int c = tid; c < S::NCELL; c += S1_THREADS)
+                        sTag[c] = 0;
+                    __syncthreads();
+                    rt = 1;
                 }
-            __syncthreads();
+                uint32_t cell[S1_K], idv[S1_K];
+                unsigned long long zk[S1_K];
+                double xv[S1_K], yv[S1_K], zv[S1_K];
+                uint32_t cn[S1_K];
+                // issue every load of the round before using any of them
+#pragma unroll
+                for (int k = 0; k < S1_K; ++k) {
+                    cell[k] = OWM_NONE;
+                    const uint32_t u = (r * S1_K + k) * 4 + wave;
+                    const uint32_t g = u / nseg, seg = u - g * nseg;
+                    const uint32_t li = g * nl_grp + sub;
+                    if (u >= nunits || sub >= nl_grp || li >= m)
+                        continue;
+                    const uint32_t a = sLfA[li], n = sLfB[li];
+                    const uint32_t o = seg * 64 + off;
+                    if (n & S1_LEAFTAG) {
+                        if (o == 0) {
+                            cell[k] = n & ~S1_LEAFTAG;
+                            cn[k] = T.lf_cnt[a];
+                            zk[k] = T.lf_zkey[a];
+                            idv[k] = T.lf_zid[a];
+                        }
+                    } else if (o < n) {
+                        const uint32_t p = a + o;
+                        xv[k] = T.xs[p];
+                        yv[k] = T.ys[p];
+                        zv[k] = T.zs[p];
+                        idv[k] = T.ids[p];
+                        cell[k] = OWM_LEAF_BIT;  // a point, classified below
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < S1_K; ++k) {
+                    if (cell[k] == OWM_NONE)
+                        continue;
+                    if (cell[k] == OWM_LEAF_BIT) {
+                        const double x = xv[k], y = yv[k];
+                        if (!(x > BL && x < BR && y > BB && y < BT)) {
+                            cell[k] = OWM_NONE;
+                            continue;
+                        }
+                        cell[k] = cls(sLy, sRy, nj, y, inv) * NXC + cls(sLx, sRx, ni, x, inv);
+                        cn[k] = 1;
+                        zk[k] = owm_eligible(zv[k]) ? owm_zkey(zv[k]) : OWM_ZKEY_NONE;
+                    }
+                    const uint32_t c = cell[k];
+                    atomicAdd(&sCnt[c], cn[k]);
+                    if (zk[k] != OWM_ZKEY_NONE) {
+                        unsigned long long old = atomicMin(&sZ[c], zk[k]);
+                        if (zk[k] < old)
+                            sTag[c] = (uint16_t)rt;
+                    } else {
+                        cell[k] = OWM_NONE;  // counted, never a minimum
+                    }
+                }
+                __syncthreads();
+                // a cell whose minimum z dropped in this round drops its old index
+#pragma unroll
+                for (int k = 0; k < S1_K; ++k)
+                    if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k] && sTag[cell[k]] == (uint16_t)rt)
+                        sId[cell[k]] = OWM_NONE;
+                __syncthreads();
+                // No barrier after this phase: a lane that reads a minimum
+                // already lowered by the next round skips an index that the
+                // next round's reset would discard anyway, and tags are never
+                // cleared, so they cannot race with the next round's marks.
+#pragma unroll
+                for (int k = 0; k < S1_K; ++k)
+                    if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k])
+                        atomicMin(&sId[cell[k]], idv[k]);
+            }
         }
+        // the class ranges of a block without leaves, and the last round's
+        // index updates, are complete before the window passes
+        __syncthreads();
 
         // separable reduction: x-pass per (class row, window column) ...
         const uint32_t nyc = 4 * nj + 1;
@@ -415,13 +483,21 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t *__restri
 
 uint64_t stage1_work_words(uint64_t nblk) { return 2 * (nblk + 1) + (nblk / SC_TILE + 2) + 8; }
 
-template <int BX, int BY>
+template <int BX, int BY, int K>
 static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, const uint32_t *start,
                          const uint32_t *list, uint32_t cap, uint32_t *min_ids)
 {
     uint32_t grid = B.nblk < 1 ? 1 : (B.nblk > 65536 ? 65536 : B.nblk);
-    hipLaunchKernelGGL((k_stage1<BX, BY>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, start, list,
+    hipLaunchKernelGGL((k_stage1<BX, BY, K>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, start, list,
                        cap, min_ids);
+}
+
+bool stage1_shape_ok(const Stage1Cfg &c)
+{
+    if (c.bx == 8 && c.by == 8)
+        return c.k == 2 || c.k == 4 || c.k == 8;
+    return c.k == 4 && ((c.bx == 16 && c.by == 8) || (c.bx == 16 && c.by == 16) ||
+                        (c.bx == 32 && c.by == 8));
 }
 
 void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
@@ -449,13 +525,17 @@ void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Sta
                        (const uint32_t *)start, list, cap, err);
     launch_add_u32(s, start + B.nblk, counter + 1, total_out);
     if (cfg.bx == 16 && cfg.by == 8)
-        stage1_shape<16, 8>(s, B, T, start, list, cap, min_ids);
+        stage1_shape<16, 8, 4>(s, B, T, start, list, cap, min_ids);
     else if (cfg.bx == 16 && cfg.by == 16)
-        stage1_shape<16, 16>(s, B, T, start, list, cap, min_ids);
+        stage1_shape<16, 16, 4>(s, B, T, start, list, cap, min_ids);
     else if (cfg.bx == 32 && cfg.by == 8)
-        stage1_shape<32, 8>(s, B, T, start, list, cap, min_ids);
+        stage1_shape<32, 8, 4>(s, B, T, start, list, cap, min_ids);
+    else if (cfg.k == 2)
+        stage1_shape<8, 8, 2>(s, B, T, start, list, cap, min_ids);
+    else if (cfg.k == 8)
+        stage1_shape<8, 8, 8>(s, B, T, start, list, cap, min_ids);
     else
-        stage1_shape<8, 8>(s, B, T, start, list, cap, min_ids);
+        stage1_shape<8, 8, 4>(s, B, T, start, list, cap, min_ids);
 }
 
 }  // namespace owm

@@ -63,14 +63,15 @@ __device__ __forceinline__ void wave_min_kid(unsigned long long &k, uint32_t &id
     }
 }
 
-__global__ void k_bbox_init(unsigned long long *bk, uint32_t ntiles)
+// The bbox reduction spreads its atomics over BBOX_REPL copies of every
+// tile's four words (copy = block index mod BBOX_REPL) so that thousands of
+// blocks do not serialise on four addresses; k_bbox_fold folds the copies.
+constexpr uint32_t BBOX_REPL = 16;
+
+__global__ void k_bbox_init(unsigned long long *bk, uint32_t nwords)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntiles; i += gridDim.x * blockDim.x) {
-        bk[4 * i + 0] = ~0ull;  // min x
-        bk[4 * i + 1] = ~0ull;  // min y
-        bk[4 * i + 2] = 0ull;   // max x
-        bk[4 * i + 3] = 0ull;   // max y
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x)
+        bk[i] = (i & 2) ? 0ull : ~0ull;  // words: min x, min y, max x, max y
 }
 
 __device__ __forceinline__ void bbox_flush(unsigned long long *bk, uint32_t t, double mnx, double mny,
@@ -86,58 +87,111 @@ __device__ __forceinline__ void bbox_flush(unsigned long long *bk, uint32_t t, d
     }
 }
 
-// Exact fp64 min/max of x and y per tile (NaNs ignored).
+// Block-wide reduction of the four accumulators, flushed by thread 0.
+__device__ __forceinline__ void bbox_block_flush(unsigned long long *bk, uint32_t t, double mnx,
+                                                 double mny, double mxx, double mxy,
+                                                 double (*red)[TB / 64])
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    mnx = wave_min_d(mnx);
+    mny = wave_min_d(mny);
+    mxx = wave_max_d(mxx);
+    mxy = wave_max_d(mxy);
+    if (lane == 0) {
+        red[0][w] = mnx;
+        red[1][w] = mny;
+        red[2][w] = mxx;
+        red[3][w] = mxy;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < TB / 64; ++k) {
+            mnx = fmin(mnx, red[0][k]);
+            mny = fmin(mny, red[1][k]);
+            mxx = fmax(mxx, red[2][k]);
+            mxy = fmax(mxy, red[3][k]);
+        }
+        bbox_flush(bk, t, mnx, mny, mxx, mxy);
+    }
+    __syncthreads();
+}
+
+// Exact fp64 min/max of x and y per tile (NaNs ignored).  A block keeps its
+// accumulators across consecutive chunks of one tile and flushes once per tile.
 __global__ __launch_bounds__(TB) void k_bbox(BatchDev B, const double *__restrict__ X,
-                                              const double *__restrict__ Y, unsigned long long *bk)
+                                              const double *__restrict__ Y, unsigned long long *bk_all)
 {
     __shared__ double red[4][TB / 64];
+    unsigned long long *bk = bk_all + (size_t)(blockIdx.x % BBOX_REPL) * 4 * B.ntiles;
     const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t tid = threadIdx.x;
+    uint32_t cur = OWM_NONE;  // tile of the accumulators (uniform over the block)
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
         uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
         uint32_t t0 = owm_find_tile(B.pt_off, B.ntiles, c0);
         uint32_t t1 = owm_find_tile(B.pt_off, B.ntiles, c1 - 1);
-        double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-        uint32_t t = t0, next = B.pt_off[t0 + 1];
-        for (uint32_t i = c0 + tid; i < c1; i += TB) {
-            while (i >= next) {
-                bbox_flush(bk, t, mnx, mny, mxx, mxy);
-                mnx = mny = INFINITY;
-                mxx = mxy = -INFINITY;
-                ++t;
-                next = B.pt_off[t + 1];
-            }
-            double x = X[i], y = Y[i];
-            mnx = fmin(mnx, x);
-            mxx = fmax(mxx, x);
-            mny = fmin(mny, y);
-            mxy = fmax(mxy, y);
+        if (t0 != cur && cur != OWM_NONE) {
+            bbox_block_flush(bk, cur, mnx, mny, mxx, mxy, red);
+            mnx = mny = INFINITY;
+            mxx = mxy = -INFINITY;
+            cur = OWM_NONE;
         }
         if (t0 != t1) {
-            bbox_flush(bk, t, mnx, mny, mxx, mxy);
+            // a chunk across tile boundaries: per-thread flushes (rare)
+            double ax = INFINITY, ay = INFINITY, bx = -INFINITY, by = -INFINITY;
+            uint32_t t = t0, next = B.pt_off[t0 + 1];
+            for (uint32_t i = c0 + tid; i < c1; i += TB) {
+                while (i >= next) {
+                    bbox_flush(bk, t, ax, ay, bx, by);
+                    ax = ay = INFINITY;
+                    bx = by = -INFINITY;
+                    ++t;
+                    next = B.pt_off[t + 1];
+                }
+                double x = X[i], y = Y[i];
+                ax = fmin(ax, x);
+                bx = fmax(bx, x);
+                ay = fmin(ay, y);
+                by = fmax(by, y);
+            }
+            bbox_flush(bk, t, ax, ay, bx, by);
             continue;
         }
-        mnx = wave_min_d(mnx);
-        mny = wave_min_d(mny);
-        mxx = wave_max_d(mxx);
-        mxy = wave_max_d(mxy);
-        if (lane == 0) {
-            red[0][w] = mnx;
-            red[1][w] = mny;
-            red[2][w] = mxx;
-            red[3][w] = mxy;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            for (int k = 1; k < TB / 64; ++k) {
-                mnx = fmin(mnx, red[0][k]);
-                mny = fmin(mny, red[1][k]);
-                mxx = fmax(mxx, red[2][k]);
-                mxy = fmax(mxy, red[3][k]);
+        cur = t0;
+        // eight independent loads per array in flight per thread
+        for (uint32_t base = c0 + tid; base < c1; base += 8 * TB) {
+            double xv[8], yv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                uint32_t i = base + k * TB;
+                xv[k] = i < c1 ? X[i] : NAN;
+                yv[k] = i < c1 ? Y[i] : NAN;
             }
-            bbox_flush(bk, t0, mnx, mny, mxx, mxy);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                mnx = fmin(mnx, xv[k]);
+                mxx = fmax(mxx, xv[k]);
+                mny = fmin(mny, yv[k]);
+                mxy = fmax(mxy, yv[k]);
+            }
         }
-        __syncthreads();
+    }
+    if (cur != OWM_NONE)
+        bbox_block_flush(bk, cur, mnx, mny, mxx, mxy, red);
+}
+
+// Fold the copies of every tile's words into copy 0.
+__global__ void k_bbox_fold(unsigned long long *bk, uint32_t ntiles)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 4 * ntiles; i += gridDim.x * blockDim.x) {
+        unsigned long long v = bk[i];
+        const bool is_max = (i & 2) != 0;
+        for (uint32_t r = 1; r < BBOX_REPL; ++r) {
+            unsigned long long u = bk[(size_t)r * 4 * ntiles + i];
+            v = is_max ? (u > v ? u : v) : (u < v ? u : v);
+        }
+        bk[i] = v;
     }
 }
 
@@ -223,84 +277,219 @@ __device__ __forceinline__ void seg_reduce(uint32_t off, uint32_t ls, F &&step)
         step(o, off + o < ls);
 }
 
-// Gather into Morton order and per-leaf aggregates.  A wave owns a group of
+// Exchange with the lane whose index differs in bit log2(STEP), or (for
+// STEP 4 and 8) with the mirrored lane of the 8- or 16-lane row: after the
+// steps 1..STEP every lane of an aligned segment of 2*STEP lanes has combined
+// all of them, which is all a commutative reduction needs.  Steps up to 8
+// are DPP moves on the VALU, 16 a swizzle, 32 a permute.
+template <int STEP>
+__device__ __forceinline__ uint32_t xchg_u(uint32_t v)
+{
+    if constexpr (STEP == 1)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad [1,0,3,2]
+    else if constexpr (STEP == 2)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad [2,3,0,1]
+    else if constexpr (STEP == 4)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row half mirror
+    else if constexpr (STEP == 8)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);  // row mirror
+    else if constexpr (STEP == 16)
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);  // xor 16 within 32
+    else
+        return (uint32_t)__shfl_xor((int)v, 32, 64);
+}
+
+template <int STEP>
+__device__ __forceinline__ unsigned long long xchg_u64(unsigned long long v)
+{
+    uint32_t lo = xchg_u<STEP>((uint32_t)v), hi = xchg_u<STEP>((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+template <int STEP>
+__device__ __forceinline__ double xchg_d(double v)
+{
+    return __longlong_as_double((long long)xchg_u64<STEP>((unsigned long long)__double_as_longlong(v)));
+}
+
+struct LeafAgg {
+    double mnx, mny, mxx, mxy;
+    uint32_t cnt, bid;
+    unsigned long long bk;
+};
+
+template <int STEP>
+__device__ __forceinline__ void leaf_agg_step(LeafAgg &a)
+{
+    const double b0 = xchg_d<STEP>(a.mnx), b1 = xchg_d<STEP>(a.mny);
+    const double b2 = xchg_d<STEP>(a.mxx), b3 = xchg_d<STEP>(a.mxy);
+    const uint32_t c = xchg_u<STEP>(a.cnt), i2 = xchg_u<STEP>(a.bid);
+    const unsigned long long k2 = xchg_u64<STEP>(a.bk);
+    a.mnx = fmin(a.mnx, b0);
+    a.mny = fmin(a.mny, b1);
+    a.mxx = fmax(a.mxx, b2);
+    a.mxy = fmax(a.mxy, b3);
+    a.cnt += c;
+    if (owm_less(k2, i2, a.bk, a.bid)) {
+        a.bk = k2;
+        a.bid = i2;
+    }
+}
+
+// Reduce over aligned segments of ls lanes (ls a power of two <= 64); every
+// lane of a segment ends with its result.  Must run with the whole wave active.
+__device__ __forceinline__ void leaf_agg_reduce(LeafAgg &a, uint32_t ls)
+{
+    if (ls > 1)
+        leaf_agg_step<1>(a);
+    if (ls > 2)
+        leaf_agg_step<2>(a);
+    if (ls > 4)
+        leaf_agg_step<4>(a);
+    if (ls > 8)
+        leaf_agg_step<8>(a);
+    if (ls > 16)
+        leaf_agg_step<16>(a);
+    if (ls > 32)
+        leaf_agg_step<32>(a);
+}
+
+__device__ __forceinline__ void leaf_agg_add(LeafAgg &a, double x, double y, double z, uint32_t v)
+{
+    if (!isnan(x) && !isnan(y)) {
+        ++a.cnt;
+        a.mnx = fmin(a.mnx, x);
+        a.mxx = fmax(a.mxx, x);
+        a.mny = fmin(a.mny, y);
+        a.mxy = fmax(a.mxy, y);
+        if (owm_eligible(z)) {
+            unsigned long long zk = owm_zkey(z);
+            if (owm_less(zk, v, a.bk, a.bid)) {
+                a.bk = zk;
+                a.bid = v;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ LeafAgg leaf_agg_empty()
+{
+    return LeafAgg{INFINITY, INFINITY, -INFINITY, -INFINITY, 0u, OWM_NONE, OWM_ZKEY_NONE};
+}
+
+__device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, const uint32_t *skeys,
+                                           uint32_t jl, uint32_t start, uint32_t npts, const LeafAgg &a)
+{
+    const uint32_t leaf = TD.leaf_off + jl;
+    T.lf_box[leaf] = make_double4(a.mnx, a.mny, a.mxx, a.mxy);
+    T.lf_cnt[leaf] = a.cnt;
+    T.lf_start[leaf] = start;
+    T.lf_npts[leaf] = npts;
+    T.lf_zkey[leaf] = a.bk;
+    T.lf_zid[leaf] = a.bid;
+    T.lf_key0[leaf] = skeys[start];
+    T.lf_parent[leaf] = OWM_NONE;
+}
+
+__device__ __forceinline__ void gather_put(TreeOut &T, uint32_t p, uint32_t v, const double4 &r)
+{
+    T.xs[p] = r.x;
+    T.ys[p] = r.y;
+    T.zs[p] = r.z;
+    T.ids[p] = v;
+    if (T.pos)
+        T.pos[v] = p;
+}
+
+constexpr int GATHER_G = 4;  // leaf groups a wave gathers at once
+
+// Gather into Morton order and per-leaf aggregates.  A wave owns groups of
 // 64/leaf_size consecutive leaves of one tile (one leaf when leaf_size > 64,
-// its points strided over the lanes): every lane gathers one point per step
-// and segmented shuffles reduce each leaf's tight bbox, count (non-NaN x and
-// y) and lowest (z, index).
+// its points strided over the lanes): every lane gathers one point per leaf
+// group and segmented butterflies reduce each leaf's tight bbox, count
+// (non-NaN x and y) and lowest (z, index).  With a power-of-two leaf size of at
+// most 64 a wave takes GATHER_G groups at once, so that all their index and
+// record loads are in flight together.
 __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t *__restrict__ vals,
                                                        const uint32_t *__restrict__ skeys,
                                                        const double4 *__restrict__ aos, TreeOut T)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = gridDim.x * (TB / 64);
-    for (uint32_t g = blockIdx.x * (TB / 64) + (threadIdx.x >> 6); g < B.ngrp; g += nw) {
-        const TileDev &TD = B.tiles[owm_find_tile(B.grp_off, B.ntiles, g)];
+    for (uint32_t g0 = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * GATHER_G; g0 < B.ngrp;
+         g0 += nw * GATHER_G) {
+        const uint32_t t = owm_find_tile(B.grp_off, B.ntiles, g0);
+        const TileDev &TD = B.tiles[t];
         const uint32_t L = TD.leaf_size;
-        const uint32_t nl_grp = L <= 64 ? 64 / L : 1;
-        const uint32_t ls = L <= 64 ? L : 64;
-        const uint32_t sub = lane / ls, off = lane - sub * ls;
-        const uint32_t jl = (g - TD.grp_off) * nl_grp + sub;
-        const bool active = sub < nl_grp && jl < TD.nleaf;
-        const uint32_t start = TD.pt_off + jl * L;
-        const uint32_t npts = active ? min(L, TD.n - jl * L) : 0u;
-        double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-        uint32_t cnt = 0, bid = OWM_NONE;
-        unsigned long long bk = OWM_ZKEY_NONE;
-        for (uint32_t k = off; k < npts; k += ls) {
-            uint32_t p = start + k;
-            uint32_t v = vals[p];
-            const double4 r = aos[v];
-            const double x = r.x, y = r.y, z = r.z;
-            T.xs[p] = x;
-            T.ys[p] = y;
-            T.zs[p] = z;
-            T.ids[p] = v;
-            if (T.pos)
-                T.pos[v] = p;
-            if (!isnan(x) && !isnan(y)) {
-                ++cnt;
-                mnx = fmin(mnx, x);
-                mxx = fmax(mxx, x);
-                mny = fmin(mny, y);
-                mxy = fmax(mxy, y);
-                if (owm_eligible(z)) {
-                    unsigned long long zk = owm_zkey(z);
-                    if (owm_less(zk, v, bk, bid)) {
-                        bk = zk;
-                        bid = v;
+        if (L <= 64 && (L & (L - 1)) == 0 && g0 + GATHER_G <= B.grp_off[t + 1]) {
+            const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane & (L - 1);
+            uint32_t v[GATHER_G], p[GATHER_G], jl[GATHER_G];
+            bool have[GATHER_G];
+#pragma unroll
+            for (int q = 0; q < GATHER_G; ++q) {
+                jl[q] = (g0 + q - TD.grp_off) * nl_grp + sub;
+                have[q] = jl[q] < TD.nleaf && jl[q] * L + off < TD.n;
+                p[q] = TD.pt_off + jl[q] * L + off;
+                v[q] = have[q] ? vals[p[q]] : 0u;
+            }
+            double4 r[GATHER_G];
+#pragma unroll
+            for (int q = 0; q < GATHER_G; ++q)
+                r[q] = have[q] ? aos[v[q]] : make_double4(NAN, NAN, NAN, 0.0);
+#pragma unroll
+            for (int q = 0; q < GATHER_G; ++q) {
+                LeafAgg a = leaf_agg_empty();
+                if (have[q]) {
+                    gather_put(T, p[q], v[q], r[q]);
+                    leaf_agg_add(a, r[q].x, r[q].y, r[q].z, v[q]);
+                }
+                leaf_agg_reduce(a, L);
+                if (off == 0 && jl[q] < TD.nleaf) {
+                    const uint32_t start = TD.pt_off + jl[q] * L;
+                    leaf_store(TD, T, skeys, jl[q], start, min(L, TD.n - jl[q] * L), a);
+                }
+            }
+            continue;
+        }
+        // any leaf size, one group at a time
+        for (uint32_t g = g0; g < min(g0 + GATHER_G, B.ngrp); ++g) {
+            const TileDev &TG = B.tiles[owm_find_tile(B.grp_off, B.ntiles, g)];
+            const uint32_t LG = TG.leaf_size;
+            const uint32_t nl_grp = LG <= 64 ? 64 / LG : 1;
+            const uint32_t ls = LG <= 64 ? LG : 64;
+            const uint32_t sub = lane / ls, off = lane - sub * ls;
+            const uint32_t jl = (g - TG.grp_off) * nl_grp + sub;
+            const bool active = sub < nl_grp && jl < TG.nleaf;
+            const uint32_t start = TG.pt_off + jl * LG;
+            const uint32_t npts = active ? min(LG, TG.n - jl * LG) : 0u;
+            LeafAgg a = leaf_agg_empty();
+            for (uint32_t k = off; k < npts; k += ls) {
+                const uint32_t pp = start + k;
+                const uint32_t vv = vals[pp];
+                const double4 rr = aos[vv];
+                gather_put(T, pp, vv, rr);
+                leaf_agg_add(a, rr.x, rr.y, rr.z, vv);
+            }
+            seg_reduce(off, ls, [&](uint32_t o, bool take) {
+                uint32_t c2 = __shfl_down(a.cnt, o, 64);
+                double w0 = __shfl_down(a.mnx, o, 64), w1 = __shfl_down(a.mny, o, 64);
+                double w2 = __shfl_down(a.mxx, o, 64), w3 = __shfl_down(a.mxy, o, 64);
+                unsigned long long k2 = __shfl_down(a.bk, o, 64);
+                uint32_t i2 = __shfl_down(a.bid, o, 64);
+                if (take) {
+                    a.cnt += c2;
+                    a.mnx = fmin(a.mnx, w0);
+                    a.mny = fmin(a.mny, w1);
+                    a.mxx = fmax(a.mxx, w2);
+                    a.mxy = fmax(a.mxy, w3);
+                    if (owm_less(k2, i2, a.bk, a.bid)) {
+                        a.bk = k2;
+                        a.bid = i2;
                     }
                 }
-            }
-        }
-        seg_reduce(off, ls, [&](uint32_t o, bool take) {
-            uint32_t c2 = __shfl_down(cnt, o, 64);
-            double a = __shfl_down(mnx, o, 64), b = __shfl_down(mny, o, 64);
-            double c = __shfl_down(mxx, o, 64), d = __shfl_down(mxy, o, 64);
-            unsigned long long k2 = __shfl_down(bk, o, 64);
-            uint32_t i2 = __shfl_down(bid, o, 64);
-            if (take) {
-                cnt += c2;
-                mnx = fmin(mnx, a);
-                mny = fmin(mny, b);
-                mxx = fmax(mxx, c);
-                mxy = fmax(mxy, d);
-                if (owm_less(k2, i2, bk, bid)) {
-                    bk = k2;
-                    bid = i2;
-                }
-            }
-        });
-        if (active && off == 0) {
-            const uint32_t leaf = TD.leaf_off + jl;
-            T.lf_box[leaf] = make_double4(mnx, mny, mxx, mxy);
-            T.lf_cnt[leaf] = cnt;
-            T.lf_start[leaf] = start;
-            T.lf_npts[leaf] = npts;
-            T.lf_zkey[leaf] = bk;
-            T.lf_zid[leaf] = bid;
-            T.lf_key0[leaf] = skeys[start];
-            T.lf_parent[leaf] = OWM_NONE;
+            });
+            if (active && off == 0)
+                leaf_store(TG, T, skeys, jl, start, npts, a);
         }
     }
 }
@@ -374,32 +563,35 @@ struct Agg {
     unsigned long long zk;
 };
 
-// Aggregates are read with device-coherent loads: the writer may be another
-// workgroup (global climb) or another wave of this workgroup (chunk climb).
+// Leaf aggregates come from an earlier kernel and are read with plain loads.
+// Node aggregates are written during the climb by another wave of the same
+// workgroup (chunk climb, SCOPE = workgroup) or by another workgroup (global
+// climb, SCOPE = agent), so they are read with loads coherent at that scope.
+template <int SCOPE>
 __device__ __forceinline__ double ld_d(const double *p)
 {
-    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, SCOPE);
     return __longlong_as_double((long long)v);
 }
 
+template <int SCOPE>
 __device__ __forceinline__ Agg load_agg(const TreeOut &T, uint32_t ref)
 {
     Agg a;
-    const double *bp;
     if (ref & OWM_LEAF_BIT) {
-        uint32_t l = ref & ~OWM_LEAF_BIT;
-        bp = (const double *)(T.lf_box + l);
-        a.cnt = __hip_atomic_load(T.lf_cnt + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a.zk = __hip_atomic_load(T.lf_zkey + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a.zid = __hip_atomic_load(T.lf_zid + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t l = ref & ~OWM_LEAF_BIT;
+        a.box = T.lf_box[l];
+        a.cnt = T.lf_cnt[l];
+        a.zk = T.lf_zkey[l];
+        a.zid = T.lf_zid[l];
     } else {
-        bp = (const double *)(T.nd_box + ref);
-        a.cnt = __hip_atomic_load(T.nd_cnt + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a.zk = __hip_atomic_load(T.nd_zkey + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a.zid = __hip_atomic_load(T.nd_zid + ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double *bp = (const double *)(T.nd_box + ref);
+        a.cnt = __hip_atomic_load(T.nd_cnt + ref, __ATOMIC_RELAXED, SCOPE);
+        a.zk = __hip_atomic_load(T.nd_zkey + ref, __ATOMIC_RELAXED, SCOPE);
+        a.zid = __hip_atomic_load(T.nd_zid + ref, __ATOMIC_RELAXED, SCOPE);
+        a.box = make_double4(ld_d<SCOPE>(bp), ld_d<SCOPE>(bp + 1), ld_d<SCOPE>(bp + 2),
+                             ld_d<SCOPE>(bp + 3));
     }
-    a.box = make_double4(ld_d(bp), ld_d(bp + 1), ld_d(bp + 2), ld_d(bp + 3));
     return a;
 }
 
@@ -428,86 +620,119 @@ __device__ __forceinline__ void store_agg(const TreeOut &T, uint32_t p, const Ag
 }
 
 constexpr uint32_t BU_CHUNK = 2048;  // leaves per workgroup in the chunk climb
+constexpr int BU_TB = 1024;           // threads per chunk-climb workgroup
+constexpr uint32_t BU_EXP_CAP = 1024;  // exports staged in LDS per chunk
 
 // Chunk climb: a workgroup owns leaves [A, E] of the batch; a node whose
 // leaf range lies inside [A, E] has both children completed by this
 // workgroup, so its arrival counter lives in LDS and needs no device-wide
 // fence.  A completed node whose parent's range leaves the chunk is exported
 // for the global climb.
-__global__ __launch_bounds__(TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exported,
+__global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exported,
                                                         uint32_t *n_exported, uint32_t cap,
                                                         uint32_t *err)
 {
     __shared__ uint32_t sFlag[BU_CHUNK];
+    // exports are gathered per chunk and reserved with one device atomic
+    __shared__ uint32_t sExp[BU_EXP_CAP];
+    __shared__ uint32_t sNexp, sBase;
     const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
     for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
         const uint32_t A = ch * BU_CHUNK, E = min(A + BU_CHUNK, B.nleaf) - 1;
-        for (uint32_t k = threadIdx.x; k < BU_CHUNK; k += TB)
+        for (uint32_t k = threadIdx.x; k < BU_CHUNK; k += BU_TB)
             sFlag[k] = 0;
+        if (threadIdx.x == 0)
+            sNexp = 0;
         __syncthreads();
-        for (uint32_t l = A + threadIdx.x; l <= E; l += TB) {
+        for (uint32_t l = A + threadIdx.x; l <= E; l += BU_TB) {
             const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
             uint32_t c = OWM_LEAF_BIT | l;
-            Agg mine = load_agg(T, c);
+            Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
             uint32_t p = T.lf_parent[l];
             for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
                 if (depth > 128) {
                     atomicOr(err, 4u);
                     break;
                 }
-                uint2 rg = T.nd_range[p];
-                uint32_t glo = TD.leaf_off + rg.x, ghi = TD.leaf_off + rg.y;
+                // everything about p in one round trip
+                const uint2 rg = T.nd_range[p];
+                const uint2 chd = T.nd_child[p];
+                const uint32_t pp = T.nd_parent[p];
+                const uint32_t glo = TD.leaf_off + rg.x, ghi = TD.leaf_off + rg.y;
                 if (glo < A || ghi > E) {
-                    uint32_t k = atomicAdd(n_exported, 1u);
-                    if (k < cap)
-                        exported[k] = c;
-                    else
-                        atomicOr(err, 8u);
+                    const uint32_t k = atomicAdd(&sNexp, 1u);
+                    if (k < BU_EXP_CAP) {
+                        sExp[k] = c;
+                    } else {
+                        const uint32_t g = atomicAdd(n_exported, 1u);
+                        if (g < cap)
+                            exported[g] = c;
+                        else
+                            atomicOr(err, 8u);
+                    }
                     break;
                 }
-                uint32_t slot = TD.leaf_off + (p - TD.node_off) - A;
+                const uint32_t slot = TD.leaf_off + (p - TD.node_off) - A;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                uint32_t old = atomicAdd(&sFlag[slot], 1u);
+                const uint32_t old = atomicAdd(&sFlag[slot], 1u);
                 if (old == 0)
                     break;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                uint2 chd = T.nd_child[p];
-                Agg other = load_agg(T, chd.x == c ? chd.y : chd.x);
+                const Agg other = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
                 mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
                 store_agg(T, p, mine);
                 c = p;
-                p = T.nd_parent[p];
+                p = pp;
             }
+        }
+        __syncthreads();
+        const uint32_t ne = min(sNexp, BU_EXP_CAP);
+        if (threadIdx.x == 0)
+            sBase = ne ? atomicAdd(n_exported, ne) : 0u;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < ne; k += BU_TB) {
+            const uint32_t g = sBase + k;
+            if (g < cap)
+                exported[g] = sExp[k];
+            else
+                atomicOr(err, 8u);
         }
         __syncthreads();
     }
 }
 
-// Global climb from the exported nodes: arrival counters in global memory
-// with device-scope release/acquire fences (only chunk-crossing nodes).
-__global__ __launch_bounds__(TB) void k_bottomup_global(TreeOut T, const uint32_t *exported,
-                                                         const uint32_t *n_exported, uint32_t cap,
-                                                         uint32_t *err)
+// Global climb from the exported nodes (those whose parent's leaf range
+// crosses a chunk), in a single workgroup: every write and arrival of this
+// climb then comes from one workgroup, so workgroup-scope release/acquire
+// suffice and no device-wide cache write-back is paid per level.  Aggregates
+// of the chunk climb were written by an earlier kernel and are visible.
+__global__ __launch_bounds__(BU_TB) void k_bottomup_global(TreeOut T, const uint32_t *exported,
+                                                            const uint32_t *n_exported, uint32_t cap,
+                                                            uint32_t *err)
 {
     const uint32_t ne = min(*n_exported, cap);
-    for (uint32_t k = blockIdx.x * TB + threadIdx.x; k < ne; k += gridDim.x * TB) {
+    for (uint32_t k = threadIdx.x; k < ne; k += BU_TB) {
         uint32_t c = exported[k];
+        Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
         uint32_t p = (c & OWM_LEAF_BIT) ? T.lf_parent[c & ~OWM_LEAF_BIT] : T.nd_parent[c];
         for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
             if (depth > 128) {
                 atomicOr(err, 4u);
                 break;
             }
-            __threadfence();
-            uint32_t old = atomicAdd(&T.nd_flag[p], 1u);
+            const uint2 chd = T.nd_child[p];
+            const uint32_t pp = T.nd_parent[p];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const uint32_t old = __hip_atomic_fetch_add(&T.nd_flag[p], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0)
                 break;
-            __threadfence();
-            uint2 chd = T.nd_child[p];
-            Agg a = load_agg(T, chd.x), b = load_agg(T, chd.y);
-            store_agg(T, p, merge_agg(a, b));
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const Agg other = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
+            mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
+            store_agg(T, p, mine);
             c = p;
-            p = T.nd_parent[p];
+            p = pp;
         }
     }
 }
@@ -522,15 +747,21 @@ static uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap)
     return (uint32_t)g;
 }
 
+uint64_t bbox_key_words(uint32_t ntiles) { return (uint64_t)BBOX_REPL * 4 * ntiles; }
+
 void launch_bbox_init(hipStream_t s, unsigned long long *bk, uint32_t ntiles)
 {
-    hipLaunchKernelGGL(k_bbox_init, dim3(grid_for(ntiles, TB, 64)), dim3(TB), 0, s, bk, ntiles);
+    uint64_t nw = bbox_key_words(ntiles);
+    hipLaunchKernelGGL(k_bbox_init, dim3(grid_for(nw, TB, 256)), dim3(TB), 0, s, bk, (uint32_t)nw);
 }
 
 void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                  unsigned long long *bk)
 {
-    hipLaunchKernelGGL(k_bbox, dim3(grid_for(B.np, CHUNK, 4096)), dim3(TB), 0, s, B, x, y, bk);
+    // about four chunks per block: fewer flushes, still every CU busy
+    hipLaunchKernelGGL(k_bbox, dim3(grid_for(B.np, 4 * CHUNK, 2048)), dim3(TB), 0, s, B, x, y, bk);
+    hipLaunchKernelGGL(k_bbox_fold, dim3(grid_for(4ull * B.ntiles, TB, 64)), dim3(TB), 0, s, bk,
+                       B.ntiles);
 }
 
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
@@ -543,7 +774,7 @@ void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const doub
 void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *vals,
                           const uint32_t *skeys, const double4 *aos, const TreeOut &T)
 {
-    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.ngrp, TB / 64, 16384)), dim3(TB), 0, s, B,
+    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.ngrp, GATHER_G * TB / 64, 16384)), dim3(TB), 0, s, B,
                        vals, skeys, aos, T);
 }
 
@@ -569,9 +800,9 @@ void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_
     launch_fill_u32(s, T.nd_flag, B.nnode, 0u);
     launch_fill_u32(s, work, 1, 0u);
     const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
-    hipLaunchKernelGGL(k_bottomup_local, dim3(nch < 1 ? 1 : (nch > 16384 ? 16384 : nch)), dim3(TB),
+    hipLaunchKernelGGL(k_bottomup_local, dim3(nch < 1 ? 1 : (nch > 16384 ? 16384 : nch)), dim3(BU_TB),
                        0, s, B, T, work + 1, work, cap, err);
-    hipLaunchKernelGGL(k_bottomup_global, dim3(64), dim3(TB), 0, s, T, (const uint32_t *)(work + 1),
+    hipLaunchKernelGGL(k_bottomup_global, dim3(1), dim3(BU_TB), 0, s, T, (const uint32_t *)(work + 1),
                        (const uint32_t *)work, cap, err);
 }
 
