@@ -61,7 +61,8 @@ def cpu_baseline(x, y, z, want_seeds, reps):
 
 
 def run_batch_bench(ctx, torch, synth, n_tiles, steps):
-    """Plain (one tile per launch chain) vs batched (all tiles in one chain)."""
+    """Autotune on the batch, then plain (one run_tile launch chain per tile)
+    vs batched (all tiles in one chain, eager and replayed from a hipGraph)."""
     import numpy as np
     tiles = synth.batch_tiles(n_tiles, seed=0)
     arrs = [t.generate() for t in tiles]
@@ -69,6 +70,9 @@ def run_batch_bench(ctx, torch, synth, n_tiles, steps):
     dev = [torch.from_numpy(np.concatenate([a[k] for a in arrs])).to("cuda:0") for k in range(3)]
     desc = [(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0)) for t in tiles]
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    table = ctx.autotune(*dev, desc)
+    t_tune = time.perf_counter() - t0
     # plain: one run_tile per tile on device-resident slices
     offs = np.cumsum([0] + [t.n for t in tiles])
     views = [[d[offs[i]:offs[i + 1]] for d in dev] for i in range(len(tiles))]
@@ -84,15 +88,28 @@ def run_batch_bench(ctx, torch, synth, n_tiles, steps):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        got = ctx.batch_run(*dev, desc)
+        eager = ctx.batch_run(*dev, desc)
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / steps
+    ctx.batch_run(*dev, desc, graph=True)  # capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = ctx.batch_run(*dev, desc, graph=True)
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
-    same = all(np.array_equal(a, b) for a, b in zip(plain, got))
+    same = all(np.array_equal(a, b) for a, b in zip(plain, got)) and \
+        all(np.array_equal(a, b) for a, b in zip(plain, eager))
     return {"tiles": len(tiles), "points": int(n_total),
             "plain_mpts": round(n_total / t_plain / 1e6, 1),
             "batched_mpts": round(n_total / t_batch / 1e6, 1),
-            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
-            "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same}
+            "batched_eager_mpts": round(n_total / t_eager / 1e6, 1),
+            "batched_ms": round(t_batch * 1e3, 3), "batched_eager_ms": round(t_eager * 1e3, 3),
+            "plain_ms": round(t_plain * 1e3, 3),
+            "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same,
+            "autotune_s": round(t_tune, 2),
+            "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k",
+                                         "sort_bits")} for e in table if e["ms"] > 0]}
 
 
 def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):

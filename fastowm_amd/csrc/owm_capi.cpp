@@ -109,8 +109,10 @@ owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_o
     return OWM_OK;
 }
 
-// Copies a host cloud into the context's upload buffer.
-static owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, const double *z,
+}  // extern "C"
+
+namespace owm {
+owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, const double *z,
                                uint64_t n, double **dx, double **dy, double **dz)
 {
     uint64_t b = al256(n * 8ull);
@@ -137,6 +139,9 @@ static owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, c
     }
     return OWM_OK;
 }
+}  // namespace owm
+
+extern "C" {
 
 static owm_status run_one(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
                           uint64_t n, const owm_params *params, uint32_t *seeds, uint64_t cap,

@@ -41,13 +41,6 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
     return OWM_E_DEVICE;
 }
 
-#define HIP_TRY(call, what)                        \
-    do {                                           \
-        hipError_t e_ = (call);                    \
-        if (e_ != hipSuccess)                      \
-            return hip_fail(ctx, e_, what);        \
-    } while (0)
-
 static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 static const char *PHASE_NAMES[PH_COUNT] = {
@@ -260,14 +253,16 @@ int bit_length(uint64_t v)
 // ------------------------------------------------------------- planning --
 
 // Per-tile tree layout: leaves, internal nodes, Morton quantisation.
-void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
-                      const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
-                      TreeLayout &L)
+void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
+               const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
+               TreeLayout &L)
 {
     const uint32_t T = (uint32_t)tiles.size();
     L.ntiles = T;
     L.tile_bits = T > 1 ? bit_length(T - 1) : 0;
-    L.key_bits = (32 - L.tile_bits) & ~1;
+    // Morton bits under the tile bits: the sort key width is sort_bits (24 or
+    // 32), but never fewer than 2 Morton bits per axis
+    L.key_bits = std::max((std::min(sort_bits, 32) - L.tile_bits) & ~1, std::min(4, (32 - L.tile_bits) & ~1));
     L.pt_off.assign(T + 1, 0);
     L.leaf_off.assign(T + 1, 0);
     L.node_off.assign(T + 1, 0);
@@ -717,7 +712,7 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
     }
     if (need_bbox) {
         std::vector<std::array<double, 4>> unit(n_tiles, {0.0, 1.0, 0.0, 1.0});
-        plan_tree(P.td, P.n, P.leaf, unit, P.L);
+        plan_tree(32, P.td, P.n, P.leaf, unit, P.L);
         const uint64_t bk_bytes = bbox_key_words(n_tiles) * 8;
         uint64_t bytes = align_up(bk_bytes, 256) + align_up((n_tiles + 1) * 4ull, 256);
         st = ctx->small.ensure(ctx, bytes);
@@ -757,7 +752,7 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
                             dec(hk[4 * t + 3])};
             }
     }
-    plan_tree(P.td, P.n, P.leaf, P.box, P.L);
+    plan_tree((int)ctx->tuner.sort_bits_for(np), P.td, P.n, P.leaf, P.box, P.L);
     return OWM_OK;
 }
 
@@ -799,6 +794,9 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
         est += 1.5 * D.nleaf * fx * fy + 64;
     }
     P.Q.s1_cap = (uint64_t)std::min(est, 4.0e9);
+    // a join that overflowed before sets the floor (and keeps captured chains valid)
+    if (ctx->s1_need_max)
+        P.Q.s1_cap = std::max<uint64_t>(P.Q.s1_cap, ctx->s1_need_max + ctx->s1_need_max / 8 + 1024);
     // OWM_S1_LIST_CAP overrides the first guess (used to exercise the re-run)
     if (const char *ov = getenv("OWM_S1_LIST_CAP"))
         P.Q.s1_cap = std::max<uint64_t>(1, strtoull(ov, nullptr, 10));
@@ -914,13 +912,124 @@ owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint3
     return OWM_OK;
 }
 
+// Key of a captured batch chain: everything its launches depend on (device
+// addresses, layout, tile table, per-launch configuration).
+static void graph_key(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                      const Prepared &P, std::vector<uint8_t> &key)
+{
+    auto put = [&key](const void *p, size_t n) {
+        const uint8_t *b = (const uint8_t *)p;
+        key.insert(key.end(), b, b + n);
+    };
+    key.clear();
+    const void *ptrs[6] = {dx, dy, dz, ctx->arena.buf.ptr, ctx->s1_list.ptr, nullptr};
+    put(ptrs, sizeof(ptrs));
+    const uint64_t nums[6] = {ctx->arena.buf.bytes, ctx->s1_list.bytes, P.Q.s1_cap,
+                              (uint64_t)P.L.key_bits, (uint64_t)P.L.tile_bits,
+                              (uint64_t)P.any_need1};
+    put(nums, sizeof(nums));
+    put(&P.cfg, sizeof(P.cfg));
+    put(P.td.data(), P.td.size() * sizeof(TileDev));
+}
+
+GraphCache::~GraphCache() { clear(); }
+
+void GraphCache::clear()
+{
+    for (auto &s : slots)
+        if (s.exec)
+            (void)hipGraphExecDestroy(s.exec);
+    slots.clear();
+}
+
+// Enqueues (or replays) the device chain of a batch: guard reset, tree build,
+// stages 1-3 and the merge.  With a graph, the chain is captured on first use
+// and replayed while its key matches.
+static owm_status enqueue_chain(owm_ctx *ctx, bool graph, const double *dx, const double *dy,
+                                const double *dz, Prepared &P, BatchDev &B, TreeOut &T,
+                                uint32_t *guard, QueryOut &qo)
+{
+    hipStream_t s = ctx->stream;
+    Arena &A = ctx->arena;
+    auto body = [&]() -> owm_status {
+        launch_fill_u32(s, guard, 64, 0u);
+        owm_status st = build_tree(ctx, s, P.L, B, dx, dy, dz, T, A, guard);
+        if (st != OWM_OK)
+            return st;
+        TreeDev TV = tree_view(T);
+        TV.X = dx;
+        TV.Y = dy;
+        st = enqueue_query(ctx, s, B, TV, P.Q, P.L.np, P.cfg, P.any_need1 ? 1u : 0u, A, qo, 1, 4,
+                           nullptr, 0);
+        if (st != OWM_OK)
+            return st;
+        launch_add_u32(s, qo.cnt + 3, guard, qo.cnt + 3);
+        return OWM_OK;
+    };
+    if (!graph)
+        return body();
+    GraphCache &G = ctx->graphs;
+    std::vector<uint8_t> key;
+    graph_key(ctx, dx, dy, dz, P, key);
+    for (size_t i = 0; i < G.slots.size(); ++i)
+        if (G.slots[i].key == key) {
+            GraphCache::Slot hit = std::move(G.slots[i]);
+            G.slots.erase(G.slots.begin() + i);
+            qo = hit.qo;
+            A.used = hit.used_after;
+            hipError_t e = hipGraphLaunch(hit.exec, s);
+            G.slots.insert(G.slots.begin(), std::move(hit));
+            if (e != hipSuccess)
+                return hip_fail(ctx, e, "graph launch");
+            return OWM_OK;
+        }
+    // capture: phase timers are host events and stay out of the graph
+    const bool prof_on = ctx->prof.on;
+    ctx->prof.on = false;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        ctx->prof.on = prof_on;
+        return hip_fail(ctx, e, "begin capture");
+    }
+    owm_status st = body();
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    ctx->prof.on = prof_on;
+    if (st != OWM_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "end capture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "graph instantiate");
+    GraphCache::Slot slot;
+    slot.key = std::move(key);
+    slot.exec = exec;
+    slot.qo = qo;
+    slot.used_after = A.used;
+    G.slots.insert(G.slots.begin(), std::move(slot));
+    while (G.slots.size() > GraphCache::MAX_SLOTS) {
+        (void)hipGraphExecDestroy(G.slots.back().exec);
+        G.slots.pop_back();
+    }
+    e = hipGraphLaunch(exec, s);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "graph launch");
+    return OWM_OK;
+}
+
 // Runs the whole pipeline on a batch whose points are in device memory
-// (concatenated SoA).  seeds/seed_off_host receive tile-local ids.
+// (concatenated SoA).  seeds/seed_off_host receive tile-local ids.  With
+// OWM_B_GRAPH the device chain is captured once per batch layout and replayed.
 owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
                      const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
                      uint64_t cap, uint64_t *seed_off_host)
 {
-    (void)flags;
     Prepared P;
     owm_status st = prepare_tree(ctx, dx, dy, tiles, n_tiles, P);
     if (st != OWM_OK)
@@ -928,48 +1037,43 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     st = prepare_query(ctx, &tiles[0].params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
-    const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) + build_scratch_bytes(P.L) +
-                          query_bytes(P.Q, P.L.np, 0) + 8192;
-    st = ctx->arena.ensure(ctx, need);
-    if (st != OWM_OK)
-        return st;
-    Arena &A = ctx->arena;
-    uint32_t *guard = (uint32_t *)A.take(256);
-    launch_fill_u32(ctx->stream, guard, 64, 0u);
-    BatchDev B;
-    st = upload_table(ctx, A, P, B);
-    if (st != OWM_OK)
-        return st;
-    TreeOut T;
-    carve_tree(A.take(tree_bytes(P.L)), P.L, T);
-    T.pos = nullptr;  // the cloud stays available: stage 3 reads LLP coordinates from it
-    st = build_tree(ctx, ctx->stream, P.L, B, dx, dy, dz, T, A, guard);
-    if (st != OWM_OK)
-        return st;
-    TreeDev TV = tree_view(T);
-    TV.X = dx;
-    TV.Y = dy;
-    const uint64_t mark = A.used;
+    bool graph = (flags & OWM_B_GRAPH) != 0;
     for (int attempt = 0;; ++attempt) {
+        const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) + build_scratch_bytes(P.L) +
+                              query_bytes(P.Q, P.L.np, 0) + 8192;
+        void *old_arena = ctx->arena.buf.ptr;
+        st = ctx->arena.ensure(ctx, need);
+        if (st != OWM_OK)
+            return st;
         st = ensure_s1_list(ctx, P.Q.s1_cap);
         if (st != OWM_OK)
             return st;
-        QueryOut qo;
-        st = enqueue_query(ctx, ctx->stream, B, TV, P.Q, P.L.np, P.cfg, P.any_need1 ? 1u : 0u, A,
-                           qo, 1, 4, nullptr, 0);
+        if (ctx->arena.buf.ptr != old_arena)
+            ctx->graphs.clear();  // captured chains point into the old arena
+        Arena &A = ctx->arena;
+        uint32_t *guard = (uint32_t *)A.take(256);
+        BatchDev B;
+        st = upload_table(ctx, A, P, B);
         if (st != OWM_OK)
             return st;
-        launch_add_u32(ctx->stream, qo.cnt + 3, guard, qo.cnt + 3);
-        uint64_t need = 0;
-        st = read_seeds(ctx, qo, P, seeds, cap, seed_off_host, &need);
-        if (st != OWM_OK || need == 0)
+        TreeOut T;
+        carve_tree(A.take(tree_bytes(P.L)), P.L, T);
+        T.pos = nullptr;  // the cloud stays available: stage 3 reads LLP coordinates from it
+        QueryOut qo;
+        st = enqueue_chain(ctx, graph, dx, dy, dz, P, B, T, guard, qo);
+        if (st != OWM_OK)
+            return st;
+        uint64_t s1_need = 0;
+        st = read_seeds(ctx, qo, P, seeds, cap, seed_off_host, &s1_need);
+        if (st != OWM_OK || s1_need == 0)
             return st;
         if (attempt > 0) {
             set_err(ctx, "stage-1 join list overflow persisted");
             return OWM_E_INTERNAL;
         }
-        P.Q.s1_cap = need + need / 8 + 1024;  // the join's exact size, then run the query again
-        A.used = mark;
+        // the join's exact size is known now: run again with room for it
+        ctx->s1_need_max = std::max<uint64_t>(ctx->s1_need_max, s1_need);
+        P.Q.s1_cap = s1_need + s1_need / 8 + 1024;
     }
 }
 

@@ -71,28 +71,53 @@ struct QueryOut {
     uint32_t *seed_off = nullptr;
 };
 
-// Size-class tuning table: leaf size and stage-1 window block per point count.
+// Size-class tuning table.  The leaf size is looked up by a tile's point
+// count; the stage-1 window block and the sort key width (Morton bits plus
+// tile bits, 24 or 32: three or four 8-bit passes) by the batch's total.
+struct TuneCfg {
+    uint32_t leaf;
+    Stage1Cfg s1;
+    uint32_t sort_bits;
+};
+
 struct Tuner {
     struct Entry {
         uint64_t max_n;
-        uint32_t leaf;
-        Stage1Cfg s1;
+        TuneCfg cfg;
+        double ms;  // time of the winning trial (0 = default, never tuned)
     };
-    std::vector<Entry> table{{~0ull, 32, {8, 8, 4}}};
+    static constexpr uint64_t CLASS_MAX[6] = {1ull << 16, 1ull << 18, 1ull << 20,
+                                              1ull << 22, 1ull << 24, ~0ull};
+    std::vector<Entry> table;
+    bool forced = false;  // a tuning trial overrides the table
+    TuneCfg force{};
+    Tuner() { reset(); }
+    void reset()
+    {
+        table.clear();
+        for (uint64_t m : CLASS_MAX)
+            table.push_back({m, {32, {8, 8, 4}, 32}, 0.0});
+    }
+    static int class_of(uint64_t n)
+    {
+        int c = 0;
+        while (c < 5 && n > CLASS_MAX[c])
+            ++c;
+        return c;
+    }
+    const TuneCfg &cfg_for(uint64_t n) const
+    {
+        if (forced)
+            return force;
+        return table[class_of(n)].cfg;
+    }
+    // a trial with force.leaf == 0 keeps the table's leaf sizes
     uint32_t leaf_for(uint64_t n) const
     {
-        for (const auto &e : table)
-            if (n <= e.max_n)
-                return e.leaf;
-        return 32;
+        return forced && force.leaf ? force.leaf : table[class_of(n)].cfg.leaf;
     }
-    Stage1Cfg stage1_for(uint64_t n) const
-    {
-        for (const auto &e : table)
-            if (n <= e.max_n)
-                return e.s1;
-        return {8, 8, 4};
-    }
+    Stage1Cfg stage1_for(uint64_t n) const { return cfg_for(n).s1; }
+    uint32_t sort_bits_for(uint64_t n) const { return cfg_for(n).sort_bits; }
 };
 
 struct Prepared {
@@ -110,7 +135,7 @@ owm_status derive_params(const owm_params &P, uint64_t n, const double *bbox, De
                          std::string &why);
 void fill_query(TileDev &T, const owm_params &P, const Derived &D);
 int bit_length(uint64_t v);
-void plan_tree(std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
+void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
                const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
                TreeLayout &L);
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
@@ -142,6 +167,16 @@ uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested);
 Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total);
 void set_err(owm_ctx *ctx, const char *fmt, ...);
 owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what);
+// Returns the failure status (with `ctx` in scope) when a HIP call fails.
+#define HIP_TRY(call, what)                        \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess)                      \
+            return hip_fail(ctx, e_, what);        \
+    } while (0)
+// Copies a host cloud into the context's upload buffer (stream-ordered).
+owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                        uint64_t n, double **dx, double **dy, double **dz);
 owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
                      const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
                      uint64_t cap, uint64_t *seed_off_host);
@@ -177,6 +212,20 @@ struct Profiler {
     ~Profiler();
 };
 
+// Captured batch chains, most recently used first.
+struct GraphCache {
+    static constexpr size_t MAX_SLOTS = 4;
+    struct Slot {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec = nullptr;
+        QueryOut qo;
+        uint64_t used_after = 0;  // arena offset after the chain's allocations
+    };
+    std::vector<Slot> slots;
+    void clear();
+    ~GraphCache();
+};
+
 }  // namespace owm
 
 struct owm_ctx {
@@ -188,6 +237,7 @@ struct owm_ctx {
     owm::Arena small;   // bbox pass
     owm::DevBuf upload; // host inputs staged to the device
     owm::DevBuf s1_list; // stage-1 leaf/block join list (grows on demand)
+    uint64_t s1_need_max = 0;  // largest join that overflowed its first estimate
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};
     ::TileDev *h_tiles = nullptr;
@@ -195,6 +245,7 @@ struct owm_ctx {
     uint32_t h_cap = 0;
     owm::ServeState *serve = nullptr;
     owm::Profiler prof;  // owned; released by owm_serve_stop
+    owm::GraphCache graphs;  // captured batch chains (OWM_B_GRAPH)
     void stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
                      const owm::QueryLayout &Q);
     uint64_t held_bytes() const;

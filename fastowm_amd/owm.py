@@ -58,6 +58,19 @@ class TileDesc(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("params", Params)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("max_n", ctypes.c_uint64), ("leaf_size", ctypes.c_uint32),
+                ("block_x", ctypes.c_uint32), ("block_y", ctypes.c_uint32),
+                ("block_k", ctypes.c_uint32), ("sort_bits", ctypes.c_uint32),
+                ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+TUNE_CLASSES = 6
+
+
 _lib = None
 
 
@@ -106,6 +119,10 @@ def lib():
             "owm_serve_stop": ([vp], ctypes.c_int),
             "owm_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
             "owm_profile_read": ([vp, P(PhaseStat), u32, P(u32)], ctypes.c_int),
+            "owm_autotune": ([vp, dp, dp, dp, ctypes.c_int, P(TileDesc), u32, P(TuneEntry), u32,
+                              P(u32)], ctypes.c_int),
+            "owm_tune_get": ([vp, P(TuneEntry), u32, P(u32)], ctypes.c_int),
+            "owm_tune_set": ([vp, P(TuneEntry), u32], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -254,9 +271,8 @@ class Context:
         self._check(st)
         return out[:ns.value].copy()
 
-    def batch_run(self, x, y, z, tiles, graph=False):
-        """tiles: list of (n, dict(wsize=, bsize=, overlap=, leaf_size=, bbox=)) in
-        cloud order.  Returns a list of seed arrays (tile-local ids)."""
+    @staticmethod
+    def _descs(tiles):
         descs = (TileDesc * len(tiles))()
         off = 0
         for i, (n, kw) in enumerate(tiles):
@@ -264,6 +280,13 @@ class Context:
             descs[i].n = n
             descs[i].params = make_params(**kw)
             off += n
+        return descs, off
+
+    def batch_run(self, x, y, z, tiles, graph=False):
+        """tiles: list of (n, dict(wsize=, bsize=, overlap=, leaf_size=, bbox=)) in
+        cloud order.  Returns a list of seed arrays (tile-local ids).  graph=True
+        captures the device chain once per batch layout and replays it."""
+        descs, off = self._descs(tiles)
         cap = off + len(tiles) + 1
         out = np.empty(cap, np.uint32)
         so = np.zeros(len(tiles) + 1, np.uint64)
@@ -277,6 +300,36 @@ class Context:
                                      out.ctypes.data, cap, so.ctypes.data)
         self._check(st)
         return [out[int(so[i]):int(so[i + 1])].copy() for i in range(len(tiles))]
+
+    # -- autotuner ----------------------------------------------------------
+    def autotune(self, x, y, z, tiles):
+        """Times leaf sizes, window blocks and sort widths on this batch (same
+        form as batch_run) and keeps the winners; returns the table."""
+        descs, _ = self._descs(tiles)
+        out = (TuneEntry * TUNE_CLASSES)()
+        n = ctypes.c_uint32()
+        if _is_device(x):
+            st = lib().owm_autotune(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), 1, descs,
+                                    len(tiles), out, TUNE_CLASSES, ctypes.byref(n))
+        else:
+            (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
+            st = lib().owm_autotune(self.h, xp, yp, zp, 0, descs, len(tiles), out, TUNE_CLASSES,
+                                    ctypes.byref(n))
+        self._check(st)
+        return [out[i].as_dict() for i in range(n.value)]
+
+    def tune_table(self):
+        out = (TuneEntry * TUNE_CLASSES)()
+        n = ctypes.c_uint32()
+        self._check(lib().owm_tune_get(self.h, out, TUNE_CLASSES, ctypes.byref(n)))
+        return [out[i].as_dict() for i in range(n.value)]
+
+    def set_tune_table(self, entries):
+        arr = (TuneEntry * len(entries))()
+        for i, e in enumerate(entries):
+            for f, _ in TuneEntry._fields_:
+                setattr(arr[i], f, e[f])
+        self._check(lib().owm_tune_set(self.h, arr, len(entries)))
 
     # -- stage-level interface (reference names) ----------------------------
     def parallel_qtree(self, x, y, z, leaf_size=32) -> Tree:

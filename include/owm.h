@@ -130,6 +130,34 @@ owm_status owm_batch_run_device(owm_ctx *ctx, const double *d_x, const double *d
                                 uint32_t flags, uint32_t *seeds, uint64_t cap,
                                 uint64_t *seed_off);
 
+/* ---- size-class autotuner -------------------------------------------------
+ * No counterpart in the reference library (there, tree granularity was
+ * tuned by sweep scripts over maxNumber).  Point-count classes: up to 2^16,
+ * 2^18, 2^20, 2^22, 2^24 and beyond.  A tile whose params.leaf_size is 0
+ * takes the leaf size of its own class; the stage-1 window block and the sort
+ * key width come from the class of the batch's total point count. */
+#define OWM_TUNE_CLASSES 6u
+typedef struct owm_tune_entry {
+    uint64_t max_n;      /* class bound: point counts up to max_n */
+    uint32_t leaf_size;  /* points per leaf (maxNumber) */
+    uint32_t block_x, block_y, block_k; /* stage-1 window block, points per lane per round */
+    uint32_t sort_bits;  /* sort key width, 24 or 32 (three or four 8-bit passes) */
+    double ms;           /* time of the winning trial; 0 = default, not tuned */
+} owm_tune_entry;
+/* Times trial configurations on the given batch (tiles as in owm_batch_run,
+ * their leaf_size ignored): leaf sizes 16..128 per tile class on the tiles of
+ * that class, then window blocks and sort widths for the whole batch.  Stores
+ * the winners in the context's table (used by every later call, and copied
+ * into serving slots at owm_serve_start) and returns the table in out
+ * (cap >= OWM_TUNE_CLASSES). */
+owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                        int device_ptrs, const owm_tile *tiles, uint32_t n_tiles,
+                        owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
+owm_status owm_tune_get(owm_ctx *ctx, owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
+/* Loads a table (e.g. one saved from owm_tune_get); classes not listed keep
+ * the defaults. */
+owm_status owm_tune_set(owm_ctx *ctx, const owm_tune_entry *in, uint32_t n);
+
 /* ---- stage-level API on a device-resident tree ---------------------------
  * Mirrors the reference's internal boundary (include/optim3_func.hpp:85-93):
  * parallel_qtree + storeMinAndNumPoints -> owm_tree_build, stage1 ->

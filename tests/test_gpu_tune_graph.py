@@ -1,0 +1,113 @@
+"""Captured batch chains (hipGraph replay) and the size-class autotuner: every
+replay and every tuned configuration gives the eager, untuned seed sets."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fastowm_amd import owm, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _batch(sizes, seed):
+    tiles, arrs = [], []
+    for i, n in enumerate(sizes):
+        t = synth.scaled(synth.config_c1(seed + i), n)
+        t.wsize = (8, 12, 16)[i % 3]
+        t.bsize = (20, 15, 21)[i % 3]
+        t.overlap = (0.8, 0.5, 0.9)[i % 3]
+        tiles.append(t)
+        arrs.append(t.generate())
+    x, y, z = (np.concatenate([a[k] for a in arrs]) for k in range(3))
+    desc = [(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0)) for t in tiles]
+    return x, y, z, desc, tiles, arrs
+
+
+def _same(a, b):
+    return len(a) == len(b) and all(np.array_equal(u, v) for u, v in zip(a, b))
+
+
+def test_graph_replays_equal_eager():
+    torch = pytest.importorskip("torch")
+    xa, ya, za, da, _, _ = _batch([30_000, 5_000, 120_000, 60_000, 9_000, 250_000], seed=100)
+    xb, yb, zb, db, _, _ = _batch([70_000, 40_000, 3_000], seed=200)
+    dev_a = [torch.from_numpy(v).to("cuda:0") for v in (xa, ya, za)]
+    dev_b = [torch.from_numpy(v).to("cuda:0") for v in (xb, yb, zb)]
+    with owm.Context(0) as ctx:
+        say("eager references")
+        ref_a = ctx.batch_run(*dev_a, da)
+        ref_b = ctx.batch_run(*dev_b, db)
+        for rep in range(3):
+            say("graph replay of batch a", rep)
+            assert _same(ctx.batch_run(*dev_a, da, graph=True), ref_a)
+            # eager work and a synchronisation between replays
+            assert _same(ctx.batch_run(*dev_b, db), ref_b)
+            torch.cuda.synchronize()
+        say("second layout captured beside the first")
+        assert _same(ctx.batch_run(*dev_b, db, graph=True), ref_b)
+        assert _same(ctx.batch_run(*dev_a, da, graph=True), ref_a)
+        assert _same(ctx.batch_run(*dev_b, db, graph=True), ref_b)
+
+
+def test_graph_from_host_arrays_matches_oracle():
+    x, y, z, desc, tiles, arrs = _batch([20_000, 45_000], seed=300)
+    with owm.Context(0) as ctx:
+        for rep in range(2):
+            say("host-input graph", rep)
+            got = ctx.batch_run(x, y, z, desc, graph=True)
+            for t, (tx, ty, tz), g in zip(tiles, arrs, got):
+                want, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
+                assert np.array_equal(g, want)
+
+
+def test_autotune_keeps_results_and_table_round_trips():
+    x, y, z, desc, tiles, arrs = _batch([30_000, 80_000, 300_000, 1_200_000, 50_000], seed=400)
+    with owm.Context(0) as ctx:
+        say("untuned batch")
+        before = ctx.batch_run(x, y, z, desc)
+        say("autotune")
+        table = ctx.autotune(x, y, z, desc)
+        assert len(table) == owm.TUNE_CLASSES
+        assert [e["max_n"] for e in table] == [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 2 ** 64 - 1]
+        for e in table:
+            assert e["leaf_size"] in (16, 32, 64, 128)
+            assert e["sort_bits"] in (24, 32)
+            assert (e["block_x"], e["block_y"], e["block_k"]) in ((8, 8, 4), (16, 8, 4), (8, 8, 2))
+        # the classes that held tiles were timed
+        assert all(table[c]["ms"] > 0 for c in (0, 1, 2, 3))
+        say("tuned batch equals untuned")
+        assert _same(ctx.batch_run(x, y, z, desc), before)
+        say("table round trip")
+        assert ctx.tune_table() == table
+        ctx.set_tune_table(table)
+        assert ctx.tune_table() == table
+        bad = dict(table[0])
+        bad["sort_bits"] = 20
+        with pytest.raises(owm.OWMError) as e:
+            ctx.set_tune_table([bad])
+        assert e.value.name == "E_ARG"
+
+
+@pytest.mark.parametrize("sort_bits,leaf,block", [(24, 16, (8, 8, 2)), (24, 128, (16, 8, 4)),
+                                                  (32, 64, (16, 8, 4))])
+def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
+    say("table entry", sort_bits, leaf, block)
+    t = synth.scaled(synth.config_c2(seed=5), 700_000)
+    x, y, z = t.generate()
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    x2, y2, z2, desc, tiles, arrs = _batch([40_000, 90_000, 20_000], seed=500)
+    with owm.Context(0) as ctx:
+        entries = ctx.tune_table()
+        for e in entries:
+            e["sort_bits"], e["leaf_size"] = sort_bits, leaf
+            e["block_x"], e["block_y"], e["block_k"] = block
+        ctx.set_tune_table(entries)
+        assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 0), want)
+        got = ctx.batch_run(x2, y2, z2, desc)
+        for tt, (tx, ty, tz), g in zip(tiles, arrs, got):
+            w, _, _ = O.run(tx, ty, tz, tt.wsize, tt.bsize, tt.overlap, 32)
+            assert np.array_equal(g, w)
