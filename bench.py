@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--leaf", type=int, default=0,
+                    help="leaf size (maxNumber) of the timed tile; 0 = autotuned")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -207,14 +209,21 @@ def main():
     dx, dy, dz = (torch.from_numpy(a).to("cuda:0") for a in (x, y, z))
     torch.cuda.synchronize()
     ctx = owm.Context(0)
+    leaf = args.leaf
+    tuned = None
+    if leaf == 0:
+        # leaf size, window block and sort width from the autotuner (untimed)
+        log("autotune")
+        tuned = ctx.autotune(dx, dy, dz, [(tile.n, dict(wsize=12, bsize=20, overlap=0.8))])
+        tuned = [e for e in tuned if e["ms"] > 0][-1]
     for _ in range(args.warmup):
-        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, 32)
+        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, leaf)
     torch.cuda.synchronize()
     ctx.profile(True)
     log(f"timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, 32)
+        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, leaf)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = ctx.profile_read()
@@ -244,8 +253,11 @@ def main():
             "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "c2: 20M points, 2x2 km, 5 pts/m2, 10% in Gaussian clusters "
-                                   "(sigma 3 m), Wsize 12, Overlap 0.8, Bsize 20, leaf 32",
-                       "points": tile.n, "seeds": int(len(seeds))},
+                                   "(sigma 3 m), Wsize 12, Overlap 0.8, Bsize 20",
+                       "points": tile.n, "seeds": int(len(seeds)),
+                       "leaf_size": leaf if leaf else (tuned["leaf_size"] if tuned else None),
+                       "tuned": {k: tuned[k] for k in ("leaf_size", "block_x", "block_y", "block_k",
+                                                       "sort_bits")} if tuned else None},
             "roofline": roofline, "phases": phases}
     if not args.no_batch:
         log("batch leg")

@@ -621,24 +621,25 @@ __device__ __forceinline__ void store_agg(const TreeOut &T, uint32_t p, const Ag
 
 constexpr uint32_t BU_CHUNK = 2048;  // leaves per workgroup in the chunk climb
 constexpr int BU_TB = 1024;           // threads per chunk-climb workgroup
-constexpr uint32_t BU_EXP_CAP = 1024;  // exports staged in LDS per chunk
+// A chunk exports at most the children of the crossing paths at its two
+// boundaries, each no longer than the depth limit.
+constexpr uint32_t BU_MAX_DEPTH = 128;
+constexpr uint32_t BU_EXP_PER_CHUNK = 2 * (BU_MAX_DEPTH + 2);
 
 // Chunk climb: a workgroup owns leaves [A, E] of the batch; a node whose
 // leaf range lies inside [A, E] has both children completed by this
 // workgroup, so its arrival counter lives in LDS and needs no device-wide
 // fence.  A completed node whose parent's range leaves the chunk is exported
 // for the global climb.
-__global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exported,
-                                                        uint32_t *n_exported, uint32_t cap,
-                                                        uint32_t *err)
+__global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exp_cnt,
+                                                        uint32_t *exported, uint32_t *err)
 {
     __shared__ uint32_t sFlag[BU_CHUNK];
-    // exports are gathered per chunk and reserved with one device atomic
-    __shared__ uint32_t sExp[BU_EXP_CAP];
-    __shared__ uint32_t sNexp, sBase;
+    __shared__ uint32_t sNexp;
     const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
     for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
         const uint32_t A = ch * BU_CHUNK, E = min(A + BU_CHUNK, B.nleaf) - 1;
+        uint32_t *slots = exported + (size_t)ch * BU_EXP_PER_CHUNK;
         for (uint32_t k = threadIdx.x; k < BU_CHUNK; k += BU_TB)
             sFlag[k] = 0;
         if (threadIdx.x == 0)
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T,
             Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
             uint32_t p = T.lf_parent[l];
             for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
-                if (depth > 128) {
+                if (depth > BU_MAX_DEPTH) {
                     atomicOr(err, 4u);
                     break;
                 }
@@ -661,15 +662,10 @@ __global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T,
                 const uint32_t glo = TD.leaf_off + rg.x, ghi = TD.leaf_off + rg.y;
                 if (glo < A || ghi > E) {
                     const uint32_t k = atomicAdd(&sNexp, 1u);
-                    if (k < BU_EXP_CAP) {
-                        sExp[k] = c;
-                    } else {
-                        const uint32_t g = atomicAdd(n_exported, 1u);
-                        if (g < cap)
-                            exported[g] = c;
-                        else
-                            atomicOr(err, 8u);
-                    }
+                    if (k < BU_EXP_PER_CHUNK)
+                        slots[k] = c;
+                    else
+                        atomicOr(err, 8u);
                     break;
                 }
                 const uint32_t slot = TD.leaf_off + (p - TD.node_off) - A;
@@ -686,53 +682,101 @@ __global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T,
             }
         }
         __syncthreads();
-        const uint32_t ne = min(sNexp, BU_EXP_CAP);
         if (threadIdx.x == 0)
-            sBase = ne ? atomicAdd(n_exported, ne) : 0u;
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < ne; k += BU_TB) {
-            const uint32_t g = sBase + k;
-            if (g < cap)
-                exported[g] = sExp[k];
-            else
-                atomicOr(err, 8u);
-        }
+            exp_cnt[ch] = min(sNexp, BU_EXP_PER_CHUNK);
         __syncthreads();
     }
 }
 
-// Global climb from the exported nodes (those whose parent's leaf range
-// crosses a chunk), in a single workgroup: every write and arrival of this
-// climb then comes from one workgroup, so workgroup-scope release/acquire
-// suffice and no device-wide cache write-back is paid per level.  Aggregates
-// of the chunk climb were written by an earlier kernel and are visible.
-__global__ __launch_bounds__(BU_TB) void k_bottomup_global(TreeOut T, const uint32_t *exported,
-                                                            const uint32_t *n_exported, uint32_t cap,
-                                                            uint32_t *err)
+__device__ __forceinline__ bool in_tile(const TileDev &TD, uint32_t ref)
 {
-    const uint32_t ne = min(*n_exported, cap);
-    for (uint32_t k = threadIdx.x; k < ne; k += BU_TB) {
-        uint32_t c = exported[k];
-        Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
-        uint32_t p = (c & OWM_LEAF_BIT) ? T.lf_parent[c & ~OWM_LEAF_BIT] : T.nd_parent[c];
-        for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
-            if (depth > 128) {
-                atomicOr(err, 4u);
-                break;
+    if (ref & OWM_LEAF_BIT) {
+        const uint32_t l = ref & ~OWM_LEAF_BIT;
+        return l >= TD.leaf_off && l < TD.leaf_off + TD.nleaf;
+    }
+    return ref >= TD.node_off && ref + 1 < TD.node_off + TD.nleaf;
+}
+
+// Global climb from the exported nodes (those whose parent's leaf range
+// crosses a chunk).  Climbs of one tile can meet, climbs of different tiles
+// never do, so one workgroup owns each tile: all writes and arrivals of a
+// tile's climb come from one workgroup and workgroup-scope release/acquire
+// suffice (no device-wide cache write-back per level).  Aggregates of the
+// chunk climb were written by an earlier kernel and are visible.
+__global__ __launch_bounds__(BU_TB) void k_bottomup_global(BatchDev B, TreeOut T,
+                                                            const uint32_t *exp_cnt,
+                                                            const uint32_t *exported, uint32_t *err)
+{
+    __shared__ uint32_t sOff[BU_TB + 1];
+    __shared__ uint32_t sWave[BU_TB / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
+        const TileDev &TD = B.tiles[t];
+        if (TD.nleaf < 2)
+            continue;
+        const uint32_t c0 = TD.leaf_off / BU_CHUNK, c1 = (TD.leaf_off + TD.nleaf - 1) / BU_CHUNK;
+        if (c0 == c1)
+            continue;  // the chunk climb finished the whole tree
+        for (uint32_t cb = c0; cb <= c1; cb += BU_TB) {
+            const uint32_t m = min((uint32_t)BU_TB, c1 - cb + 1);
+            // exclusive scan of the export counts of chunks cb .. cb+m-1
+            const uint32_t v = tid < m ? exp_cnt[cb + tid] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t u = __shfl_up(inc, o, 64);
+                if (lane >= (uint32_t)o)
+                    inc += u;
             }
-            const uint2 chd = T.nd_child[p];
-            const uint32_t pp = T.nd_parent[p];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            const uint32_t old = __hip_atomic_fetch_add(&T.nd_flag[p], 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0)
-                break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const Agg other = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
-            mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
-            store_agg(T, p, mine);
-            c = p;
-            p = pp;
+            if (lane == 63)
+                sWave[w] = inc;
+            __syncthreads();
+            uint32_t add = 0, total = 0;
+            for (uint32_t k = 0; k < BU_TB / 64; ++k) {
+                if (k < w)
+                    add += sWave[k];
+                total += sWave[k];
+            }
+            sOff[tid] = add + inc - v;
+            if (tid == 0)
+                sOff[BU_TB] = total;
+            __syncthreads();
+            for (uint32_t i = tid; i < total; i += BU_TB) {
+                uint32_t lo = 0, hi = m;  // last chunk with sOff <= i
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (sOff[mid] <= i)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                uint32_t c = exported[(size_t)(cb + lo) * BU_EXP_PER_CHUNK + (i - sOff[lo])];
+                if (!in_tile(TD, c))
+                    continue;  // a chunk shared with a neighbouring tile
+                Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
+                uint32_t p = (c & OWM_LEAF_BIT) ? T.lf_parent[c & ~OWM_LEAF_BIT] : T.nd_parent[c];
+                for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
+                    if (depth > BU_MAX_DEPTH) {
+                        atomicOr(err, 4u);
+                        break;
+                    }
+                    const uint2 chd = T.nd_child[p];
+                    const uint32_t pp = T.nd_parent[p];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    const uint32_t old = __hip_atomic_fetch_add(&T.nd_flag[p], 1u, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (old == 0)
+                        break;
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    const Agg other =
+                        load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
+                    mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
+                    store_agg(T, p, mine);
+                    c = p;
+                    p = pp;
+                }
+            }
+            __syncthreads();  // sOff is rewritten for the next piece
         }
     }
 }
@@ -787,23 +831,25 @@ void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
 
 uint64_t bottomup_work_words(uint64_t nleaf, uint32_t ntiles)
 {
-    uint64_t chunks = (nleaf + BU_CHUNK - 1) / BU_CHUNK;
-    return 1 + 160 * (chunks + ntiles) + 1024;
+    (void)ntiles;
+    const uint64_t chunks = (nleaf + BU_CHUNK - 1) / BU_CHUNK;
+    return chunks * (1 + BU_EXP_PER_CHUNK) + 64;
 }
 
 void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *work,
                      uint64_t work_words, uint32_t *err)
 {
+    (void)work_words;
     if (B.nnode == 0)
         return;
-    const uint32_t cap = (uint32_t)(work_words - 1);
     launch_fill_u32(s, T.nd_flag, B.nnode, 0u);
-    launch_fill_u32(s, work, 1, 0u);
     const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
+    uint32_t *exp_cnt = work, *exported = work + nch;
     hipLaunchKernelGGL(k_bottomup_local, dim3(nch < 1 ? 1 : (nch > 16384 ? 16384 : nch)), dim3(BU_TB),
-                       0, s, B, T, work + 1, work, cap, err);
-    hipLaunchKernelGGL(k_bottomup_global, dim3(1), dim3(BU_TB), 0, s, T, (const uint32_t *)(work + 1),
-                       (const uint32_t *)work, cap, err);
+                       0, s, B, T, exp_cnt, exported, err);
+    const uint32_t g = B.ntiles < 1 ? 1 : (B.ntiles > 4096 ? 4096 : B.ntiles);
+    hipLaunchKernelGGL(k_bottomup_global, dim3(g), dim3(BU_TB), 0, s, B, T, (const uint32_t *)exp_cnt,
+                       (const uint32_t *)exported, err);
 }
 
 }  // namespace owm
