@@ -108,7 +108,7 @@ def run_batch_bench(ctx, torch, synth, n_tiles, steps):
             "plain_ms": round(t_plain * 1e3, 3),
             "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same,
             "autotune_s": round(t_tune, 2),
-            "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k",
+            "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k", "summaries",
                                          "sort_bits")} for e in table if e["ms"] > 0]}
 
 
@@ -256,7 +256,7 @@ def main():
                                    "(sigma 3 m), Wsize 12, Overlap 0.8, Bsize 20",
                        "points": tile.n, "seeds": int(len(seeds)),
                        "leaf_size": leaf if leaf else (tuned["leaf_size"] if tuned else None),
-                       "tuned": {k: tuned[k] for k in ("leaf_size", "block_x", "block_y", "block_k",
+                       "tuned": {k: tuned[k] for k in ("leaf_size", "block_x", "block_y", "block_k", "summaries",
                                                        "sort_bits")} if tuned else None},
             "roofline": roofline, "phases": phases}
     if not args.no_batch:

@@ -31,7 +31,8 @@ struct TileDev {
     uint32_t win_off, cell_off, blk_off, nbx;
     uint32_t min_num_points, stage2_need;
     int32_t h, hb;
-    uint32_t bsize, pad0;
+    uint32_t bsize;
+    uint32_t lcs;  // stage 1 reads leaf cell summaries (else every leaf point by point)
     double displace, initx, inity, initxg, inityg;
     double inv_disp;  // 1 / displace (estimates only; exact compares follow)
 };
@@ -64,6 +65,17 @@ struct TreeDev {
     const uint32_t *nd_cnt, *nd_zid;
     const unsigned long long *nd_zkey;
     const uint2 *nd_child;
+};
+
+// Leaf cell summaries of stage 1: per leaf up to LCS_SLOTS aggregates of runs
+// of its points that share a global window-edge class in x and y.
+#define LCS_SLOTS 16u
+#define LCS_RAW 0xFFFFFFFFu  // the leaf has more classes than slots
+struct LeafCells {
+    uint32_t *ne;            // [nleaf] summaries of the leaf, or LCS_RAW
+    unsigned long long *kc;  // [nleaf * LCS_SLOTS] class key (gy << 16 | gx) | count << 32
+    uint32_t *id;            // lowest (z, index) of the summary
+    unsigned long long *zk;
 };
 
 #ifdef __HIPCC__

@@ -515,6 +515,8 @@ uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
     b += align_up(1024, 256);                                           // counters
     b += align_up(stage1_work_words(Q.nblk) * 4ull, 256);              // stage-1 join
     b += align_up((Q.ntiles + 1) * 4ull, 256);                          // seed_off
+    if (Q.lcs_leaves)
+        b += align_up(leaf_cells_bytes(Q.lcs_leaves), 256);             // leaf cell summaries
     return b;
 }
 
@@ -541,7 +543,9 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     uint32_t *cnt = (uint32_t *)scratch.take(1024);
     uint32_t *seed_off = (uint32_t *)scratch.take((Q.ntiles + 1) * 4ull);
     uint32_t *s1work = (uint32_t *)scratch.take(stage1_work_words(Q.nblk) * 4ull);
-    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off || !s1work) {
+    uint8_t *lcs_mem = Q.lcs_leaves ? scratch.take(leaf_cells_bytes(Q.lcs_leaves)) : nullptr;
+    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off || !s1work ||
+        (Q.lcs_leaves && !lcs_mem)) {
         set_err(ctx, "scratch arena exhausted (query)");
         return OWM_E_INTERNAL;
     }
@@ -552,7 +556,10 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     launch_fill_u32(s, cnt, 16, 0u);
     if (from <= 1) {
         int pi = ctx->prof.begin(s, PH_STAGE1);
-        launch_stage1(s, B, TV, cfg, s1work, (uint32_t *)ctx->s1_list.ptr,
+        LeafCells LC{};
+        if (lcs_mem)
+            LC = carve_leaf_cells(lcs_mem, Q.lcs_leaves);
+        launch_stage1(s, B, TV, cfg, lcs_mem ? &LC : nullptr, s1work, (uint32_t *)ctx->s1_list.ptr,
                       (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4), W.k0, cnt + 3,
                       cnt + 6);
         // every point read once (x, y, z, index) and one minimum written per window
@@ -670,10 +677,10 @@ uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested)
 Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total)
 {
     Stage1Cfg c = ctx->tuner.stage1_for(np_total);
-    // "bx,by,k" forces one window-block shape (for experiments)
+    // "bx,by,k,lcs" forces one stage-1 configuration (for experiments)
     if (const char *ov = getenv("OWM_S1_SHAPE")) {
         Stage1Cfg f{};
-        if (sscanf(ov, "%u,%u,%u", &f.bx, &f.by, &f.k) == 3 && stage1_shape_ok(f))
+        if (sscanf(ov, "%u,%u,%u,%u", &f.bx, &f.by, &f.k, &f.lcs) == 4 && stage1_shape_ok(f))
             c = f;
     }
     return c;
@@ -794,6 +801,28 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
         est += 1.5 * D.nleaf * fx * fy + 64;
     }
     P.Q.s1_cap = (uint64_t)std::min(est, 4.0e9);
+    // Leaf cell summaries pay off when a leaf spans few window-edge classes:
+    // estimate the classes a leaf of side sqrt(L / density) meets (edges every
+    // D, or every D/2 when left and right edges do not coincide).
+    P.Q.lcs_leaves = 0;
+    for (uint32_t t = 0; t < n_tiles; ++t) {
+        TileDev &D = P.td[t];
+        D.lcs = 0;
+        if (!P.cfg.lcs || D.nleaf == 0 || D.ncols == 0 || D.leaf_size > 64 || D.ncols > 16383 ||
+            D.nrows > 8191 || P.L.nleaf >= (1ull << 27))
+            continue;
+        const double dens = std::max(D.n / std::max(1e-9, (double)D.ncols * D.nrows * D.displace *
+                                                             D.displace), 1e-9);
+        const double side = std::sqrt(D.leaf_size / dens);
+        const double q = 2.0 * D.h / D.displace;
+        const double spacing = std::fabs(q - std::round(q)) < 1e-6 ? D.displace : D.displace / 2;
+        // a Morton-ordered leaf enters a class about twice on average
+        const double runs = 2.0 * (side / spacing + 1.0) * (side / spacing + 1.0);
+        if (runs <= LCS_SLOTS) {
+            D.lcs = 1;
+            P.Q.lcs_leaves = P.L.nleaf;
+        }
+    }
     // a join that overflowed before sets the floor (and keeps captured chains valid)
     if (ctx->s1_need_max)
         P.Q.s1_cap = std::max<uint64_t>(P.Q.s1_cap, ctx->s1_need_max + ctx->s1_need_max / 8 + 1024);

@@ -58,6 +58,7 @@ struct QueryLayout {
     uint32_t ntiles = 0;
     uint64_t nwin = 0, ncell = 0, nblk = 0;
     uint64_t s1_cap = 0;  // capacity of the stage-1 leaf/block join list
+    uint64_t lcs_leaves = 0;  // leaves with summary slots (0: no tile uses summaries)
     std::vector<uint32_t> win_off, cell_off, blk_off;
 };
 
@@ -96,7 +97,7 @@ struct Tuner {
     {
         table.clear();
         for (uint64_t m : CLASS_MAX)
-            table.push_back({m, {32, {8, 8, 4}, 32}, 0.0});
+            table.push_back({m, {32, {8, 8, 4, 0}, 32}, 0.0});
     }
     static int class_of(uint64_t n)
     {
@@ -127,7 +128,7 @@ struct Prepared {
     std::vector<std::array<double, 4>> box;
     TreeLayout L;
     QueryLayout Q;
-    Stage1Cfg cfg{8, 8, 4};
+    Stage1Cfg cfg{8, 8, 4, 0};
     bool any_need1 = false;
 };
 

@@ -54,15 +54,20 @@ void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_
 struct Stage1Cfg {
     uint32_t bx, by;  // window block shape
     uint32_t k;       // points per lane per binning round
+    uint32_t lcs;     // 1: leaf cell summaries where a tile's leaves are small enough
 };
 bool stage1_shape_ok(const Stage1Cfg &c);
 // work: stage1_work_words(nblk) u32; list: leaf ids per window block, capacity
 // cap.  *total_out receives the number of list entries needed; when it
 // exceeds cap, err gets bit 16 and the call must be repeated with a larger list.
 uint64_t stage1_work_words(uint64_t nblk);
+// lc: leaf cell summaries, built here for the tiles whose TileDev.lcs is set
+// (null when no tile uses them; only shapes with stage1_shape_ok(lcs=1)).
 void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *min_ids, uint32_t *err,
-                   uint32_t *total_out);
+                   const LeafCells *lc, uint32_t *work, uint32_t *list, uint32_t cap,
+                   uint32_t *min_ids, uint32_t *err, uint32_t *total_out);
+uint64_t leaf_cells_bytes(uint64_t nleaf);
+LeafCells carve_leaf_cells(uint8_t *base, uint64_t nleaf);
 // sorted window minima -> LLP list (ascending) + count
 void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
                            uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,

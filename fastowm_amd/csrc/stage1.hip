@@ -25,6 +25,7 @@ namespace owm {
 constexpr int S1_THREADS = 256;
 constexpr int S1_CHUNK = S1_THREADS;  // leaves whose metadata is staged at once
 constexpr uint32_t S1_LEAFTAG = 0x80000000u;
+constexpr uint32_t S1_POINT = 0xFFFFFFFEu;  // item is a point, classified after the loads
 
 // #{k : E[k] < x} and #{k : E[k] <= x} over a strictly increasing edge list
 // E[0..n) spaced ~1/inv apart: estimate, then correct with exact compares.
@@ -154,6 +155,139 @@ __global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t 
     }
 }
 
+// ---------------------------------------------------- leaf cell summaries --
+//
+// Global edge classes: over the multiset of all left edges L_i and right
+// edges R_i of a tile's windows (one axis), G(x) = #{e < x} + #{e <= x}.
+// Points with equal G lie in exactly the same windows along that axis, and
+// for every edge e of the multiset, e < x <=> G(e) < G(x) and e <= x <=>
+// G(e) <= G(x).  A window block's local class of x (the same count over the
+// block's own edges, a subset) is therefore a function of G(x) alone, and so
+// is "strictly inside the block" (G(BL) < G(x) < G(BR)).  A leaf's points
+// with equal (Gx, Gy) can thus be binned as one aggregate (count, lowest
+// (z, index)) by every block; k_lcs builds one summary per run of equal
+// classes in the leaf's Morton order (up to LCS_SLOTS), and stage 1 reads
+// them instead of the points (leaves with more runs stay point by point).
+
+__device__ __forceinline__ double edge_at(double init, double disp, double hd, int i, bool right)
+{
+    const double c = owm_wcenter(init, disp, (uint32_t)i);
+    return right ? __dadd_rn(c, hd) : __dsub_rn(c, hd);
+}
+
+// 2 #{edges of one side < x} + [x is such an edge]; the edges of one side
+// are strictly increasing in the window index.  The estimate from the step
+// is exact but for rounding near an edge, so two exact edge compares confirm
+// it and a walk fixes it otherwise.
+__device__ __forceinline__ uint32_t gside(double x, double init, double disp, double inv, double hd,
+                                          bool right, uint32_t n)
+{
+    const double f = (x - (right ? init + hd : init - hd)) * inv;
+    int k = f >= 0.0 ? (f < (double)n ? (int)f + 1 : (int)n) : 0;
+    const double elo = edge_at(init, disp, hd, k > 0 ? k - 1 : 0, right);
+    double ehi = edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
+    if (!((k == 0 || elo < x) && (k == (int)n || !(ehi < x)))) {
+        while (k > 0 && !(edge_at(init, disp, hd, k - 1, right) < x))
+            --k;
+        while (k < (int)n && edge_at(init, disp, hd, k, right) < x)
+            ++k;
+        ehi = edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
+    }
+    return 2u * (uint32_t)k + ((k < (int)n && ehi == x) ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t gclass(double x, double init, double disp, double inv, double hd,
+                                           uint32_t n)
+{
+    return gside(x, init, disp, inv, hd, false, n) + gside(x, init, disp, inv, hd, true, n);
+}
+
+// #{a[k] < g} + #{a[k] <= g} over a strictly increasing array of n classes
+__device__ __forceinline__ uint32_t icnt2(const uint32_t *a, uint32_t n, uint32_t g)
+{
+    uint32_t lo = 0, hi = n;  // first k with a[k] >= g
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < g)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return 2 * lo + (lo < n && a[lo] == g ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_lcs(BatchDev B, TreeDev T, LeafCells LC)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = gridDim.x * 4;
+    for (uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6); g < B.ngrp; g += nw) {
+        const TileDev &TD = B.tiles[owm_find_tile(B.grp_off, B.ntiles, g)];
+        if (!TD.lcs)
+            continue;
+        const uint32_t L = TD.leaf_size;  // <= 64 when summaries are on
+        const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane - sub * L;
+        const uint32_t jl = (g - TD.grp_off) * nl_grp + sub;
+        const bool active = sub < nl_grp && jl < TD.nleaf;
+        const uint32_t npts = active ? min(L, TD.n - jl * L) : 0u;
+        const bool in = off < npts;
+        const double hd = (double)(float)TD.h, inv = TD.inv_disp, disp = TD.displace;
+        uint32_t key = OWM_NONE, id = OWM_NONE, cnt = 0;
+        unsigned long long zk = OWM_ZKEY_NONE;
+        if (in) {
+            const uint32_t p = TD.pt_off + jl * L + off;
+            const double x = T.xs[p], y = T.ys[p], z = T.zs[p];
+            id = T.ids[p];
+            // points outside the outermost edges lie in no window
+            if (x > edge_at(TD.initx, disp, hd, 0, false) &&
+                x < edge_at(TD.initx, disp, hd, TD.ncols - 1, true) &&
+                y > edge_at(TD.inity, disp, hd, 0, false) &&
+                y < edge_at(TD.inity, disp, hd, TD.nrows - 1, true)) {
+                const uint32_t gx = gclass(x, TD.initx, disp, inv, hd, TD.ncols);
+                const uint32_t gy = gclass(y, TD.inity, disp, inv, hd, TD.nrows);
+                key = (gy << 16) | gx;
+                cnt = 1;
+                zk = owm_eligible(z) ? owm_zkey(z) : OWM_ZKEY_NONE;
+            }
+        }
+        // 1. runs of equal keys within a leaf (Morton order): segmented scan
+        const uint32_t prev = __shfl_up(key, 1, 64);
+        const bool head = in && (off == 0 || prev != key);
+        const unsigned long long heads = __ballot(head);
+        const uint32_t start = 63 - __clzll(heads & ((2ull << lane) - 1ull) | 1ull);
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t c2 = __shfl_up(cnt, o, 64);
+            const unsigned long long k2 = __shfl_up(zk, o, 64);
+            const uint32_t i2 = __shfl_up(id, o, 64);
+            if (in && lane >= start + o) {
+                cnt += c2;
+                if (owm_less(k2, i2, zk, id)) {
+                    zk = k2;
+                    id = i2;
+                }
+            }
+        }
+        const bool tail = in && (off == npts - 1 || (lane < 63 && ((heads >> (lane + 1)) & 1ull)));
+        // 2. every run's tail holds the run's summary; a key seen again later
+        // in the leaf simply gets another slot (the bins add up the same)
+        const bool emit = tail && key != OWM_NONE;
+        const unsigned long long emits = __ballot(emit);
+        const unsigned long long seg = (L == 64 ? ~0ull : ((1ull << L) - 1ull)) << (sub * L);
+        const uint32_t ne = (uint32_t)__popcll(emits & seg);
+        if (active) {
+            const uint32_t leaf = TD.leaf_off + jl;
+            if (emit && ne <= LCS_SLOTS) {
+                const size_t e = (size_t)leaf * LCS_SLOTS + __popcll(emits & seg & ((1ull << lane) - 1ull));
+                LC.kc[e] = (unsigned long long)key | ((unsigned long long)cnt << 32);
+                LC.id[e] = id;
+                LC.zk[e] = zk;
+            }
+            if (off == 0)
+                LC.ne[leaf] = ne <= LCS_SLOTS ? ne : LCS_RAW;
+        }
+    }
+}
+
 template <int BX, int BY>
 struct S1Shape {
     static constexpr int NXC = 4 * BX + 1;
@@ -161,13 +295,17 @@ struct S1Shape {
     static constexpr int NCELL = NXC * NYC;
     // the window pass arrays and the leaf chunk metadata are never live together
     static constexpr int M1 = NYC * BX;
-    static constexpr int META_BYTES = 2 * S1_CHUNK * 4;
+    static constexpr int META_BYTES = 3 * S1_CHUNK * 4;
     static constexpr int M1_BYTES = M1 * 16;
     static constexpr int UNION_BYTES = M1_BYTES > META_BYTES ? M1_BYTES : META_BYTES;
 };
 
-template <int BX, int BY, int S1_K>
-__global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
+constexpr uint32_t S1_ENT_LEAVES = 64 / LCS_SLOTS;  // summary leaves per unit of 64 lanes
+
+// USE_LCS compiles the leaf-summary items in (their registers cost occupancy
+// when no tile uses them).
+template <int BX, int BY, int S1_K, bool USE_LCS>
+__global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
                                                         const uint32_t *__restrict__ blk_start,
                                                         const uint32_t *__restrict__ list,
                                                         uint32_t cap,
@@ -176,16 +314,20 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
     using S = S1Shape<BX, BY>;
     constexpr int NXC = S::NXC;
     __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
+    __shared__ uint32_t sGLx[BX], sGRx[BX], sGLy[BY], sGRy[BY];  // global classes of the edges
     __shared__ uint32_t sCxl[BX], sCxh[BX], sCyl[BY], sCyh[BY];
     __shared__ uint32_t sCnt[S::NCELL];
     __shared__ unsigned long long sZ[S::NCELL];
     __shared__ uint32_t sId[S::NCELL];
     __shared__ uint16_t sTag[S::NCELL];  // round in which the cell's minimum z last dropped
+    __shared__ uint32_t sNraw, sNent;
     __shared__ __align__(16) unsigned char sUnion[S::UNION_BYTES];
-    // leaf chunk: first sorted position (or leaf index) and point count (or
-    // S1_LEAFTAG | class cell when the leaf is binned whole from its aggregate)
+    // leaf chunk: point leaves (first sorted position, and point count or
+    // S1_LEAFTAG | class cell when binned whole from the leaf aggregate), and
+    // summary leaves (leaf << 5 | number of summaries)
     uint32_t *sLfA = reinterpret_cast<uint32_t *>(sUnion);
     uint32_t *sLfB = sLfA + S1_CHUNK;
+    uint32_t *sEnt = sLfB + S1_CHUNK;
     unsigned long long *m1z = reinterpret_cast<unsigned long long *>(sUnion);
     uint32_t *m1cnt = reinterpret_cast<uint32_t *>(m1z + S::M1);
     uint32_t *m1id = m1cnt + S::M1;
@@ -199,16 +341,25 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
         const uint32_t ni = min((uint32_t)BX, TD.ncols - i0), nj = min((uint32_t)BY, TD.nrows - j0);
         const double hd = (double)(float)TD.h;
         const double inv = 1.0 / TD.displace;
+        const bool lcs = USE_LCS && TD.lcs != 0;
 
         if (tid < ni) {
             double c = owm_wcenter(TD.initx, TD.displace, i0 + tid);
             sLx[tid] = __dsub_rn(c, hd);
             sRx[tid] = __dadd_rn(c, hd);
+            if (lcs) {
+                sGLx[tid] = gclass(sLx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
+                sGRx[tid] = gclass(sRx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
+            }
         }
         if (tid >= 64 && tid < 64 + nj) {
             double c = owm_wcenter(TD.inity, TD.displace, j0 + tid - 64);
             sLy[tid - 64] = __dsub_rn(c, hd);
             sRy[tid - 64] = __dadd_rn(c, hd);
+            if (lcs) {
+                sGLy[tid - 64] = gclass(sLy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
+                sGRy[tid - 64] = gclass(sRy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
+            }
         }
         for (int c = tid; c < S::NCELL; c += S1_THREADS) {
             sCnt[c] = 0;
@@ -229,6 +380,7 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
             sCyh[tid - 64] = 2 * (cnt_lt(sLy, nj, r, inv) + cnt_lt(sRy, nj, r, inv));
         }
         const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
+        const uint32_t GBL = sGLx[0], GBR = sGRx[ni - 1], GBB = sGLy[0], GBT = sGRy[nj - 1];
 
         const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
         const uint32_t nl = s1 - s0;
@@ -237,32 +389,48 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
         const uint32_t ls = L <= 64 ? L : 64;
         const uint32_t nseg = L <= 64 ? 1 : (L + 63) / 64;
         const uint32_t sub = lane / ls, off = lane - sub * ls;
+        const uint32_t esub = lane / LCS_SLOTS, eslot = lane - esub * LCS_SLOTS;
 
         for (uint32_t ch = 0; ch < nl; ch += S1_CHUNK) {
             const uint32_t m = min((uint32_t)S1_CHUNK, nl - ch);
             // the class ranges (first chunk) and the previous chunk's rounds
             // are complete before the metadata is overwritten
+            if (tid == 0) {
+                sNraw = 0;
+                sNent = 0;
+            }
             __syncthreads();
             if (tid < m) {
                 const uint32_t leaf = list[s0 + ch + tid];
-                const uint32_t np = T.lf_npts[leaf], st = T.lf_start[leaf];
-                const double4 bx = T.lf_box[leaf];
-                uint32_t a = st, n = np;
-                // whole leaf inside one class cell: bin its aggregate (a class
-                // cell is never wider than the window step)
-                if (nseg == 1 && np > 0 && bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace &&
-                    bx.x > BL && bx.z < BR && bx.y > BB && bx.w < BT) {
-                    uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
-                    if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
-                        a = leaf;
-                        n = S1_LEAFTAG | (cy * NXC + cx);
+                const uint32_t ne = lcs ? LC.ne[leaf] : LCS_RAW;
+                if (ne != LCS_RAW) {
+                    if (ne)
+                        sEnt[atomicAdd(&sNent, 1u)] = (leaf << 5) | ne;
+                } else {
+                    const uint32_t np = T.lf_npts[leaf], st = T.lf_start[leaf];
+                    const double4 bx = T.lf_box[leaf];
+                    uint32_t a = st, n = np;
+                    // whole leaf inside one class cell: bin its aggregate (a class
+                    // cell is never wider than the window step)
+                    if (nseg == 1 && np > 0 && bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace &&
+                        bx.x > BL && bx.z < BR && bx.y > BB && bx.w < BT) {
+                        uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
+                        if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
+                            a = leaf;
+                            n = S1_LEAFTAG | (cy * NXC + cx);
+                        }
+                    }
+                    if (n) {
+                        const uint32_t k = atomicAdd(&sNraw, 1u);
+                        sLfA[k] = a;
+                        sLfB[k] = n;
                     }
                 }
-                sLfA[tid] = a;
-                sLfB[tid] = n;
             }
             __syncthreads();
-            const uint32_t nunits = (m + nl_grp - 1) / nl_grp * nseg;
+            const uint32_t mraw = sNraw, ment = sNent;
+            const uint32_t neu = (ment + S1_ENT_LEAVES - 1) / S1_ENT_LEAVES;
+            const uint32_t nunits = neu + (mraw + nl_grp - 1) / nl_grp * nseg;
             const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
 
             for (uint32_t r = 0; r < rounds; ++r) {
@@ -282,9 +450,28 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
                 for (int k = 0; k < S1_K; ++k) {
                     cell[k] = OWM_NONE;
                     const uint32_t u = (r * S1_K + k) * 4 + wave;
-                    const uint32_t g = u / nseg, seg = u - g * nseg;
+                    if (u >= nunits)
+                        continue;
+                    if (USE_LCS && u < neu) {
+                        // a leaf summary: one slot per lane
+                        const uint32_t li = u * S1_ENT_LEAVES + esub;
+                        if (li >= ment)
+                            continue;
+                        const uint32_t v = sEnt[li];
+                        if (eslot >= (v & 31u))
+                            continue;
+                        const size_t e = (size_t)(v >> 5) * LCS_SLOTS + eslot;
+                        const unsigned long long kc = LC.kc[e];
+                        zk[k] = LC.zk[e];
+                        idv[k] = LC.id[e];
+                        cn[k] = (uint32_t)(kc >> 32);
+                        cell[k] = S1_LEAFTAG | (uint32_t)kc;  // classes, mapped below
+                        continue;
+                    }
+                    const uint32_t uu = u - neu;
+                    const uint32_t g = uu / nseg, seg = uu - g * nseg;
                     const uint32_t li = g * nl_grp + sub;
-                    if (u >= nunits || sub >= nl_grp || li >= m)
+                    if (sub >= nl_grp || li >= mraw)
                         continue;
                     const uint32_t a = sLfA[li], n = sLfB[li];
                     const uint32_t o = seg * 64 + off;
@@ -301,14 +488,14 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
                         yv[k] = T.ys[p];
                         zv[k] = T.zs[p];
                         idv[k] = T.ids[p];
-                        cell[k] = OWM_LEAF_BIT;  // a point, classified below
+                        cell[k] = S1_POINT;
                     }
                 }
 #pragma unroll
                 for (int k = 0; k < S1_K; ++k) {
                     if (cell[k] == OWM_NONE)
                         continue;
-                    if (cell[k] == OWM_LEAF_BIT) {
+                    if (cell[k] == S1_POINT) {
                         const double x = xv[k], y = yv[k];
                         if (!(x > BL && x < BR && y > BB && y < BT)) {
                             cell[k] = OWM_NONE;
@@ -317,6 +504,14 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T,
                         cell[k] = cls(sLy, sRy, nj, y, inv) * NXC + cls(sLx, sRx, ni, x, inv);
                         cn[k] = 1;
                         zk[k] = owm_eligible(zv[k]) ? owm_zkey(zv[k]) : OWM_ZKEY_NONE;
+                    } else if (USE_LCS && (cell[k] & S1_LEAFTAG)) {
+                        const uint32_t gx = cell[k] & 0xFFFFu, gy = (cell[k] >> 16) & 0x7FFFu;
+                        if (!(gx > GBL && gx < GBR && gy > GBB && gy < GBT)) {
+                            cell[k] = OWM_NONE;
+                            continue;
+                        }
+                        cell[k] = (icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy)) * NXC +
+                                  icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
                     }
                     const uint32_t c = cell[k];
                     atomicAdd(&sCnt[c], cn[k]);
@@ -483,29 +678,62 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t *__restri
 
 uint64_t stage1_work_words(uint64_t nblk) { return 2 * (nblk + 1) + (nblk / SC_TILE + 2) + 8; }
 
-template <int BX, int BY, int K>
-static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, const uint32_t *start,
-                         const uint32_t *list, uint32_t cap, uint32_t *min_ids)
+template <int BX, int BY, int K, bool LCS>
+static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, const LeafCells &LC,
+                         const uint32_t *start, const uint32_t *list, uint32_t cap, uint32_t *min_ids)
 {
     uint32_t grid = B.nblk < 1 ? 1 : (B.nblk > 65536 ? 65536 : B.nblk);
-    hipLaunchKernelGGL((k_stage1<BX, BY, K>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, start, list,
-                       cap, min_ids);
+    hipLaunchKernelGGL((k_stage1<BX, BY, K, LCS>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, LC,
+                       start, list, cap, min_ids);
 }
 
 bool stage1_shape_ok(const Stage1Cfg &c)
 {
+    if (c.lcs > 1)
+        return false;
     if (c.bx == 8 && c.by == 8)
-        return c.k == 2 || c.k == 4 || c.k == 8;
-    return c.k == 4 && ((c.bx == 16 && c.by == 8) || (c.bx == 16 && c.by == 16) ||
-                        (c.bx == 32 && c.by == 8));
+        return c.k == 4 || (!c.lcs && (c.k == 2 || c.k == 8));
+    if (c.bx == 16 && c.by == 8)
+        return c.k == 4;
+    return !c.lcs && c.k == 4 && ((c.bx == 16 && c.by == 16) || (c.bx == 32 && c.by == 8));
+}
+
+static uint64_t al256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+uint64_t leaf_cells_bytes(uint64_t nleaf)
+{
+    const uint64_t ns = nleaf * LCS_SLOTS;
+    return al256(nleaf * 4) + al256(ns * 8) + al256(ns * 4) + al256(ns * 8);
+}
+
+LeafCells carve_leaf_cells(uint8_t *base, uint64_t nleaf)
+{
+    const uint64_t ns = nleaf * LCS_SLOTS;
+    LeafCells c;
+    uint64_t o = 0;
+    c.ne = (uint32_t *)(base + o);
+    o += al256(nleaf * 4);
+    c.kc = (unsigned long long *)(base + o);
+    o += al256(ns * 8);
+    c.id = (uint32_t *)(base + o);
+    o += al256(ns * 4);
+    c.zk = (unsigned long long *)(base + o);
+    return c;
 }
 
 void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *min_ids, uint32_t *err,
-                   uint32_t *total_out)
+                   const LeafCells *lc, uint32_t *work, uint32_t *list, uint32_t cap,
+                   uint32_t *min_ids, uint32_t *err, uint32_t *total_out)
 {
     if (B.nblk == 0)
         return;
+    LeafCells LC{};
+    if (lc) {
+        LC = *lc;
+        uint32_t g = (B.ngrp + 3) / 4;
+        g = g < 1 ? 1 : (g > 16384 ? 16384 : g);
+        hipLaunchKernelGGL(k_lcs, dim3(g), dim3(256), 0, s, B, T, LC);
+    }
     // work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter
     uint32_t *cnt = work;
     uint32_t *start = work + (B.nblk + 1);
@@ -524,18 +752,25 @@ void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Sta
     hipLaunchKernelGGL(k_s1_bin<1>, dim3(g), dim3(256), 0, s, B, T, cfg.bx, cfg.by, cnt,
                        (const uint32_t *)start, list, cap, err);
     launch_add_u32(s, start + B.nblk, counter + 1, total_out);
-    if (cfg.bx == 16 && cfg.by == 8)
-        stage1_shape<16, 8, 4>(s, B, T, start, list, cap, min_ids);
-    else if (cfg.bx == 16 && cfg.by == 16)
-        stage1_shape<16, 16, 4>(s, B, T, start, list, cap, min_ids);
-    else if (cfg.bx == 32 && cfg.by == 8)
-        stage1_shape<32, 8, 4>(s, B, T, start, list, cap, min_ids);
-    else if (cfg.k == 2)
-        stage1_shape<8, 8, 2>(s, B, T, start, list, cap, min_ids);
-    else if (cfg.k == 8)
-        stage1_shape<8, 8, 8>(s, B, T, start, list, cap, min_ids);
-    else
-        stage1_shape<8, 8, 4>(s, B, T, start, list, cap, min_ids);
+    const bool use = lc != nullptr;
+    if (cfg.bx == 16 && cfg.by == 8) {
+        if (use)
+            stage1_shape<16, 8, 4, true>(s, B, T, LC, start, list, cap, min_ids);
+        else
+            stage1_shape<16, 8, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (cfg.bx == 16 && cfg.by == 16) {
+        stage1_shape<16, 16, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (cfg.bx == 32 && cfg.by == 8) {
+        stage1_shape<32, 8, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (cfg.k == 2) {
+        stage1_shape<8, 8, 2, false>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (cfg.k == 8) {
+        stage1_shape<8, 8, 8, false>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (use) {
+        stage1_shape<8, 8, 4, true>(s, B, T, LC, start, list, cap, min_ids);
+    } else {
+        stage1_shape<8, 8, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+    }
 }
 
 }  // namespace owm

@@ -61,7 +61,8 @@ class TileDesc(ctypes.Structure):
 class TuneEntry(ctypes.Structure):
     _fields_ = [("max_n", ctypes.c_uint64), ("leaf_size", ctypes.c_uint32),
                 ("block_x", ctypes.c_uint32), ("block_y", ctypes.c_uint32),
-                ("block_k", ctypes.c_uint32), ("sort_bits", ctypes.c_uint32),
+                ("block_k", ctypes.c_uint32), ("summaries", ctypes.c_uint32),
+                ("sort_bits", ctypes.c_uint32),
                 ("ms", ctypes.c_double)]
 
     def as_dict(self):

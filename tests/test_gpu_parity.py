@@ -211,3 +211,60 @@ def test_errors(ctx):
     # the context stays usable after errors
     want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
     assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 32), want)
+
+
+def _lattice_cloud(n, seed, side=200):
+    """Integer coordinates over [0, side]: with Wsize 10, Overlap 0.9 (step 1 m)
+    every window edge is an integer, so many points lie exactly on edges."""
+    rng = np.random.default_rng(seed)
+    x = rng.integers(0, side + 1, n).astype(np.float64)
+    y = rng.integers(0, side + 1, n).astype(np.float64)
+    z = np.round(rng.normal(50.0, 3.0, n), 1)  # 0.1 m steps: many equal z
+    x[0], y[0], x[1], y[1] = 0.0, 0.0, float(side), float(side)  # pin the bbox
+    return x, y, z
+
+
+def _stage1_table(ctx, summaries, block=(16, 8, 4)):
+    entries = ctx.tune_table()
+    for e in entries:
+        e["block_x"], e["block_y"], e["block_k"] = block
+        e["summaries"] = 1 if summaries == "on" else 0
+    ctx.set_tune_table(entries)
+
+
+@pytest.fixture
+def restore_table(ctx):
+    saved = ctx.tune_table()
+    yield
+    ctx.set_tune_table(saved)
+
+
+@pytest.mark.parametrize("summaries", ["on", "off"])
+@pytest.mark.parametrize("ws,bs,ov,leaf", [(10, 20, 0.9, 16), (10, 15, 0.5, 32), (12, 20, 0.8, 64),
+                                           (7, 11, 0.5, 8)])
+def test_points_on_edges_with_and_without_summaries(ctx, restore_table, summaries, ws, bs, ov, leaf):
+    """Stage 1 with leaf cell summaries (classes from the global edge multiset)
+    and point by point must agree with the brute-force window scan, also where
+    points sit exactly on window edges and z ties."""
+    _stage1_table(ctx, summaries, (16, 8, 4) if leaf % 2 else (8, 8, 4))
+    say("lattice", summaries, ws, bs, ov, leaf)
+    x, y, z = _lattice_cloud(40_000, seed=ws * 100 + leaf)
+    box = synth.bbox(x, y)
+    p = O.params(ws, bs, ov, box, len(x), leaf)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=leaf)
+    gm, _ = ctx.stage1(tree, ws, ov, bs)
+    assert np.array_equal(gm, O.stage1_brute(x, y, z, p))
+    tree.free()
+    want, _, _ = O.run(x, y, z, ws, bs, ov, leaf)
+    assert np.array_equal(ctx.run_tile(x, y, z, ws, bs, ov, leaf), want)
+
+
+@pytest.mark.parametrize("summaries", ["on", "off"])
+def test_c2_scaled_with_and_without_summaries(ctx, restore_table, summaries):
+    _stage1_table(ctx, summaries)
+    say("c2 scaled", summaries)
+    t = synth.scaled(synth.config_c2(seed=3), 2_000_000)
+    x, y, z = t.generate()
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    for leaf in (16, 32, 64):
+        assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, leaf), want), leaf

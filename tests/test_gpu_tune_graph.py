@@ -76,7 +76,8 @@ def test_autotune_keeps_results_and_table_round_trips():
         for e in table:
             assert e["leaf_size"] in (16, 32, 64, 128)
             assert e["sort_bits"] in (24, 32)
-            assert (e["block_x"], e["block_y"], e["block_k"]) in ((8, 8, 4), (16, 8, 4), (8, 8, 2))
+            assert (e["block_x"], e["block_y"], e["block_k"], e["summaries"]) in (
+                (8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1))
         # the classes that held tiles were timed
         assert all(table[c]["ms"] > 0 for c in (0, 1, 2, 3))
         say("tuned batch equals untuned")
@@ -92,8 +93,9 @@ def test_autotune_keeps_results_and_table_round_trips():
         assert e.value.name == "E_ARG"
 
 
-@pytest.mark.parametrize("sort_bits,leaf,block", [(24, 16, (8, 8, 2)), (24, 128, (16, 8, 4)),
-                                                  (32, 64, (16, 8, 4))])
+@pytest.mark.parametrize("sort_bits,leaf,block", [(24, 16, (8, 8, 2, 0)), (24, 128, (16, 8, 4, 0)),
+                                                  (32, 64, (16, 8, 4, 0)), (24, 32, (16, 8, 4, 1)),
+                                                  (32, 16, (8, 8, 4, 1))])
 def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
     say("table entry", sort_bits, leaf, block)
     t = synth.scaled(synth.config_c2(seed=5), 700_000)
@@ -104,7 +106,7 @@ def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
         entries = ctx.tune_table()
         for e in entries:
             e["sort_bits"], e["leaf_size"] = sort_bits, leaf
-            e["block_x"], e["block_y"], e["block_k"] = block
+            e["block_x"], e["block_y"], e["block_k"], e["summaries"] = block
         ctx.set_tune_table(entries)
         assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 0), want)
         got = ctx.batch_run(x2, y2, z2, desc)
