@@ -242,9 +242,35 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
     const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
     const uint32_t shift = 64 - B.key_bits;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
+        const uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
         uint32_t t = owm_find_tile(B.pt_off, B.ntiles, c0);
         uint32_t next = B.pt_off[t + 1];
+        if (c1 <= next) {
+            // one tile: eight independent point loads in flight per thread
+            const TileDev &T = B.tiles[t];
+            for (uint32_t base = c0 + threadIdx.x; base < c1; base += 8 * TB) {
+                double xv[8], yv[8], zv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t i = base + j * TB;
+                    xv[j] = i < c1 ? X[i] : 0.0;
+                    yv[j] = i < c1 ? Y[i] : 0.0;
+                    zv[j] = i < c1 ? Z[i] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t i = base + j * TB;
+                    if (i < c1) {
+                        const uint32_t k = T.tile_key | (uint32_t)(morton64(T, xv[j], yv[j]) >> shift);
+                        keys[i] = k;
+                        aos[i] = make_double4(xv[j], yv[j], zv[j], 0.0);
+                        for (int p = 0; p < passes; ++p)
+                            atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+                    }
+                }
+            }
+            continue;
+        }
         for (uint32_t i = c0 + threadIdx.x; i < c1; i += TB) {
             while (i >= next)
                 next = B.pt_off[++t + 1];
