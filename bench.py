@@ -26,6 +26,34 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# kernel family behind each timed phase, and the algorithmic bytes per unit the
+# library's phase timers count (the per-launch figure is that times the units)
+PHASE_KERNEL = {"stage1": "owm::k_stage1", "gather_leaves": "owm::k_gather_leaves",
+                "sort_tree_pass": "owm::k_onesweep", "morton": "owm::k_morton",
+                "bbox": "owm::k_bbox", "bottomup": "owm::k_bottomup_local+global",
+                "stage1_join": "owm::k_s1_bin+k_scan_excl"}
+PHASE_BYTES = {"stage1": "24 B x N x Wsize/Displace + 4 B x windows",
+               "gather_leaves": "76 B x N + 48 B x leaves", "sort_tree_pass": "24 B x N per pass",
+               "morton": "28 B x N", "bbox": "16 B x N", "bottomup": "120 B x leaves",
+               "karras": "24 B x leaves", "sort_windows_pass": "12 B x windows per pass",
+               "stage2_compact": "8 B x windows", "stage1_join": "72 B x leaves + 12 B x blocks"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by the copy-kernel factor),
+    or None when no such measurement is committed."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    k = t.get("kernels", {}).get(kernel)
+    if k is None:
+        return None, None
+    return int(k["bytes_per_launch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -177,6 +205,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 512-tile batch leg (for profiling it)")
     ap.add_argument("--leaf", type=int, default=0,
                     help="leaf size (maxNumber) of the timed tile; 0 = autotuned")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -202,6 +232,12 @@ def main():
     import numpy as np
     import torch
     from fastowm_amd import owm, synth
+
+    if args.batch_only:
+        with owm.Context(0) as bctx:
+            print(json.dumps({"batch": run_batch_bench(bctx, torch, synth, args.batch_tiles,
+                                                       max(1, args.steps // 4))}), flush=True)
+        return
 
     tile = synth.config_c2(seed=0)
     log("generating", tile.name)
@@ -243,10 +279,18 @@ def main():
     per_launch_bytes = dp["bytes"] / dp["calls"]
     per_launch_s = dp["ms"] * 1e-3 / dp["calls"]
     achieved = per_launch_bytes / per_launch_s / 1e9
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
+    kernel = PHASE_KERNEL.get(dname, dname)
+    if dname == "stage1" and tuned:
+        kernel = "owm::k_stage1<%d, %d, %d, %s>" % (tuned["block_x"], tuned["block_y"],
+                                                    tuned["block_k"],
+                                                    "true" if tuned["summaries"] else "false")
+    traffic, traffic_src = measured_traffic(kernel)
+    roofline = {"bound": "hbm", "kernel": kernel, "phase": dname, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "launch_ms": round(per_launch_s * 1e3, 4),
-                "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+                "traffic": traffic, "traffic_source": traffic_src,
+                "launch_ms": round(per_launch_s * 1e3, 4),
+                "algorithmic_bytes_per_launch": int(per_launch_bytes),
+                "bytes_per_unit": PHASE_BYTES.get(dname)}
 
     line = {"metric": "Mpoints/s end-to-end OWM (tree + stages 1-3), plain GPU, 20M-point tile",
             "value": round(value, 1), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,

@@ -45,7 +45,8 @@ static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a
 
 static const char *PHASE_NAMES[PH_COUNT] = {
     "bbox", "morton", "sort_tree_pass", "gather_leaves", "karras", "bottomup", "stage1",
-    "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds"};
+    "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
+    "stage1_summaries"};
 
 const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
 
@@ -448,8 +449,10 @@ static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_
         launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
                         S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
                         iota && p == 0, true, 0);
-        // per key: read key (+ value unless generated), write key (+ value)
-        uint64_t per = keys_only ? 8 : ((iota && p == 0) ? 12 : 16);
+        // algorithmic bytes per key and pass: 24 for the tree's (key, index)
+        // pairs, 12 for the window minima (with the run-length pass, 44 per
+        // window over three passes), 8 for the seed merge (keys only)
+        uint64_t per = phase == PH_SORT_TREE ? 24 : (phase == PH_SORT_WIN ? 12 : 8);
         ctx->prof.end(s, pi, per * n_max, n_max);
         std::swap(kin, kout);
         std::swap(vin, vout);
@@ -481,21 +484,23 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     launch_fill_u32(s, S.hist, 4 * 256, 0u);
     int pi = ctx->prof.begin(s, PH_MORTON);
     launch_morton(s, B, dx, dy, dz, S.k0, aos, S.hist, passes);
-    ctx->prof.end(s, pi, 60ull * L.np, L.np);  // read x, y, z; write key + 32-byte record
+    // algorithmic bytes per point: read x, y; write a key and an index (the
+    // kernel also writes the 32-byte record the gather reads)
+    ctx->prof.end(s, pi, 28ull * L.np, L.np);
     int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
     const uint32_t *skeys = which ? S.k1 : S.k0;
     const uint32_t *svals = which ? S.v1 : S.v0;
     pi = ctx->prof.begin(s, PH_GATHER);
     launch_gather_leaves(s, B, svals, skeys, aos, T);
-    // per point: read index + 32-byte record, write x,y,z + index (+ inverse);
-    // per leaf: key + record
-    ctx->prof.end(s, pi, (T.pos ? 68ull : 64ull) * L.np + 68ull * L.nleaf, L.np);
+    // algorithmic bytes: gather 52 per point (index and x, y, z read, x, y, z
+    // written), leaves 24 per point + 48 per leaf
+    ctx->prof.end(s, pi, 76ull * L.np + 48ull * L.nleaf, L.np);
     pi = ctx->prof.begin(s, PH_KARRAS);
     launch_karras(s, B, T);
-    ctx->prof.end(s, pi, 24ull * L.nnode, L.nnode);
+    ctx->prof.end(s, pi, 24ull * L.nleaf, L.nnode);  // with the bottom-up: 144 per leaf
     pi = ctx->prof.begin(s, PH_BOTTOMUP);
     launch_bottomup(s, B, T, bu_work, bu_words, guard);
-    ctx->prof.end(s, pi, 4ull * L.nleaf + 160ull * L.nnode, L.nnode);
+    ctx->prof.end(s, pi, 120ull * L.nleaf, L.nnode);
     HIP_TRY(hipGetLastError(), "tree build launch");
     return OWM_OK;
 }
@@ -555,15 +560,24 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     out.seed_off = seed_off;
     launch_fill_u32(s, cnt, 16, 0u);
     if (from <= 1) {
-        int pi = ctx->prof.begin(s, PH_STAGE1);
+        const uint32_t s1cap = (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4);
+        uint32_t *list = (uint32_t *)ctx->s1_list.ptr;
+        int pi = ctx->prof.begin(s, PH_S1_JOIN);
+        launch_stage1_join(s, B, TV, cfg, s1work, list, s1cap, cnt + 3, cnt + 6);
+        // per leaf: bbox + count read by both join passes; per block: count, start
+        ctx->prof.end(s, pi, 72ull * B.nleaf + 12ull * Q.nblk, B.nleaf);
         LeafCells LC{};
-        if (lcs_mem)
+        if (lcs_mem) {
             LC = carve_leaf_cells(lcs_mem, Q.lcs_leaves);
-        launch_stage1(s, B, TV, cfg, lcs_mem ? &LC : nullptr, s1work, (uint32_t *)ctx->s1_list.ptr,
-                      (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4), W.k0, cnt + 3,
-                      cnt + 6);
-        // every point read once (x, y, z, index) and one minimum written per window
-        ctx->prof.end(s, pi, 28ull * np + 4ull * Q.nwin, np);
+            pi = ctx->prof.begin(s, PH_LCS);
+            launch_leaf_cells(s, B, TV, LC);
+            ctx->prof.end(s, pi, 28ull * np + 4ull * B.nleaf, np);  // points read once
+        }
+        pi = ctx->prof.begin(s, PH_STAGE1);
+        launch_stage1_bins(s, B, TV, cfg, lcs_mem ? &LC : nullptr, s1work, list, s1cap, W.k0);
+        // the algorithmic stage-1 traffic: every point streamed once per covering
+        // window row (24 B x Wsize/Displace) and one minimum written per window
+        ctx->prof.end(s, pi, Q.s1_alg_bytes + 4ull * Q.nwin, np);
         out.min_ids = W.k0;
     }
     if (to <= 1) {
@@ -578,7 +592,7 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         out.sorted_min_ids = sorted;
         int pi = ctx->prof.begin(s, PH_STAGE2);
         launch_stage2_compact(s, B, sorted, (uint32_t)nwin, any_need1, SD.k0, cnt + 0, lb1, cnt + 4);
-        ctx->prof.end(s, pi, 4ull * nwin, nwin);
+        ctx->prof.end(s, pi, 8ull * nwin, nwin);
     } else {
         if (n_in)
             HIP_TRY(hipMemcpyAsync(SD.k0, h_in, n_in * 4ull, hipMemcpyHostToDevice, s), "llp H2D");
@@ -770,6 +784,7 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
 {
     const uint32_t n_tiles = (uint32_t)P.td.size();
     P.any_need1 = false;
+    double s1_alg = 0.0;
     for (uint32_t t = 0; t < n_tiles; ++t) {
         const owm_params &pp = *(const owm_params *)((const uint8_t *)params + (size_t)t * stride_bytes);
         Derived D;
@@ -782,7 +797,9 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
         }
         fill_query(P.td[t], pp, D);
         P.any_need1 |= pp.overlap == 0.0;
+        s1_alg += 24.0 * (double)P.td[t].n * ((double)pp.wsize / D.displace);
     }
+    P.Q.s1_alg_bytes = (uint64_t)s1_alg;
     P.cfg = pick_stage1(ctx, np_total);
     P.Q.ntiles = n_tiles;
     plan_query(P.td, P.cfg, P.Q);

@@ -59,6 +59,7 @@ struct QueryLayout {
     uint64_t nwin = 0, ncell = 0, nblk = 0;
     uint64_t s1_cap = 0;  // capacity of the stage-1 leaf/block join list
     uint64_t lcs_leaves = 0;  // leaves with summary slots (0: no tile uses summaries)
+    uint64_t s1_alg_bytes = 0;  // sum over tiles of 24 n Wsize / Displace
     std::vector<uint32_t> win_off, cell_off, blk_off;
 };
 
@@ -189,7 +190,7 @@ struct ServeState;
 
 enum Phase {
     PH_BBOX, PH_MORTON, PH_SORT_TREE, PH_GATHER, PH_KARRAS, PH_BOTTOMUP, PH_STAGE1,
-    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_COUNT
+    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_COUNT
 };
 
 const char *phase_name(int ph);

@@ -57,15 +57,21 @@ struct Stage1Cfg {
     uint32_t lcs;     // 1: leaf cell summaries where a tile's leaves are small enough
 };
 bool stage1_shape_ok(const Stage1Cfg &c);
-// work: stage1_work_words(nblk) u32; list: leaf ids per window block, capacity
-// cap.  *total_out receives the number of list entries needed; when it
-// exceeds cap, err gets bit 16 and the call must be repeated with a larger list.
 uint64_t stage1_work_words(uint64_t nblk);
-// lc: leaf cell summaries, built here for the tiles whose TileDev.lcs is set
-// (null when no tile uses them; only shapes with stage1_shape_ok(lcs=1)).
-void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   const LeafCells *lc, uint32_t *work, uint32_t *list, uint32_t cap,
-                   uint32_t *min_ids, uint32_t *err, uint32_t *total_out);
+// Stage 1 in three launches.  The join lists, per window block, the leaves
+// whose bbox meets it (work: stage1_work_words(nblk) u32; list capacity cap;
+// *total_out receives the entries needed, and bit 16 of err is set when they
+// exceed cap, after which the call must be repeated with a larger list).  The
+// leaf cell summaries are built for the tiles whose TileDev.lcs is set.  The
+// bins write one minimum index per window (lc null: no tile uses summaries;
+// else only shapes with stage1_shape_ok(lcs=1)).
+void launch_stage1_join(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                        uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *err,
+                        uint32_t *total_out);
+void launch_leaf_cells(hipStream_t s, const BatchDev &B, const TreeDev &T, const LeafCells &lc);
+void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                        const LeafCells *lc, const uint32_t *work, const uint32_t *list,
+                        uint32_t cap, uint32_t *min_ids);
 uint64_t leaf_cells_bytes(uint64_t nleaf);
 LeafCells carve_leaf_cells(uint8_t *base, uint64_t nleaf);
 // sorted window minima -> LLP list (ascending) + count

@@ -721,20 +721,20 @@ LeafCells carve_leaf_cells(uint8_t *base, uint64_t nleaf)
     return c;
 }
 
-void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
-                   const LeafCells *lc, uint32_t *work, uint32_t *list, uint32_t cap,
-                   uint32_t *min_ids, uint32_t *err, uint32_t *total_out)
+void launch_leaf_cells(hipStream_t s, const BatchDev &B, const TreeDev &T, const LeafCells &lc)
+{
+    uint32_t g = (B.ngrp + 3) / 4;
+    g = g < 1 ? 1 : (g > 16384 ? 16384 : g);
+    hipLaunchKernelGGL(k_lcs, dim3(g), dim3(256), 0, s, B, T, lc);
+}
+
+// work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter
+void launch_stage1_join(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                        uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *err,
+                        uint32_t *total_out)
 {
     if (B.nblk == 0)
         return;
-    LeafCells LC{};
-    if (lc) {
-        LC = *lc;
-        uint32_t g = (B.ngrp + 3) / 4;
-        g = g < 1 ? 1 : (g > 16384 ? 16384 : g);
-        hipLaunchKernelGGL(k_lcs, dim3(g), dim3(256), 0, s, B, T, LC);
-    }
-    // work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter
     uint32_t *cnt = work;
     uint32_t *start = work + (B.nblk + 1);
     uint32_t *lookback = start + (B.nblk + 1);
@@ -752,6 +752,18 @@ void launch_stage1(hipStream_t s, const BatchDev &B, const TreeDev &T, const Sta
     hipLaunchKernelGGL(k_s1_bin<1>, dim3(g), dim3(256), 0, s, B, T, cfg.bx, cfg.by, cnt,
                        (const uint32_t *)start, list, cap, err);
     launch_add_u32(s, start + B.nblk, counter + 1, total_out);
+}
+
+void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
+                        const LeafCells *lc, const uint32_t *work, const uint32_t *list,
+                        uint32_t cap, uint32_t *min_ids)
+{
+    if (B.nblk == 0)
+        return;
+    LeafCells LC{};
+    if (lc)
+        LC = *lc;
+    const uint32_t *start = work + (B.nblk + 1);
     const bool use = lc != nullptr;
     if (cfg.bx == 16 && cfg.by == 8) {
         if (use)
