@@ -109,6 +109,26 @@ owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_o
     return OWM_OK;
 }
 
+owm_status owm_query_seed_count(const owm_params *params, uint64_t n, const double *bbox_or_null,
+                                uint64_t *max_seeds)
+{
+    if (!params || !max_seeds)
+        return OWM_E_ARG;
+    const double *b = (params->flags & OWM_F_HAVE_BBOX) ? params->bbox : bbox_or_null;
+    if (!b) {
+        *max_seeds = n;
+        return OWM_OK;
+    }
+    Derived D;
+    std::string why;
+    owm_status st = derive_params(*params, n, b, D, why);
+    if (st != OWM_OK)
+        return st;
+    const uint64_t w = (uint64_t)D.ncols * D.nrows + (uint64_t)D.ncolsg * D.nrowsg;
+    *max_seeds = w < n ? w : n;
+    return OWM_OK;
+}
+
 }  // extern "C"
 
 namespace owm {

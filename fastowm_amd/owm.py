@@ -98,6 +98,7 @@ def lib():
             "owm_status_str": ([ctypes.c_int], ctypes.c_char_p),
             "owm_ctx_device_bytes": ([vp], u64),
             "owm_derive": ([P(Params), u64, dp, P(Derived)], ctypes.c_int),
+            "owm_query_seed_count": ([P(Params), u64, dp, P(u64)], ctypes.c_int),
             "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_run_tile_device": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_batch_run": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp], ctypes.c_int),
@@ -142,6 +143,18 @@ def make_params(wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None) -> Para
         for k in range(4):
             p.bbox[k] = float(bbox[k])
     return p
+
+
+def query_seed_count(wsize, bsize, overlap, n, bbox=None) -> int:
+    """Upper bound on the seeds of an n-point tile (seed-buffer sizing)."""
+    p = make_params(wsize, bsize, overlap, 0, None)
+    out = ctypes.c_uint64()
+    bb = (ctypes.c_double * 4)(*bbox) if bbox is not None else None
+    st = lib().owm_query_seed_count(ctypes.byref(p), n, ctypes.cast(bb, ctypes.c_void_p) if bb else None,
+                                    ctypes.byref(out))
+    if st != 0:
+        raise OWMError(st, "owm_query_seed_count")
+    return out.value
 
 
 def derive(wsize, bsize, overlap, n, bbox) -> Derived:

@@ -94,6 +94,12 @@ owm_status owm_profile_read(owm_ctx *ctx, owm_phase_stat *out, uint32_t cap, uin
 /* Parameter derivation only (no device work): owm_optim3.cpp:92-132. */
 owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_or_null,
                       owm_derived *out);
+/* Seed-buffer sizing: an upper bound on the seeds of an n-point tile,
+ * min(n, ncols*nrows + ncolsg*nrowsg) (every seed is a window minimum or a
+ * stage-3 addition); n when neither params nor bbox_or_null give a bbox.  The
+ * reference sizes its minIDs buffer from nCols*nRows (owm_optim3.cpp:134). */
+owm_status owm_query_seed_count(const owm_params *params, uint64_t n, const double *bbox_or_null,
+                                uint64_t *max_seeds);
 
 /* ---- whole pipeline, one tile -------------------------------------------
  * Replaces the timed loop of the reference driver: tree construction

