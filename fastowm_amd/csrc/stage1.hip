@@ -293,11 +293,10 @@ struct S1Shape {
     static constexpr int NXC = 4 * BX + 1;
     static constexpr int NYC = 4 * BY + 1;
     static constexpr int NCELL = NXC * NYC;
-    // the window pass arrays and the leaf chunk metadata are never live together
+    // the x-pass results (class row x window column) reuse the cell arrays
     static constexpr int M1 = NYC * BX;
-    static constexpr int META_BYTES = 3 * S1_CHUNK * 4;
-    static constexpr int M1_BYTES = M1 * 16;
-    static constexpr int UNION_BYTES = M1_BYTES > META_BYTES ? M1_BYTES : META_BYTES;
+    static constexpr int M1_PER_THREAD = (M1 + S1_THREADS - 1) / S1_THREADS;
+    static_assert(M1 <= NCELL, "x-pass results fit the cell arrays");
 };
 
 constexpr uint32_t S1_ENT_LEAVES = 64 / LCS_SLOTS;  // summary leaves per unit of 64 lanes
@@ -319,18 +318,15 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
     __shared__ uint32_t sCnt[S::NCELL];
     __shared__ unsigned long long sZ[S::NCELL];
     __shared__ uint32_t sId[S::NCELL];
-    __shared__ uint16_t sTag[S::NCELL];  // round in which the cell's minimum z last dropped
     __shared__ uint32_t sNraw, sNent;
-    __shared__ __align__(16) unsigned char sUnion[S::UNION_BYTES];
     // leaf chunk: point leaves (first sorted position, and point count or
     // S1_LEAFTAG | class cell when binned whole from the leaf aggregate), and
     // summary leaves (leaf << 5 | number of summaries)
-    uint32_t *sLfA = reinterpret_cast<uint32_t *>(sUnion);
-    uint32_t *sLfB = sLfA + S1_CHUNK;
-    uint32_t *sEnt = sLfB + S1_CHUNK;
-    unsigned long long *m1z = reinterpret_cast<unsigned long long *>(sUnion);
-    uint32_t *m1cnt = reinterpret_cast<uint32_t *>(m1z + S::M1);
-    uint32_t *m1id = m1cnt + S::M1;
+    __shared__ uint32_t sLfA[S1_CHUNK], sLfB[S1_CHUNK];
+    __shared__ uint32_t sEnt[USE_LCS ? S1_CHUNK : 1];
+    unsigned long long *m1z = sZ;
+    uint32_t *m1cnt = sCnt;
+    uint32_t *m1id = sId;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -365,9 +361,7 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
             sCnt[c] = 0;
             sZ[c] = OWM_ZKEY_NONE;
             sId[c] = OWM_NONE;
-            sTag[c] = 0;
         }
-        uint32_t rt = 0;  // round tag
         __syncthreads();
         // window (i, j) = classes [2#{e <= L_i}, 2#{e < R_i}] (likewise y)
         if (tid < ni) {
@@ -434,15 +428,9 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
             const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
 
             for (uint32_t r = 0; r < rounds; ++r) {
-                if (++rt == 0x10000u) {
-                    __syncthreads();
-                    for (int c = tid; c < S::NCELL; c += S1_THREADS)
-                        sTag[c] = 0;
-                    __syncthreads();
-                    rt = 1;
-                }
                 uint32_t cell[S1_K], idv[S1_K];
                 unsigned long long zk[S1_K];
+                uint32_t dropped = 0;  // bit k: item k lowered its cell's minimum
                 double xv[S1_K], yv[S1_K], zv[S1_K];
                 uint32_t cn[S1_K];
                 // issue every load of the round before using any of them
@@ -515,25 +503,25 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
                     }
                     const uint32_t c = cell[k];
                     atomicAdd(&sCnt[c], cn[k]);
-                    if (zk[k] != OWM_ZKEY_NONE) {
-                        unsigned long long old = atomicMin(&sZ[c], zk[k]);
-                        if (zk[k] < old)
-                            sTag[c] = (uint16_t)rt;
-                    } else {
+                    if (zk[k] != OWM_ZKEY_NONE)
+                        dropped |= (atomicMin(&sZ[c], zk[k]) > zk[k] ? 1u : 0u) << k;
+                    else
                         cell[k] = OWM_NONE;  // counted, never a minimum
-                    }
                 }
                 __syncthreads();
-                // a cell whose minimum z dropped in this round drops its old index
+                // A cell whose minimum dropped in this round drops its old
+                // index: a lane holding the final minimum that found a larger
+                // one before its own update saw the drop (lanes that only tie
+                // an earlier minimum found it equal and keep the index).
 #pragma unroll
                 for (int k = 0; k < S1_K; ++k)
-                    if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k] && sTag[cell[k]] == (uint16_t)rt)
+                    if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k] && ((dropped >> k) & 1u))
                         sId[cell[k]] = OWM_NONE;
                 __syncthreads();
                 // No barrier after this phase: a lane that reads a minimum
                 // already lowered by the next round skips an index that the
-                // next round's reset would discard anyway, and tags are never
-                // cleared, so they cannot race with the next round's marks.
+                // next round's reset discards anyway; the next round's resets
+                // follow a barrier that every update of this round precedes.
 #pragma unroll
                 for (int k = 0; k < S1_K; ++k)
                     if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k])
@@ -544,24 +532,40 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
         // index updates, are complete before the window passes
         __syncthreads();
 
-        // separable reduction: x-pass per (class row, window column) ...
+        // separable reduction: x-pass per (class row, window column), held in
+        // registers until every cell has been read, then stored over the cells ...
         const uint32_t nyc = 4 * nj + 1;
-        for (uint32_t q = tid; q < nyc * ni; q += S1_THREADS) {
-            uint32_t rr = q / ni, w = q - rr * ni;
-            uint32_t cnt = 0, id = OWM_NONE;
-            unsigned long long z = OWM_ZKEY_NONE;
+        uint32_t xc[S::M1_PER_THREAD], xi[S::M1_PER_THREAD];
+        unsigned long long xz[S::M1_PER_THREAD];
+#pragma unroll
+        for (int t = 0; t < S::M1_PER_THREAD; ++t) {
+            const uint32_t q = tid + t * S1_THREADS;
+            xc[t] = 0;
+            xi[t] = OWM_NONE;
+            xz[t] = OWM_ZKEY_NONE;
+            if (q >= nyc * ni)
+                continue;
+            const uint32_t rr = q / ni, w = q - rr * ni;
             const uint32_t chi = min(sCxh[w], 4 * ni);
             for (uint32_t c = sCxl[w]; c <= chi; ++c) {
-                uint32_t cc = rr * NXC + c;
-                cnt += sCnt[cc];
-                if (owm_less(sZ[cc], sId[cc], z, id)) {
-                    z = sZ[cc];
-                    id = sId[cc];
+                const uint32_t cc = rr * NXC + c;
+                xc[t] += sCnt[cc];
+                if (owm_less(sZ[cc], sId[cc], xz[t], xi[t])) {
+                    xz[t] = sZ[cc];
+                    xi[t] = sId[cc];
                 }
             }
-            m1cnt[rr * BX + w] = cnt;
-            m1id[rr * BX + w] = id;
-            m1z[rr * BX + w] = z;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < S::M1_PER_THREAD; ++t) {
+            const uint32_t q = tid + t * S1_THREADS;
+            if (q >= nyc * ni)
+                continue;
+            const uint32_t rr = q / ni, w = q - rr * ni;
+            m1cnt[rr * BX + w] = xc[t];
+            m1id[rr * BX + w] = xi[t];
+            m1z[rr * BX + w] = xz[t];
         }
         __syncthreads();
         // ... then y-pass per window
