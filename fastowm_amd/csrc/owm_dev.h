@@ -50,6 +50,7 @@ struct BatchDev {
     const uint32_t *cell_off;  // [ntiles+1]
     const uint32_t *blk_off;   // [ntiles+1]
     const uint32_t *grp_off;   // [ntiles+1]
+    const uint32_t *sort_units;  // [ntiles+1] first sort tile of each tile's segment
 };
 
 // Tree arrays of a batch.

@@ -260,10 +260,10 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
 {
     const uint32_t T = (uint32_t)tiles.size();
     L.ntiles = T;
-    L.tile_bits = T > 1 ? bit_length(T - 1) : 0;
-    // Morton bits under the tile bits: the sort key width is sort_bits (24 or
-    // 32), but never fewer than 2 Morton bits per axis
-    L.key_bits = std::max((std::min(sort_bits, 32) - L.tile_bits) & ~1, std::min(4, (32 - L.tile_bits) & ~1));
+    // The tree sort is segmented by tile, so the keys hold Morton bits only:
+    // sort_bits of them (16, 24 or 32: two to four 8-bit passes).
+    L.tile_bits = 0;
+    L.key_bits = std::max(std::min(sort_bits, 32) & ~1, 4);
     L.pt_off.assign(T + 1, 0);
     L.leaf_off.assign(T + 1, 0);
     L.node_off.assign(T + 1, 0);
@@ -277,7 +277,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
         D.leaf_off = L.leaf_off[t];
         D.node_off = L.node_off[t];
         D.root = D.nleaf == 0 ? OWM_NONE : (D.nleaf == 1 ? (OWM_LEAF_BIT | D.leaf_off) : D.node_off);
-        D.tile_key = L.key_bits >= 32 ? 0u : (t << L.key_bits);
+        D.tile_key = 0;
         const auto &b = qbox[t];
         double side = std::max(b[1] - b[0], b[3] - b[2]);
         if (!(side > 0.0) || !std::isfinite(side))
@@ -294,6 +294,11 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
     }
     L.ngrp = L.grp_off[T];
     L.np = L.pt_off[T];
+    // sort tiles of each tile's segment (a sort tile never straddles tiles)
+    const uint64_t tk = sort_tile_keys((uint32_t)std::min<uint64_t>(L.np, 0xFFFFFFFFu));
+    L.sort_units.assign(T + 1, 0);
+    for (uint32_t t = 0; t < T; ++t)
+        L.sort_units[t + 1] = L.sort_units[t] + (uint32_t)((n[t] + tk - 1) / tk);
     L.nleaf = L.leaf_off[T];
     L.nnode = L.node_off[T];
 }
@@ -392,11 +397,13 @@ TreeDev tree_view(const TreeOut &T)
 }
 
 // Scratch of the tree build (keys, values, histograms, look-back flags).
-uint64_t sort_scratch_bytes(uint64_t n, int passes)
+// nseg segments: one histogram per segment, and up to one partial sort tile more
+// per segment.
+uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg)
 {
-    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1));
-    return align_up(n * 4ull, 256) * 4 + align_up(4 * 256 * 4, 256) + align_up(64, 256) +
-           align_up((uint64_t)passes * tiles * 256 * 4, 256);
+    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
+    return align_up(n * 4ull, 256) * 4 + align_up((uint64_t)nseg * SORT_SEG_HIST * 4, 256) +
+           align_up(64, 256) + align_up((uint64_t)passes * tiles * 256 * 4, 256);
 }
 
 struct SortBufs {
@@ -407,11 +414,11 @@ struct SortBufs {
 uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    return sort_scratch_bytes(L.np, passes) + align_up(L.np * 32ull, 256) +
+    return sort_scratch_bytes(L.np, passes, L.ntiles) + align_up(L.np * 32ull, 256) +
            align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
 }
 
-static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
+static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1)
 {
     SortBufs S;
     uint64_t o = 0;
@@ -420,12 +427,12 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
         o += align_up(bytes, 256);
         return p;
     };
-    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1));
+    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
     S.k0 = (uint32_t *)take(n * 4ull);
     S.v0 = (uint32_t *)take(n * 4ull);
     S.k1 = (uint32_t *)take(n * 4ull);
     S.v1 = (uint32_t *)take(n * 4ull);
-    S.hist = (uint32_t *)take(4 * 256 * 4);
+    S.hist = (uint32_t *)take((uint64_t)nseg * SORT_SEG_HIST * 4);
     S.counters = (uint32_t *)take(64);  // per-pass tile counters
     S.lookback = (uint32_t *)take((uint64_t)passes * tiles * 256 * 4);
     return S;
@@ -434,21 +441,25 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes)
 // Runs the LSD passes; keys in S.k0 (values implicit iota when iota).  Returns
 // which buffer pair holds the result (0 -> k0/v0, 1 -> k1/v1).
 static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
-                    const uint32_t *d_n, int passes, bool keys_only, bool iota, bool hist_ready)
+                    const uint32_t *d_n, int passes, bool keys_only, bool iota, bool hist_ready,
+                    const SortSegs &seg = SortSegs(), uint32_t seg_tiles = 0)
 {
     if (!hist_ready) {
         launch_fill_u32(s, S.hist, 4 * 256, 0u);
         launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
     }
-    launch_sort_scan(s, S.hist, passes, S.counters);
-    uint32_t tiles = sort_tiles(std::max<uint32_t>(n_max, 1));
+    uint32_t tiles = seg.pt ? seg_tiles : sort_tiles(std::max<uint32_t>(n_max, 1));
+    if (seg.pt)
+        launch_sort_scan_seg(s, S.hist, seg.pt, seg.nseg, passes, S.counters);
+    else
+        launch_sort_scan(s, S.hist, passes, S.counters);
     launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
     for (int p = 0; p < passes; ++p) {
         int pi = ctx->prof.begin(s, phase);
         launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
                         S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
-                        iota && p == 0, true, 0);
+                        iota && p == 0, true, 0, seg, seg_tiles);
         // algorithmic bytes per key and pass: 24 for the tree's (key, index)
         // pairs, 12 for the window minima (with the run-length pass, 44 per
         // window over three passes), 8 for the seed merge (keys only)
@@ -468,12 +479,12 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     if (L.np == 0)
         return OWM_OK;
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes));
+    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes, L.ntiles));
     if (!sb) {
         set_err(ctx, "scratch arena exhausted (tree sort)");
         return OWM_E_INTERNAL;
     }
-    SortBufs S = carve_sort(sb, L.np, passes);
+    SortBufs S = carve_sort(sb, L.np, passes, L.ntiles);
     const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
     uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
     double4 *aos = (double4 *)scratch.take(L.np * 32ull);
@@ -481,13 +492,19 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         set_err(ctx, "scratch arena exhausted (bottom-up)");
         return OWM_E_INTERNAL;
     }
-    launch_fill_u32(s, S.hist, 4 * 256, 0u);
+    launch_fill_u32(s, S.hist, (uint64_t)L.ntiles * SORT_SEG_HIST, 0u);
     int pi = ctx->prof.begin(s, PH_MORTON);
     launch_morton(s, B, dx, dy, dz, S.k0, aos, S.hist, passes);
     // algorithmic bytes per point: read x, y; write a key and an index (the
     // kernel also writes the 32-byte record the gather reads)
     ctx->prof.end(s, pi, 28ull * L.np, L.np);
-    int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true);
+    SortSegs seg;
+    seg.pt = B.pt_off;
+    seg.units = B.sort_units;
+    seg.nseg = L.ntiles;
+    seg.digit_stride = SORT_SEG_HIST;
+    int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true,
+                         seg, L.sort_units[L.ntiles]);
     const uint32_t *skeys = which ? S.k1 : S.k0;
     const uint32_t *svals = which ? S.v1 : S.v0;
     pi = ctx->prof.begin(s, PH_GATHER);
@@ -513,8 +530,8 @@ uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
     int passes = (bits + 7) / 8;
     uint64_t nseed_max = std::max<uint64_t>(Q.nwin, n_in) + Q.ncell + 1;
     uint64_t b = 0;
-    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes);
-    b += sort_scratch_bytes(nseed_max, passes);                           // seeds sort
+    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes, 1);
+    b += sort_scratch_bytes(nseed_max, passes, 1);                        // seeds sort
     b += align_up(std::max<uint64_t>(Q.ncell, 1) * 4ull, 256) * 2;      // cell flag / res
     b += align_up((nseed_max / 4096 + 2) * 4ull, 256) * 2;               // compaction look-back
     b += align_up(1024, 256);                                           // counters
@@ -539,8 +556,8 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     const uint64_t nwin = from == 2 ? n_in : Q.nwin;
     const uint64_t nw = std::max<uint64_t>(nwin, 1);
     const uint64_t nseed_max = std::max<uint64_t>(nwin, from == 3 ? n_in : 0) + Q.ncell + 1;
-    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes));
-    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes));
+    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes, 1));
+    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes, 1));
     uint32_t *cell_flag = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
     uint32_t *cell_res = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
     uint32_t *lb1 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
@@ -859,17 +876,17 @@ owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
     TileDev *d_tiles = (TileDev *)A.take(n_tiles * sizeof(TileDev));
-    uint32_t *d_off[7];
-    for (int k = 0; k < 7; ++k)
+    uint32_t *d_off[8];
+    for (int k = 0; k < 8; ++k)
         d_off[k] = (uint32_t *)A.take((n_tiles + 1) * 4ull);
-    if (!d_tiles || !d_off[6]) {
+    if (!d_tiles || !d_off[7]) {
         set_err(ctx, "scratch arena exhausted (table)");
         return OWM_E_INTERNAL;
     }
     ctx->stage_table(P.td, P.L, P.Q);
     HIP_TRY(hipMemcpyAsync(d_tiles, ctx->h_tiles, n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s),
             "upload tiles");
-    for (int k = 0; k < 7; ++k)
+    for (int k = 0; k < 8; ++k)
         HIP_TRY(hipMemcpyAsync(d_off[k], ctx->h_off[k], (n_tiles + 1) * 4ull, hipMemcpyHostToDevice, s),
                 "upload offsets");
     B = BatchDev{};
@@ -889,13 +906,14 @@ owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
     B.cell_off = d_off[4];
     B.blk_off = d_off[5];
     B.grp_off = d_off[6];
+    B.sort_units = d_off[7];
     B.ngrp = (uint32_t)P.L.ngrp;
     return OWM_OK;
 }
 
 uint64_t table_bytes(uint32_t n_tiles)
 {
-    return align_up((uint64_t)n_tiles * sizeof(TileDev), 256) + 7 * align_up((n_tiles + 1) * 4ull, 256);
+    return align_up((uint64_t)n_tiles * sizeof(TileDev), 256) + 8 * align_up((n_tiles + 1) * 4ull, 256);
 }
 
 // Reads the counts, checks the device guards, copies the seeds (tile-local).
@@ -1145,9 +1163,10 @@ void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, c
     // the previous upload from these buffers must have completed
     (void)hipStreamSynchronize(stream);
     memcpy(h_tiles, td.data(), T * sizeof(TileDev));
-    const std::vector<uint32_t> *src[7] = {&L.pt_off, &L.leaf_off, &L.node_off, &Q.win_off,
-                                           &Q.cell_off, &Q.blk_off, &L.grp_off};
-    for (int k = 0; k < 7; ++k)
+    const std::vector<uint32_t> *src[8] = {&L.pt_off,   &L.leaf_off, &L.node_off,
+                                           &Q.win_off,  &Q.cell_off, &Q.blk_off,
+                                           &L.grp_off,  &L.sort_units};
+    for (int k = 0; k < 8; ++k)
         memcpy(h_off[k], src[k]->data(), (T + 1) * 4);
 }
 

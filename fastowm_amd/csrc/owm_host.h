@@ -52,6 +52,7 @@ struct TreeLayout {
     int tile_bits = 0, key_bits = 32;
     uint64_t np = 0, nleaf = 0, nnode = 0, ngrp = 0;
     std::vector<uint32_t> pt_off, leaf_off, node_off, grp_off;
+    std::vector<uint32_t> sort_units;  // first sort tile of every tile's segment
 };
 
 struct QueryLayout {
@@ -74,8 +75,8 @@ struct QueryOut {
 };
 
 // Size-class tuning table.  The leaf size is looked up by a tile's point
-// count; the stage-1 window block and the sort key width (Morton bits plus
-// tile bits, 24 or 32: three or four 8-bit passes) by the batch's total.
+// count; the stage-1 window block and the sort key width (Morton bits per
+// tile, 16, 24 or 32: two to four 8-bit passes) by the batch's total.
 struct TuneCfg {
     uint32_t leaf;
     Stage1Cfg s1;
@@ -142,7 +143,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
                TreeLayout &L);
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
 void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
-uint64_t sort_scratch_bytes(uint64_t n, int passes);
+uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg);
 uint64_t build_scratch_bytes(const TreeLayout &L);
 uint64_t table_bytes(uint32_t n_tiles);
 owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const BatchDev &B,
@@ -243,7 +244,7 @@ struct owm_ctx {
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};
     ::TileDev *h_tiles = nullptr;
-    uint32_t *h_off[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t *h_off[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t h_cap = 0;
     owm::ServeState *serve = nullptr;
     owm::Profiler prof;  // owned; released by owm_serve_stop

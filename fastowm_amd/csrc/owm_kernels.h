@@ -6,16 +6,32 @@
 namespace owm {
 
 // ---- radix sort (sort.hip) ----
+// Segments sorted independently: [pt[t], pt[t+1]) for t < nseg, sort tiles
+// units[t] .. units[t+1]-1 (units[t] = sum of ceil(n_s / sort_tile_keys(n))
+// over s < t), digit offsets of segment t at digit_off + t * digit_stride.
+// pt == null: one segment [0, n).
+struct SortSegs {
+    const uint32_t *pt = nullptr;
+    const uint32_t *units = nullptr;
+    uint32_t nseg = 0;
+    uint32_t digit_stride = 0;
+};
 uint32_t sort_tiles(uint32_t n);
+uint32_t sort_tile_keys(uint32_t n);  // keys per sort tile for an n-key sort
+constexpr uint32_t SORT_SEG_HIST = 4 * 256;  // histogram words per segment (all passes)
+// hist[t * SORT_SEG_HIST + p * 256 + d] -> segment base + exclusive offsets
+void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,
+                          int passes, uint32_t *counters);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 void launch_sort_hist(hipStream_t s, const uint32_t *keys, const uint32_t *d_n, uint32_t n_max,
                       int passes, uint32_t *hist);
 void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *counters);
+// seg_tiles: sort tiles over all segments (units[nseg]) when seg.pt is set
 void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
                      uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n, uint32_t n_max,
                      int shift, const uint32_t *digit_off, uint32_t *lookback,
                      uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
-                     uint32_t grid_cap);
+                     uint32_t grid_cap, const SortSegs &seg = SortSegs(), uint32_t seg_tiles = 0);
 
 // ---- tree (tree.hip) ----
 struct TreeOut {
@@ -37,6 +53,7 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 // (min x, min y, max x, max y as order keys per tile).
 uint64_t bbox_key_words(uint32_t ntiles);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
+// hist: per tile SORT_SEG_HIST digit counters of every pass
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                    const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes);
 // T.pos may be null (no inverse permutation is written then)

@@ -1,9 +1,9 @@
 // Size-class autotuner: times trial configurations of the whole batch chain
 // and keeps the fastest per size class.  Knobs: leaf size (16..128, per tile
 // class), stage-1 window block (bx x by windows, k points per lane per round,
-// with or without leaf cell summaries) and the sort key width (24 or 32 bits:
-// three or four 8-bit radix passes), the last two per class of the batch
-// total.  This is the GPU counterpart of the reference's tuning of its tree
+// with or without leaf cell summaries) and the sort key width (16, 24 or 32
+// Morton bits: two to four 8-bit radix passes), the last two per class of the
+// batch total.  This is the GPU counterpart of the reference's tuning of its tree
 // granularity by sweeps over maxNumber and TBB chunk sizes (CPP-TBB-CPU
 // scripts and Results/*.csv); there the sweep is a script, here it is a
 // library call whose table the batch and serving paths use.
@@ -20,7 +20,7 @@ namespace {
 
 constexpr uint32_t LEAVES[] = {16, 32, 64, 128};
 constexpr Stage1Cfg BLOCKS[] = {{8, 8, 4, 0}, {16, 8, 4, 0}, {8, 8, 2, 0}, {16, 8, 4, 1}};
-constexpr uint32_t SORT_BITS[] = {32, 24};
+constexpr uint32_t SORT_BITS[] = {32, 24, 16};
 constexpr uint64_t SUB_POINTS = 8ull << 20;  // points per leaf-size trial batch
 
 void to_entry(const Tuner::Entry &e, owm_tune_entry &o)
@@ -102,7 +102,7 @@ owm_status owm_tune_set(owm_ctx *ctx, const owm_tune_entry *in, uint32_t n)
         const owm_tune_entry &e = in[k];
         TuneCfg c{e.leaf_size, {e.block_x, e.block_y, e.block_k, e.summaries}, e.sort_bits};
         if (c.leaf == 0 || c.leaf > 1024 || !stage1_shape_ok(c.s1) ||
-            (c.sort_bits != 24 && c.sort_bits != 32)) {
+            (c.sort_bits != 16 && c.sort_bits != 24 && c.sort_bits != 32)) {
             set_err(ctx, "tuning entry %u: unsupported configuration", k);
             return OWM_E_ARG;
         }

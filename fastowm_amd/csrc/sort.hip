@@ -10,6 +10,12 @@
 //
 // Stability: keys are ranked in (wave, item, lane) order, which is input order,
 // so equal keys keep their input order and the sort equals std::stable_sort.
+//
+// Segmented form: the tiles of a batch are contiguous segments of the input
+// that are sorted each on its own.  Sort tiles never straddle a segment, every
+// segment has its own digit offsets, and a segment's first sort tile starts
+// its look-back chain, so the keys hold Morton bits only and every pass
+// permutes within a segment.
 #include "owm_kernels.h"
 
 namespace owm {
@@ -24,6 +30,8 @@ constexpr int WAVES = BLOCK / 64;
 // tiles for more workgroups.
 constexpr uint32_t BIG_SORT = 1u << 21;
 static inline int sort_items(uint32_t n) { return n >= BIG_SORT ? 32 : 16; }
+constexpr uint32_t SEG_HIST = SORT_SEG_HIST;
+static_assert(SEG_HIST == 4 * RADIX, "four passes of 8-bit digits per segment");
 constexpr uint32_t LB_AGG = 1u << 30;
 constexpr uint32_t LB_INC = 2u << 30;
 constexpr uint32_t LB_CNT = (1u << 30) - 1;
@@ -76,6 +84,27 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const uint32_t *__restrict_
     }
 }
 
+// Per segment t (one workgroup each): hist[t][p][d] -> pt[t] + exclusive
+// offsets within the segment; workgroup 0 also zeroes the tile counters.
+__global__ __launch_bounds__(BLOCK) void k_sort_scan_seg(uint32_t *hist, const uint32_t *pt,
+                                                          uint32_t nseg, int passes,
+                                                          uint32_t *counters)
+{
+    __shared__ uint32_t tmp[WAVES];
+    for (uint32_t t = blockIdx.x; t < nseg; t += gridDim.x) {
+        uint32_t *h = hist + (size_t)t * SEG_HIST;
+        const uint32_t b = pt[t];
+        for (int p = 0; p < passes; ++p) {
+            uint32_t v = h[p * RADIX + threadIdx.x];
+            uint32_t e = block_excl_scan(v, tmp);
+            h[p * RADIX + threadIdx.x] = b + e;
+            __syncthreads();
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 8)
+        counters[threadIdx.x] = 0;
+}
+
 // hist[p][d] -> exclusive offsets; also zeroes the per-pass tile counters.
 __global__ __launch_bounds__(BLOCK) void k_sort_scan(uint32_t *hist, int passes, uint32_t *counters)
 {
@@ -98,7 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
                                                      const uint32_t *d_n, uint32_t n_host, int shift,
                                                      const uint32_t *__restrict__ digit_off,
                                                      uint32_t *lookback, uint32_t *tile_counter,
-                                                     int write_keys)
+                                                     int write_keys, SortSegs seg)
 {
     constexpr int TILE_T = BLOCK * ITEMS_T;
     __shared__ uint32_t s_tile;
@@ -109,8 +138,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
     __shared__ uint32_t s_keys[TILE_T];
     __shared__ uint32_t s_vals[KEYS_ONLY ? 1 : TILE_T];
 
-    const uint32_t n = d_n ? *d_n : n_host;
-    const uint32_t ntiles = (n + TILE_T - 1) / TILE_T;
+    const uint32_t n_all = d_n ? *d_n : n_host;
+    const uint32_t ntiles = seg.pt ? seg.units[seg.nseg] : (n_all + TILE_T - 1) / TILE_T;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
 
@@ -123,7 +152,21 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
         const uint32_t tile = s_tile;
         if (tile >= ntiles)
             break;
-        const uint32_t base = tile * TILE_T;
+        // the tile's segment: [base, n) of the input, digit offsets doff
+        uint32_t base, n;
+        bool first;
+        const uint32_t *doff = digit_off;
+        if (seg.pt) {
+            const uint32_t t = owm_find_tile(seg.units, seg.nseg, tile);
+            base = seg.pt[t] + (tile - seg.units[t]) * TILE_T;
+            n = seg.pt[t + 1];
+            first = tile == seg.units[t];
+            doff = digit_off + (size_t)t * seg.digit_stride;
+        } else {
+            base = tile * TILE_T;
+            n = n_all;
+            first = tile == 0;
+        }
 
         uint32_t key[ITEMS_T], val[ITEMS_T], rank[ITEMS_T];
 #pragma unroll
@@ -164,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
         s_whist[3][d] = c0 + c1 + c2;
         uint32_t *flag = lookback + (size_t)tile * RADIX + d;
         uint32_t prefix = 0;
-        if (tile == 0) {
+        if (first) {
             __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -183,7 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
             __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_glob[d] = digit_off[d] + prefix;
+        s_glob[d] = doff[d] + prefix;
         uint32_t ex = block_excl_scan(total, s_tmp);
         s_excl[d] = ex;
         __syncthreads();
@@ -251,44 +294,54 @@ void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *count
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(BLOCK), 0, s, hist, passes, counters);
 }
 
+void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,
+                          int passes, uint32_t *counters)
+{
+    const uint32_t g = nseg < 1 ? 1 : (nseg > 4096 ? 4096 : nseg);
+    hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, hist, pt, nseg, passes, counters);
+}
+
+uint32_t sort_tile_keys(uint32_t n) { return (uint32_t)(BLOCK * sort_items(n)); }
+
 template <int IT>
 static void onesweep_items(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
                            uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n,
                            uint32_t n_max, int shift, const uint32_t *digit_off, uint32_t *lookback,
-                           uint32_t *tile_counter, bool keys_only, bool iota, int wk, uint32_t grid)
+                           uint32_t *tile_counter, bool keys_only, bool iota, int wk, uint32_t grid,
+                           const SortSegs &seg)
 {
     if (keys_only)
         hipLaunchKernelGGL((k_onesweep<true, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
                            vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
+                           tile_counter, wk, seg);
     else if (iota)
         hipLaunchKernelGGL((k_onesweep<false, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
                            vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
+                           tile_counter, wk, seg);
     else
         hipLaunchKernelGGL((k_onesweep<false, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
                            vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off, lookback,
-                           tile_counter, wk);
+                           tile_counter, wk, seg);
 }
 
 void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
                      uint32_t *keys_out, uint32_t *vals_out, const uint32_t *d_n, uint32_t n_max,
                      int shift, const uint32_t *digit_off, uint32_t *lookback,
                      uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
-                     uint32_t grid_cap)
+                     uint32_t grid_cap, const SortSegs &seg, uint32_t seg_tiles)
 {
     // the tile size follows n_max; a device count d_n <= n_max uses the same
-    uint32_t tiles = sort_tiles(n_max);
+    uint32_t tiles = seg.pt ? seg_tiles : sort_tiles(n_max);
     uint32_t grid = tiles < 1 ? 1 : tiles;
     if (grid_cap && grid > grid_cap)
         grid = grid_cap;
     int wk = write_keys ? 1 : 0;
     if (sort_items(n_max) == 32)
         onesweep_items<32>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
-                           lookback, tile_counter, keys_only, iota, wk, grid);
+                           lookback, tile_counter, keys_only, iota, wk, grid, seg);
     else
         onesweep_items<16>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
-                           lookback, tile_counter, keys_only, iota, wk, grid);
+                           lookback, tile_counter, keys_only, iota, wk, grid, seg);
 }
 
 }  // namespace owm

@@ -235,9 +235,25 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
                                                 uint32_t *__restrict__ keys, double4 *__restrict__ aos,
                                                 uint32_t *hist, int passes)
 {
+    // digit counts of the tile `cur` (uniform over the block), flushed to the
+    // tile's histogram when the block moves to another tile
     __shared__ uint32_t h[4][256];
     for (int i = threadIdx.x; i < 4 * 256; i += TB)
         (&h[0][0])[i] = 0;
+    uint32_t cur = OWM_NONE;
+    auto flush = [&]() {
+        __syncthreads();
+        uint32_t *g = hist + (size_t)cur * SORT_SEG_HIST;
+        for (int i = threadIdx.x; i < passes * 256; i += TB) {
+            const uint32_t c = (&h[0][0])[i];
+            if (c) {
+                atomicAdd(&g[i], c);
+                (&h[0][0])[i] = 0;
+            }
+        }
+        __syncthreads();
+        cur = OWM_NONE;
+    };
     __syncthreads();
     const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
     const uint32_t shift = 64 - B.key_bits;
@@ -245,8 +261,12 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
         const uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
         uint32_t t = owm_find_tile(B.pt_off, B.ntiles, c0);
         uint32_t next = B.pt_off[t + 1];
-        if (c1 <= next) {
+        const bool single = c1 <= next;
+        if (cur != OWM_NONE && (!single || t != cur))
+            flush();
+        if (single) {
             // one tile: eight independent point loads in flight per thread
+            cur = t;
             const TileDev &T = B.tiles[t];
             for (uint32_t base = c0 + threadIdx.x; base < c1; base += 8 * TB) {
                 double xv[8], yv[8], zv[8];
@@ -271,6 +291,7 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
             }
             continue;
         }
+        // a chunk across tiles: counts go straight to each tile's histogram
         for (uint32_t i = c0 + threadIdx.x; i < c1; i += TB) {
             while (i >= next)
                 next = B.pt_off[++t + 1];
@@ -280,16 +301,13 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
             uint32_t k = T.tile_key | (uint32_t)(m >> shift);
             keys[i] = k;
             aos[i] = make_double4(x, y, Z[i], 0.0);
+            uint32_t *g = hist + (size_t)t * SORT_SEG_HIST;
             for (int p = 0; p < passes; ++p)
-                atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+                atomicAdd(&g[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += TB) {
-        uint32_t c = (&h[0][0])[i];
-        if (c)
-            atomicAdd(&hist[i], c);
-    }
+    if (cur != OWM_NONE)
+        flush();
 }
 
 // Segmented reductions over contiguous lane segments of length ls (a segment

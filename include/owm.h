@@ -148,7 +148,7 @@ typedef struct owm_tune_entry {
     uint32_t leaf_size;  /* points per leaf (maxNumber) */
     uint32_t block_x, block_y, block_k; /* stage-1 window block, points per lane per round */
     uint32_t summaries;  /* 1: stage 1 reads per-leaf summaries of equal window-edge classes */
-    uint32_t sort_bits;  /* sort key width, 24 or 32 (three or four 8-bit passes) */
+    uint32_t sort_bits;  /* Morton bits of the per-tile sort key: 16, 24 or 32 (2-4 passes) */
     double ms;           /* time of the winning trial; 0 = default, not tuned */
 } owm_tune_entry;
 /* Times trial configurations on the given batch (tiles as in owm_batch_run,

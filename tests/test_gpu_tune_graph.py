@@ -75,7 +75,7 @@ def test_autotune_keeps_results_and_table_round_trips():
         assert [e["max_n"] for e in table] == [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 2 ** 64 - 1]
         for e in table:
             assert e["leaf_size"] in (16, 32, 64, 128)
-            assert e["sort_bits"] in (24, 32)
+            assert e["sort_bits"] in (16, 24, 32)
             assert (e["block_x"], e["block_y"], e["block_k"], e["summaries"]) in (
                 (8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1))
         # the classes that held tiles were timed
@@ -95,7 +95,7 @@ def test_autotune_keeps_results_and_table_round_trips():
 
 @pytest.mark.parametrize("sort_bits,leaf,block", [(24, 16, (8, 8, 2, 0)), (24, 128, (16, 8, 4, 0)),
                                                   (32, 64, (16, 8, 4, 0)), (24, 32, (16, 8, 4, 1)),
-                                                  (32, 16, (8, 8, 4, 1))])
+                                                  (32, 16, (8, 8, 4, 1)), (16, 32, (16, 8, 4, 0))])
 def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
     say("table entry", sort_bits, leaf, block)
     t = synth.scaled(synth.config_c2(seed=5), 700_000)
