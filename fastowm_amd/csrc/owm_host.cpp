@@ -44,7 +44,7 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
 static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 static const char *PHASE_NAMES[PH_COUNT] = {
-    "bbox", "morton", "sort_tree_pass", "gather_leaves", "karras", "bottomup", "stage1",
+    "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "bottomup", "stage1",
     "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
     "stage1_summaries"};
 
@@ -410,11 +410,13 @@ struct SortBufs {
     uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
 };
 
-// Device scratch of one tree build: the sort buffers and the bottom-up work list.
+// Device scratch of one tree build: the sort buffers, one set of sorted point
+// records (two when more than two passes), and the bottom-up work list.
 uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    return sort_scratch_bytes(L.np, passes, L.ntiles) + align_up(L.np * 32ull, 256) +
+    const int rec_sets = passes > 2 ? 2 : 1;
+    return sort_scratch_bytes(L.np, passes, L.ntiles) + rec_sets * 3 * align_up(L.np * 8ull, 256) +
            align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
 }
 
@@ -441,30 +443,23 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg 
 // Runs the LSD passes; keys in S.k0 (values implicit iota when iota).  Returns
 // which buffer pair holds the result (0 -> k0/v0, 1 -> k1/v1).
 static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
-                    const uint32_t *d_n, int passes, bool keys_only, bool iota, bool hist_ready,
-                    const SortSegs &seg = SortSegs(), uint32_t seg_tiles = 0)
+                    const uint32_t *d_n, int passes, bool keys_only, bool iota)
 {
-    if (!hist_ready) {
-        launch_fill_u32(s, S.hist, 4 * 256, 0u);
-        launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
-    }
-    uint32_t tiles = seg.pt ? seg_tiles : sort_tiles(std::max<uint32_t>(n_max, 1));
-    if (seg.pt)
-        launch_sort_scan_seg(s, S.hist, seg.pt, seg.nseg, passes, S.counters);
-    else
-        launch_sort_scan(s, S.hist, passes, S.counters);
+    launch_fill_u32(s, S.hist, 4 * 256, 0u);
+    launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
+    const uint32_t tiles = sort_tiles(std::max<uint32_t>(n_max, 1));
+    launch_sort_scan(s, S.hist, passes, S.counters);
     launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
     for (int p = 0; p < passes; ++p) {
         int pi = ctx->prof.begin(s, phase);
         launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
                         S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
-                        iota && p == 0, true, 0, seg, seg_tiles);
-        // algorithmic bytes per key and pass: 24 for the tree's (key, index)
-        // pairs, 12 for the window minima (with the run-length pass, 44 per
-        // window over three passes), 8 for the seed merge (keys only)
-        uint64_t per = phase == PH_SORT_TREE ? 24 : (phase == PH_SORT_WIN ? 12 : 8);
-        ctx->prof.end(s, pi, per * n_max, n_max);
+                        iota && p == 0, true, 0);
+        // algorithmic bytes per key and pass: 12 for the window minima (with
+        // the run-length pass, 44 per window over three passes), 8 for the
+        // seed merge (keys only)
+        ctx->prof.end(s, pi, (phase == PH_SORT_WIN ? 12ull : 8ull) * n_max, n_max);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -487,31 +482,58 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     SortBufs S = carve_sort(sb, L.np, passes, L.ntiles);
     const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
     uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
-    double4 *aos = (double4 *)scratch.take(L.np * 32ull);
-    if (!bu_work || !aos) {
-        set_err(ctx, "scratch arena exhausted (bottom-up)");
+    bool have = bu_work != nullptr;
+    // record sets between passes: (x, y, z) here, indices in the values buffers
+    RecOut rec[2] = {};
+    for (int k = 0; k < (passes > 2 ? 2 : 1); ++k) {
+        rec[k].x = (double *)scratch.take(L.np * 8ull);
+        rec[k].y = (double *)scratch.take(L.np * 8ull);
+        rec[k].z = (double *)scratch.take(L.np * 8ull);
+        rec[k].id = k ? S.v1 : S.v0;
+        have = have && rec[k].x && rec[k].y && rec[k].z;
+    }
+    if (!have) {
+        set_err(ctx, "scratch arena exhausted (tree build)");
         return OWM_E_INTERNAL;
     }
     launch_fill_u32(s, S.hist, (uint64_t)L.ntiles * SORT_SEG_HIST, 0u);
     int pi = ctx->prof.begin(s, PH_MORTON);
-    launch_morton(s, B, dx, dy, dz, S.k0, aos, S.hist, passes);
-    // algorithmic bytes per point: read x, y; write a key and an index (the
-    // kernel also writes the 32-byte record the gather reads)
-    ctx->prof.end(s, pi, 28ull * L.np, L.np);
+    launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
+    // algorithmic bytes per point: read x, y; write a key
+    ctx->prof.end(s, pi, 20ull * L.np, L.np);
     SortSegs seg;
     seg.pt = B.pt_off;
     seg.units = B.sort_units;
     seg.nseg = L.ntiles;
     seg.digit_stride = SORT_SEG_HIST;
-    int which = run_sort(ctx, PH_SORT_TREE, s, S, (uint32_t)L.np, nullptr, passes, false, true, true,
-                         seg, L.sort_units[L.ntiles]);
-    const uint32_t *skeys = which ? S.k1 : S.k0;
-    const uint32_t *svals = which ? S.v1 : S.v0;
-    pi = ctx->prof.begin(s, PH_GATHER);
-    launch_gather_leaves(s, B, svals, skeys, aos, T);
-    // algorithmic bytes: gather 52 per point (index and x, y, z read, x, y, z
-    // written), leaves 24 per point + 48 per leaf
-    ctx->prof.end(s, pi, 76ull * L.np + 48ull * L.nleaf, L.np);
+    const uint32_t tiles = L.sort_units[L.ntiles];
+    launch_sort_scan_seg(s, S.hist, seg.pt, seg.nseg, passes, S.counters);
+    launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
+    // passes move (key, x, y, z, index): the cloud -> rec[0] -> rec[1] -> ... -> the tree
+    const RecOut tree_rec{T.xs, T.ys, T.zs, T.ids, T.pos};
+    uint32_t *kin = S.k0, *kout = S.k1;
+    for (int p = 0; p < passes; ++p) {
+        RecIn in;
+        if (p == 0) {
+            in = RecIn{dx, dy, dz, nullptr};
+        } else {
+            const RecOut &r = rec[(p - 1) & 1];
+            in = RecIn{r.x, r.y, r.z, r.id};
+        }
+        const RecOut &out = p == passes - 1 ? tree_rec : rec[p & 1];
+        pi = ctx->prof.begin(s, PH_SORT_TREE);
+        launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p, S.hist + 256 * p,
+                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0, seg, tiles);
+        // algorithmic bytes per point and pass: a key and a 28-byte record
+        // read and written
+        ctx->prof.end(s, pi, 64ull * L.np, L.np);
+        std::swap(kin, kout);
+    }
+    const uint32_t *skeys = kin;
+    pi = ctx->prof.begin(s, PH_LEAVES);
+    launch_leaf_aggs(s, B, skeys, T);
+    // algorithmic bytes: x, y, z and index read per point; 48 per leaf written
+    ctx->prof.end(s, pi, 28ull * L.np + 48ull * L.nleaf, L.np);
     pi = ctx->prof.begin(s, PH_KARRAS);
     launch_karras(s, B, T);
     ctx->prof.end(s, pi, 24ull * L.nleaf, L.nnode);  // with the bottom-up: 144 per leaf
@@ -604,7 +626,7 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     if (from <= 2) {
         if (from == 2 && nwin)
             HIP_TRY(hipMemcpyAsync(W.k0, h_in, nwin * 4ull, hipMemcpyHostToDevice, s), "min_ids H2D");
-        int which = run_sort(ctx, PH_SORT_WIN, s, W, (uint32_t)nwin, nullptr, passes, true, false, false);
+        int which = run_sort(ctx, PH_SORT_WIN, s, W, (uint32_t)nwin, nullptr, passes, true, false);
         const uint32_t *sorted = which ? W.k1 : W.k0;
         out.sorted_min_ids = sorted;
         int pi = ctx->prof.begin(s, PH_STAGE2);
@@ -635,7 +657,7 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         HIP_TRY(hipGetLastError(), "stage 3 launch");
         return OWM_OK;
     }
-    int which = run_sort(ctx, PH_MERGE, s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false, false);
+    int which = run_sort(ctx, PH_MERGE, s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false);
     out.seeds = which ? SD.k1 : SD.k0;
     launch_seed_ranges(s, B, out.seeds, cnt + 2, seed_off);
     HIP_TRY(hipGetLastError(), "query launch");

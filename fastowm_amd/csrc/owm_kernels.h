@@ -33,6 +33,23 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
                      uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
                      uint32_t grid_cap, const SortSegs &seg = SortSegs(), uint32_t seg_tiles = 0);
 
+// Point records of the tree sort: coordinates and batch indices (id unused by
+// a first pass, which numbers the points itself; pos may be null).
+struct RecIn {
+    const double *x, *y, *z;
+    const uint32_t *id;
+};
+struct RecOut {
+    double *x, *y, *z;
+    uint32_t *id, *pos;
+};
+// One record-moving pass of the segmented tree sort over n points (segments
+// seg, seg_tiles sort tiles in all); iota on the first pass.
+void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
+                         uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
+                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
+                         bool iota, const SortSegs &seg, uint32_t seg_tiles);
+
 // ---- tree (tree.hip) ----
 struct TreeOut {
     double *xs, *ys, *zs;
@@ -55,10 +72,9 @@ uint64_t bbox_key_words(uint32_t ntiles);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
 // hist: per tile SORT_SEG_HIST digit counters of every pass
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes);
-// T.pos may be null (no inverse permutation is written then)
-void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *sorted_vals,
-                          const uint32_t *sorted_keys, const double4 *aos, const TreeOut &T);
+                   uint32_t *keys, uint32_t *hist, int passes);
+// Leaf aggregates over the points in Morton order (T.xs, ys, zs, ids).
+void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const uint32_t *sorted_keys, const TreeOut &T);
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
 // Bottom-up aggregation: chunk-local climb with LDS arrival flags, then a
 // global climb over the nodes whose leaf range crosses a chunk.  `work` holds

@@ -16,6 +16,11 @@
 // segment has its own digit offsets, and a segment's first sort tile starts
 // its look-back chain, so the keys hold Morton bits only and every pass
 // permutes within a segment.
+//
+// Record form (the tree sort): every pass moves the points' x, y, z and batch
+// index with the keys, the first reading the caller's coordinate arrays and
+// the last writing the tree's, so the cloud ends in Morton order without a
+// gather by index (random 32-byte reads run at a fraction of HBM bandwidth).
 #include "owm_kernels.h"
 
 namespace owm {
@@ -119,6 +124,124 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(uint32_t *hist, int passes,
         counters[threadIdx.x] = 0;
 }
 
+// LDS state of one onesweep tile.
+struct SweepLds {
+    uint32_t whist[WAVES][RADIX];  // per-wave digit counts, then wave-exclusive prefixes
+    uint32_t excl[RADIX];          // digit start within the tile
+    uint32_t glob[RADIX];          // digit start in the output minus excl
+    uint32_t tmp[WAVES];
+    uint32_t tile;
+};
+
+// Takes the next tile from the counter and resolves its segment: input range
+// [base, n), digit offsets doff and whether it starts its look-back chain.
+// Returns false (uniformly) when no tile is left.
+template <int TILE_T>
+__device__ __forceinline__ bool sweep_claim(SweepLds &L, uint32_t *tile_counter, uint32_t n_all,
+                                            const SortSegs &seg, const uint32_t *digit_off,
+                                            uint32_t &tile, uint32_t &base, uint32_t &n, bool &first,
+                                            const uint32_t *&doff)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ntiles = seg.pt ? seg.units[seg.nseg] : (n_all + TILE_T - 1) / TILE_T;
+    if (tid == 0)
+        L.tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < WAVES * RADIX; i += BLOCK)
+        (&L.whist[0][0])[i] = 0;
+    __syncthreads();
+    tile = L.tile;
+    if (tile >= ntiles)
+        return false;
+    doff = digit_off;
+    if (seg.pt) {
+        const uint32_t t = owm_find_tile(seg.units, seg.nseg, tile);
+        base = seg.pt[t] + (tile - seg.units[t]) * TILE_T;
+        n = seg.pt[t + 1];
+        first = tile == seg.units[t];
+        doff = digit_off + (size_t)t * seg.digit_stride;
+    } else {
+        base = tile * TILE_T;
+        n = n_all;
+        first = tile == 0;
+    }
+    return true;
+}
+
+// Stable in-wave ranks: item k of lane l of wave w is element
+// base + w*64*ITEMS_T + k*64 + l; rank[k] counts the wave's earlier elements of
+// the same digit, and L.whist[w][d] ends as the wave's count of digit d.  FULL:
+// the tile holds ITEMS_T keys per thread (no element is past n).  PACK: rank[k]
+// is d << 16 | rank, and rank may be key itself (each key is read before its
+// rank is written).
+template <int ITEMS_T, bool FULL = false, bool PACK = false>
+__device__ __forceinline__ void sweep_rank(SweepLds &L, const uint32_t (&key)[ITEMS_T],
+                                           uint32_t base, uint32_t n, int shift,
+                                           uint32_t (&rank)[ITEMS_T])
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < ITEMS_T; ++k) {
+        const uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
+        const bool valid = FULL || idx < n;
+        const uint32_t d = (key[k] >> shift) & (RADIX - 1);
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t before = __popcll(m & lt_mask);
+        const uint32_t old = L.whist[w][d];
+        if (valid && before == 0)
+            L.whist[w][d] = old + (uint32_t)__popcll(m);
+        rank[k] = PACK ? d << 16 | (old + before) : old + before;
+    }
+}
+
+// After sweep_rank and a barrier: wave-exclusive prefixes per digit, the tile's
+// digit counts published and its global digit bases resolved by decoupled
+// look-back.  Ends with a barrier; then a key of digit d and rank r in wave w
+// sits at tile slot s = excl[d] + whist[w][d] + r and goes to output position
+// glob[d] + s (glob holds the digit's output start minus its tile start,
+// modulo 2^32).
+__device__ __forceinline__ void sweep_offsets(SweepLds &L, uint32_t tile, bool first,
+                                              const uint32_t *doff, uint32_t *lookback)
+{
+    const uint32_t d = threadIdx.x;
+    const uint32_t c0 = L.whist[0][d], c1 = L.whist[1][d], c2 = L.whist[2][d], c3 = L.whist[3][d];
+    const uint32_t total = c0 + c1 + c2 + c3;
+    L.whist[0][d] = 0;
+    L.whist[1][d] = c0;
+    L.whist[2][d] = c0 + c1;
+    L.whist[3][d] = c0 + c1 + c2;
+    uint32_t *flag = lookback + (size_t)tile * RADIX + d;
+    uint32_t prefix = 0;
+    if (first) {
+        __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t j = (int32_t)tile - 1;
+        for (;;) {
+            const uint32_t f = __hip_atomic_load(lookback + (size_t)j * RADIX + d, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t st = f & ~LB_CNT;
+            if (st == 0)
+                continue;  // predecessor still ranking (it holds an earlier tile, so it is resident)
+            prefix += f & LB_CNT;
+            if (st == LB_INC)
+                break;
+            --j;
+        }
+        __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t ex = block_excl_scan(total, L.tmp);
+    L.excl[d] = ex;
+    L.glob[d] = doff[d] + prefix - ex;
+    __syncthreads();
+}
+
 template <bool KEYS_ONLY, bool IOTA, int ITEMS_T>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__ keys_in,
                                                      const uint32_t *__restrict__ vals_in,
@@ -130,44 +253,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
                                                      int write_keys, SortSegs seg)
 {
     constexpr int TILE_T = BLOCK * ITEMS_T;
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_whist[WAVES][RADIX];
-    __shared__ uint32_t s_excl[RADIX];
-    __shared__ uint32_t s_glob[RADIX];
-    __shared__ uint32_t s_tmp[WAVES];
+    __shared__ SweepLds L;
     __shared__ uint32_t s_keys[TILE_T];
     __shared__ uint32_t s_vals[KEYS_ONLY ? 1 : TILE_T];
 
     const uint32_t n_all = d_n ? *d_n : n_host;
-    const uint32_t ntiles = seg.pt ? seg.units[seg.nseg] : (n_all + TILE_T - 1) / TILE_T;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-
-    for (;;) {
-        if (tid == 0)
-            s_tile = atomicAdd(tile_counter, 1u);
-        for (int i = tid; i < WAVES * RADIX; i += BLOCK)
-            (&s_whist[0][0])[i] = 0;
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= ntiles)
-            break;
-        // the tile's segment: [base, n) of the input, digit offsets doff
-        uint32_t base, n;
-        bool first;
-        const uint32_t *doff = digit_off;
-        if (seg.pt) {
-            const uint32_t t = owm_find_tile(seg.units, seg.nseg, tile);
-            base = seg.pt[t] + (tile - seg.units[t]) * TILE_T;
-            n = seg.pt[t + 1];
-            first = tile == seg.units[t];
-            doff = digit_off + (size_t)t * seg.digit_stride;
-        } else {
-            base = tile * TILE_T;
-            n = n_all;
-            first = tile == 0;
-        }
-
+    uint32_t tile, base, n;
+    bool first;
+    const uint32_t *doff;
+    while (sweep_claim<TILE_T>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
         uint32_t key[ITEMS_T], val[ITEMS_T], rank[ITEMS_T];
 #pragma unroll
         for (int k = 0; k < ITEMS_T; ++k) {
@@ -177,66 +272,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
             if (!KEYS_ONLY)
                 val[k] = valid ? (IOTA ? idx : vals_in[idx]) : 0u;
         }
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k) {
-            uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
-            bool valid = idx < n;
-            uint32_t d = (key[k] >> shift) & (RADIX - 1);
-            unsigned long long m = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < RB; ++b) {
-                bool bit = (d >> b) & 1u;
-                unsigned long long bb = __ballot(bit);
-                m &= bit ? bb : ~bb;
-            }
-            uint32_t before = __popcll(m & lt_mask);
-            uint32_t old = s_whist[w][d];
-            if (valid && before == 0)
-                s_whist[w][d] = old + (uint32_t)__popcll(m);
-            rank[k] = old + before;
-        }
+        sweep_rank<ITEMS_T>(L, key, base, n, shift, rank);
         __syncthreads();
-
-        // per digit: wave-exclusive prefixes, tile total, look-back, global base
-        const uint32_t d = tid;
-        uint32_t c0 = s_whist[0][d], c1 = s_whist[1][d], c2 = s_whist[2][d], c3 = s_whist[3][d];
-        uint32_t total = c0 + c1 + c2 + c3;
-        s_whist[0][d] = 0;
-        s_whist[1][d] = c0;
-        s_whist[2][d] = c0 + c1;
-        s_whist[3][d] = c0 + c1 + c2;
-        uint32_t *flag = lookback + (size_t)tile * RADIX + d;
-        uint32_t prefix = 0;
-        if (first) {
-            __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int32_t j = (int32_t)tile - 1;
-            for (;;) {
-                uint32_t f = __hip_atomic_load(lookback + (size_t)j * RADIX + d, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t st = f & ~LB_CNT;
-                if (st == 0)
-                    continue;  // predecessor still ranking (it holds an earlier tile, so it is resident)
-                prefix += f & LB_CNT;
-                if (st == LB_INC)
-                    break;
-                --j;
-            }
-            __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_glob[d] = doff[d] + prefix;
-        uint32_t ex = block_excl_scan(total, s_tmp);
-        s_excl[d] = ex;
-        __syncthreads();
-
+        sweep_offsets(L, tile, first, doff, lookback);
 #pragma unroll
         for (int k = 0; k < ITEMS_T; ++k) {
             uint32_t idx = base + w * (64 * ITEMS_T) + k * 64 + lane;
             if (idx < n) {
                 uint32_t dd = (key[k] >> shift) & (RADIX - 1);
-                uint32_t slot = s_excl[dd] + s_whist[w][dd] + rank[k];
+                uint32_t slot = L.excl[dd] + L.whist[w][dd] + rank[k];
                 s_keys[slot] = key[k];
                 if (!KEYS_ONLY)
                     s_vals[slot] = val[k];
@@ -247,13 +291,155 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
         for (uint32_t s = tid; s < cnt; s += BLOCK) {
             uint32_t k = s_keys[s];
             uint32_t dd = (k >> shift) & (RADIX - 1);
-            uint32_t dst = s_glob[dd] + (s - s_excl[dd]);
+            uint32_t dst = L.glob[dd] + s;
             if (write_keys)
                 keys_out[dst] = k;
             if (!KEYS_ONLY)
                 vals_out[dst] = s_vals[s];
         }
         __syncthreads();
+    }
+}
+
+// One tile of a record-moving pass (see k_sort_records).  FULL: the tile holds
+// TILE_T points, so no load or store needs a bound check; otherwise loads are
+// clamped to the tile's last point and the items past it take the unused tile
+// slots past cnt, so that the LDS writes need no check either.
+template <bool IOTA, bool POS, int ITEMS_T, bool FULL>
+__device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t *s_dig,
+                                             const uint32_t *__restrict__ keys_in, const RecIn &in,
+                                             uint32_t *__restrict__ keys_out, const RecOut &out,
+                                             uint32_t tile, uint32_t base, uint32_t cnt, bool first,
+                                             const uint32_t *doff, uint32_t *lookback, int shift)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t p0 = w * (64 * ITEMS_T) + lane;  // tile position of the thread's item 0
+    unsigned long long *const s_u64 = reinterpret_cast<unsigned long long *>(s_buf);
+    // element of item k; o is p0 passed through an empty asm, so that every
+    // field computes its own addresses (otherwise the compiler keeps the 64-bit
+    // addresses of all items live from one field to the next, and spills)
+    auto fresh = [&]() {
+        uint32_t o = p0;
+        asm volatile("" : "+v"(o));
+        return o;
+    };
+    auto at = [&](uint32_t o, int k) { return base + (FULL ? o + k * 64 : min(o + k * 64, cnt - 1)); };
+    double v[ITEMS_T];
+    auto load = [&](const double *__restrict__ src) {
+        const uint32_t o = fresh();
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            v[k] = src[at(o, k)];
+    };
+    // per item its tile slot (16 bits, two per word)
+    uint32_t slot2[ITEMS_T / 2];
+    {
+        // keys, then (in place) digit << 16 | in-wave rank
+        uint32_t kr[ITEMS_T];
+        const uint32_t o = fresh();
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            kr[k] = keys_in[at(o, k)];
+        load(in.x);
+        sweep_rank<ITEMS_T, FULL, true>(L, kr, 0, cnt, shift, kr);
+        __syncthreads();
+        sweep_offsets(L, tile, first, doff, lookback);
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k) {
+            const uint32_t dd = kr[k] >> 16;
+            uint32_t sl = L.excl[dd] + L.whist[w][dd] + (kr[k] & 0xFFFFu);
+            if (!FULL)
+                sl = p0 + k * 64 < cnt ? sl : p0 + k * 64;
+            s_dig[sl] = (uint8_t)dd;
+            s_buf[sl] = v[k];
+            if (k & 1)
+                slot2[k / 2] |= sl << 16;
+            else
+                slot2[k / 2] = sl;
+        }
+    }
+    __syncthreads();
+    auto slot_of = [&](int k) { return (slot2[k / 2] >> (16 * (k & 1))) & 0xFFFFu; };
+    // the staged field goes out (then a barrier); the next one, already
+    // loaded, comes in (then a barrier).  Destinations are looked up per slot
+    // (digit, then the digit's base) in a rolled loop: unrolled, the compiler
+    // hoists every lookup and address and runs out of registers.
+    auto store = [&](double *__restrict__ to) {
+#pragma unroll 2
+        for (uint32_t s = tid; s < cnt; s += BLOCK)
+            to[L.glob[s_dig[s]] + s] = s_buf[s];
+        __syncthreads();
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            s_buf[slot_of(k)] = v[k];
+        __syncthreads();
+    };
+    load(in.y);
+    store(out.x);
+    stage();
+    load(in.z);
+    store(out.y);
+    stage();
+    // key and index travel together as one 8-byte word
+    {
+        unsigned long long kv[ITEMS_T];
+        const uint32_t o = fresh();
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k) {
+            const uint32_t id = IOTA ? base + o + k * 64 : in.id[at(o, k)];
+            kv[k] = ((unsigned long long)id << 32) | keys_in[at(o, k)];
+        }
+        store(out.z);
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            s_u64[slot_of(k)] = kv[k];
+        __syncthreads();
+    }
+    for (uint32_t s = tid; s < cnt; s += BLOCK) {
+        const unsigned long long kv = s_u64[s];
+        const uint32_t dst = L.glob[s_dig[s]] + s, id = (uint32_t)(kv >> 32);
+        keys_out[dst] = (uint32_t)kv;
+        out.id[dst] = id;
+        if (POS)
+            out.pos[id] = dst;
+    }
+    __syncthreads();
+}
+
+// One pass of the segmented tree sort that moves whole point records with the
+// keys, so that the last pass leaves the cloud in Morton order and no gather
+// by index follows.  Records are x, y, z (8 bytes each) and the point's batch
+// index.  The first pass reads the caller's coordinate arrays and numbers the
+// points itself (IOTA); the last writes the tree's arrays (and, with POS, each
+// point's sorted position at its index).  Ranks come from the keys alone; each
+// field is then staged through one LDS buffer in digit order and written out,
+// so consecutive lanes write consecutive addresses.  The loads of a field are
+// issued before the stores of the previous one (those of x before the
+// ranking), so that their latency hides behind that work.
+template <bool IOTA, bool POS, int ITEMS_T>
+__global__ __launch_bounds__(BLOCK, 2) void k_sort_records(const uint32_t *__restrict__ keys_in,
+                                                            RecIn in, uint32_t *__restrict__ keys_out,
+                                                            RecOut out, uint32_t n_all, int shift,
+                                                            const uint32_t *__restrict__ digit_off,
+                                                            uint32_t *lookback, uint32_t *tile_counter,
+                                                            SortSegs seg)
+{
+    constexpr int TILE_T = BLOCK * ITEMS_T;
+    __shared__ SweepLds L;
+    __shared__ double s_buf[TILE_T];
+    __shared__ uint8_t s_dig[TILE_T];  // digit of every tile slot
+    uint32_t tile, base, n;
+    bool first;
+    const uint32_t *doff;
+    while (sweep_claim<TILE_T>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
+        if (n - base >= (uint32_t)TILE_T)
+            records_tile<IOTA, POS, ITEMS_T, true>(L, s_buf, s_dig, keys_in, in, keys_out, out, tile,
+                                                   base, TILE_T, first, doff, lookback, shift);
+        else
+            records_tile<IOTA, POS, ITEMS_T, false>(L, s_buf, s_dig, keys_in, in, keys_out, out, tile,
+                                                    base, n - base, first, doff, lookback, shift);
     }
 }
 
@@ -342,6 +528,40 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
     else
         onesweep_items<16>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
                            lookback, tile_counter, keys_only, iota, wk, grid, seg);
+}
+
+template <int IT>
+static void sort_records_items(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
+                               uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
+                               const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
+                               bool iota, const SortSegs &seg, uint32_t grid)
+{
+    if (iota && out.pos)
+        hipLaunchKernelGGL((k_sort_records<true, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+    else if (iota)
+        hipLaunchKernelGGL((k_sort_records<true, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+    else if (out.pos)
+        hipLaunchKernelGGL((k_sort_records<false, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+    else
+        hipLaunchKernelGGL((k_sort_records<false, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
+                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+}
+
+void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
+                         uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
+                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
+                         bool iota, const SortSegs &seg, uint32_t seg_tiles)
+{
+    const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
+    if (sort_items(n) == 32)
+        sort_records_items<32>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
+                               tile_counter, iota, seg, grid);
+    else
+        sort_records_items<16>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
+                               tile_counter, iota, seg, grid);
 }
 
 }  // namespace owm

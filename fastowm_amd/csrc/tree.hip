@@ -225,15 +225,11 @@ __device__ __forceinline__ unsigned long long morton64(const TileDev &T, double 
     return spread2(qx) | (spread2(qy) << 1);
 }
 
-// Sort keys (tile bits above the top key_bits of the Morton code) plus the
-// digit histograms of every radix pass.  The cloud is also packed into 32-byte
-// (x, y, z, 0) records, so that the gather after the sort reads one aligned
-// sector per point instead of three scattered 8-byte words.
+// Sort keys (the top key_bits of each point's Morton code within its tile)
+// plus each tile's digit histograms of every radix pass.
 __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restrict__ X,
                                                 const double *__restrict__ Y,
-                                                const double *__restrict__ Z,
-                                                uint32_t *__restrict__ keys, double4 *__restrict__ aos,
-                                                uint32_t *hist, int passes)
+                                                uint32_t *__restrict__ keys, uint32_t *hist, int passes)
 {
     // digit counts of the tile `cur` (uniform over the block), flushed to the
     // tile's histogram when the block moves to another tile
@@ -269,13 +265,12 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
             cur = t;
             const TileDev &T = B.tiles[t];
             for (uint32_t base = c0 + threadIdx.x; base < c1; base += 8 * TB) {
-                double xv[8], yv[8], zv[8];
+                double xv[8], yv[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const uint32_t i = base + j * TB;
                     xv[j] = i < c1 ? X[i] : 0.0;
                     yv[j] = i < c1 ? Y[i] : 0.0;
-                    zv[j] = i < c1 ? Z[i] : 0.0;
                 }
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
@@ -283,7 +278,6 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
                     if (i < c1) {
                         const uint32_t k = T.tile_key | (uint32_t)(morton64(T, xv[j], yv[j]) >> shift);
                         keys[i] = k;
-                        aos[i] = make_double4(xv[j], yv[j], zv[j], 0.0);
                         for (int p = 0; p < passes; ++p)
                             atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
                     }
@@ -300,7 +294,6 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
             unsigned long long m = morton64(T, x, y);
             uint32_t k = T.tile_key | (uint32_t)(m >> shift);
             keys[i] = k;
-            aos[i] = make_double4(x, y, Z[i], 0.0);
             uint32_t *g = hist + (size_t)t * SORT_SEG_HIST;
             for (int p = 0; p < passes; ++p)
                 atomicAdd(&g[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
@@ -435,58 +428,45 @@ __device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, const 
     T.lf_parent[leaf] = OWM_NONE;
 }
 
-__device__ __forceinline__ void gather_put(TreeOut &T, uint32_t p, uint32_t v, const double4 &r)
-{
-    T.xs[p] = r.x;
-    T.ys[p] = r.y;
-    T.zs[p] = r.z;
-    T.ids[p] = v;
-    if (T.pos)
-        T.pos[v] = p;
-}
+constexpr int LEAF_G = 4;  // leaf groups a wave reduces at once
 
-constexpr int GATHER_G = 4;  // leaf groups a wave gathers at once
-
-// Gather into Morton order and per-leaf aggregates.  A wave owns groups of
+// Per-leaf aggregates over the points in Morton order.  A wave owns groups of
 // 64/leaf_size consecutive leaves of one tile (one leaf when leaf_size > 64,
-// its points strided over the lanes): every lane gathers one point per leaf
+// its points strided over the lanes): every lane reads one point per leaf
 // group and segmented butterflies reduce each leaf's tight bbox, count
 // (non-NaN x and y) and lowest (z, index).  With a power-of-two leaf size of at
-// most 64 a wave takes GATHER_G groups at once, so that all their index and
-// record loads are in flight together.
-__global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t *__restrict__ vals,
-                                                       const uint32_t *__restrict__ skeys,
-                                                       const double4 *__restrict__ aos, TreeOut T)
+// most 64 a wave takes LEAF_G groups at once, so that all their loads are in
+// flight together.
+__global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__restrict__ skeys,
+                                                   TreeOut T)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = gridDim.x * (TB / 64);
-    for (uint32_t g0 = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * GATHER_G; g0 < B.ngrp;
-         g0 += nw * GATHER_G) {
+    for (uint32_t g0 = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * LEAF_G; g0 < B.ngrp;
+         g0 += nw * LEAF_G) {
         const uint32_t t = owm_find_tile(B.grp_off, B.ntiles, g0);
         const TileDev &TD = B.tiles[t];
         const uint32_t L = TD.leaf_size;
-        if (L <= 64 && (L & (L - 1)) == 0 && g0 + GATHER_G <= B.grp_off[t + 1]) {
+        if (L <= 64 && (L & (L - 1)) == 0 && g0 + LEAF_G <= B.grp_off[t + 1]) {
             const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane & (L - 1);
-            uint32_t v[GATHER_G], p[GATHER_G], jl[GATHER_G];
-            bool have[GATHER_G];
+            uint32_t v[LEAF_G], jl[LEAF_G];
+            double x[LEAF_G], y[LEAF_G], z[LEAF_G];
+            bool have[LEAF_G];
 #pragma unroll
-            for (int q = 0; q < GATHER_G; ++q) {
+            for (int q = 0; q < LEAF_G; ++q) {
                 jl[q] = (g0 + q - TD.grp_off) * nl_grp + sub;
                 have[q] = jl[q] < TD.nleaf && jl[q] * L + off < TD.n;
-                p[q] = TD.pt_off + jl[q] * L + off;
-                v[q] = have[q] ? vals[p[q]] : 0u;
+                const uint32_t p = TD.pt_off + jl[q] * L + off;
+                v[q] = have[q] ? T.ids[p] : 0u;
+                x[q] = have[q] ? T.xs[p] : NAN;
+                y[q] = have[q] ? T.ys[p] : NAN;
+                z[q] = have[q] ? T.zs[p] : NAN;
             }
-            double4 r[GATHER_G];
 #pragma unroll
-            for (int q = 0; q < GATHER_G; ++q)
-                r[q] = have[q] ? aos[v[q]] : make_double4(NAN, NAN, NAN, 0.0);
-#pragma unroll
-            for (int q = 0; q < GATHER_G; ++q) {
+            for (int q = 0; q < LEAF_G; ++q) {
                 LeafAgg a = leaf_agg_empty();
-                if (have[q]) {
-                    gather_put(T, p[q], v[q], r[q]);
-                    leaf_agg_add(a, r[q].x, r[q].y, r[q].z, v[q]);
-                }
+                if (have[q])
+                    leaf_agg_add(a, x[q], y[q], z[q], v[q]);
                 leaf_agg_reduce(a, L);
                 if (off == 0 && jl[q] < TD.nleaf) {
                     const uint32_t start = TD.pt_off + jl[q] * L;
@@ -496,7 +476,7 @@ __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t
             continue;
         }
         // any leaf size, one group at a time
-        for (uint32_t g = g0; g < min(g0 + GATHER_G, B.ngrp); ++g) {
+        for (uint32_t g = g0; g < min(g0 + LEAF_G, B.ngrp); ++g) {
             const TileDev &TG = B.tiles[owm_find_tile(B.grp_off, B.ntiles, g)];
             const uint32_t LG = TG.leaf_size;
             const uint32_t nl_grp = LG <= 64 ? 64 / LG : 1;
@@ -509,10 +489,7 @@ __global__ __launch_bounds__(TB) void k_gather_leaves(BatchDev B, const uint32_t
             LeafAgg a = leaf_agg_empty();
             for (uint32_t k = off; k < npts; k += ls) {
                 const uint32_t pp = start + k;
-                const uint32_t vv = vals[pp];
-                const double4 rr = aos[vv];
-                gather_put(T, pp, vv, rr);
-                leaf_agg_add(a, rr.x, rr.y, rr.z, vv);
+                leaf_agg_add(a, T.xs[pp], T.ys[pp], T.zs[pp], T.ids[pp]);
             }
             seg_reduce(off, ls, [&](uint32_t o, bool take) {
                 uint32_t c2 = __shfl_down(a.cnt, o, 64);
@@ -853,17 +830,16 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 }
 
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   const double *z, uint32_t *keys, double4 *aos, uint32_t *hist, int passes)
+                   uint32_t *keys, uint32_t *hist, int passes)
 {
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, z, keys,
-                       aos, hist, passes);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, keys,
+                       hist, passes);
 }
 
-void launch_gather_leaves(hipStream_t s, const BatchDev &B, const uint32_t *vals,
-                          const uint32_t *skeys, const double4 *aos, const TreeOut &T)
+void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const uint32_t *skeys, const TreeOut &T)
 {
-    hipLaunchKernelGGL(k_gather_leaves, dim3(grid_for(B.ngrp, GATHER_G * TB / 64, 16384)), dim3(TB), 0, s, B,
-                       vals, skeys, aos, T);
+    hipLaunchKernelGGL(k_leaf_aggs, dim3(grid_for(B.ngrp, LEAF_G * TB / 64, 16384)), dim3(TB), 0, s,
+                       B, skeys, T);
 }
 
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
