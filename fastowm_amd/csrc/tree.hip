@@ -430,49 +430,89 @@ __device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, const 
 
 constexpr int LEAF_G = 4;  // leaf groups a wave reduces at once
 
+template <typename V>
+__device__ __forceinline__ V shfl_any(V v, int src)
+{
+    if constexpr (sizeof(V) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src, 64);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src, 64);
+        return __builtin_bit_cast(V, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(V, __shfl(__builtin_bit_cast(int, v), src, 64));
+    }
+}
+
 // Per-leaf aggregates over the points in Morton order.  A wave owns groups of
 // 64/leaf_size consecutive leaves of one tile (one leaf when leaf_size > 64,
 // its points strided over the lanes): every lane reads one point per leaf
 // group and segmented butterflies reduce each leaf's tight bbox, count
 // (non-NaN x and y) and lowest (z, index).  With a power-of-two leaf size of at
 // most 64 a wave takes LEAF_G groups at once, so that all their loads are in
-// flight together.
+// flight together, and the leaves' records are then written by consecutive
+// lanes (one store per field for all of them).  The group index is uniform over
+// the wave, so the tile lookup and the tile's fields are scalar loads.
 __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__restrict__ skeys,
                                                    TreeOut T)
 {
     const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (TB / 64);
-    for (uint32_t g0 = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * LEAF_G; g0 < B.ngrp;
-         g0 += nw * LEAF_G) {
+    for (uint32_t g0 = (blockIdx.x * (TB / 64) + wv) * LEAF_G; g0 < B.ngrp; g0 += nw * LEAF_G) {
         const uint32_t t = owm_find_tile(B.grp_off, B.ntiles, g0);
         const TileDev &TD = B.tiles[t];
         const uint32_t L = TD.leaf_size;
         if (L <= 64 && (L & (L - 1)) == 0 && g0 + LEAF_G <= B.grp_off[t + 1]) {
             const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane & (L - 1);
-            uint32_t v[LEAF_G], jl[LEAF_G];
+            const uint32_t jl0 = (g0 - TD.grp_off) * nl_grp;  // first leaf of the groups
+            uint32_t v[LEAF_G];
             double x[LEAF_G], y[LEAF_G], z[LEAF_G];
             bool have[LEAF_G];
 #pragma unroll
             for (int q = 0; q < LEAF_G; ++q) {
-                jl[q] = (g0 + q - TD.grp_off) * nl_grp + sub;
-                have[q] = jl[q] < TD.nleaf && jl[q] * L + off < TD.n;
-                const uint32_t p = TD.pt_off + jl[q] * L + off;
-                v[q] = have[q] ? T.ids[p] : 0u;
-                x[q] = have[q] ? T.xs[p] : NAN;
-                y[q] = have[q] ? T.ys[p] : NAN;
-                z[q] = have[q] ? T.zs[p] : NAN;
+                const uint32_t jl = jl0 + q * nl_grp + sub, e = jl * L + off;
+                have[q] = jl < TD.nleaf && e < TD.n;
+                // loads clamped to the tile's last point (no branch per load)
+                const uint32_t p = TD.pt_off + min(e, TD.n - 1);
+                v[q] = T.ids[p];
+                x[q] = T.xs[p];
+                y[q] = T.ys[p];
+                z[q] = T.zs[p];
             }
+            // lane m ends with the aggregate of leaf jl0 + m when the groups hold
+            // at most 64 leaves (leaf_size >= 4); smaller leaves are stored
+            // group by group from each leaf's first lane
+            const bool gather = LEAF_G * nl_grp <= 64;
+            LeafAgg mine = leaf_agg_empty();
 #pragma unroll
             for (int q = 0; q < LEAF_G; ++q) {
                 LeafAgg a = leaf_agg_empty();
                 if (have[q])
                     leaf_agg_add(a, x[q], y[q], z[q], v[q]);
                 leaf_agg_reduce(a, L);
-                if (off == 0 && jl[q] < TD.nleaf) {
-                    const uint32_t start = TD.pt_off + jl[q] * L;
-                    leaf_store(TD, T, skeys, jl[q], start, min(L, TD.n - jl[q] * L), a);
+                if (!gather) {
+                    const uint32_t jl = jl0 + q * nl_grp + sub;
+                    if (off == 0 && jl < TD.nleaf)
+                        leaf_store(TD, T, skeys, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), a);
+                    continue;
                 }
+                const bool take = lane / nl_grp == (uint32_t)q;
+                if (nl_grp > 1) {
+                    const int src = (int)((lane & (nl_grp - 1)) * L);
+                    a.mnx = shfl_any(a.mnx, src);
+                    a.mny = shfl_any(a.mny, src);
+                    a.mxx = shfl_any(a.mxx, src);
+                    a.mxy = shfl_any(a.mxy, src);
+                    a.cnt = shfl_any(a.cnt, src);
+                    a.bid = shfl_any(a.bid, src);
+                    a.bk = shfl_any(a.bk, src);
+                }
+                if (take)
+                    mine = a;
             }
+            const uint32_t jl = jl0 + lane;
+            if (gather && lane < LEAF_G * nl_grp && jl < TD.nleaf)
+                leaf_store(TD, T, skeys, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), mine);
             continue;
         }
         // any leaf size, one group at a time
