@@ -295,7 +295,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
     L.ngrp = L.grp_off[T];
     L.np = L.pt_off[T];
     // sort tiles of each tile's segment (a sort tile never straddles tiles)
-    const uint64_t tk = sort_tile_keys((uint32_t)std::min<uint64_t>(L.np, 0xFFFFFFFFu));
+    const uint64_t tk = record_tile_keys((uint32_t)std::min<uint64_t>(L.np, 0xFFFFFFFFu));
     L.sort_units.assign(T + 1, 0);
     for (uint32_t t = 0; t < T; ++t)
         L.sort_units[t + 1] = L.sort_units[t] + (uint32_t)((n[t] + tk - 1) / tk);

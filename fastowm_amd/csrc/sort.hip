@@ -52,7 +52,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
     return v;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over a block of NW waves.
+template <int NW = WAVES>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp)
 {
     uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -124,20 +125,21 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan(uint32_t *hist, int passes,
         counters[threadIdx.x] = 0;
 }
 
-// LDS state of one onesweep tile.
+// LDS state of one onesweep tile (a block of NW waves).
+template <int NW = WAVES>
 struct SweepLds {
-    uint32_t whist[WAVES][RADIX];  // per-wave digit counts, then wave-exclusive prefixes
-    uint32_t excl[RADIX];          // digit start within the tile
-    uint32_t glob[RADIX];          // digit start in the output minus excl
-    uint32_t tmp[WAVES];
+    uint32_t whist[NW][RADIX];  // per-wave digit counts, then wave-exclusive prefixes
+    uint32_t excl[RADIX];       // digit start within the tile
+    uint32_t glob[RADIX];       // digit start in the output minus excl
+    uint32_t tmp[NW];
     uint32_t tile;
 };
 
 // Takes the next tile from the counter and resolves its segment: input range
 // [base, n), digit offsets doff and whether it starts its look-back chain.
 // Returns false (uniformly) when no tile is left.
-template <int TILE_T>
-__device__ __forceinline__ bool sweep_claim(SweepLds &L, uint32_t *tile_counter, uint32_t n_all,
+template <int TILE_T, int NW>
+__device__ __forceinline__ bool sweep_claim(SweepLds<NW> &L, uint32_t *tile_counter, uint32_t n_all,
                                             const SortSegs &seg, const uint32_t *digit_off,
                                             uint32_t &tile, uint32_t &base, uint32_t &n, bool &first,
                                             const uint32_t *&doff)
@@ -146,7 +148,7 @@ __device__ __forceinline__ bool sweep_claim(SweepLds &L, uint32_t *tile_counter,
     const uint32_t ntiles = seg.pt ? seg.units[seg.nseg] : (n_all + TILE_T - 1) / TILE_T;
     if (tid == 0)
         L.tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < WAVES * RADIX; i += BLOCK)
+    for (int i = tid; i < NW * RADIX; i += NW * 64)
         (&L.whist[0][0])[i] = 0;
     __syncthreads();
     tile = L.tile;
@@ -173,8 +175,8 @@ __device__ __forceinline__ bool sweep_claim(SweepLds &L, uint32_t *tile_counter,
 // the tile holds ITEMS_T keys per thread (no element is past n).  PACK: rank[k]
 // is d << 16 | rank, and rank may be key itself (each key is read before its
 // rank is written).
-template <int ITEMS_T, bool FULL = false, bool PACK = false>
-__device__ __forceinline__ void sweep_rank(SweepLds &L, const uint32_t (&key)[ITEMS_T],
+template <int ITEMS_T, bool FULL = false, bool PACK = false, int NW = WAVES>
+__device__ __forceinline__ void sweep_rank(SweepLds<NW> &L, const uint32_t (&key)[ITEMS_T],
                                            uint32_t base, uint32_t n, int shift,
                                            uint32_t (&rank)[ITEMS_T])
 {
@@ -202,43 +204,50 @@ __device__ __forceinline__ void sweep_rank(SweepLds &L, const uint32_t (&key)[IT
 
 // After sweep_rank and a barrier: wave-exclusive prefixes per digit, the tile's
 // digit counts published and its global digit bases resolved by decoupled
-// look-back.  Ends with a barrier; then a key of digit d and rank r in wave w
-// sits at tile slot s = excl[d] + whist[w][d] + r and goes to output position
-// glob[d] + s (glob holds the digit's output start minus its tile start,
-// modulo 2^32).
-__device__ __forceinline__ void sweep_offsets(SweepLds &L, uint32_t tile, bool first,
+// look-back (thread d < RADIX handles digit d).  Ends with a barrier; then a key
+// of digit d and rank r in wave w sits at tile slot
+// s = excl[d] + whist[w][d] + r and goes to output position glob[d] + s (glob
+// holds the digit's output start minus its tile start, modulo 2^32).
+template <int NW>
+__device__ __forceinline__ void sweep_offsets(SweepLds<NW> &L, uint32_t tile, bool first,
                                               const uint32_t *doff, uint32_t *lookback)
 {
     const uint32_t d = threadIdx.x;
-    const uint32_t c0 = L.whist[0][d], c1 = L.whist[1][d], c2 = L.whist[2][d], c3 = L.whist[3][d];
-    const uint32_t total = c0 + c1 + c2 + c3;
-    L.whist[0][d] = 0;
-    L.whist[1][d] = c0;
-    L.whist[2][d] = c0 + c1;
-    L.whist[3][d] = c0 + c1 + c2;
-    uint32_t *flag = lookback + (size_t)tile * RADIX + d;
-    uint32_t prefix = 0;
-    if (first) {
-        __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int32_t j = (int32_t)tile - 1;
-        for (;;) {
-            const uint32_t f = __hip_atomic_load(lookback + (size_t)j * RADIX + d, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t st = f & ~LB_CNT;
-            if (st == 0)
-                continue;  // predecessor still ranking (it holds an earlier tile, so it is resident)
-            prefix += f & LB_CNT;
-            if (st == LB_INC)
-                break;
-            --j;
+    uint32_t total = 0, prefix = 0;
+    if (d < RADIX) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = L.whist[w][d];
+            L.whist[w][d] = total;
+            total += c;
         }
-        __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t *flag = lookback + (size_t)tile * RADIX + d;
+        if (first) {
+            __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int32_t j = (int32_t)tile - 1;
+            for (;;) {
+                const uint32_t f = __hip_atomic_load(lookback + (size_t)j * RADIX + d, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t st = f & ~LB_CNT;
+                if (st == 0)
+                    continue;  // predecessor still ranking (it holds an earlier tile, so it is resident)
+                prefix += f & LB_CNT;
+                if (st == LB_INC)
+                    break;
+                --j;
+            }
+            __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
-    const uint32_t ex = block_excl_scan(total, L.tmp);
-    L.excl[d] = ex;
-    L.glob[d] = doff[d] + prefix - ex;
+    // waves past the digits add zeros after them, so digit prefixes are exact
+    const uint32_t ex = block_excl_scan<NW>(total, L.tmp);
+    if (d < RADIX) {
+        L.excl[d] = ex;
+        L.glob[d] = doff[d] + prefix - ex;
+    }
     __syncthreads();
 }
 
@@ -253,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
                                                      int write_keys, SortSegs seg)
 {
     constexpr int TILE_T = BLOCK * ITEMS_T;
-    __shared__ SweepLds L;
+    __shared__ SweepLds<> L;
     __shared__ uint32_t s_keys[TILE_T];
     __shared__ uint32_t s_vals[KEYS_ONLY ? 1 : TILE_T];
 
@@ -305,8 +314,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // TILE_T points, so no load or store needs a bound check; otherwise loads are
 // clamped to the tile's last point and the items past it take the unused tile
 // slots past cnt, so that the LDS writes need no check either.
-template <bool IOTA, bool POS, int ITEMS_T, bool FULL>
-__device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t *s_dig,
+template <bool IOTA, bool POS, int ITEMS_T, int NW, bool FULL>
+__device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uint8_t *s_dig,
                                              const uint32_t *__restrict__ keys_in, const RecIn &in,
                                              uint32_t *__restrict__ keys_out, const RecOut &out,
                                              uint32_t tile, uint32_t base, uint32_t cnt, bool first,
@@ -341,7 +350,7 @@ __device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t
         for (int k = 0; k < ITEMS_T; ++k)
             kr[k] = keys_in[at(o, k)];
         load(in.x);
-        sweep_rank<ITEMS_T, FULL, true>(L, kr, 0, cnt, shift, kr);
+        sweep_rank<ITEMS_T, FULL, true, NW>(L, kr, 0, cnt, shift, kr);
         __syncthreads();
         sweep_offsets(L, tile, first, doff, lookback);
 #pragma unroll
@@ -366,7 +375,7 @@ __device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t
     // hoists every lookup and address and runs out of registers.
     auto store = [&](double *__restrict__ to) {
 #pragma unroll 2
-        for (uint32_t s = tid; s < cnt; s += BLOCK)
+        for (uint32_t s = tid; s < cnt; s += NW * 64)
             to[L.glob[s_dig[s]] + s] = s_buf[s];
         __syncthreads();
     };
@@ -397,7 +406,7 @@ __device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t
             s_u64[slot_of(k)] = kv[k];
         __syncthreads();
     }
-    for (uint32_t s = tid; s < cnt; s += BLOCK) {
+    for (uint32_t s = tid; s < cnt; s += NW * 64) {
         const unsigned long long kv = s_u64[s];
         const uint32_t dst = L.glob[s_dig[s]] + s, id = (uint32_t)(kv >> 32);
         keys_out[dst] = (uint32_t)kv;
@@ -418,28 +427,29 @@ __device__ __forceinline__ void records_tile(SweepLds &L, double *s_buf, uint8_t
 // so consecutive lanes write consecutive addresses.  The loads of a field are
 // issued before the stores of the previous one (those of x before the
 // ranking), so that their latency hides behind that work.
-template <bool IOTA, bool POS, int ITEMS_T>
-__global__ __launch_bounds__(BLOCK, 2) void k_sort_records(const uint32_t *__restrict__ keys_in,
-                                                            RecIn in, uint32_t *__restrict__ keys_out,
-                                                            RecOut out, uint32_t n_all, int shift,
-                                                            const uint32_t *__restrict__ digit_off,
-                                                            uint32_t *lookback, uint32_t *tile_counter,
-                                                            SortSegs seg)
+template <bool IOTA, bool POS, int ITEMS_T, int NW>
+__global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t *__restrict__ keys_in,
+                                                                   RecIn in, uint32_t *__restrict__ keys_out,
+                                                                   RecOut out, uint32_t n_all, int shift,
+                                                                   const uint32_t *__restrict__ digit_off,
+                                                                   uint32_t *lookback,
+                                                                   uint32_t *tile_counter, SortSegs seg)
 {
-    constexpr int TILE_T = BLOCK * ITEMS_T;
-    __shared__ SweepLds L;
+    constexpr int TILE_T = NW * 64 * ITEMS_T;
+    __shared__ SweepLds<NW> L;
     __shared__ double s_buf[TILE_T];
     __shared__ uint8_t s_dig[TILE_T];  // digit of every tile slot
     uint32_t tile, base, n;
     bool first;
     const uint32_t *doff;
-    while (sweep_claim<TILE_T>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
+    while (sweep_claim<TILE_T, NW>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
         if (n - base >= (uint32_t)TILE_T)
-            records_tile<IOTA, POS, ITEMS_T, true>(L, s_buf, s_dig, keys_in, in, keys_out, out, tile,
-                                                   base, TILE_T, first, doff, lookback, shift);
+            records_tile<IOTA, POS, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+                                                       tile, base, TILE_T, first, doff, lookback, shift);
         else
-            records_tile<IOTA, POS, ITEMS_T, false>(L, s_buf, s_dig, keys_in, in, keys_out, out, tile,
-                                                    base, n - base, first, doff, lookback, shift);
+            records_tile<IOTA, POS, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+                                                        tile, base, n - base, first, doff, lookback,
+                                                        shift);
     }
 }
 
@@ -487,7 +497,6 @@ void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uin
     hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, hist, pt, nseg, passes, counters);
 }
 
-uint32_t sort_tile_keys(uint32_t n) { return (uint32_t)(BLOCK * sort_items(n)); }
 
 template <int IT>
 static void onesweep_items(hipStream_t s, const uint32_t *keys_in, const uint32_t *vals_in,
@@ -530,24 +539,31 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
                            lookback, tile_counter, keys_only, iota, wk, grid, seg);
 }
 
-template <int IT>
-static void sort_records_items(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
+// Record sort tiles: 16384 points (8 waves x 32) for sorts of >= BIG_SORT
+// points, where each digit's run per tile is long enough (64 points) that the
+// scattered writes mostly fill whole cache lines; 4096 (4 waves x 16) below.
+static inline bool big_records(uint32_t n) { return n >= BIG_SORT; }
+uint32_t record_tile_keys(uint32_t n) { return big_records(n) ? 8 * 64 * 32 : 4 * 64 * 16; }
+
+template <int IT, int NW>
+static void sort_records_shape(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                                uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
                                bool iota, const SortSegs &seg, uint32_t grid)
 {
+    const dim3 b(NW * 64);
     if (iota && out.pos)
-        hipLaunchKernelGGL((k_sort_records<true, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        hipLaunchKernelGGL((k_sort_records<true, true, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
+                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
     else if (iota)
-        hipLaunchKernelGGL((k_sort_records<true, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        hipLaunchKernelGGL((k_sort_records<true, false, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
+                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
     else if (out.pos)
-        hipLaunchKernelGGL((k_sort_records<false, true, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        hipLaunchKernelGGL((k_sort_records<false, true, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
+                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
     else
-        hipLaunchKernelGGL((k_sort_records<false, false, IT>), dim3(grid), dim3(BLOCK), 0, s, keys_in,
-                           in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        hipLaunchKernelGGL((k_sort_records<false, false, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
+                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
 }
 
 void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
@@ -556,12 +572,12 @@ void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in
                          bool iota, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
-    if (sort_items(n) == 32)
-        sort_records_items<32>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                               tile_counter, iota, seg, grid);
+    if (big_records(n))
+        sort_records_shape<32, 8>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
+                                  tile_counter, iota, seg, grid);
     else
-        sort_records_items<16>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                               tile_counter, iota, seg, grid);
+        sort_records_shape<16, 4>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
+                                  tile_counter, iota, seg, grid);
 }
 
 }  // namespace owm
