@@ -59,11 +59,40 @@ __device__ __forceinline__ uint32_t cnt_2(const double *E, uint32_t n, double x,
     return 2 * k + (k < n && E[k] == x ? 1u : 0u);
 }
 
-// class of a coordinate against left edges L and right edges R (n each)
-__device__ __forceinline__ uint32_t cls(const double *L, const double *R, uint32_t n, double x,
-                                        double inv)
+// Edge lists of one window block along one axis, with the rounding tolerance
+// of the arithmetic class below.  Edges are E_i = fl(fl(init + fl(i*disp)) +- h);
+// every value involved is at most M = max(|init|, |first edge|, |last edge|) + h
+// in magnitude, so E_i = E_0 + i*disp + e with |e| <= 2^-50 M, and
+// f = fl(fl(x - E_0) * inv) is within 2^-50 M / disp of (x - E_0) / disp for
+// x in the block.  Hence when f is farther than tol = 2^-46 M / disp from every
+// integer, x lies strictly between E_floor(f) and E_floor(f)+1 (the bound holds
+// four times over), and #{E < x} = #{E <= x} = clamp(floor(f) + 1, 0, n).
+struct AxisEdges {
+    const double *L, *R;
+    uint32_t n;
+    double l0, r0, inv, tol;
+};
+
+__device__ __forceinline__ uint32_t cnt_2f(const double *E, uint32_t n, double e0, double x,
+                                           double inv, double tol)
 {
-    return cnt_2(L, n, x, inv) + cnt_2(R, n, x, inv);
+    const double f = (x - e0) * inv, fl = floor(f), fr = f - fl;
+    if (fr > tol && fr < 1.0 - tol)
+        return 2u * (uint32_t)fmin(fmax(fl + 1.0, 0.0), (double)n);
+    return cnt_2(E, n, x, inv);  // near an edge: exact compares
+}
+
+// class of a coordinate against the block's left and right edges
+__device__ __forceinline__ uint32_t cls_f(const AxisEdges &A, double x)
+{
+    return cnt_2f(A.L, A.n, A.l0, x, A.inv, A.tol) + cnt_2f(A.R, A.n, A.r0, x, A.inv, A.tol);
+}
+
+__device__ __forceinline__ AxisEdges axis_edges(const double *L, const double *R, uint32_t n,
+                                                double init, double hd, double inv)
+{
+    const double m = fmax(fabs(init), fmax(fabs(L[0]), fabs(R[n - 1]))) + hd;
+    return AxisEdges{L, R, n, L[0], R[0], inv, ldexp(m, -46) * inv};
 }
 
 // Windows [a, b] whose open extent (c_i - h, c_i + h) can hold a point of
@@ -304,7 +333,7 @@ constexpr uint32_t S1_ENT_LEAVES = 64 / LCS_SLOTS;  // summary leaves per unit o
 // USE_LCS compiles the leaf-summary items in (their registers cost occupancy
 // when no tile uses them).
 template <int BX, int BY, int S1_K, bool USE_LCS>
-__global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
+__global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
                                                         const uint32_t *__restrict__ blk_start,
                                                         const uint32_t *__restrict__ list,
                                                         uint32_t cap,
@@ -374,6 +403,8 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
             sCyh[tid - 64] = 2 * (cnt_lt(sLy, nj, r, inv) + cnt_lt(sRy, nj, r, inv));
         }
         const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
+        const AxisEdges AX = axis_edges(sLx, sRx, ni, TD.initx, hd, inv);
+        const AxisEdges AY = axis_edges(sLy, sRy, nj, TD.inity, hd, inv);
         const uint32_t GBL = sGLx[0], GBR = sGRx[ni - 1], GBB = sGLy[0], GBT = sGRy[nj - 1];
 
         const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
@@ -408,8 +439,8 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
                     // cell is never wider than the window step)
                     if (nseg == 1 && np > 0 && bx.z - bx.x < TD.displace && bx.w - bx.y < TD.displace &&
                         bx.x > BL && bx.z < BR && bx.y > BB && bx.w < BT) {
-                        uint32_t cx = cls(sLx, sRx, ni, bx.x, inv), cy = cls(sLy, sRy, nj, bx.y, inv);
-                        if (cx == cls(sLx, sRx, ni, bx.z, inv) && cy == cls(sLy, sRy, nj, bx.w, inv)) {
+                        uint32_t cx = cls_f(AX, bx.x), cy = cls_f(AY, bx.y);
+                        if (cx == cls_f(AX, bx.z) && cy == cls_f(AY, bx.w)) {
                             a = leaf;
                             n = S1_LEAFTAG | (cy * NXC + cx);
                         }
@@ -489,7 +520,7 @@ __global__ __launch_bounds__(S1_THREADS) void k_stage1(BatchDev B, TreeDev T, Le
                             cell[k] = OWM_NONE;
                             continue;
                         }
-                        cell[k] = cls(sLy, sRy, nj, y, inv) * NXC + cls(sLx, sRx, ni, x, inv);
+                        cell[k] = cls_f(AY, y) * NXC + cls_f(AX, x);
                         cn[k] = 1;
                         zk[k] = owm_eligible(zv[k]) ? owm_zkey(zv[k]) : OWM_ZKEY_NONE;
                     } else if (USE_LCS && (cell[k] & S1_LEAFTAG)) {

@@ -278,6 +278,54 @@ def test_points_on_edges_with_and_without_summaries(ctx, restore_table, summarie
     assert np.array_equal(ctx.run_tile(x, y, z, ws, bs, ov, leaf), want)
 
 
+def _near_edge_cloud(n, seed, side, origin):
+    """Integer lattice (window edges are integers with Wsize 10, Overlap 0.9)
+    moved by 0, 1-3 ulps, 1e-9, 1e-6 or 0.5 either way, at the origin or at
+    projected-grid magnitudes: points just inside and just outside the edges,
+    where the arithmetic class estimate must hand over to exact compares."""
+    rng = np.random.default_rng(seed)
+    x0 = origin[0] + rng.integers(0, side + 1, n).astype(np.float64)
+    y0 = origin[1] + rng.integers(0, side + 1, n).astype(np.float64)
+
+    def nudge(v):
+        kind = rng.integers(0, 9, len(v))
+        out = v.copy()
+        for k, steps in ((1, 1), (2, 2), (3, 3)):
+            m = kind == k
+            sgn = np.where(rng.integers(0, 2, m.sum()) == 1, np.inf, -np.inf)
+            w = out[m]
+            for _ in range(steps):
+                w = np.nextafter(w, sgn)
+            out[m] = w
+        for k, d in ((4, 1e-9), (5, 1e-6), (6, 0.5)):
+            m = kind == k
+            out[m] += d * np.where(rng.integers(0, 2, m.sum()) == 1, 1.0, -1.0)
+        return out
+
+    x, y = nudge(x0), nudge(y0)
+    z = np.round(rng.normal(50.0, 3.0, n), 1)
+    x[0], y[0], x[1], y[1] = origin[0], origin[1], origin[0] + side, origin[1] + side
+    return x, y, z
+
+
+@pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
+def test_points_ulps_from_edges(ctx, restore_table, origin):
+    """Stage 1 classifies by arithmetic away from edges and by exact compares
+    near them; points a few ulps either side of an edge, at small and at
+    projected-grid coordinates, must give the brute-force window scan."""
+    _stage1_table(ctx, "off")
+    say("near edges", origin)
+    x, y, z = _near_edge_cloud(40_000, seed=int(origin[0]) % 1000 + 7, side=200, origin=origin)
+    box = synth.bbox(x, y)
+    p = O.params(10, 20, 0.9, box, len(x), 16)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=16)
+    gm, _ = ctx.stage1(tree, 10, 0.9, 20)
+    assert np.array_equal(gm, O.stage1_brute(x, y, z, p))
+    tree.free()
+    want, _, _ = O.run(x, y, z, 10, 20, 0.9, 16)
+    assert np.array_equal(ctx.run_tile(x, y, z, 10, 20, 0.9, 16), want)
+
+
 @pytest.mark.parametrize("summaries", ["on", "off"])
 def test_c2_scaled_with_and_without_summaries(ctx, restore_table, summaries):
     _stage1_table(ctx, summaries)
