@@ -43,6 +43,20 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
 
 static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
+// Regions of the pinned readback buffer for n_tiles tiles.
+struct IoLayout {
+    uint64_t ptoff, bbox, counts, seed_off, bytes;
+    explicit IoLayout(uint32_t n_tiles)
+    {
+        const uint64_t off = align_up((n_tiles + 1) * 4ull, 256);
+        ptoff = 0;
+        bbox = off;
+        counts = bbox + align_up(n_tiles * 32ull, 256);
+        seed_off = counts + 256;
+        bytes = seed_off + off;
+    }
+};
+
 static const char *PHASE_NAMES[PH_COUNT] = {
     "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "bottomup", "stage1",
     "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
@@ -658,8 +672,10 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         return OWM_OK;
     }
     int which = run_sort(ctx, PH_MERGE, s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false);
-    out.seeds = which ? SD.k1 : SD.k0;
-    launch_seed_ranges(s, B, out.seeds, cnt + 2, seed_off);
+    uint32_t *merged = which ? SD.k1 : SD.k0;
+    out.seeds = merged;
+    launch_seed_ranges(s, B, merged, cnt + 2, seed_off);
+    launch_seeds_local(s, B, merged, cnt + 2, (uint32_t)nseed_max);
     HIP_TRY(hipGetLastError(), "query launch");
     return OWM_OK;
 }
@@ -780,8 +796,14 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
             return st;
         unsigned long long *bk = (unsigned long long *)ctx->small.take(bk_bytes);
         uint32_t *ptoff = (uint32_t *)ctx->small.take((n_tiles + 1) * 4ull);
-        HIP_TRY(hipMemcpyAsync(ptoff, P.L.pt_off.data(), (n_tiles + 1) * 4ull,
-                               hipMemcpyHostToDevice, s),
+        const IoLayout io(n_tiles);
+        uint8_t *h = ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
+        if (!h) {
+            set_err(ctx, "pinned host allocation failed");
+            return OWM_E_NOMEM;
+        }
+        memcpy(h + io.ptoff, P.L.pt_off.data(), (n_tiles + 1) * 4ull);
+        HIP_TRY(hipMemcpyAsync(ptoff, h + io.ptoff, (n_tiles + 1) * 4ull, hipMemcpyHostToDevice, s),
                 "upload offsets");
         BatchDev B0{};
         B0.ntiles = n_tiles;
@@ -792,8 +814,8 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
         if (np)
             launch_bbox(s, B0, dx, dy, bk);
         ctx->prof.end(s, pi, 16ull * np, np);
-        std::vector<unsigned long long> hk(4ull * n_tiles);
-        HIP_TRY(hipMemcpyAsync(hk.data(), bk, n_tiles * 32ull, hipMemcpyDeviceToHost, s), "bbox D2H");
+        const unsigned long long *hk = (const unsigned long long *)(h + io.bbox);
+        HIP_TRY(hipMemcpyAsync(h + io.bbox, bk, n_tiles * 32ull, hipMemcpyDeviceToHost, s), "bbox D2H");
         HIP_TRY(hipStreamSynchronize(s), "bbox sync");
         ctx->prof.collect();
         auto dec = [](unsigned long long k) {
@@ -897,20 +919,23 @@ owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
 {
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
-    TileDev *d_tiles = (TileDev *)A.take(n_tiles * sizeof(TileDev));
-    uint32_t *d_off[8];
-    for (int k = 0; k < 8; ++k)
-        d_off[k] = (uint32_t *)A.take((n_tiles + 1) * 4ull);
-    if (!d_tiles || !d_off[7]) {
+    // tiles, then the eight offset arrays, as staged by stage_table
+    uint8_t *d = A.take(table_bytes(n_tiles));
+    if (!d) {
         set_err(ctx, "scratch arena exhausted (table)");
         return OWM_E_INTERNAL;
     }
-    ctx->stage_table(P.td, P.L, P.Q);
-    HIP_TRY(hipMemcpyAsync(d_tiles, ctx->h_tiles, n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s),
-            "upload tiles");
-    for (int k = 0; k < 8; ++k)
-        HIP_TRY(hipMemcpyAsync(d_off[k], ctx->h_off[k], (n_tiles + 1) * 4ull, hipMemcpyHostToDevice, s),
-                "upload offsets");
+    if (!ctx->stage_table(P.td, P.L, P.Q)) {
+        set_err(ctx, "pinned host allocation failed");
+        return OWM_E_NOMEM;
+    }
+    HIP_TRY(hipMemcpyAsync(d, ctx->h_tab, table_bytes(n_tiles), hipMemcpyHostToDevice, s),
+            "upload table");
+    TileDev *d_tiles = (TileDev *)d;
+    uint32_t *d_off[8];
+    uint64_t o = align_up((uint64_t)n_tiles * sizeof(TileDev), 256);
+    for (int k = 0; k < 8; ++k, o += align_up((n_tiles + 1) * 4ull, 256))
+        d_off[k] = (uint32_t *)(d + o);
     B = BatchDev{};
     B.ntiles = n_tiles;
     B.np = (uint32_t)P.L.np;
@@ -972,10 +997,16 @@ owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint3
 {
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
-    uint32_t hc[16];
-    std::vector<uint32_t> hoff(n_tiles + 1);
-    HIP_TRY(hipMemcpyAsync(hc, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, s), "counts D2H");
-    HIP_TRY(hipMemcpyAsync(hoff.data(), qo.seed_off, (n_tiles + 1) * 4ull, hipMemcpyDeviceToHost, s),
+    const IoLayout io(n_tiles);
+    uint8_t *h = ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
+    if (!h) {
+        set_err(ctx, "pinned host allocation failed");
+        return OWM_E_NOMEM;
+    }
+    const uint32_t *hc = (const uint32_t *)(h + io.counts);
+    const uint32_t *hoff = (const uint32_t *)(h + io.seed_off);
+    HIP_TRY(hipMemcpyAsync(h + io.counts, qo.cnt, 16 * 4, hipMemcpyDeviceToHost, s), "counts D2H");
+    HIP_TRY(hipMemcpyAsync(h + io.seed_off, qo.seed_off, (n_tiles + 1) * 4ull, hipMemcpyDeviceToHost, s),
             "ranges D2H");
     HIP_TRY(hipStreamSynchronize(s), "pipeline");
     ctx->prof.collect();
@@ -990,11 +1021,9 @@ owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint3
                 (unsigned long long)ns);
         return OWM_E_CAPACITY;
     }
+    // indices are tile-local already (launch_seeds_local)
     if (ns)
         HIP_TRY(hipMemcpy(seeds, qo.seeds, ns * 4ull, hipMemcpyDeviceToHost), "seeds D2H");
-    for (uint32_t t = 0; t < n_tiles; ++t)
-        for (uint64_t i = hoff[t]; i < hoff[t + 1]; ++i)
-            seeds[i] -= P.L.pt_off[t];
     return OWM_OK;
 }
 
@@ -1168,28 +1197,42 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
 using owm::TreeLayout;
 using owm::QueryLayout;
 
-void owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, const QueryLayout &Q)
+uint8_t *owm_ctx::pinned(uint8_t *&buf, uint64_t &cap, uint64_t need)
+{
+    if (need <= cap)
+        return buf;
+    // copies from or into the old buffer must have completed
+    (void)hipStreamSynchronize(stream);
+    if (buf)
+        (void)hipHostFree(buf);
+    buf = nullptr;
+    cap = 0;
+    const uint64_t want = need + need / 4 + 4096;
+    if (hipHostMalloc((void **)&buf, want, 0) != hipSuccess) {
+        buf = nullptr;
+        return nullptr;
+    }
+    cap = want;
+    return buf;
+}
+
+bool owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, const QueryLayout &Q)
 {
     const uint32_t T = (uint32_t)td.size();
-    if (h_cap < T + 1) {
-        if (h_tiles)
-            (void)hipHostFree(h_tiles);
-        for (auto &p : h_off)
-            if (p)
-                (void)hipHostFree(p);
-        h_cap = T + 1 + 64;
-        (void)hipHostMalloc((void **)&h_tiles, h_cap * sizeof(TileDev), 0);
-        for (auto &p : h_off)
-            (void)hipHostMalloc((void **)&p, h_cap * 4, 0);
-    }
-    // the previous upload from these buffers must have completed
+    const uint64_t bytes = owm::table_bytes(T);
+    // the previous upload from this buffer must have completed
     (void)hipStreamSynchronize(stream);
-    memcpy(h_tiles, td.data(), T * sizeof(TileDev));
+    uint8_t *h = pinned(h_tab, h_tab_cap, bytes);
+    if (!h)
+        return false;
+    memcpy(h, td.data(), T * sizeof(TileDev));
     const std::vector<uint32_t> *src[8] = {&L.pt_off,   &L.leaf_off, &L.node_off,
                                            &Q.win_off,  &Q.cell_off, &Q.blk_off,
                                            &L.grp_off,  &L.sort_units};
-    for (int k = 0; k < 8; ++k)
-        memcpy(h_off[k], src[k]->data(), (T + 1) * 4);
+    uint64_t o = owm::align_up((uint64_t)T * sizeof(TileDev), 256);
+    for (int k = 0; k < 8; ++k, o += owm::align_up((T + 1) * 4ull, 256))
+        memcpy(h + o, src[k]->data(), (T + 1) * 4);
+    return true;
 }
 
 uint64_t owm_ctx::room_for(const owm::DevBuf &b) const

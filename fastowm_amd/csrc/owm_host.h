@@ -243,13 +243,19 @@ struct owm_ctx {
     uint64_t s1_need_max = 0;  // largest join that overflowed its first estimate
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};
-    ::TileDev *h_tiles = nullptr;
-    uint32_t *h_off[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    uint32_t h_cap = 0;
+    // pinned host staging: the tile table (tiles, then eight offset arrays, in
+    // the device layout, uploaded by one copy) and small readbacks (tile point
+    // offsets, bbox keys, counts, seed ranges)
+    uint8_t *h_tab = nullptr;
+    uint64_t h_tab_cap = 0;
+    uint8_t *h_io = nullptr;
+    uint64_t h_io_cap = 0;
+    uint8_t *pinned(uint8_t *&buf, uint64_t &cap, uint64_t need);
     owm::ServeState *serve = nullptr;
     owm::Profiler prof;  // owned; released by owm_serve_stop
     owm::GraphCache graphs;  // captured batch chains (OWM_B_GRAPH)
-    void stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
+    // copies the tile table into h_tab (false: no pinned memory)
+    bool stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
                      const owm::QueryLayout &Q);
     uint64_t held_bytes() const;
     // Largest size buffer `b` may grow to without exceeding the budget.

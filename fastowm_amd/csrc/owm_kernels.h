@@ -120,6 +120,9 @@ void launch_compact_valid(hipStream_t s, const uint32_t *in, uint32_t n, uint32_
                           uint32_t *tile_counter);
 void launch_seed_ranges(hipStream_t s, const BatchDev &B, const uint32_t *seeds,
                         const uint32_t *d_nseeds, uint32_t *seed_off);
+// seeds (sorted batch indices, count at d_nseeds <= n_max) -> tile-local indices
+void launch_seeds_local(hipStream_t s, const BatchDev &B, uint32_t *seeds, const uint32_t *d_nseeds,
+                        uint32_t n_max);
 void launch_add_u32(hipStream_t s, const uint32_t *a, const uint32_t *b, uint32_t *out);
 
 }  // namespace owm

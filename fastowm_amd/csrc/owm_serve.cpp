@@ -275,11 +275,10 @@ static owm_status copy_result(owm_ctx *ctx, Request &r, uint32_t *seeds, uint64_
 
 owm_ctx::~owm_ctx()
 {
-    if (h_tiles)
-        (void)hipHostFree(h_tiles);
-    for (auto &p : h_off)
-        if (p)
-            (void)hipHostFree(p);
+    if (h_tab)
+        (void)hipHostFree(h_tab);
+    if (h_io)
+        (void)hipHostFree(h_io);
     if (stream)
         (void)hipStreamDestroy(stream);
 }

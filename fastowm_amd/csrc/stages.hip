@@ -321,6 +321,18 @@ __global__ void k_seed_ranges(BatchDev B, const uint32_t *__restrict__ seeds, co
     }
 }
 
+// Batch indices of the merged seeds -> indices within their tiles (after the
+// ranges: the seeds are sorted by batch index, so the tile of seed i is the one
+// whose point range holds it).
+__global__ void k_seeds_local(BatchDev B, uint32_t *__restrict__ seeds, const uint32_t *d_ns)
+{
+    const uint32_t ns = *d_ns;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        const uint32_t v = seeds[i];
+        seeds[i] = v - B.pt_off[owm_find_tile(B.pt_off, B.ntiles, v)];
+    }
+}
+
 __global__ void k_add_u32(const uint32_t *a, const uint32_t *b, uint32_t *out) { *out = *a + *b; }
 
 // ----------------------------------------------------------------- launch --
@@ -372,6 +384,16 @@ void launch_seed_ranges(hipStream_t s, const BatchDev &B, const uint32_t *seeds,
 {
     uint32_t g = (B.ntiles + 1 + 255) / 256;
     hipLaunchKernelGGL(k_seed_ranges, dim3(g), dim3(256), 0, s, B, seeds, d_nseeds, seed_off);
+}
+
+void launch_seeds_local(hipStream_t s, const BatchDev &B, uint32_t *seeds, const uint32_t *d_nseeds,
+                        uint32_t n_max)
+{
+    if (B.ntiles < 2)
+        return;  // one tile: batch and tile indices coincide
+    uint32_t g = (n_max + 255) / 256;
+    g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
+    hipLaunchKernelGGL(k_seeds_local, dim3(g), dim3(256), 0, s, B, seeds, d_nseeds);
 }
 
 void launch_add_u32(hipStream_t s, const uint32_t *a, const uint32_t *b, uint32_t *out)

@@ -222,6 +222,15 @@ class Context:
             raise OWMError(st, f"cannot create a context on device {device}")
         self.h = h
         self._trees = weakref.WeakSet()
+        self._seed_buf = np.empty(0, np.uint32)
+
+    def _seeds_out(self, cap):
+        """Reusable host buffer the library copies seeds into (results are
+        copied out of it): a fresh array per call would cost the page faults
+        of an n-sized allocation every time.  It only grows."""
+        if len(self._seed_buf) < cap:
+            self._seed_buf = np.empty(cap, np.uint32)
+        return self._seed_buf
 
     def close(self):
         if getattr(self, "h", None):
@@ -273,7 +282,7 @@ class Context:
         p = make_params(wsize, bsize, overlap, leaf_size, bbox)
         n = len(x)
         cap = n + 1 if cap is None else cap
-        out = np.empty(max(cap, 1), np.uint32)
+        out = self._seeds_out(max(cap, 1))
         ns = ctypes.c_uint64()
         if _is_device(x):
             st = lib().owm_run_tile_device(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), n,
@@ -302,7 +311,7 @@ class Context:
         captures the device chain once per batch layout and replays it."""
         descs, off = self._descs(tiles)
         cap = off + len(tiles) + 1
-        out = np.empty(cap, np.uint32)
+        out = self._seeds_out(cap)
         so = np.zeros(len(tiles) + 1, np.uint64)
         flags = B_GRAPH if graph else 0
         if _is_device(x):
