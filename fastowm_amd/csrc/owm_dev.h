@@ -22,7 +22,6 @@ struct TileDev {
     uint32_t leaf_off, nleaf;
     uint32_t node_off, leaf_size;
     uint32_t root;      // child reference of the root (OWM_LEAF_BIT | leaf) or internal
-    uint32_t tile_key;  // high Morton key bits identifying the tile in a batch sort
     uint32_t grp_off;   // first leaf group (64/leaf_size leaves per wave) of the tile
     uint32_t pad_t;
     double qminx, qminy, qscale;  // quantisation: q = (x - qminx) * qscale in [0, 2^32)

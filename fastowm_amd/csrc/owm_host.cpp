@@ -291,7 +291,6 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
         D.leaf_off = L.leaf_off[t];
         D.node_off = L.node_off[t];
         D.root = D.nleaf == 0 ? OWM_NONE : (D.nleaf == 1 ? (OWM_LEAF_BIT | D.leaf_off) : D.node_off);
-        D.tile_key = 0;
         const auto &b = qbox[t];
         double side = std::max(b[1] - b[0], b[3] - b[2]);
         if (!(side > 0.0) || !std::isfinite(side))
@@ -536,16 +535,17 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         }
         const RecOut &out = p == passes - 1 ? tree_rec : rec[p & 1];
         pi = ctx->prof.begin(s, PH_SORT_TREE);
+        const bool last = p == passes - 1;
         launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p, S.hist + 256 * p,
-                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0, seg, tiles);
+                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0, last,
+                            seg, tiles);
         // algorithmic bytes per point and pass: a key and a 28-byte record
-        // read and written
-        ctx->prof.end(s, pi, 64ull * L.np, L.np);
+        // read and written (the last pass writes no key)
+        ctx->prof.end(s, pi, (last ? 60ull : 64ull) * L.np, L.np);
         std::swap(kin, kout);
     }
-    const uint32_t *skeys = kin;
     pi = ctx->prof.begin(s, PH_LEAVES);
-    launch_leaf_aggs(s, B, skeys, T);
+    launch_leaf_aggs(s, B, T);
     // algorithmic bytes: x, y, z and index read per point; 48 per leaf written
     ctx->prof.end(s, pi, 28ull * L.np + 48ull * L.nleaf, L.np);
     pi = ctx->prof.begin(s, PH_KARRAS);

@@ -44,11 +44,12 @@ struct RecOut {
     uint32_t *id, *pos;
 };
 // One record-moving pass of the segmented tree sort over n points (segments
-// seg, seg_tiles sort tiles in all); iota on the first pass.
+// seg, seg_tiles sort tiles in all): the first pass numbers the points itself,
+// the last writes no keys (keys_out unused).
 void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                          uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
                          const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool iota, const SortSegs &seg, uint32_t seg_tiles);
+                         bool first, bool last, const SortSegs &seg, uint32_t seg_tiles);
 
 // ---- tree (tree.hip) ----
 struct TreeOut {
@@ -73,8 +74,9 @@ void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t nti
 // hist: per tile SORT_SEG_HIST digit counters of every pass
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                    uint32_t *keys, uint32_t *hist, int passes);
-// Leaf aggregates over the points in Morton order (T.xs, ys, zs, ids).
-void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const uint32_t *sorted_keys, const TreeOut &T);
+// Leaf aggregates over the points in Morton order (T.xs, ys, zs, ids); each
+// leaf's first key is recomputed from its first point.
+void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T);
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
 // Bottom-up aggregation: chunk-local climb with LDS arrival flags, then a
 // global climb over the nodes whose leaf range crosses a chunk.  `work` holds

@@ -19,8 +19,9 @@
 //
 // Record form (the tree sort): every pass moves the points' x, y, z and batch
 // index with the keys, the first reading the caller's coordinate arrays and
-// the last writing the tree's, so the cloud ends in Morton order without a
-// gather by index (random 32-byte reads run at a fraction of HBM bandwidth).
+// the last writing the tree's (without keys: the tree recomputes the few it
+// needs), so the cloud ends in Morton order without a gather by index (random
+// 32-byte reads run at a fraction of HBM bandwidth).
 #include "owm_kernels.h"
 
 namespace owm {
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // TILE_T points, so no load or store needs a bound check; otherwise loads are
 // clamped to the tile's last point and the items past it take the unused tile
 // slots past cnt, so that the LDS writes need no check either.
-template <bool IOTA, bool POS, int ITEMS_T, int NW, bool FULL>
+template <bool IOTA, bool LAST, bool POS, int ITEMS_T, int NW, bool FULL>
 __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uint8_t *s_dig,
                                              const uint32_t *__restrict__ keys_in, const RecIn &in,
                                              uint32_t *__restrict__ keys_out, const RecOut &out,
@@ -391,6 +392,28 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
     load(in.z);
     store(out.y);
     stage();
+    if constexpr (LAST) {
+        // the last pass writes no keys: the index alone
+        const uint32_t o = fresh();
+        uint32_t id[ITEMS_T];
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            id[k] = IOTA ? base + o + k * 64 : in.id[at(o, k)];
+        store(out.z);
+        uint32_t *const s_u32 = reinterpret_cast<uint32_t *>(s_buf);
+#pragma unroll
+        for (int k = 0; k < ITEMS_T; ++k)
+            s_u32[slot_of(k)] = id[k];
+        __syncthreads();
+        for (uint32_t s = tid; s < cnt; s += NW * 64) {
+            const uint32_t v = s_u32[s], dst = L.glob[s_dig[s]] + s;
+            out.id[dst] = v;
+            if (POS)
+                out.pos[v] = dst;
+        }
+        __syncthreads();
+        return;
+    }
     // key and index travel together as one 8-byte word
     {
         unsigned long long kv[ITEMS_T];
@@ -408,11 +431,9 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
     }
     for (uint32_t s = tid; s < cnt; s += NW * 64) {
         const unsigned long long kv = s_u64[s];
-        const uint32_t dst = L.glob[s_dig[s]] + s, id = (uint32_t)(kv >> 32);
+        const uint32_t dst = L.glob[s_dig[s]] + s;
         keys_out[dst] = (uint32_t)kv;
-        out.id[dst] = id;
-        if (POS)
-            out.pos[id] = dst;
+        out.id[dst] = (uint32_t)(kv >> 32);
     }
     __syncthreads();
 }
@@ -422,12 +443,13 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
 // by index follows.  Records are x, y, z (8 bytes each) and the point's batch
 // index.  The first pass reads the caller's coordinate arrays and numbers the
 // points itself (IOTA); the last writes the tree's arrays (and, with POS, each
-// point's sorted position at its index).  Ranks come from the keys alone; each
+// point's sorted position at its index; LAST: no keys are written).  Ranks come
+// from the keys alone; each
 // field is then staged through one LDS buffer in digit order and written out,
 // so consecutive lanes write consecutive addresses.  The loads of a field are
 // issued before the stores of the previous one (those of x before the
 // ranking), so that their latency hides behind that work.
-template <bool IOTA, bool POS, int ITEMS_T, int NW>
+template <bool IOTA, bool LAST, bool POS, int ITEMS_T, int NW>
 __global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t *__restrict__ keys_in,
                                                                    RecIn in, uint32_t *__restrict__ keys_out,
                                                                    RecOut out, uint32_t n_all, int shift,
@@ -444,10 +466,10 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t
     const uint32_t *doff;
     while (sweep_claim<TILE_T, NW>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
         if (n - base >= (uint32_t)TILE_T)
-            records_tile<IOTA, POS, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+            records_tile<IOTA, LAST, POS, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
                                                        tile, base, TILE_T, first, doff, lookback, shift);
         else
-            records_tile<IOTA, POS, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+            records_tile<IOTA, LAST, POS, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
                                                         tile, base, n - base, first, doff, lookback,
                                                         shift);
     }
@@ -549,35 +571,39 @@ template <int IT, int NW>
 static void sort_records_shape(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                                uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                               bool iota, const SortSegs &seg, uint32_t grid)
+                               bool first, bool last, const SortSegs &seg, uint32_t grid)
 {
     const dim3 b(NW * 64);
-    if (iota && out.pos)
-        hipLaunchKernelGGL((k_sort_records<true, true, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
-                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
-    else if (iota)
-        hipLaunchKernelGGL((k_sort_records<true, false, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
-                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+#define OWM_REC(IOTA, LAST, POS)                                                                  \
+    hipLaunchKernelGGL((k_sort_records<IOTA, LAST, POS, IT, NW>), dim3(grid), b, 0, s, keys_in, in, \
+                       keys_out, out, n, shift, digit_off, lookback, tile_counter, seg)
+    if (first && last && out.pos)
+        OWM_REC(true, true, true);  // a single pass
+    else if (first && last)
+        OWM_REC(true, true, false);
+    else if (first)
+        OWM_REC(true, false, false);
+    else if (!last)
+        OWM_REC(false, false, false);
     else if (out.pos)
-        hipLaunchKernelGGL((k_sort_records<false, true, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
-                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        OWM_REC(false, true, true);
     else
-        hipLaunchKernelGGL((k_sort_records<false, false, IT, NW>), dim3(grid), b, 0, s, keys_in, in,
-                           keys_out, out, n, shift, digit_off, lookback, tile_counter, seg);
+        OWM_REC(false, true, false);
+#undef OWM_REC
 }
 
 void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                          uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
                          const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool iota, const SortSegs &seg, uint32_t seg_tiles)
+                         bool first, bool last, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
     if (big_records(n))
         sort_records_shape<32, 8>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, iota, seg, grid);
+                                  tile_counter, first, last, seg, grid);
     else
         sort_records_shape<16, 4>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, iota, seg, grid);
+                                  tile_counter, first, last, seg, grid);
 }
 
 }  // namespace owm

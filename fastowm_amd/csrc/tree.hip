@@ -225,6 +225,14 @@ __device__ __forceinline__ unsigned long long morton64(const TileDev &T, double 
     return spread2(qx) | (spread2(qy) << 1);
 }
 
+// Sort key of a point: the top bits of its Morton code (shift = 64 - key bits).
+// The tree pass recomputes a leaf's first key from its first point with this
+// same function, so the last sort pass need not write keys.
+__device__ __forceinline__ uint32_t sort_key(const TileDev &T, double x, double y, uint32_t shift)
+{
+    return (uint32_t)(morton64(T, x, y) >> shift);
+}
+
 // Sort keys (the top key_bits of each point's Morton code within its tile)
 // plus each tile's digit histograms of every radix pass.
 __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restrict__ X,
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
                 for (int j = 0; j < 8; ++j) {
                     const uint32_t i = base + j * TB;
                     if (i < c1) {
-                        const uint32_t k = T.tile_key | (uint32_t)(morton64(T, xv[j], yv[j]) >> shift);
+                        const uint32_t k = sort_key(T, xv[j], yv[j], shift);
                         keys[i] = k;
                         for (int p = 0; p < passes; ++p)
                             atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
@@ -291,8 +299,7 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
                 next = B.pt_off[++t + 1];
             const TileDev &T = B.tiles[t];
             const double x = X[i], y = Y[i];
-            unsigned long long m = morton64(T, x, y);
-            uint32_t k = T.tile_key | (uint32_t)(m >> shift);
+            const uint32_t k = sort_key(T, x, y, shift);
             keys[i] = k;
             uint32_t *g = hist + (size_t)t * SORT_SEG_HIST;
             for (int p = 0; p < passes; ++p)
@@ -414,8 +421,8 @@ __device__ __forceinline__ LeafAgg leaf_agg_empty()
     return LeafAgg{INFINITY, INFINITY, -INFINITY, -INFINITY, 0u, OWM_NONE, OWM_ZKEY_NONE};
 }
 
-__device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, const uint32_t *skeys,
-                                           uint32_t jl, uint32_t start, uint32_t npts, const LeafAgg &a)
+__device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, uint32_t key0, uint32_t jl,
+                                           uint32_t start, uint32_t npts, const LeafAgg &a)
 {
     const uint32_t leaf = TD.leaf_off + jl;
     T.lf_box[leaf] = make_double4(a.mnx, a.mny, a.mxx, a.mxy);
@@ -424,7 +431,7 @@ __device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, const 
     T.lf_npts[leaf] = npts;
     T.lf_zkey[leaf] = a.bk;
     T.lf_zid[leaf] = a.bid;
-    T.lf_key0[leaf] = skeys[start];
+    T.lf_key0[leaf] = key0;
     T.lf_parent[leaf] = OWM_NONE;
 }
 
@@ -452,9 +459,9 @@ __device__ __forceinline__ V shfl_any(V v, int src)
 // flight together, and the leaves' records are then written by consecutive
 // lanes (one store per field for all of them).  The group index is uniform over
 // the wave, so the tile lookup and the tile's fields are scalar loads.
-__global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__restrict__ skeys,
-                                                   TreeOut T)
+__global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
 {
+    const uint32_t shift = 64 - B.key_bits;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (TB / 64);
@@ -484,6 +491,7 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__
             // group by group from each leaf's first lane
             const bool gather = LEAF_G * nl_grp <= 64;
             LeafAgg mine = leaf_agg_empty();
+            double mine_x = 0.0, mine_y = 0.0;  // the leaf's first point
 #pragma unroll
             for (int q = 0; q < LEAF_G; ++q) {
                 LeafAgg a = leaf_agg_empty();
@@ -492,8 +500,9 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__
                 leaf_agg_reduce(a, L);
                 if (!gather) {
                     const uint32_t jl = jl0 + q * nl_grp + sub;
+                    const uint32_t k0 = off == 0 && have[q] ? sort_key(TD, x[q], y[q], shift) : 0u;
                     if (off == 0 && jl < TD.nleaf)
-                        leaf_store(TD, T, skeys, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), a);
+                        leaf_store(TD, T, k0, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), a);
                     continue;
                 }
                 const bool take = lane / nl_grp == (uint32_t)q;
@@ -507,12 +516,19 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__
                     a.bid = shfl_any(a.bid, src);
                     a.bk = shfl_any(a.bk, src);
                 }
-                if (take)
+                const int first = nl_grp > 1 ? (int)((lane & (nl_grp - 1)) * L) : 0;
+                const double fx = shfl_any(x[q], first), fy = shfl_any(y[q], first);
+                if (take) {
                     mine = a;
+                    mine_x = fx;
+                    mine_y = fy;
+                }
             }
+            // every leaf's first key from its first point, once per wave
+            const uint32_t mine_key = sort_key(TD, mine_x, mine_y, shift);
             const uint32_t jl = jl0 + lane;
             if (gather && lane < LEAF_G * nl_grp && jl < TD.nleaf)
-                leaf_store(TD, T, skeys, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), mine);
+                leaf_store(TD, T, mine_key, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), mine);
             continue;
         }
         // any leaf size, one group at a time
@@ -527,9 +543,13 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__
             const uint32_t start = TG.pt_off + jl * LG;
             const uint32_t npts = active ? min(LG, TG.n - jl * LG) : 0u;
             LeafAgg a = leaf_agg_empty();
+            uint32_t k0 = 0;
             for (uint32_t k = off; k < npts; k += ls) {
                 const uint32_t pp = start + k;
-                leaf_agg_add(a, T.xs[pp], T.ys[pp], T.zs[pp], T.ids[pp]);
+                const double px = T.xs[pp], py = T.ys[pp];
+                if (k == 0)
+                    k0 = sort_key(TG, px, py, shift);
+                leaf_agg_add(a, px, py, T.zs[pp], T.ids[pp]);
             }
             seg_reduce(off, ls, [&](uint32_t o, bool take) {
                 uint32_t c2 = __shfl_down(a.cnt, o, 64);
@@ -550,7 +570,7 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, const uint32_t *__
                 }
             });
             if (active && off == 0)
-                leaf_store(TG, T, skeys, jl, start, npts, a);
+                leaf_store(TG, T, k0, jl, start, npts, a);
         }
     }
 }
@@ -876,10 +896,10 @@ void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const doub
                        hist, passes);
 }
 
-void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const uint32_t *skeys, const TreeOut &T)
+void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T)
 {
     hipLaunchKernelGGL(k_leaf_aggs, dim3(grid_for(B.ngrp, LEAF_G * TB / 64, 16384)), dim3(TB), 0, s,
-                       B, skeys, T);
+                       B, T);
 }
 
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
