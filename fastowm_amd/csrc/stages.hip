@@ -188,6 +188,7 @@ __global__ void k_s3_mark(BatchDev B, TreeDev T, const uint32_t *__restrict__ ll
 }
 
 constexpr int S3_STACK = 1024;
+constexpr uint32_t S3_PART = 256;  // partially covered leaves a wave defers per cell
 
 // One wave per cell without an LLP: lowest (z, index) strictly inside.
 __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
@@ -195,9 +196,11 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
                                                    uint32_t *__restrict__ cell_res, uint32_t *err)
 {
     __shared__ uint32_t sStack[4][S3_STACK];
+    __shared__ uint32_t sPart[4][S3_PART];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     uint32_t *stk = sStack[w];
+    uint32_t *part = sPart[w];
     const uint32_t nw = gridDim.x * 4;
     for (uint32_t c = blockIdx.x * 4 + w; c < B.ncell; c += nw) {
         if (cell_flag[c]) {
@@ -216,6 +219,7 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
         uint32_t cnt = 0, bid = OWM_NONE;
         unsigned long long bk = OWM_ZKEY_NONE;
         uint32_t nstack = (TD.n > 0 && TD.root != OWM_NONE) ? 1u : 0u;
+        uint32_t npart = 0;  // leaves the cell's edge crosses, scanned by the whole wave below
         if (lane == 0)
             stk[0] = TD.root;
         __builtin_amdgcn_wave_barrier();
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
             uint32_t node = lane < m ? stk[nstack - 1 - lane] : 0u;
             __builtin_amdgcn_wave_barrier();
             nstack -= m;
-            bool push = false;
+            bool push = false, crossed = false;
             if (lane < m) {
                 double4 bx;
                 uint32_t nc;
@@ -253,28 +257,40 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
                             bid = id;
                         }
                     } else if (node & OWM_LEAF_BIT) {
-                        uint32_t l = node & ~OWM_LEAF_BIT;
-                        uint32_t s0 = T.lf_start[l], np = T.lf_npts[l];
-                        for (uint32_t p = s0; p < s0 + np; ++p) {
-                            double x = T.xs[p], y = T.ys[p];
-                            if (x > b0 && x < b2 && y > b1 && y < b3) {
-                                ++cnt;
-                                double z = T.zs[p];
-                                if (owm_eligible(z)) {
-                                    unsigned long long k = owm_zkey(z);
-                                    uint32_t id = T.ids[p];
-                                    if (owm_less(k, id, bk, bid)) {
-                                        bk = k;
-                                        bid = id;
-                                    }
-                                }
-                            }
-                        }
+                        crossed = true;
                     } else {
                         push = true;
                     }
                 }
             }
+            // crossed leaves go to the wave's list; past its capacity a lane
+            // scans its leaf itself
+            const unsigned long long cm = __ballot(crossed);
+            if (crossed) {
+                const uint32_t at = npart + (uint32_t)__popcll(cm & lt_mask);
+                if (at < S3_PART) {
+                    part[at] = node & ~OWM_LEAF_BIT;
+                } else {
+                    const uint32_t l = node & ~OWM_LEAF_BIT;
+                    const uint32_t s0 = T.lf_start[l], np = T.lf_npts[l];
+                    for (uint32_t p = s0; p < s0 + np; ++p) {
+                        const double x = T.xs[p], y = T.ys[p];
+                        if (x > b0 && x < b2 && y > b1 && y < b3) {
+                            ++cnt;
+                            const double z = T.zs[p];
+                            if (owm_eligible(z)) {
+                                const unsigned long long k = owm_zkey(z);
+                                const uint32_t id = T.ids[p];
+                                if (owm_less(k, id, bk, bid)) {
+                                    bk = k;
+                                    bid = id;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            npart = min(npart + (uint32_t)__popcll(cm), S3_PART);
             unsigned long long pm = __ballot(push);
             if (push) {
                 uint2 ch = T.nd_child[node];
@@ -285,6 +301,45 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
             nstack += 2 * __popcll(pm);
             __builtin_amdgcn_wave_barrier();
         }
+        // crossed leaves: one point per lane, four leaves' loads in flight
+        for (uint32_t i = 0; i < npart; i += 4) {
+            uint32_t s0[4], np[4], mx = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t l = i + q < npart ? part[i + q] : OWM_NONE;
+                s0[q] = l != OWM_NONE ? T.lf_start[l] : 0u;
+                np[q] = l != OWM_NONE ? T.lf_npts[l] : 0u;
+                mx = max(mx, np[q]);
+            }
+            for (uint32_t o = 0; o < mx; o += 64) {
+                double x[4], y[4], z[4];
+                uint32_t id[4];
+                bool in[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    in[q] = o + lane < np[q];
+                    const uint32_t p = s0[q] + (in[q] ? o + lane : 0u);
+                    x[q] = T.xs[p];
+                    y[q] = T.ys[p];
+                    z[q] = T.zs[p];
+                    id[q] = T.ids[p];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (in[q] && x[q] > b0 && x[q] < b2 && y[q] > b1 && y[q] < b3) {
+                        ++cnt;
+                        if (owm_eligible(z[q])) {
+                            const unsigned long long k = owm_zkey(z[q]);
+                            if (owm_less(k, id[q], bk, bid)) {
+                                bk = k;
+                                bid = id[q];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             cnt += __shfl_xor(cnt, o, 64);

@@ -326,6 +326,24 @@ def test_points_ulps_from_edges(ctx, restore_table, origin):
     assert np.array_equal(ctx.run_tile(x, y, z, 10, 20, 0.9, 16), want)
 
 
+def test_stage3_many_crossed_leaves(ctx):
+    """Stage 3 defers the leaves a cell's edge crosses to a wave-wide scan (256
+    per cell at most).  With no LLPs every 60 m cell is queried, and its edges
+    cross more than 256 two-point leaves at 5 points/m2, so the lanes past the
+    list scan their leaves themselves; the result is the brute-force grid scan."""
+    say("stage 3 crossed leaves")
+    t = synth.scaled(synth.config_c2(seed=9), 2_400_000)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    p = O.params(12, 60, 0.8, box, len(x), 2)
+    none = np.zeros(0, np.uint32)
+    want = np.sort(O.stage3_brute(x, y, z, p, none))
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=2)
+    got = ctx.stage3(tree, none, 12, 0.8, 60)
+    tree.free()
+    assert len(want) > 100 and np.array_equal(np.sort(got), want)
+
+
 @pytest.mark.parametrize("summaries", ["on", "off"])
 def test_c2_scaled_with_and_without_summaries(ctx, restore_table, summaries):
     _stage1_table(ctx, summaries)
