@@ -82,6 +82,22 @@ void owm_ctx_destroy(owm_ctx *ctx)
 
 uint64_t owm_ctx_device_bytes(const owm_ctx *ctx) { return ctx ? ctx->held_bytes() : 0; }
 
+owm_status owm_host_alloc(uint64_t bytes, void **out)
+{
+    if (!out)
+        return OWM_E_ARG;
+    *out = nullptr;
+    if (bytes == 0)
+        return OWM_OK;
+    return hipHostMalloc(out, bytes, 0) == hipSuccess ? OWM_OK : OWM_E_NOMEM;
+}
+
+void owm_host_free(void *ptr)
+{
+    if (ptr)
+        (void)hipHostFree(ptr);
+}
+
 owm_status owm_derive(const owm_params *params, uint64_t n, const double *bbox_or_null,
                       owm_derived *out)
 {

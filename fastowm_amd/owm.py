@@ -97,6 +97,8 @@ def lib():
             "owm_last_error": ([vp], ctypes.c_char_p),
             "owm_status_str": ([ctypes.c_int], ctypes.c_char_p),
             "owm_ctx_device_bytes": ([vp], u64),
+            "owm_host_alloc": ([u64, P(vp)], ctypes.c_int),
+            "owm_host_free": ([vp], None),
             "owm_derive": ([P(Params), u64, dp, P(Derived)], ctypes.c_int),
             "owm_query_seed_count": ([P(Params), u64, dp, P(u64)], ctypes.c_int),
             "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
@@ -223,14 +225,29 @@ class Context:
         self.h = h
         self._trees = weakref.WeakSet()
         self._seed_buf = np.empty(0, np.uint32)
+        self._seed_mem = None
 
     def _seeds_out(self, cap):
-        """Reusable host buffer the library copies seeds into (results are
-        copied out of it): a fresh array per call would cost the page faults
-        of an n-sized allocation every time.  It only grows."""
+        """Reusable page-locked host buffer the library copies seeds into
+        (results are copied out of it): a fresh array per call would cost the
+        page faults of an n-sized allocation every time, and pageable memory
+        a staged copy.  It only grows."""
         if len(self._seed_buf) < cap:
-            self._seed_buf = np.empty(cap, np.uint32)
+            self._free_seed_buf()
+            p = ctypes.c_void_p()
+            if lib().owm_host_alloc(cap * 4, ctypes.byref(p)) == 0 and p.value:
+                self._seed_mem = p
+                buf = (ctypes.c_uint32 * cap).from_address(p.value)
+                self._seed_buf = np.frombuffer(buf, np.uint32)
+            else:
+                self._seed_buf = np.empty(cap, np.uint32)
         return self._seed_buf
+
+    def _free_seed_buf(self):
+        self._seed_buf = np.empty(0, np.uint32)
+        if self._seed_mem is not None:
+            lib().owm_host_free(self._seed_mem)
+            self._seed_mem = None
 
     def close(self):
         if getattr(self, "h", None):
@@ -238,6 +255,7 @@ class Context:
                 t.free()
             lib().owm_ctx_destroy(self.h)
             self.h = None
+            self._free_seed_buf()
 
     def __enter__(self):
         return self
@@ -322,7 +340,8 @@ class Context:
             st = lib().owm_batch_run(self.h, xp, yp, zp, descs, len(tiles), flags,
                                      out.ctypes.data, cap, so.ctypes.data)
         self._check(st)
-        return [out[int(so[i]):int(so[i + 1])].copy() for i in range(len(tiles))]
+        # one copy out of the reusable buffer, then a view per tile
+        return np.split(out[:int(so[-1])].copy(), so[1:-1].astype(np.intp))
 
     # -- autotuner ----------------------------------------------------------
     def autotune(self, x, y, z, tiles):

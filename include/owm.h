@@ -74,6 +74,13 @@ const char *owm_status_str(owm_status s);
 /* Device bytes currently held by the context's arenas. */
 uint64_t owm_ctx_device_bytes(const owm_ctx *ctx);
 
+/* Page-locked host memory for seed buffers and host clouds: copies between it
+ * and the device run as one DMA instead of staged pageable chunks.  No
+ * reference counterpart (the reference's buffers are plain vectors).
+ * owm_host_free(NULL) is a no-op. */
+owm_status owm_host_alloc(uint64_t bytes, void **out);
+void owm_host_free(void *ptr);
+
 /* Per-phase device timing, measured with HIP events recorded on the
  * context's own stream around each kernel family.  `bytes` is the phase's
  * algorithmic HBM traffic (what the work must read and write once), so
