@@ -4,9 +4,9 @@
 // derives each tile's parameters exactly as the reference driver does
 // (CPP-TBB-CPU/application/owm_optim3.cpp:92-132), lays every per-tile range
 // out in one concatenated index space, uploads the tile table, and enqueues
-// the device chain: Morton keys -> radix sort -> gather + leaves -> Karras ->
-// bottom-up -> stage 1 -> sort + stage 2 -> stage 3 -> merge + sort of the
-// seeds.  Only the bbox (when the caller has none) and the final seed count
+// the device chain: Morton keys -> record radix sort (the cloud itself ends in
+// Morton order) -> leaves -> Karras -> bottom-up -> stage 1 -> sort + stage 2
+// -> stage 3 -> merge + sort of the seeds.  Only the bbox (when the caller has none) and the final seed count
 // come back to the host.
 #include "owm_host.h"
 

@@ -1,7 +1,7 @@
 // LSD radix sort of u32 keys (optionally with u32 values) for gfx950.
 //
 // One upfront histogram pass counts the digits of every pass; each pass is
-// then a single "onesweep" kernel: a workgroup takes the next 4096-key tile
+// then a single "onesweep" kernel: a workgroup takes the next tile of keys
 // from an atomic counter (so every predecessor tile is already resident),
 // ranks its keys stably with wave64 ballot matching, publishes its digit
 // counts, resolves its global digit offsets by decoupled look-back over the

@@ -1,7 +1,8 @@
 // Spatial index of the OWM pipeline on gfx950: bbox reduction, 2D Morton
-// keys, gather into Morton order with per-leaf aggregates, Karras LBVH over the
-// leaves and an atomic-flag bottom-up pass that propagates each node's tight
-// bbox, point count and lowest (z, index).
+// keys (the record sort in sort.hip then leaves the cloud in Morton order),
+// per-leaf aggregates, Karras LBVH over the leaves and an atomic-flag
+// bottom-up pass that propagates each node's tight bbox, point count and
+// lowest (z, index).
 //
 // Reference counterparts: ordered/cuda/ordered_builder.inl (get_morton_code
 // :33-58, order_points :60-74, init_leafs_size :78-126, init_nodes5 :129-205,
