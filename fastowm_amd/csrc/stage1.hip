@@ -157,28 +157,37 @@ __global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t 
         }
         for (uint32_t k = 0; __ballot(k < nb) != 0ull; ++k) {
             const uint32_t key = k < nb ? boff + (j0 + k / wi) * nbx + (i0 + k % wi) : OWM_NONE;
+            // match lanes by block (scalar work only), then every leader issues
+            // its group's atomic in one instruction: one round trip per k, not
+            // one per distinct block
             unsigned long long pending = __ballot(key != OWM_NONE);
+            uint32_t leader_of = 0, rank = 0, group = 0;
             while (pending) {
                 const int leader = __ffsll((long long)pending) - 1;
                 const uint32_t lk = __builtin_amdgcn_readlane(key, leader);
                 const unsigned long long peers = __ballot(key == lk);
-                if (MODE == 0) {
-                    if ((int)lane == leader)
-                        atomicAdd(&cnt[lk], (uint32_t)__popcll(peers));
-                } else {
-                    uint32_t b0 = 0;
-                    if ((int)lane == leader)
-                        b0 = atomicAdd(&cnt[lk], (uint32_t)__popcll(peers));
-                    b0 = __builtin_amdgcn_readlane(b0, leader);
-                    if (key == lk) {
-                        const uint32_t e = start[lk] + b0 + (uint32_t)__popcll(peers & below);
-                        if (e < cap)
-                            list[e] = l;
-                        else
-                            atomicOr(err, 16u);
-                    }
+                if (key == lk) {
+                    leader_of = (uint32_t)leader;
+                    rank = (uint32_t)__popcll(peers & below);
                 }
+                if ((int)lane == leader)
+                    group = (uint32_t)__popcll(peers);
                 pending &= ~peers;
+            }
+            const bool lead = key != OWM_NONE && lane == leader_of;
+            if (MODE == 0) {
+                if (lead)
+                    atomicAdd(&cnt[key], group);
+            } else {
+                uint32_t b0 = lead ? atomicAdd(&cnt[key], group) : 0u;
+                b0 = (uint32_t)__shfl((int)b0, (int)leader_of, 64);
+                if (key != OWM_NONE) {
+                    const uint32_t e = start[key] + b0 + rank;
+                    if (e < cap)
+                        list[e] = l;
+                    else
+                        atomicOr(err, 16u);
+                }
             }
         }
     }
