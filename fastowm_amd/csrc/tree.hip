@@ -436,7 +436,7 @@ __device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, uint32
     T.lf_parent[leaf] = OWM_NONE;
 }
 
-constexpr int LEAF_G = 4;  // leaf groups a wave reduces at once
+constexpr int LEAF_G = 8;  // leaf groups a wave reduces at once
 
 template <typename V>
 __device__ __forceinline__ V shfl_any(V v, int src)
