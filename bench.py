@@ -34,7 +34,7 @@ PHASE_KERNEL = {"stage1": "owm::k_stage1", "leaf_aggs": "owm::k_leaf_aggs",
                 "stage1_join": "owm::k_s1_bin+k_scan_excl"}
 PHASE_BYTES = {"stage1": "24 B x N x Wsize/Displace + 4 B x windows",
                "leaf_aggs": "28 B x N + 48 B x leaves",
-               "sort_tree_pass": "64 B x N (first pass: key + x, y, z, index read and written), 60 B x N (last pass: no key written)",
+               "sort_tree_pass": "60 B x N per pass with 16-bit keys (first pass: key + x, y, z read, key + x, y, z, index written; last: key + x, y, z, index read, x, y, z, index written)",
                "morton": "20 B x N", "bbox": "16 B x N", "bottomup": "120 B x leaves",
                "karras": "24 B x leaves", "sort_windows_pass": "12 B x windows per pass",
                "stage2_compact": "8 B x windows", "stage1_join": "72 B x leaves + 12 B x blocks"}

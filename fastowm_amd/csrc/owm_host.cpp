@@ -540,8 +540,9 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
                             S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0, last,
                             seg, tiles);
         // algorithmic bytes per point and pass: a key and a 28-byte record
-        // read and written (the last pass writes no key)
-        ctx->prof.end(s, pi, (last ? 60ull : 64ull) * L.np, L.np);
+        // read and written, except that the first pass reads no index (it
+        // numbers the points) and the last writes no key
+        ctx->prof.end(s, pi, (p == 0 || last ? 60ull : 64ull) * L.np, L.np);
         std::swap(kin, kout);
     }
     pi = ctx->prof.begin(s, PH_LEAVES);
