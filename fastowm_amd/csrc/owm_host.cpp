@@ -308,7 +308,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
     L.ngrp = L.grp_off[T];
     L.np = L.pt_off[T];
     // sort tiles of each tile's segment (a sort tile never straddles tiles)
-    const uint64_t tk = record_tile_keys((uint32_t)std::min<uint64_t>(L.np, 0xFFFFFFFFu));
+    const uint64_t tk = record_tile_keys((uint32_t)std::min<uint64_t>(L.np, 0xFFFFFFFFu), T);
     L.sort_units.assign(T + 1, 0);
     for (uint32_t t = 0; t < T; ++t)
         L.sort_units[t + 1] = L.sort_units[t] + (uint32_t)((n[t] + tk - 1) / tk);
@@ -412,9 +412,11 @@ TreeDev tree_view(const TreeOut &T)
 // Scratch of the tree build (keys, values, histograms, look-back flags).
 // nseg segments: one histogram per segment, and up to one partial sort tile more
 // per segment.
-uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg)
+// lb_tiles: look-back rows per pass (0: the key/value sort's tile count, plus
+// one partial tile per segment)
+uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles)
 {
-    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
+    const uint64_t tiles = lb_tiles ? lb_tiles : sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
     return align_up(n * 4ull, 256) * 4 + align_up((uint64_t)nseg * SORT_SEG_HIST * 4, 256) +
            align_up(64, 256) + align_up((uint64_t)passes * tiles * 256 * 4, 256);
 }
@@ -429,11 +431,13 @@ uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
     const int rec_sets = passes > 2 ? 2 : 1;
-    return sort_scratch_bytes(L.np, passes, L.ntiles) + rec_sets * 3 * align_up(L.np * 8ull, 256) +
+    return sort_scratch_bytes(L.np, passes, L.ntiles, L.sort_units[L.ntiles] + 1) +
+           rec_sets * 3 * align_up(L.np * 8ull, 256) +
            align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
 }
 
-static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1)
+static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1,
+                           uint64_t lb_tiles = 0)
 {
     SortBufs S;
     uint64_t o = 0;
@@ -442,7 +446,7 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg 
         o += align_up(bytes, 256);
         return p;
     };
-    uint64_t tiles = sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
+    const uint64_t tiles = lb_tiles ? lb_tiles : sort_tiles((uint32_t)std::max<uint64_t>(n, 1)) + nseg;
     S.k0 = (uint32_t *)take(n * 4ull);
     S.v0 = (uint32_t *)take(n * 4ull);
     S.k1 = (uint32_t *)take(n * 4ull);
@@ -487,12 +491,14 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     if (L.np == 0)
         return OWM_OK;
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes, L.ntiles));
+    // record-sort tiles (their size follows the segments, not the key/value sort)
+    const uint64_t lb_tiles = L.sort_units[L.ntiles] + 1;
+    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes, L.ntiles, lb_tiles));
     if (!sb) {
         set_err(ctx, "scratch arena exhausted (tree sort)");
         return OWM_E_INTERNAL;
     }
-    SortBufs S = carve_sort(sb, L.np, passes, L.ntiles);
+    SortBufs S = carve_sort(sb, L.np, passes, L.ntiles, lb_tiles);
     const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
     uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
     bool have = bu_work != nullptr;
@@ -567,8 +573,8 @@ uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
     int passes = (bits + 7) / 8;
     uint64_t nseed_max = std::max<uint64_t>(Q.nwin, n_in) + Q.ncell + 1;
     uint64_t b = 0;
-    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes, 1);
-    b += sort_scratch_bytes(nseed_max, passes, 1);                        // seeds sort
+    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes, 1, 0);
+    b += sort_scratch_bytes(nseed_max, passes, 1, 0);                     // seeds sort
     b += align_up(std::max<uint64_t>(Q.ncell, 1) * 4ull, 256) * 2;      // cell flag / res
     b += align_up((nseed_max / 4096 + 2) * 4ull, 256) * 2;               // compaction look-back
     b += align_up(1024, 256);                                           // counters
@@ -593,8 +599,8 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     const uint64_t nwin = from == 2 ? n_in : Q.nwin;
     const uint64_t nw = std::max<uint64_t>(nwin, 1);
     const uint64_t nseed_max = std::max<uint64_t>(nwin, from == 3 ? n_in : 0) + Q.ncell + 1;
-    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes, 1));
-    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes, 1));
+    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes, 1, 0));
+    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes, 1, 0));
     uint32_t *cell_flag = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
     uint32_t *cell_res = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
     uint32_t *lb1 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);

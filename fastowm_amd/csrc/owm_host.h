@@ -143,7 +143,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
                TreeLayout &L);
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
 void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
-uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg);
+uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles);
 uint64_t build_scratch_bytes(const TreeLayout &L);
 uint64_t table_bytes(uint32_t n_tiles);
 owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const BatchDev &B,

@@ -7,7 +7,7 @@ namespace owm {
 
 // ---- radix sort (sort.hip) ----
 // Segments sorted independently: [pt[t], pt[t+1]) for t < nseg, sort tiles
-// units[t] .. units[t+1]-1 (units[t] = sum of ceil(n_s / record_tile_keys(n))
+// units[t] .. units[t+1]-1 (units[t] = sum of ceil(n_s / record_tile_keys(n, nseg))
 // over s < t), digit offsets of segment t at digit_off + t * digit_stride.
 // pt == null: one segment [0, n).
 struct SortSegs {
@@ -17,7 +17,8 @@ struct SortSegs {
     uint32_t digit_stride = 0;
 };
 uint32_t sort_tiles(uint32_t n);
-uint32_t record_tile_keys(uint32_t n);  // points per tile of an n-point record sort
+// points per tile of an n-point record sort over nseg segments
+uint32_t record_tile_keys(uint32_t n, uint32_t nseg);
 constexpr uint32_t SORT_SEG_HIST = 4 * 256;  // histogram words per segment (all passes)
 // hist[t * SORT_SEG_HIST + p * 256 + d] -> segment base + exclusive offsets
 void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,

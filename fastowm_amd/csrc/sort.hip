@@ -562,10 +562,14 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
 }
 
 // Record sort tiles: 16384 points (8 waves x 32) for sorts of >= BIG_SORT
-// points, where each digit's run per tile is long enough (64 points) that the
-// scattered writes mostly fill whole cache lines; 4096 (4 waves x 16) below.
-static inline bool big_records(uint32_t n) { return n >= BIG_SORT; }
-uint32_t record_tile_keys(uint32_t n) { return big_records(n) ? 8 * 64 * 32 : 4 * 64 * 16; }
+// points whose segments average >= 64K points, where each digit's run per tile
+// is long enough (64 points) that the scattered writes mostly fill whole cache
+// lines; 4096 (4 waves x 16) otherwise (every segment ends in a partial tile).
+static inline bool big_records(uint32_t n, uint32_t nseg)
+{
+    return n >= BIG_SORT && n / (nseg ? nseg : 1u) >= 65536u;
+}
+uint32_t record_tile_keys(uint32_t n, uint32_t nseg) { return big_records(n, nseg) ? 8 * 64 * 32 : 4 * 64 * 16; }
 
 template <int IT, int NW>
 static void sort_records_shape(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
@@ -598,7 +602,7 @@ void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in
                          bool first, bool last, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
-    if (big_records(n))
+    if (big_records(n, seg.nseg))
         sort_records_shape<32, 8>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
                                   tile_counter, first, last, seg, grid);
     else

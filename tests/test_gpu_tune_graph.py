@@ -113,3 +113,19 @@ def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
         for tt, (tx, ty, tz), g in zip(tiles, arrs, got):
             w, _, _ = O.run(tx, ty, tz, tt.wsize, tt.bsize, tt.overlap, 32)
             assert np.array_equal(g, w)
+
+
+def test_many_small_tiles_over_two_million_points():
+    """A batch above 2M points whose tiles average under 64K points sorts with
+    4096-point tiles (every tile of the batch ends in a partial sort tile);
+    each tile's seeds equal the oracle's, eager and replayed."""
+    sizes = [40_000 + 3_000 * (i % 11) for i in range(44)]
+    assert sum(sizes) > 2 ** 21 and sum(sizes) / len(sizes) < 65536
+    x, y, z, desc, tiles, arrs = _batch(sizes, seed=600)
+    with owm.Context(0) as ctx:
+        say("many small tiles", len(sizes), sum(sizes))
+        got = ctx.batch_run(x, y, z, desc)
+        for t, (tx, ty, tz), g in zip(tiles, arrs, got):
+            want, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
+            assert np.array_equal(g, want)
+        assert _same(ctx.batch_run(x, y, z, desc, graph=True), got)
