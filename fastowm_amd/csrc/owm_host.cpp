@@ -542,9 +542,10 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         const RecOut &out = p == passes - 1 ? tree_rec : rec[p & 1];
         pi = ctx->prof.begin(s, PH_SORT_TREE);
         const bool last = p == passes - 1;
-        launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p, S.hist + 256 * p,
-                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0, last,
-                            seg, tiles);
+        launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p,
+                            L.tile_bits + L.key_bits, S.hist + 256 * p,
+                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0,
+                            last, seg, tiles);
         // algorithmic bytes per point and pass: a key and a 28-byte record
         // read and written, except that the first pass reads no index (it
         // numbers the points) and the last writes no key

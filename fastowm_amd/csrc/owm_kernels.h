@@ -45,12 +45,13 @@ struct RecOut {
     uint32_t *id, *pos;
 };
 // One record-moving pass of the segmented tree sort over n points (segments
-// seg, seg_tiles sort tiles in all): the first pass numbers the points itself,
-// the last writes no keys (keys_out unused).
+// seg, seg_tiles sort tiles in all, keys of key_bits bits): the first pass
+// numbers the points itself, the last writes no keys (keys_out unused).
 void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                          uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
-                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool first, bool last, const SortSegs &seg, uint32_t seg_tiles);
+                         int key_bits, const uint32_t *digit_off, uint32_t *lookback,
+                         uint32_t *tile_counter, bool first, bool last, const SortSegs &seg,
+                         uint32_t seg_tiles);
 
 // ---- tree (tree.hip) ----
 struct TreeOut {

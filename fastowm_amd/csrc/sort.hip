@@ -175,8 +175,8 @@ __device__ __forceinline__ bool sweep_claim(SweepLds<NW> &L, uint32_t *tile_coun
 // the same digit, and L.whist[w][d] ends as the wave's count of digit d.  FULL:
 // the tile holds ITEMS_T keys per thread (no element is past n).  PACK: rank[k]
 // is d << 16 | rank, and rank may be key itself (each key is read before its
-// rank is written).
-template <int ITEMS_T, bool FULL = false, bool PACK = false, int NW = WAVES>
+// rank is written); HI: the key's byte above the digit goes to bits 24-31.
+template <int ITEMS_T, bool FULL = false, bool PACK = false, int NW = WAVES, bool HI = false>
 __device__ __forceinline__ void sweep_rank(SweepLds<NW> &L, const uint32_t (&key)[ITEMS_T],
                                            uint32_t base, uint32_t n, int shift,
                                            uint32_t (&rank)[ITEMS_T])
@@ -199,7 +199,8 @@ __device__ __forceinline__ void sweep_rank(SweepLds<NW> &L, const uint32_t (&key
         const uint32_t old = L.whist[w][d];
         if (valid && before == 0)
             L.whist[w][d] = old + (uint32_t)__popcll(m);
-        rank[k] = PACK ? d << 16 | (old + before) : old + before;
+        const uint32_t hi = HI ? ((key[k] >> (shift + RB)) & 255u) << 24 : 0u;
+        rank[k] = PACK ? hi | d << 16 | (old + before) : old + before;
     }
 }
 
@@ -315,7 +316,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // TILE_T points, so no load or store needs a bound check; otherwise loads are
 // clamped to the tile's last point and the items past it take the unused tile
 // slots past cnt, so that the LDS writes need no check either.
-template <bool IOTA, bool LAST, bool POS, int ITEMS_T, int NW, bool FULL>
+// K16: 16-bit keys sorted by their low byte (the first of two passes): a key is
+// its digit and the byte above it, kept from the ranking, so the keys are not
+// read a second time for the key/index staging.
+template <bool IOTA, bool LAST, bool POS, bool K16, int ITEMS_T, int NW, bool FULL>
 __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uint8_t *s_dig,
                                              const uint32_t *__restrict__ keys_in, const RecIn &in,
                                              uint32_t *__restrict__ keys_out, const RecOut &out,
@@ -341,22 +345,27 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
         for (int k = 0; k < ITEMS_T; ++k)
             v[k] = src[at(o, k)];
     };
-    // per item its tile slot (16 bits, two per word)
-    uint32_t slot2[ITEMS_T / 2];
+    // per item its tile slot (16 bits, two per word); K16: the key's high
+    // byte (four per word)
+    uint32_t slot2[ITEMS_T / 2], hi4[K16 ? ITEMS_T / 4 : 1];
     {
-        // keys, then (in place) digit << 16 | in-wave rank
+        // keys, then (in place) [high byte << 24 |] digit << 16 | in-wave rank
         uint32_t kr[ITEMS_T];
         const uint32_t o = fresh();
 #pragma unroll
         for (int k = 0; k < ITEMS_T; ++k)
             kr[k] = keys_in[at(o, k)];
         load(in.x);
-        sweep_rank<ITEMS_T, FULL, true, NW>(L, kr, 0, cnt, shift, kr);
+        sweep_rank<ITEMS_T, FULL, true, NW, K16>(L, kr, 0, cnt, shift, kr);
         __syncthreads();
         sweep_offsets(L, tile, first, doff, lookback);
 #pragma unroll
         for (int k = 0; k < ITEMS_T; ++k) {
-            const uint32_t dd = kr[k] >> 16;
+            const uint32_t dd = (kr[k] >> 16) & 255u;
+            if constexpr (K16) {
+                const uint32_t h = kr[k] >> 24;
+                hi4[k / 4] = (k & 3) ? hi4[k / 4] | h << (8 * (k & 3)) : h;
+            }
             uint32_t sl = L.excl[dd] + L.whist[w][dd] + (kr[k] & 0xFFFFu);
             if (!FULL)
                 sl = p0 + k * 64 < cnt ? sl : p0 + k * 64;
@@ -421,7 +430,12 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
 #pragma unroll
         for (int k = 0; k < ITEMS_T; ++k) {
             const uint32_t id = IOTA ? base + o + k * 64 : in.id[at(o, k)];
-            kv[k] = ((unsigned long long)id << 32) | keys_in[at(o, k)];
+            uint32_t key;
+            if constexpr (K16)
+                key = ((hi4[k / 4] >> (8 * (k & 3))) & 255u) << 8 | s_dig[slot_of(k)];
+            else
+                key = keys_in[at(o, k)];
+            kv[k] = ((unsigned long long)id << 32) | key;
         }
         store(out.z);
 #pragma unroll
@@ -449,7 +463,7 @@ __device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uin
 // so consecutive lanes write consecutive addresses.  The loads of a field are
 // issued before the stores of the previous one (those of x before the
 // ranking), so that their latency hides behind that work.
-template <bool IOTA, bool LAST, bool POS, int ITEMS_T, int NW>
+template <bool IOTA, bool LAST, bool POS, bool K16, int ITEMS_T, int NW>
 __global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t *__restrict__ keys_in,
                                                                    RecIn in, uint32_t *__restrict__ keys_out,
                                                                    RecOut out, uint32_t n_all, int shift,
@@ -466,10 +480,10 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t
     const uint32_t *doff;
     while (sweep_claim<TILE_T, NW>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
         if (n - base >= (uint32_t)TILE_T)
-            records_tile<IOTA, LAST, POS, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+            records_tile<IOTA, LAST, POS, K16, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
                                                        tile, base, TILE_T, first, doff, lookback, shift);
         else
-            records_tile<IOTA, LAST, POS, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
+            records_tile<IOTA, LAST, POS, K16, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
                                                         tile, base, n - base, first, doff, lookback,
                                                         shift);
     }
@@ -575,39 +589,43 @@ template <int IT, int NW>
 static void sort_records_shape(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                                uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                               bool first, bool last, const SortSegs &seg, uint32_t grid)
+                               bool first, bool last, bool k16, const SortSegs &seg, uint32_t grid)
 {
     const dim3 b(NW * 64);
-#define OWM_REC(IOTA, LAST, POS)                                                                  \
-    hipLaunchKernelGGL((k_sort_records<IOTA, LAST, POS, IT, NW>), dim3(grid), b, 0, s, keys_in, in, \
-                       keys_out, out, n, shift, digit_off, lookback, tile_counter, seg)
+#define OWM_REC(IOTA, LAST, POS, K16)                                                              \
+    hipLaunchKernelGGL((k_sort_records<IOTA, LAST, POS, K16, IT, NW>), dim3(grid), b, 0, s, keys_in, \
+                       in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg)
     if (first && last && out.pos)
-        OWM_REC(true, true, true);  // a single pass
+        OWM_REC(true, true, true, false);  // a single pass
     else if (first && last)
-        OWM_REC(true, true, false);
+        OWM_REC(true, true, false, false);
+    else if (first && k16)
+        OWM_REC(true, false, false, true);
     else if (first)
-        OWM_REC(true, false, false);
+        OWM_REC(true, false, false, false);
     else if (!last)
-        OWM_REC(false, false, false);
+        OWM_REC(false, false, false, false);
     else if (out.pos)
-        OWM_REC(false, true, true);
+        OWM_REC(false, true, true, false);
     else
-        OWM_REC(false, true, false);
+        OWM_REC(false, true, false, false);
 #undef OWM_REC
 }
 
 void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
                          uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
-                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool first, bool last, const SortSegs &seg, uint32_t seg_tiles)
+                         int key_bits, const uint32_t *digit_off, uint32_t *lookback,
+                         uint32_t *tile_counter, bool first, bool last, const SortSegs &seg,
+                         uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
+    const bool k16 = key_bits <= 16 && shift == 0;
     if (big_records(n, seg.nseg))
         sort_records_shape<32, 8>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, first, last, seg, grid);
+                                  tile_counter, first, last, k16, seg, grid);
     else
         sort_records_shape<16, 4>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, first, last, seg, grid);
+                                  tile_counter, first, last, k16, seg, grid);
 }
 
 }  // namespace owm
