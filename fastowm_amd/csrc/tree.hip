@@ -451,6 +451,68 @@ __device__ __forceinline__ V shfl_any(V v, int src)
     }
 }
 
+// Leaf sizes 16, 32, 64 and 128: the wave reads its LEAF_G groups in rounds
+// of 512 points, a segment of 8 lanes taking 64 consecutive points (8 per
+// lane, each load a 64-byte run per segment).  Each lane first folds its own
+// points of every leaf, then three butterflies inside the segment (four when a
+// leaf spans two segments) finish the leaves.  Lane k of a segment reads its
+// points of the segment's k-th leaf rotated by k, so that it holds that
+// leaf's first point and stores the leaf.
+template <int LL>
+__device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint32_t g0,
+                                            uint32_t lane, uint32_t shift)
+{
+    constexpr int NK = LL <= 64 ? 64 / LL : 1;  // leaves of a lane's points
+    constexpr int QL = 8 / NK;                  // a lane's points per leaf
+    constexpr int GPTS = LL <= 64 ? 64 : LL;    // points per group
+    const uint32_t seg = lane >> 3, sub = lane & 7;
+#pragma unroll 1
+    for (int r = 0; r < GPTS / 64; ++r) {
+        const uint32_t e0 = (g0 - TD.grp_off) * GPTS + r * 512 + seg * 64;
+        uint32_t v[8];
+        double x[8], y[8], z[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t k = q / QL;
+            const uint32_t e = e0 + q * 8 + ((sub - k) & 7);
+            // loads clamped to the tile's last point (no branch per load)
+            const uint32_t p = TD.pt_off + min(e, TD.n - 1);
+            v[q] = T.ids[p];
+            x[q] = T.xs[p];
+            y[q] = T.ys[p];
+            z[q] = T.zs[p];
+        }
+        LeafAgg a[NK];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            a[k] = leaf_agg_empty();
+#pragma unroll
+            for (int q = k * QL; q < (k + 1) * QL; ++q)
+                if (e0 + q * 8 + ((sub - k) & 7) < TD.n)
+                    leaf_agg_add(a[k], x[q], y[q], z[q], v[q]);
+            leaf_agg_step<1>(a[k]);
+            leaf_agg_step<2>(a[k]);
+            leaf_agg_step<4>(a[k]);
+            if (LL > 64)
+                leaf_agg_step<8>(a[k]);
+        }
+        LeafAgg mine = a[0];
+        double fx = x[0], fy = y[0];
+#pragma unroll
+        for (int k = 1; k < NK; ++k)
+            if (sub == (uint32_t)k) {
+                mine = a[k];
+                fx = x[k * QL];
+                fy = y[k * QL];
+            }
+        const uint32_t jl = e0 / LL + sub;
+        const bool store = LL <= 64 ? sub < (uint32_t)NK : (sub == 0 && (seg & 1) == 0);
+        if (store && jl < TD.nleaf)
+            leaf_store(TD, T, sort_key(TD, fx, fy, shift), jl, TD.pt_off + jl * LL,
+                       min((uint32_t)LL, TD.n - jl * LL), mine);
+    }
+}
+
 // Per-leaf aggregates over the points in Morton order.  A wave owns groups of
 // 64/leaf_size consecutive leaves of one tile (one leaf when leaf_size > 64,
 // its points strided over the lanes): every lane reads one point per leaf
@@ -470,6 +532,17 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
         const uint32_t t = owm_find_tile(B.grp_off, B.ntiles, g0);
         const TileDev &TD = B.tiles[t];
         const uint32_t L = TD.leaf_size;
+        if (g0 + LEAF_G <= B.grp_off[t + 1] && (L == 16 || L == 32 || L == 64 || L == 128)) {
+            if (L == 16)
+                leaf_rounds<16>(TD, T, g0, lane, shift);
+            else if (L == 32)
+                leaf_rounds<32>(TD, T, g0, lane, shift);
+            else if (L == 64)
+                leaf_rounds<64>(TD, T, g0, lane, shift);
+            else
+                leaf_rounds<128>(TD, T, g0, lane, shift);
+            continue;
+        }
         if (L <= 64 && (L & (L - 1)) == 0 && g0 + LEAF_G <= B.grp_off[t + 1]) {
             const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane & (L - 1);
             const uint32_t jl0 = (g0 - TD.grp_off) * nl_grp;  // first leaf of the groups
