@@ -30,13 +30,13 @@ HBM_PEAK_GBS = 8000.0
 # library's phase timers count (the per-launch figure is that times the units)
 PHASE_KERNEL = {"stage1": "owm::k_stage1", "leaf_aggs": "owm::k_leaf_aggs",
                 "sort_tree_pass": "owm::k_sort_records", "morton": "owm::k_morton",
-                "bbox": "owm::k_bbox", "bottomup": "owm::k_bottomup_local+global",
+                "bbox": "owm::k_bbox", "agg_levels": "owm::k_agg_levels",
                 "stage1_join": "owm::k_s1_bin+k_scan_excl"}
 PHASE_BYTES = {"stage1": "24 B x N x Wsize/Displace + 4 B x windows",
                "leaf_aggs": "28 B x N + 48 B x leaves",
                "sort_tree_pass": "60 B x N per pass with 16-bit keys (first pass: key + x, y, z read, key + x, y, z, index written; last: key + x, y, z, index read, x, y, z, index written)",
-               "morton": "20 B x N", "bbox": "16 B x N", "bottomup": "120 B x leaves",
-               "karras": "24 B x leaves", "sort_windows_pass": "12 B x windows per pass",
+               "morton": "20 B x N", "bbox": "16 B x N", "agg_levels": "48 B x (leaves + level entries)",
+               "karras": "24 B x leaves + 76 B x nodes", "sort_windows_pass": "12 B x windows per pass",
                "stage2_compact": "8 B x windows", "stage1_join": "72 B x leaves + 12 B x blocks"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
 

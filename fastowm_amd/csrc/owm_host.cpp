@@ -58,7 +58,7 @@ struct IoLayout {
 };
 
 static const char *PHASE_NAMES[PH_COUNT] = {
-    "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "bottomup", "stage1",
+    "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "agg_levels", "stage1",
     "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
     "stage1_summaries"};
 
@@ -349,7 +349,7 @@ uint64_t tree_bytes(const TreeLayout &L)
     b += align_up(L.nleaf * 4ull, 256) * 6;               // cnt start npts zid parent key0
     b += align_up(L.nleaf * 8ull, 256);                   // lf_zkey
     b += align_up(std::max<uint64_t>(L.nnode, 1) * 32ull, 256);  // nd_box
-    b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 4;  // cnt zid parent flag
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 3;  // cnt zid parent
     b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 3;  // zkey child range
     return b;
 }
@@ -380,7 +380,6 @@ void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
     T.nd_cnt = (uint32_t *)take(nn * 4ull);
     T.nd_zid = (uint32_t *)take(nn * 4ull);
     T.nd_parent = (uint32_t *)take(nn * 4ull);
-    T.nd_flag = (uint32_t *)take(nn * 4ull);
     T.nd_zkey = (unsigned long long *)take(nn * 8ull);
     T.nd_child = (uint2 *)take(nn * 8ull);
     T.nd_range = (uint2 *)take(nn * 8ull);
@@ -425,15 +424,23 @@ struct SortBufs {
     uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
 };
 
+// Aggregate levels of nleaf leaves: box, count, z key, z index per entry.
+static uint64_t agg_levels_bytes(uint64_t nleaf)
+{
+    uint64_t e = 0;
+    agg_levels_layout(nleaf, &e);
+    e = std::max<uint64_t>(e, 1);
+    return align_up(e * 32, 256) + 2 * align_up(e * 4, 256) + align_up(e * 8, 256);
+}
+
 // Device scratch of one tree build: the sort buffers, one set of sorted point
-// records (two when more than two passes), and the bottom-up work list.
+// records (two when more than two passes), and the aggregate levels.
 uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
     const int rec_sets = passes > 2 ? 2 : 1;
     return sort_scratch_bytes(L.np, passes, L.ntiles, L.sort_units[L.ntiles] + 1) +
-           rec_sets * 3 * align_up(L.np * 8ull, 256) +
-           align_up(bottomup_work_words(L.nleaf, L.ntiles) * 4ull, 256);
+           rec_sets * 3 * align_up(L.np * 8ull, 256) + agg_levels_bytes(L.nleaf);
 }
 
 static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1,
@@ -499,9 +506,14 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         return OWM_E_INTERNAL;
     }
     SortBufs S = carve_sort(sb, L.np, passes, L.ntiles, lb_tiles);
-    const uint64_t bu_words = bottomup_work_words(L.nleaf, L.ntiles);
-    uint32_t *bu_work = (uint32_t *)scratch.take(bu_words * 4ull);
-    bool have = bu_work != nullptr;
+    uint64_t lv_entries = 0;
+    AggLevels V = agg_levels_layout(L.nleaf, &lv_entries);
+    lv_entries = std::max<uint64_t>(lv_entries, 1);
+    V.box = (double4 *)scratch.take(lv_entries * 32);
+    V.cnt = (uint32_t *)scratch.take(lv_entries * 4);
+    V.zid = (uint32_t *)scratch.take(lv_entries * 4);
+    V.zk = (unsigned long long *)scratch.take(lv_entries * 8);
+    bool have = V.box && V.cnt && V.zid && V.zk;
     // record sets between passes: (x, y, z) here, indices in the values buffers
     RecOut rec[2] = {};
     for (int k = 0; k < (passes > 2 ? 2 : 1); ++k) {
@@ -556,12 +568,15 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     launch_leaf_aggs(s, B, T);
     // algorithmic bytes: x, y, z and index read per point; 48 per leaf written
     ctx->prof.end(s, pi, 28ull * L.np + 48ull * L.nleaf, L.np);
+    pi = ctx->prof.begin(s, PH_LEVELS);
+    launch_agg_levels(s, T, V);
+    // algorithmic bytes: 48 per leaf read, 48 per level entry written
+    ctx->prof.end(s, pi, 48ull * (L.nleaf + lv_entries), L.nleaf);
     pi = ctx->prof.begin(s, PH_KARRAS);
-    launch_karras(s, B, T);
-    ctx->prof.end(s, pi, 24ull * L.nleaf, L.nnode);  // with the bottom-up: 144 per leaf
-    pi = ctx->prof.begin(s, PH_BOTTOMUP);
-    launch_bottomup(s, B, T, bu_work, bu_words, guard);
-    ctx->prof.end(s, pi, 120ull * L.nleaf, L.nnode);
+    launch_karras(s, B, T, V);
+    // per node: the neighbours' first keys, a range/children/parent record and
+    // an aggregate written (28 + 48 B), the blocks of its range read
+    ctx->prof.end(s, pi, 24ull * L.nleaf + 76ull * L.nnode, L.nnode);
     HIP_TRY(hipGetLastError(), "tree build launch");
     return OWM_OK;
 }

@@ -61,7 +61,7 @@ struct TreeOut {
     uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid, *lf_parent, *lf_key0;
     unsigned long long *lf_zkey;
     double4 *nd_box;
-    uint32_t *nd_cnt, *nd_zid, *nd_parent, *nd_flag;
+    uint32_t *nd_cnt, *nd_zid, *nd_parent;
     unsigned long long *nd_zkey;
     uint2 *nd_child, *nd_range;  // children; leaf range [lo, hi] (tile-local leaf indices)
 };
@@ -79,13 +79,23 @@ void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const doub
 // Leaf aggregates over the points in Morton order (T.xs, ys, zs, ids); each
 // leaf's first key is recomputed from its first point.
 void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T);
-void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T);
-// Bottom-up aggregation: chunk-local climb with LDS arrival flags, then a
-// global climb over the nodes whose leaf range crosses a chunk.  `work` holds
-// bottomup_work_words(nleaf, ntiles) u32 of scratch.
-uint64_t bottomup_work_words(uint64_t nleaf, uint32_t ntiles);
-void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *work,
-                     uint64_t work_words, uint32_t *err);
+// Aggregate levels: level k (1..nlev) holds the merged aggregate of every
+// aligned block of 2^k leaves of the batch, size[k] entries from entry off[k].
+constexpr uint32_t AGG_MAX_LEV = 32;
+struct AggLevels {
+    double4 *box;
+    uint32_t *cnt, *zid;
+    unsigned long long *zk;
+    uint32_t nlev;
+    uint32_t off[AGG_MAX_LEV + 1], size[AGG_MAX_LEV + 1];
+};
+// Level sizes for nleaf leaves (pointers unset); *entries = entries of all levels.
+AggLevels agg_levels_layout(uint64_t nleaf, uint64_t *entries);
+// Levels from the leaf aggregates (one launch per 11 levels).
+void launch_agg_levels(hipStream_t s, const TreeOut &T, const AggLevels &V);
+// Karras tree over each tile's leaves; every node's aggregate is merged from
+// the levels' blocks that tile its leaf range (levels built first).
+void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T, const AggLevels &V);
 
 // ---- stages (stages.hip) ----
 struct Stage1Cfg {

@@ -649,6 +649,130 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
     }
 }
 
+// ------------------------------------------------------ aggregate levels --
+// Level k >= 1 holds, for every aligned block of 2^k leaves of the batch (by
+// batch leaf index), the merged aggregate of its leaves; level 0 is the leaves'
+// own aggregates.  An internal node's aggregate merges the at most two blocks
+// per level that tile its leaf range, so no bottom-up climb is needed: every
+// node is computed on its own, where k_karras finds its range.  A block may
+// span tiles, but the blocks that tile a node's range lie inside that range.
+// The merge (min/max box, count sum, lowest (z, index)) is exact and
+// independent of order.
+
+__device__ __forceinline__ void agg_merge(LeafAgg &a, const LeafAgg &b)
+{
+    a.mnx = fmin(a.mnx, b.mnx);
+    a.mny = fmin(a.mny, b.mny);
+    a.mxx = fmax(a.mxx, b.mxx);
+    a.mxy = fmax(a.mxy, b.mxy);
+    a.cnt += b.cnt;
+    if (owm_less(b.bk, b.bid, a.bk, a.bid)) {
+        a.bk = b.bk;
+        a.bid = b.bid;
+    }
+}
+
+__device__ __forceinline__ LeafAgg level_load(const TreeOut &T, const AggLevels &V, uint32_t k,
+                                              uint32_t i)
+{
+    LeafAgg a;
+    double4 b;
+    if (k == 0) {
+        b = T.lf_box[i];
+        a.cnt = T.lf_cnt[i];
+        a.bk = T.lf_zkey[i];
+        a.bid = T.lf_zid[i];
+    } else {
+        const uint32_t o = V.off[k] + i;
+        b = V.box[o];
+        a.cnt = V.cnt[o];
+        a.bk = V.zk[o];
+        a.bid = V.zid[o];
+    }
+    a.mnx = b.x;
+    a.mny = b.y;
+    a.mxx = b.z;
+    a.mxy = b.w;
+    return a;
+}
+
+__device__ __forceinline__ void level_store(const AggLevels &V, uint32_t k, uint32_t i,
+                                            const LeafAgg &a)
+{
+    if (k > V.nlev || i >= V.size[k])
+        return;
+    const uint32_t o = V.off[k] + i;
+    V.box[o] = make_double4(a.mnx, a.mny, a.mxx, a.mxy);
+    V.cnt[o] = a.cnt;
+    V.zk[o] = a.bk;
+    V.zid[o] = a.bid;
+}
+
+constexpr int LV_TB = 256;
+constexpr int LV_PER = 8;                    // level-k0 entries per thread
+constexpr uint32_t LV_SPAN = LV_TB * LV_PER;  // 2^11 entries per workgroup
+constexpr uint32_t LV_STEPS = 11;             // levels made per launch
+
+// Levels k0+1 .. k0+11 from level k0 (n_in entries): three inside a thread
+// (8 entries), six across the wave (butterflies), two across the four waves.
+__global__ __launch_bounds__(LV_TB) void k_agg_levels(TreeOut T, AggLevels V, uint32_t k0,
+                                                       uint32_t n_in)
+{
+    __shared__ LeafAgg sW[LV_TB / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (uint32_t g = blockIdx.x; g * LV_SPAN < n_in; g += gridDim.x) {
+        const uint32_t e0 = g * LV_SPAN + tid * LV_PER;
+        LeafAgg a[LV_PER];
+#pragma unroll
+        for (int j = 0; j < LV_PER; ++j)
+            a[j] = e0 + j < n_in ? level_load(T, V, k0, e0 + j) : leaf_agg_empty();
+#pragma unroll
+        for (int lv = 1; lv <= 3; ++lv) {
+#pragma unroll
+            for (int i = 0; i < (LV_PER >> lv); ++i) {
+                agg_merge(a[2 * i], a[2 * i + 1]);
+                a[i] = a[2 * i];
+                level_store(V, k0 + lv, (e0 >> lv) + i, a[i]);
+            }
+        }
+        LeafAgg m = a[0];  // level k0+3 entry g * 256 + tid
+        const uint32_t q = g * LV_TB + tid;
+        leaf_agg_step<1>(m);
+        if ((lane & 1) == 0)
+            level_store(V, k0 + 4, q >> 1, m);
+        leaf_agg_step<2>(m);
+        if ((lane & 3) == 0)
+            level_store(V, k0 + 5, q >> 2, m);
+        leaf_agg_step<4>(m);
+        if ((lane & 7) == 0)
+            level_store(V, k0 + 6, q >> 3, m);
+        leaf_agg_step<8>(m);
+        if ((lane & 15) == 0)
+            level_store(V, k0 + 7, q >> 4, m);
+        leaf_agg_step<16>(m);
+        if ((lane & 31) == 0)
+            level_store(V, k0 + 8, q >> 5, m);
+        leaf_agg_step<32>(m);
+        if (lane == 0) {
+            level_store(V, k0 + 9, q >> 6, m);
+            sW[w] = m;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            LeafAgg u = sW[2 * tid];
+            agg_merge(u, sW[2 * tid + 1]);
+            level_store(V, k0 + 10, g * 2 + tid, u);
+        } else if (tid == 2) {
+            LeafAgg u = sW[0];
+            agg_merge(u, sW[1]);
+            agg_merge(u, sW[2]);
+            agg_merge(u, sW[3]);
+            level_store(V, k0 + 11, g, u);
+        }
+        __syncthreads();  // sW is rewritten by the next block
+    }
+}
+
 __device__ __forceinline__ int kdelta(const uint32_t *key0, uint32_t nl, int a, int b)
 {
     if (b < 0 || b >= (int)nl)
@@ -661,7 +785,9 @@ __device__ __forceinline__ int kdelta(const uint32_t *key0, uint32_t nl, int a, 
 
 // Karras (2012) binary radix tree over the leaves of each tile, keys = Morton
 // key of each leaf's first point, ties broken by leaf index.
-__global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T)
+// Each node's aggregate (bbox, count, lowest (z, index)) merges the aggregate
+// levels' blocks that tile its leaf range.
+__global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T, AggLevels V)
 {
     for (uint32_t g = blockIdx.x * TB + threadIdx.x; g < B.nnode; g += gridDim.x * TB) {
         uint32_t t = owm_find_tile(B.node_off, B.ntiles, g);
@@ -709,230 +835,21 @@ __global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T)
         T.nd_range[g] = make_uint2((uint32_t)lo, (uint32_t)hi);
         if (i == 0)
             T.nd_parent[g] = OWM_NONE;
-    }
-}
-
-struct Agg {
-    double4 box;
-    uint32_t cnt, zid;
-    unsigned long long zk;
-};
-
-// Leaf aggregates come from an earlier kernel and are read with plain loads.
-// Node aggregates are written during the climb by another wave of the same
-// workgroup (chunk climb, SCOPE = workgroup) or by another workgroup (global
-// climb, SCOPE = agent), so they are read with loads coherent at that scope.
-template <int SCOPE>
-__device__ __forceinline__ double ld_d(const double *p)
-{
-    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, SCOPE);
-    return __longlong_as_double((long long)v);
-}
-
-template <int SCOPE>
-__device__ __forceinline__ Agg load_agg(const TreeOut &T, uint32_t ref)
-{
-    Agg a;
-    if (ref & OWM_LEAF_BIT) {
-        const uint32_t l = ref & ~OWM_LEAF_BIT;
-        a.box = T.lf_box[l];
-        a.cnt = T.lf_cnt[l];
-        a.zk = T.lf_zkey[l];
-        a.zid = T.lf_zid[l];
-    } else {
-        const double *bp = (const double *)(T.nd_box + ref);
-        a.cnt = __hip_atomic_load(T.nd_cnt + ref, __ATOMIC_RELAXED, SCOPE);
-        a.zk = __hip_atomic_load(T.nd_zkey + ref, __ATOMIC_RELAXED, SCOPE);
-        a.zid = __hip_atomic_load(T.nd_zid + ref, __ATOMIC_RELAXED, SCOPE);
-        a.box = make_double4(ld_d<SCOPE>(bp), ld_d<SCOPE>(bp + 1), ld_d<SCOPE>(bp + 2),
-                             ld_d<SCOPE>(bp + 3));
-    }
-    return a;
-}
-
-__device__ __forceinline__ Agg merge_agg(const Agg &a, const Agg &b)
-{
-    Agg r;
-    r.box = make_double4(fmin(a.box.x, b.box.x), fmin(a.box.y, b.box.y), fmax(a.box.z, b.box.z),
-                         fmax(a.box.w, b.box.w));
-    r.cnt = a.cnt + b.cnt;
-    if (owm_less(b.zk, b.zid, a.zk, a.zid)) {
-        r.zk = b.zk;
-        r.zid = b.zid;
-    } else {
-        r.zk = a.zk;
-        r.zid = a.zid;
-    }
-    return r;
-}
-
-__device__ __forceinline__ void store_agg(const TreeOut &T, uint32_t p, const Agg &a)
-{
-    T.nd_box[p] = a.box;
-    T.nd_cnt[p] = a.cnt;
-    T.nd_zkey[p] = a.zk;
-    T.nd_zid[p] = a.zid;
-}
-
-constexpr uint32_t BU_CHUNK = 2048;  // leaves per workgroup in the chunk climb
-constexpr int BU_TB = 1024;           // threads per chunk-climb workgroup
-// A chunk exports at most the children of the crossing paths at its two
-// boundaries, each no longer than the depth limit.
-constexpr uint32_t BU_MAX_DEPTH = 128;
-constexpr uint32_t BU_EXP_PER_CHUNK = 2 * (BU_MAX_DEPTH + 2);
-
-// Chunk climb: a workgroup owns leaves [A, E] of the batch; a node whose
-// leaf range lies inside [A, E] has both children completed by this
-// workgroup, so its arrival counter lives in LDS and needs no device-wide
-// fence.  A completed node whose parent's range leaves the chunk is exported
-// for the global climb.
-__global__ __launch_bounds__(BU_TB) void k_bottomup_local(BatchDev B, TreeOut T, uint32_t *exp_cnt,
-                                                        uint32_t *exported, uint32_t *err)
-{
-    __shared__ uint32_t sFlag[BU_CHUNK];
-    __shared__ uint32_t sNexp;
-    const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
-    for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-        const uint32_t A = ch * BU_CHUNK, E = min(A + BU_CHUNK, B.nleaf) - 1;
-        uint32_t *slots = exported + (size_t)ch * BU_EXP_PER_CHUNK;
-        for (uint32_t k = threadIdx.x; k < BU_CHUNK; k += BU_TB)
-            sFlag[k] = 0;
-        if (threadIdx.x == 0)
-            sNexp = 0;
-        __syncthreads();
-        for (uint32_t l = A + threadIdx.x; l <= E; l += BU_TB) {
-            const TileDev &TD = B.tiles[owm_find_tile(B.leaf_off, B.ntiles, l)];
-            uint32_t c = OWM_LEAF_BIT | l;
-            Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
-            uint32_t p = T.lf_parent[l];
-            for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
-                if (depth > BU_MAX_DEPTH) {
-                    atomicOr(err, 4u);
-                    break;
-                }
-                // everything about p in one round trip
-                const uint2 rg = T.nd_range[p];
-                const uint2 chd = T.nd_child[p];
-                const uint32_t pp = T.nd_parent[p];
-                const uint32_t glo = TD.leaf_off + rg.x, ghi = TD.leaf_off + rg.y;
-                if (glo < A || ghi > E) {
-                    const uint32_t k = atomicAdd(&sNexp, 1u);
-                    if (k < BU_EXP_PER_CHUNK)
-                        slots[k] = c;
-                    else
-                        atomicOr(err, 8u);
-                    break;
-                }
-                const uint32_t slot = TD.leaf_off + (p - TD.node_off) - A;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                const uint32_t old = atomicAdd(&sFlag[slot], 1u);
-                if (old == 0)
-                    break;
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                const Agg other = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
-                mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
-                store_agg(T, p, mine);
-                c = p;
-                p = pp;
-            }
+        // blocks of [leaf_off + lo, leaf_off + hi] level by level (half-open)
+        LeafAgg acc = leaf_agg_empty();
+        uint32_t ea = TD.leaf_off + (uint32_t)lo, ee = TD.leaf_off + (uint32_t)hi + 1;
+        for (uint32_t k = 0; ea < ee; ++k) {
+            if (ea & 1)
+                agg_merge(acc, level_load(T, V, k, ea++));
+            if (ee & 1)
+                agg_merge(acc, level_load(T, V, k, --ee));
+            ea >>= 1;
+            ee >>= 1;
         }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            exp_cnt[ch] = min(sNexp, BU_EXP_PER_CHUNK);
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ bool in_tile(const TileDev &TD, uint32_t ref)
-{
-    if (ref & OWM_LEAF_BIT) {
-        const uint32_t l = ref & ~OWM_LEAF_BIT;
-        return l >= TD.leaf_off && l < TD.leaf_off + TD.nleaf;
-    }
-    return ref >= TD.node_off && ref + 1 < TD.node_off + TD.nleaf;
-}
-
-// Global climb from the exported nodes (those whose parent's leaf range
-// crosses a chunk).  Climbs of one tile can meet, climbs of different tiles
-// never do, so one workgroup owns each tile: all writes and arrivals of a
-// tile's climb come from one workgroup and workgroup-scope release/acquire
-// suffice (no device-wide cache write-back per level).  Aggregates of the
-// chunk climb were written by an earlier kernel and are visible.
-__global__ __launch_bounds__(BU_TB) void k_bottomup_global(BatchDev B, TreeOut T,
-                                                            const uint32_t *exp_cnt,
-                                                            const uint32_t *exported, uint32_t *err)
-{
-    __shared__ uint32_t sOff[BU_TB + 1];
-    __shared__ uint32_t sWave[BU_TB / 64];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
-        const TileDev &TD = B.tiles[t];
-        if (TD.nleaf < 2)
-            continue;
-        const uint32_t c0 = TD.leaf_off / BU_CHUNK, c1 = (TD.leaf_off + TD.nleaf - 1) / BU_CHUNK;
-        if (c0 == c1)
-            continue;  // the chunk climb finished the whole tree
-        for (uint32_t cb = c0; cb <= c1; cb += BU_TB) {
-            const uint32_t m = min((uint32_t)BU_TB, c1 - cb + 1);
-            // exclusive scan of the export counts of chunks cb .. cb+m-1
-            const uint32_t v = tid < m ? exp_cnt[cb + tid] : 0u;
-            uint32_t inc = v;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t u = __shfl_up(inc, o, 64);
-                if (lane >= (uint32_t)o)
-                    inc += u;
-            }
-            if (lane == 63)
-                sWave[w] = inc;
-            __syncthreads();
-            uint32_t add = 0, total = 0;
-            for (uint32_t k = 0; k < BU_TB / 64; ++k) {
-                if (k < w)
-                    add += sWave[k];
-                total += sWave[k];
-            }
-            sOff[tid] = add + inc - v;
-            if (tid == 0)
-                sOff[BU_TB] = total;
-            __syncthreads();
-            for (uint32_t i = tid; i < total; i += BU_TB) {
-                uint32_t lo = 0, hi = m;  // last chunk with sOff <= i
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (sOff[mid] <= i)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                uint32_t c = exported[(size_t)(cb + lo) * BU_EXP_PER_CHUNK + (i - sOff[lo])];
-                if (!in_tile(TD, c))
-                    continue;  // a chunk shared with a neighbouring tile
-                Agg mine = load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, c);
-                uint32_t p = (c & OWM_LEAF_BIT) ? T.lf_parent[c & ~OWM_LEAF_BIT] : T.nd_parent[c];
-                for (uint32_t depth = 0; p != OWM_NONE; ++depth) {
-                    if (depth > BU_MAX_DEPTH) {
-                        atomicOr(err, 4u);
-                        break;
-                    }
-                    const uint2 chd = T.nd_child[p];
-                    const uint32_t pp = T.nd_parent[p];
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    const uint32_t old = __hip_atomic_fetch_add(&T.nd_flag[p], 1u, __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (old == 0)
-                        break;
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    const Agg other =
-                        load_agg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, chd.x == c ? chd.y : chd.x);
-                    mine = chd.x == c ? merge_agg(mine, other) : merge_agg(other, mine);
-                    store_agg(T, p, mine);
-                    c = p;
-                    p = pp;
-                }
-            }
-            __syncthreads();  // sOff is rewritten for the next piece
-        }
+        T.nd_box[g] = make_double4(acc.mnx, acc.mny, acc.mxx, acc.mxy);
+        T.nd_cnt[g] = acc.cnt;
+        T.nd_zkey[g] = acc.bk;
+        T.nd_zid[g] = acc.bid;
     }
 }
 
@@ -976,34 +893,40 @@ void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T)
                        B, T);
 }
 
-void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T)
+void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T, const AggLevels &V)
 {
     if (B.nnode == 0)
         return;
-    hipLaunchKernelGGL(k_karras, dim3(grid_for(B.nnode, TB, 8192)), dim3(TB), 0, s, B, T);
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(B.nnode, TB, 8192)), dim3(TB), 0, s, B, T, V);
 }
 
-uint64_t bottomup_work_words(uint64_t nleaf, uint32_t ntiles)
+AggLevels agg_levels_layout(uint64_t nleaf, uint64_t *entries)
 {
-    (void)ntiles;
-    const uint64_t chunks = (nleaf + BU_CHUNK - 1) / BU_CHUNK;
-    return chunks * (1 + BU_EXP_PER_CHUNK) + 64;
+    AggLevels V{};
+    uint64_t o = 0;
+    uint32_t k = 0;
+    V.size[0] = (uint32_t)nleaf;
+    // a node's range needs levels up to floor(log2(nleaf))
+    while (k < AGG_MAX_LEV && (nleaf >> k) > 1) {
+        ++k;
+        V.off[k] = (uint32_t)o;
+        V.size[k] = (uint32_t)((nleaf + (1ull << k) - 1) >> k);
+        o += V.size[k];
+    }
+    V.nlev = k;
+    *entries = o;
+    return V;
 }
 
-void launch_bottomup(hipStream_t s, const BatchDev &B, const TreeOut &T, uint32_t *work,
-                     uint64_t work_words, uint32_t *err)
+void launch_agg_levels(hipStream_t s, const TreeOut &T, const AggLevels &V)
 {
-    (void)work_words;
-    if (B.nnode == 0)
-        return;
-    launch_fill_u32(s, T.nd_flag, B.nnode, 0u);
-    const uint32_t nch = (B.nleaf + BU_CHUNK - 1) / BU_CHUNK;
-    uint32_t *exp_cnt = work, *exported = work + nch;
-    hipLaunchKernelGGL(k_bottomup_local, dim3(nch < 1 ? 1 : (nch > 16384 ? 16384 : nch)), dim3(BU_TB),
-                       0, s, B, T, exp_cnt, exported, err);
-    const uint32_t g = B.ntiles < 1 ? 1 : (B.ntiles > 4096 ? 4096 : B.ntiles);
-    hipLaunchKernelGGL(k_bottomup_global, dim3(g), dim3(BU_TB), 0, s, B, T, (const uint32_t *)exp_cnt,
-                       (const uint32_t *)exported, err);
+    uint64_t n_in = V.size[0];
+    for (uint32_t k0 = 0; k0 < V.nlev; k0 += LV_STEPS) {
+        const uint64_t blocks = (n_in + LV_SPAN - 1) / LV_SPAN;
+        hipLaunchKernelGGL(k_agg_levels, dim3(blocks > 8192 ? 8192 : (uint32_t)blocks), dim3(LV_TB),
+                           0, s, T, V, k0, (uint32_t)n_in);
+        n_in = blocks;
+    }
 }
 
 }  // namespace owm
