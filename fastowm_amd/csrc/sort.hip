@@ -35,7 +35,10 @@ constexpr int WAVES = BLOCK / 64;
 // one look-back, so large sorts use 8192-key tiles; small sorts keep 4096-key
 // tiles for more workgroups.
 constexpr uint32_t BIG_SORT = 1u << 21;
-static inline int sort_items(uint32_t n) { return n >= BIG_SORT ? 32 : 16; }
+// key/value sort tile: 8192 keys per 256 threads from BIG_SORT keys; smaller
+// sorts (window minima, seeds) take 2048- or 1024-key tiles so that their few
+// hundred thousand keys still spread over every CU
+static inline int sort_items(uint32_t n) { return n >= BIG_SORT ? 32 : (n >= (1u << 18) ? 8 : 4); }
 constexpr uint32_t SEG_HIST = SORT_SEG_HIST;
 static_assert(SEG_HIST == 4 * RADIX, "four passes of 8-bit digits per segment");
 constexpr uint32_t LB_AGG = 1u << 30;
@@ -567,12 +570,16 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
     if (grid_cap && grid > grid_cap)
         grid = grid_cap;
     int wk = write_keys ? 1 : 0;
-    if (sort_items(n_max) == 32)
+    const int it = sort_items(n_max);
+    if (it == 32)
         onesweep_items<32>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
                            lookback, tile_counter, keys_only, iota, wk, grid, seg);
+    else if (it == 8)
+        onesweep_items<8>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
+                          lookback, tile_counter, keys_only, iota, wk, grid, seg);
     else
-        onesweep_items<16>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
-                           lookback, tile_counter, keys_only, iota, wk, grid, seg);
+        onesweep_items<4>(s, keys_in, vals_in, keys_out, vals_out, d_n, n_max, shift, digit_off,
+                          lookback, tile_counter, keys_only, iota, wk, grid, seg);
 }
 
 // Record sort tiles: 16384 points (8 waves x 32) for sorts of >= BIG_SORT
