@@ -572,6 +572,18 @@ owm_status owm_profile_enable(owm_ctx *ctx, int enable)
     GUARD_END(ctx)
 }
 
+owm_status owm_profile_phases(owm_ctx *ctx, uint32_t mask)
+{
+    if (!ctx)
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->prof.collect();
+    ctx->prof.mask = mask ? mask : ~0u;
+    return OWM_OK;
+    GUARD_END(ctx)
+}
+
 owm_status owm_profile_read(owm_ctx *ctx, owm_phase_stat *out, uint32_t cap, uint32_t *n)
 {
     if (!ctx || !n || (cap && !out))

@@ -66,7 +66,7 @@ const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
 
 int Profiler::begin(hipStream_t s, int ph)
 {
-    if (!on)
+    if (!on || !((mask >> ph) & 1u))
         return -1;
     hipEvent_t ev[2];
     for (auto &e : ev) {
@@ -79,17 +79,18 @@ int Profiler::begin(hipStream_t s, int ph)
         }
     }
     (void)hipEventRecord(ev[0], s);
-    pending.push_back({ph, ev[0], ev[1], 0, 0});
+    pending.push_back({ph, ev[0], ev[1], 0, 0, 1});
     return (int)pending.size() - 1;
 }
 
-void Profiler::end(hipStream_t s, int idx, uint64_t b, uint64_t it)
+void Profiler::end(hipStream_t s, int idx, uint64_t b, uint64_t it, uint32_t launches)
 {
     if (idx < 0 || idx >= (int)pending.size())
         return;
     (void)hipEventRecord(pending[idx].b, s);
     pending[idx].bytes = b;
     pending[idx].items = it;
+    pending[idx].launches = launches;
 }
 
 void Profiler::collect()
@@ -98,7 +99,7 @@ void Profiler::collect()
         float t = 0.f;
         if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
             ms[r.ph] += t;
-            calls[r.ph] += 1;
+            calls[r.ph] += r.launches;
             bytes[r.ph] += r.bytes;
             items[r.ph] += r.items;
         }
@@ -475,18 +476,19 @@ static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_
     launch_sort_scan(s, S.hist, passes, S.counters);
     launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
+    const int pi = ctx->prof.begin(s, phase);
     for (int p = 0; p < passes; ++p) {
-        int pi = ctx->prof.begin(s, phase);
         launch_onesweep(s, kin, vin, kout, vout, d_n, n_max, 8 * p, S.hist + 256 * p,
                         S.lookback + (uint64_t)p * tiles * 256, S.counters + p, keys_only,
                         iota && p == 0, true, 0);
-        // algorithmic bytes per key and pass: 12 for the window minima (with
-        // the run-length pass, 44 per window over three passes), 8 for the
-        // seed merge (keys only)
-        ctx->prof.end(s, pi, (phase == PH_SORT_WIN ? 12ull : 8ull) * n_max, n_max);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
+    // algorithmic bytes per key and pass: 12 for the window minima (with the
+    // run-length pass, 44 per window over three passes), 8 for the seed merge
+    // (keys only)
+    ctx->prof.end(s, pi, (phase == PH_SORT_WIN ? 12ull : 8ull) * n_max * passes,
+                  (uint64_t)n_max * passes, (uint32_t)passes);
     return (passes & 1) ? 1 : 0;
 }
 
@@ -543,6 +545,8 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     // passes move (key, x, y, z, index): the cloud -> rec[0] -> rec[1] -> ... -> the tree
     const RecOut tree_rec{T.xs, T.ys, T.zs, T.ids, T.pos};
     uint32_t *kin = S.k0, *kout = S.k1;
+    uint64_t sort_bytes = 0;
+    pi = ctx->prof.begin(s, PH_SORT_TREE);
     for (int p = 0; p < passes; ++p) {
         RecIn in;
         if (p == 0) {
@@ -552,7 +556,6 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
             in = RecIn{r.x, r.y, r.z, r.id};
         }
         const RecOut &out = p == passes - 1 ? tree_rec : rec[p & 1];
-        pi = ctx->prof.begin(s, PH_SORT_TREE);
         const bool last = p == passes - 1;
         launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p,
                             L.tile_bits + L.key_bits, S.hist + 256 * p,
@@ -561,9 +564,10 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         // algorithmic bytes per point and pass: a key and a 28-byte record
         // read and written, except that the first pass reads no index (it
         // numbers the points) and the last writes no key
-        ctx->prof.end(s, pi, (p == 0 || last ? 60ull : 64ull) * L.np, L.np);
+        sort_bytes += (p == 0 || last ? 60ull : 64ull) * L.np;
         std::swap(kin, kout);
     }
+    ctx->prof.end(s, pi, sort_bytes, (uint64_t)L.np * passes, (uint32_t)passes);
     pi = ctx->prof.begin(s, PH_LEAVES);
     launch_leaf_aggs(s, B, T);
     // algorithmic bytes: x, y, z and index read per point; 48 per leaf written
@@ -729,6 +733,8 @@ owm_status Arena::ensure(owm_ctx *ctx, uint64_t bytes)
     if (e != hipSuccess)
         return hip_fail(ctx, e, "sync before arena growth");
     e = buf.reserve(bytes, ctx->room_for(buf));
+    if (this == &ctx->arena)
+        ctx->graphs.clear();  // captured chains point into the old arena
     if (e != hipSuccess)
         return hip_fail(ctx, e, "arena allocation");
     return OWM_OK;

@@ -199,17 +199,20 @@ const char *phase_name(int ph);
 // HIP-event timing of the kernel families on the context's stream.
 struct Profiler {
     bool on = false;
+    uint32_t mask = ~0u;  // phases that record events (bit k = phase k)
     struct Rec {
         int ph;
         hipEvent_t a, b;
         uint64_t bytes, items;
+        uint32_t launches;
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
     double ms[PH_COUNT] = {};
     uint64_t calls[PH_COUNT] = {}, bytes[PH_COUNT] = {}, items[PH_COUNT] = {};
     int begin(hipStream_t s, int ph);
-    void end(hipStream_t s, int idx, uint64_t bytes, uint64_t items);
+    // launches: kernel launches of the phase between begin and end
+    void end(hipStream_t s, int idx, uint64_t bytes, uint64_t items, uint32_t launches = 1);
     void collect();  // after the stream has been synchronised
     void reset();
     ~Profiler();

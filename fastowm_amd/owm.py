@@ -122,6 +122,7 @@ def lib():
             "owm_serve_evict": ([vp, u64], ctypes.c_int),
             "owm_serve_stop": ([vp], ctypes.c_int),
             "owm_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
+            "owm_profile_phases": ([vp, u32], ctypes.c_int),
             "owm_profile_read": ([vp, P(PhaseStat), u32, P(u32)], ctypes.c_int),
             "owm_autotune": ([vp, dp, dp, dp, ctypes.c_int, P(TileDesc), u32, P(TuneEntry), u32,
                               P(u32)], ctypes.c_int),
@@ -279,8 +280,15 @@ class Context:
         if st:
             raise OWMError(st, self.last_error())
 
-    def profile(self, enable: bool = True):
-        """Start (zeroing) or stop per-phase HIP-event timing."""
+    def profile(self, enable: bool = True, phases=None):
+        """Start (zeroing) or stop per-phase HIP-event timing; `phases` (names
+        as in profile_read) limits the events to those phases."""
+        mask = 0
+        if enable and phases:
+            names = list(self.profile_read())
+            for p in phases:
+                mask |= 1 << names.index(p)
+        self._check(lib().owm_profile_phases(self.h, mask))
         self._check(lib().owm_profile_enable(self.h, 1 if enable else 0))
 
     def profile_read(self):

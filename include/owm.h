@@ -96,6 +96,11 @@ typedef struct owm_phase_stat {
 } owm_phase_stat;
 /* enable != 0 starts recording and zeroes the counters; 0 stops. */
 owm_status owm_profile_enable(owm_ctx *ctx, int enable);
+/* Phases that record events while profiling is on: bit k = the k-th phase of
+ * owm_profile_read (0 = all, the default).  Each recorded phase costs two
+ * events on the stream per occurrence, so timing one kernel family leaves the
+ * others undisturbed. */
+owm_status owm_profile_phases(owm_ctx *ctx, uint32_t mask);
 owm_status owm_profile_read(owm_ctx *ctx, owm_phase_stat *out, uint32_t cap, uint32_t *n);
 
 /* Parameter derivation only (no device work): owm_optim3.cpp:92-132. */
