@@ -43,6 +43,27 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what)
 
 static inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
+// Buffers a chain needs zeroed, cleared together by one launch right after
+// they are carved (every one of them is zero before its first use, and none is
+// written in between).
+struct ZeroList {
+    FillRanges r;
+    void add(hipStream_t s, void *p, uint64_t words)
+    {
+        if (!p || !words)
+            return;
+        if (r.count == FILL_MAX)
+            launch(s);
+        r.ptr[r.count] = (uint32_t *)p;
+        r.len[r.count++] = words;
+    }
+    void launch(hipStream_t s)
+    {
+        launch_fill_ranges(s, r);
+        r.count = 0;
+    }
+};
+
 // Regions of the pinned readback buffer for n_tiles tiles.
 struct IoLayout {
     uint64_t ptoff, bbox, counts, seed_off, bytes;
@@ -470,11 +491,10 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg 
 static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
                     const uint32_t *d_n, int passes, bool keys_only, bool iota)
 {
-    launch_fill_u32(s, S.hist, 4 * 256, 0u);
+    // S.hist and S.lookback were zeroed with the chain's other buffers
     launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
     const uint32_t tiles = sort_tiles(std::max<uint32_t>(n_max, 1));
     launch_sort_scan(s, S.hist, passes, S.counters);
-    launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     uint32_t *kin = S.k0, *vin = S.v0, *kout = S.k1, *vout = S.v1;
     const int pi = ctx->prof.begin(s, phase);
     for (int p = 0; p < passes; ++p) {
@@ -529,7 +549,12 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         set_err(ctx, "scratch arena exhausted (tree build)");
         return OWM_E_INTERNAL;
     }
-    launch_fill_u32(s, S.hist, (uint64_t)L.ntiles * SORT_SEG_HIST, 0u);
+    const uint32_t tiles = L.sort_units[L.ntiles];
+    ZeroList z;
+    z.add(s, guard, 64);
+    z.add(s, S.hist, (uint64_t)L.ntiles * SORT_SEG_HIST);
+    z.add(s, S.lookback, (uint64_t)passes * tiles * 256);
+    z.launch(s);
     int pi = ctx->prof.begin(s, PH_MORTON);
     launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
     // algorithmic bytes per point: read x, y; write a key
@@ -539,9 +564,7 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     seg.units = B.sort_units;
     seg.nseg = L.ntiles;
     seg.digit_stride = SORT_SEG_HIST;
-    const uint32_t tiles = L.sort_units[L.ntiles];
     launch_sort_scan_seg(s, S.hist, seg.pt, seg.nseg, passes, S.counters);
-    launch_fill_u32(s, S.lookback, (uint64_t)passes * tiles * 256, 0u);
     // passes move (key, x, y, z, index): the cloud -> rec[0] -> rec[1] -> ... -> the tree
     const RecOut tree_rec{T.xs, T.ys, T.zs, T.ids, T.pos};
     uint32_t *kin = S.k0, *kout = S.k1;
@@ -638,7 +661,19 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     SortBufs SD = carve_sort(p2, nseed_max, passes);
     out.cnt = cnt;
     out.seed_off = seed_off;
-    launch_fill_u32(s, cnt, 16, 0u);
+    // counters, join work, both sorts' histograms and look-back, both
+    // compactions' look-back, stage-3 cell flags
+    ZeroList z;
+    z.add(s, cnt, 16);
+    z.add(s, s1work, stage1_work_words(Q.nblk));
+    z.add(s, W.hist, 4 * 256);
+    z.add(s, W.lookback, (uint64_t)passes * sort_tiles((uint32_t)nw) * 256);
+    z.add(s, lb1, nseed_max / 4096 + 2);
+    z.add(s, cell_flag, std::max<uint64_t>(Q.ncell, 1));
+    z.add(s, lb2, nseed_max / 4096 + 2);
+    z.add(s, SD.hist, 4 * 256);
+    z.add(s, SD.lookback, (uint64_t)passes * sort_tiles((uint32_t)nseed_max) * 256);
+    z.launch(s);
     if (from <= 1) {
         const uint32_t s1cap = (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4);
         uint32_t *list = (uint32_t *)ctx->s1_list.ptr;
@@ -1096,7 +1131,7 @@ static owm_status enqueue_chain(owm_ctx *ctx, bool graph, const double *dx, cons
     hipStream_t s = ctx->stream;
     Arena &A = ctx->arena;
     auto body = [&]() -> owm_status {
-        launch_fill_u32(s, guard, 64, 0u);
+        // build_tree zeroes the guard with its own scratch
         owm_status st = build_tree(ctx, s, P.L, B, dx, dy, dz, T, A, guard);
         if (st != OWM_OK)
             return st;

@@ -24,6 +24,14 @@ constexpr uint32_t SORT_SEG_HIST = 4 * 256;  // histogram words per segment (all
 void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,
                           int passes, uint32_t *counters);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+// Zeroes up to FILL_MAX ranges (u32 words) in one launch.
+constexpr uint32_t FILL_MAX = 16;
+struct FillRanges {
+    uint32_t count = 0;
+    uint32_t *ptr[FILL_MAX] = {};
+    uint64_t len[FILL_MAX] = {};
+};
+void launch_fill_ranges(hipStream_t s, const FillRanges &r);
 void launch_sort_hist(hipStream_t s, const uint32_t *keys, const uint32_t *d_n, uint32_t n_max,
                       int passes, uint32_t *hist);
 void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *counters);

@@ -505,6 +505,29 @@ uint32_t sort_tiles(uint32_t n)
     return (n + tile - 1) / tile;
 }
 
+// One range per blockIdx.y.
+__global__ void k_fill_ranges(FillRanges r)
+{
+    uint32_t *p = r.ptr[blockIdx.y];
+    const uint64_t n = r.len[blockIdx.y];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+void launch_fill_ranges(hipStream_t s, const FillRanges &r)
+{
+    uint64_t mx = 0;
+    for (uint32_t k = 0; k < r.count; ++k)
+        mx = r.len[k] > mx ? r.len[k] : mx;
+    if (r.count == 0 || mx == 0)
+        return;
+    uint64_t blocks = (mx + 255) / 256;
+    if (blocks > 1024)
+        blocks = 1024;
+    hipLaunchKernelGGL(k_fill_ranges, dim3((unsigned)blocks, r.count), dim3(256), 0, s, r);
+}
+
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v)
 {
     if (n == 0)

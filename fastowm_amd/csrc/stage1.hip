@@ -772,7 +772,8 @@ void launch_leaf_cells(hipStream_t s, const BatchDev &B, const TreeDev &T, const
     hipLaunchKernelGGL(k_lcs, dim3(g), dim3(256), 0, s, B, T, lc);
 }
 
-// work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter
+// work: cnt[nblk] | start[nblk+1] | look-back[...] | tile counter, all zero on
+// entry.
 void launch_stage1_join(hipStream_t s, const BatchDev &B, const TreeDev &T, const Stage1Cfg &cfg,
                         uint32_t *work, uint32_t *list, uint32_t cap, uint32_t *err,
                         uint32_t *total_out)
@@ -784,7 +785,6 @@ void launch_stage1_join(hipStream_t s, const BatchDev &B, const TreeDev &T, cons
     uint32_t *lookback = start + (B.nblk + 1);
     const uint32_t nt = B.nblk / SC_TILE + 2;
     uint32_t *counter = lookback + nt;
-    launch_fill_u32(s, work, stage1_work_words(B.nblk), 0u);
     uint32_t g = (B.nleaf + 255) / 256;
     g = g < 1 ? 1 : (g > 8192 ? 8192 : g);
     hipLaunchKernelGGL(k_s1_bin<0>, dim3(g), dim3(256), 0, s, B, T, cfg.bx, cfg.by, cnt,

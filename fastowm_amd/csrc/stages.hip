@@ -398,12 +398,11 @@ static uint32_t cp_grid(uint32_t n)
     return t < 1 ? 1 : (t > 4096 ? 4096 : t);
 }
 
+// lookback ((nwin + CP_TILE - 1) / CP_TILE + 1 words) and tile_counter start at zero.
 void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
                            uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,
                            uint32_t *tile_counter)
 {
-    launch_fill_u32(s, lookback, (nwin + CP_TILE - 1) / CP_TILE + 1, 0u);
-    launch_fill_u32(s, tile_counter, 1, 0u);
     hipLaunchKernelGGL(k_compact<0>, dim3(cp_grid(nwin)), dim3(CP_BLOCK), 0, s, B, sorted, nwin,
                        any_need1, llp, (const uint32_t *)nullptr, n_llp, lookback, tile_counter);
 }
@@ -412,9 +411,8 @@ void launch_compact_valid(hipStream_t s, const uint32_t *in, uint32_t n, uint32_
                           const uint32_t *d_out_base, uint32_t *d_count, uint32_t *lookback,
                           uint32_t *tile_counter)
 {
+    // lookback ((n + CP_TILE - 1) / CP_TILE + 1 words) and tile_counter start at zero
     BatchDev B{};
-    launch_fill_u32(s, lookback, (n + CP_TILE - 1) / CP_TILE + 1, 0u);
-    launch_fill_u32(s, tile_counter, 1, 0u);
     hipLaunchKernelGGL(k_compact<1>, dim3(cp_grid(n)), dim3(CP_BLOCK), 0, s, B, in, n, 0u, out,
                        d_out_base, d_count, lookback, tile_counter);
 }
@@ -425,7 +423,7 @@ void launch_stage3(hipStream_t s, const BatchDev &B, const TreeDev &T, const uin
 {
     if (B.ncell == 0)
         return;
-    launch_fill_u32(s, cell_flag, B.ncell, 0u);
+    // cell_flag (B.ncell words) starts at zero
     uint32_t g = (n_llp_max + 255) / 256;
     g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
     hipLaunchKernelGGL(k_s3_mark, dim3(g), dim3(256), 0, s, B, T, llp, n_llp, cell_flag);
