@@ -218,6 +218,30 @@ def test_c3_full_size_roofs_with_ties(ctx):
     assert np.array_equal(seeds, want2)
 
 
+def test_profile_phase_mask(ctx):
+    """Phase timing limited to one phase records that phase only (one event
+    pair per multi-pass sort, counted as its passes) and leaves the seeds as
+    they are; with no mask every phase of the chain is timed."""
+    say("profile phase mask")
+    t = _tile(60_000, seed=41)
+    x, y, z = t.generate()
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    ctx.profile(True, phases=["sort_tree_pass"])
+    got = ctx.run_tile(x, y, z, 12, 20, 0.8, 32)
+    only = ctx.profile_read()
+    ctx.profile(True)
+    ctx.run_tile(x, y, z, 12, 20, 0.8, 32)
+    full = ctx.profile_read()
+    ctx.profile(False)
+    assert np.array_equal(got, want)
+    timed = {k for k, v in only.items() if v["calls"]}
+    assert timed == {"sort_tree_pass"}
+    assert only["sort_tree_pass"]["calls"] >= 2 and only["sort_tree_pass"]["ms"] > 0
+    assert only["sort_tree_pass"]["calls"] == full["sort_tree_pass"]["calls"]
+    for k in ("bbox", "morton", "leaf_aggs", "karras", "agg_levels", "stage1", "stage3"):
+        assert full[k]["calls"] >= 1, k
+
+
 def test_errors(ctx):
     say("errors")
     x, y, z = _tile(2000).generate()
