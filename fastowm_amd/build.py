@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-function", "-Wno-unused-variable"]
-HIP_SOURCES = ["sort.hip", "tree.hip", "stage1.hip", "stages.hip", "owm_host.cpp", "owm_capi.cpp", "owm_serve.cpp", "owm_tune.cpp"]
+HIP_SOURCES = ["sort.hip", "tree.hip", "stage1.hip", "stages.hip", "xyz.hip", "owm_host.cpp",
+               "owm_capi.cpp", "owm_serve.cpp", "owm_tune.cpp", "owm_xyz.cpp"]
 
 
 def _run(cmd):

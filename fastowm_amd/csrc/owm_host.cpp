@@ -1310,6 +1310,7 @@ uint64_t owm_ctx::room_for(const owm::DevBuf &b) const
 uint64_t owm_ctx::held_bytes() const
 {
     uint64_t b = arena.buf.bytes + small.buf.bytes + upload.bytes + s1_list.bytes;
+    b += xyz_text.bytes + xyz_work.bytes + xyz_cloud.bytes;
     b += trees_bytes.load();
     return b;
 }

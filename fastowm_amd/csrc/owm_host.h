@@ -243,6 +243,9 @@ struct owm_ctx {
     owm::Arena small;   // bbox pass
     owm::DevBuf upload; // host inputs staged to the device
     owm::DevBuf s1_list; // stage-1 leaf/block join list (grows on demand)
+    // .xyz reader: the text and index, parse work (and the cloud when no line
+    // was blank), the compacted cloud
+    owm::DevBuf xyz_text, xyz_work, xyz_cloud;
     uint64_t s1_need_max = 0;  // largest join that overflowed its first estimate
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};

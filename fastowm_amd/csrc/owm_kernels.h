@@ -105,6 +105,25 @@ void launch_agg_levels(hipStream_t s, const TreeOut &T, const AggLevels &V);
 // the levels' blocks that tile its leaf range (levels built first).
 void launch_karras(hipStream_t s, const BatchDev &B, const TreeOut &T, const AggLevels &V);
 
+// ---- .xyz reader (xyz.hip) ----
+// The text buffer is zero-padded to a whole number of xyz_chunk_bytes() plus
+// 16 bytes.  index: newline counts per chunk, their exclusive scan (chunk_off
+// has nchunks+1 entries, the last the total) and every newline's position.
+uint64_t xyz_chunk_bytes();
+void launch_xyz_count(hipStream_t s, const void *text, uint64_t padded_bytes, uint32_t *chunk_cnt,
+                      uint64_t *chunk_off);
+void launch_xyz_newlines(hipStream_t s, const void *text, uint64_t padded_bytes,
+                         const uint64_t *chunk_off, uint64_t *nlpos);
+void launch_xyz_parse(hipStream_t s, const void *text, uint64_t bytes, const uint64_t *nlpos,
+                      uint64_t nnl, uint64_t nlines, double *x, double *y, double *z,
+                      uint64_t *blank, uint64_t *host, uint64_t *host_start, uint32_t cap,
+                      uint32_t *counts);
+void launch_xyz_scatter(hipStream_t s, const uint64_t *line, const double *v, uint64_t n, double *x,
+                        double *y, double *z);
+void launch_xyz_compact(hipStream_t s, const double *x, const double *y, const double *z,
+                        uint64_t nlines, const uint64_t *blank, uint64_t nb, double *ox, double *oy,
+                        double *oz);
+
 // ---- stages (stages.hip) ----
 struct Stage1Cfg {
     uint32_t bx, by;  // window block shape

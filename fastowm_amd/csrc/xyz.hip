@@ -1,0 +1,378 @@
+// LiDAR .xyz text -> fp64 SoA cloud on the device.
+//
+// The reference reads its clouds with fscanf("%lf %lf %lf") and skips the
+// rest of each line (readXYZfile, CPP-TBB-CPU/src/optim3_func.cpp:665-722).
+// Here the text of the data section sits in device memory: one pass counts
+// the newlines of every 16 KB chunk, a scan turns the counts into line
+// numbers, a second pass writes every newline's position, and one thread per
+// line parses its three numbers.  A number is converted exactly when its
+// decimal mantissa M (up to 19 digits) and exponent q satisfy M <= 2^53 and
+// |q| <= 22: M and 10^|q| are then exact doubles and one IEEE multiply or
+// divide gives the correctly rounded value (Clinger, "How to read floating
+// point numbers accurately", PLDI 1990), i.e. the double strtod and fscanf
+// return.  Any other line (more digits, large exponents, inf/nan, hex, odd
+// separators) is listed for the host, which parses it with strtod; blank lines
+// are listed too and dropped by a compaction.
+#include "owm_kernels.h"
+
+namespace owm {
+
+constexpr int XY_TB = 256;
+constexpr uint32_t XY_PER_THREAD = 64;                // bytes per thread and chunk
+constexpr uint32_t XY_CHUNK = XY_TB * XY_PER_THREAD;  // 16 KB
+
+// 10^0 .. 10^22, every one an exact double
+__constant__ double XY_P10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                  1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                  1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+// '\n' bytes in a 16-byte word (exact per byte: no carry crosses a byte)
+__device__ __forceinline__ uint32_t nl_count16(uint4 w)
+{
+    uint32_t c = 0;
+    const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = v[k] ^ 0x0A0A0A0Au;  // a newline byte becomes zero
+        const uint32_t nz = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+        c += (uint32_t)__popc(~nz & 0x80808080u);
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *s_w)
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    if (lane == 0)
+        s_w[w] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int k = 0; k < XY_TB / 64; ++k)
+        t += s_w[k];
+    __syncthreads();
+    return t;
+}
+
+// Newlines per 16 KB chunk (the buffer is zero-padded to whole chunks).
+__global__ __launch_bounds__(XY_TB) void k_xyz_count(const uint4 *__restrict__ text,
+                                                     uint64_t nchunks, uint32_t *chunk_cnt)
+{
+    __shared__ uint32_t s_w[XY_TB / 64];
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint4 *p = text + (c * XY_CHUNK + threadIdx.x * XY_PER_THREAD) / 16;
+        uint32_t n = 0;
+#pragma unroll
+        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j)
+            n += nl_count16(p[j]);
+        n = block_sum(n, s_w);
+        if (threadIdx.x == 0)
+            chunk_cnt[c] = n;
+    }
+}
+
+// Exclusive scan of the chunk counts in one workgroup; out[n] = total.
+__global__ __launch_bounds__(1024) void k_xyz_scan(const uint32_t *__restrict__ in, uint64_t n,
+                                                   uint64_t *__restrict__ out)
+{
+    __shared__ uint64_t s_part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (n + 1023) / 1024, a = t * per, b = a + per < n ? a + per : n;
+    uint64_t sum = 0;
+    for (uint64_t i = a; i < b; ++i)
+        sum += in[i];
+    s_part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint64_t v = t >= o ? s_part[t - o] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = s_part[t] - sum;
+    for (uint64_t i = a; i < b; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == 1023)
+        out[n] = s_part[1023];
+}
+
+// Position of every newline, in order: nlpos[k] = offset of the k-th '\n'.
+__global__ __launch_bounds__(XY_TB) void k_xyz_newlines(const uint4 *__restrict__ text,
+                                                        uint64_t nchunks,
+                                                        const uint64_t *__restrict__ chunk_off,
+                                                        uint64_t *__restrict__ nlpos)
+{
+    __shared__ uint32_t s_c[XY_TB];
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t base = c * XY_CHUNK + threadIdx.x * XY_PER_THREAD;
+        const uint4 *p = text + base / 16;
+        uint4 w[XY_PER_THREAD / 16];
+        uint32_t n = 0;
+#pragma unroll
+        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
+            w[j] = p[j];
+            n += nl_count16(w[j]);
+        }
+        s_c[threadIdx.x] = n;
+        __syncthreads();
+        // exclusive prefix over the block's threads (a simple LDS scan)
+        for (uint32_t o = 1; o < XY_TB; o <<= 1) {
+            const uint32_t v = threadIdx.x >= o ? s_c[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s_c[threadIdx.x] += v;
+            __syncthreads();
+        }
+        uint64_t k = chunk_off[c] + s_c[threadIdx.x] - n;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
+            const uint32_t v[4] = {w[j].x, w[j].y, w[j].z, w[j].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (((v[q] >> (8 * b)) & 255u) == 10u)
+                        nlpos[k++] = base + 16 * j + 4 * q + b;
+        }
+    }
+}
+
+// Byte reader over the 8-byte words of the text (the buffer is padded, so
+// the word holding the last byte is always in bounds).
+struct TextCursor {
+    const unsigned long long *w;
+    uint64_t cached = ~0ull;
+    unsigned long long word = 0;
+    __device__ __forceinline__ uint32_t at(uint64_t p)
+    {
+        if ((p >> 3) != cached) {
+            cached = p >> 3;
+            word = w[cached];
+        }
+        return (uint32_t)(word >> (8 * (p & 7))) & 255u;
+    }
+};
+
+__device__ __forceinline__ bool xyz_space(uint32_t c)
+{
+    return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+
+// One number at p (leading blanks skipped); false: not one the device
+// converts exactly (the host then parses the line).
+__device__ __forceinline__ bool xyz_number(TextCursor &T, uint64_t &p, uint64_t e, double &out)
+{
+    while (p < e && xyz_space(T.at(p)))
+        ++p;
+    if (p >= e)
+        return false;
+    uint32_t c = T.at(p);
+    const bool neg = c == '-';
+    if (c == '-' || c == '+') {
+        ++p;
+        c = p < e ? T.at(p) : 0u;
+    }
+    unsigned long long m = 0;
+    int sig = 0, frac = 0, digits = 0;
+    while (p < e && (c = T.at(p)) >= '0' && c <= '9') {
+        if (m != 0 || c != '0') {
+            if (++sig > 19)
+                return false;
+            m = m * 10 + (c - '0');
+        }
+        ++digits;
+        ++p;
+    }
+    if (p < e && T.at(p) == '.') {
+        ++p;
+        while (p < e && (c = T.at(p)) >= '0' && c <= '9') {
+            if (m != 0 || c != '0') {
+                if (++sig > 19)
+                    return false;
+                m = m * 10 + (c - '0');
+            }
+            ++frac;
+            ++digits;
+            ++p;
+        }
+    }
+    if (digits == 0)
+        return false;
+    int ex = 0;
+    if (p < e && ((c = T.at(p)) == 'e' || c == 'E')) {
+        ++p;
+        bool eneg = false;
+        if (p < e && ((c = T.at(p)) == '-' || c == '+')) {
+            eneg = c == '-';
+            ++p;
+        }
+        int ed = 0;
+        while (p < e && (c = T.at(p)) >= '0' && c <= '9') {
+            if (++ed > 4)
+                return false;
+            ex = ex * 10 + (int)(c - '0');
+            ++p;
+        }
+        if (ed == 0)
+            return false;
+        ex = eneg ? -ex : ex;
+    }
+    // a number ends at a blank or at the end of the line
+    if (p < e && !xyz_space(T.at(p)))
+        return false;
+    double v;
+    if (m == 0) {
+        v = 0.0;
+    } else {
+        const int q = ex - frac;
+        if (m > (1ull << 53) || q < -22 || q > 22)
+            return false;
+        v = (double)m;
+        v = q < 0 ? __ddiv_rn(v, XY_P10[-q]) : __dmul_rn(v, XY_P10[q]);
+    }
+    out = neg ? -v : v;
+    return true;
+}
+
+// One thread per line: parsed, blank (listed) or left to the host (listed
+// with its start); lists hold cap entries, counts[0..1] count past it.
+__global__ __launch_bounds__(XY_TB) void k_xyz_parse(const unsigned long long *__restrict__ text,
+                                                     uint64_t bytes,
+                                                     const uint64_t *__restrict__ nlpos,
+                                                     uint64_t nnl, uint64_t nlines, double *x,
+                                                     double *y, double *z, uint64_t *blank,
+                                                     uint64_t *host, uint64_t *host_start,
+                                                     uint32_t cap, uint32_t *counts)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * XY_TB + threadIdx.x; i < nlines;
+         i += (uint64_t)gridDim.x * XY_TB) {
+        const uint64_t s = i ? nlpos[i - 1] + 1 : 0, e = i < nnl ? nlpos[i] : bytes;
+        TextCursor T{text};
+        uint64_t p = s;
+        while (p < e && xyz_space(T.at(p)))
+            ++p;
+        if (p >= e) {
+            const uint32_t slot = atomicAdd(&counts[0], 1u);
+            if (slot < cap)
+                blank[slot] = i;
+            continue;
+        }
+        double v[3];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            ok = ok && xyz_number(T, p, e, v[k]);
+        if (!ok) {
+            const uint32_t slot = atomicAdd(&counts[1], 1u);
+            if (slot < cap) {
+                host[slot] = i;
+                host_start[slot] = s;
+            }
+            continue;
+        }
+        x[i] = v[0];
+        y[i] = v[1];
+        z[i] = v[2];
+    }
+}
+
+// The host's values for the lines it parsed.
+__global__ void k_xyz_scatter(const uint64_t *__restrict__ line, const double *__restrict__ v,
+                              uint64_t n, double *x, double *y, double *z)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = line[k];
+        x[i] = v[3 * k];
+        y[i] = v[3 * k + 1];
+        z[i] = v[3 * k + 2];
+    }
+}
+
+// Drops the blank lines (blank: ascending line numbers): point
+// i - #{blank < i} takes line i.
+__global__ void k_xyz_compact(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, uint64_t nlines,
+                              const uint64_t *__restrict__ blank, uint64_t nb, double *ox,
+                              double *oy, double *oz)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nlines;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = nb;  // #{blank < i}
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (blank[mid] < i)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < nb && blank[lo] == i)
+            continue;
+        ox[i - lo] = x[i];
+        oy[i - lo] = y[i];
+        oz[i - lo] = z[i];
+    }
+}
+
+static uint32_t xy_grid(uint64_t work, uint32_t per, uint32_t cap)
+{
+    const uint64_t g = (work + per - 1) / per;
+    return g < 1 ? 1u : (g > cap ? cap : (uint32_t)g);
+}
+
+uint64_t xyz_chunk_bytes() { return XY_CHUNK; }
+
+void launch_xyz_count(hipStream_t s, const void *text, uint64_t padded_bytes, uint32_t *chunk_cnt,
+                      uint64_t *chunk_off)
+{
+    const uint64_t nch = padded_bytes / XY_CHUNK;
+    hipLaunchKernelGGL(k_xyz_count, dim3(xy_grid(nch, 1, 16384)), dim3(XY_TB), 0, s,
+                       (const uint4 *)text, nch, chunk_cnt);
+    hipLaunchKernelGGL(k_xyz_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)chunk_cnt, nch,
+                       chunk_off);
+}
+
+void launch_xyz_newlines(hipStream_t s, const void *text, uint64_t padded_bytes,
+                         const uint64_t *chunk_off, uint64_t *nlpos)
+{
+    const uint64_t nch = padded_bytes / XY_CHUNK;
+    hipLaunchKernelGGL(k_xyz_newlines, dim3(xy_grid(nch, 1, 16384)), dim3(XY_TB), 0, s,
+                       (const uint4 *)text, nch, chunk_off, nlpos);
+}
+
+void launch_xyz_parse(hipStream_t s, const void *text, uint64_t bytes, const uint64_t *nlpos,
+                      uint64_t nnl, uint64_t nlines, double *x, double *y, double *z,
+                      uint64_t *blank, uint64_t *host, uint64_t *host_start, uint32_t cap,
+                      uint32_t *counts)
+{
+    if (nlines == 0)
+        return;
+    hipLaunchKernelGGL(k_xyz_parse, dim3(xy_grid(nlines, XY_TB, 65536)), dim3(XY_TB), 0, s,
+                       (const unsigned long long *)text, bytes, nlpos, nnl, nlines, x, y, z, blank,
+                       host, host_start, cap, counts);
+}
+
+void launch_xyz_scatter(hipStream_t s, const uint64_t *line, const double *v, uint64_t n, double *x,
+                        double *y, double *z)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_xyz_scatter, dim3(xy_grid(n, 256, 4096)), dim3(256), 0, s, line, v, n, x,
+                       y, z);
+}
+
+void launch_xyz_compact(hipStream_t s, const double *x, const double *y, const double *z,
+                        uint64_t nlines, const uint64_t *blank, uint64_t nb, double *ox, double *oy,
+                        double *oz)
+{
+    if (nlines == 0)
+        return;
+    hipLaunchKernelGGL(k_xyz_compact, dim3(xy_grid(nlines, 256, 16384)), dim3(256), 0, s, x, y, z,
+                       nlines, blank, nb, ox, oy, oz);
+}
+
+}  // namespace owm

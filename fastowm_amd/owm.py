@@ -58,6 +58,12 @@ class TileDesc(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64), ("params", Params)]
 
 
+class CloudDesc(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("n", ctypes.c_uint64), ("bbox", ctypes.c_double * 4), ("flags", ctypes.c_uint32),
+                ("host_lines", ctypes.c_uint64)]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("max_n", ctypes.c_uint64), ("leaf_size", ctypes.c_uint32),
                 ("block_x", ctypes.c_uint32), ("block_y", ctypes.c_uint32),
@@ -103,6 +109,8 @@ def lib():
             "owm_query_seed_count": ([P(Params), u64, dp, P(u64)], ctypes.c_int),
             "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_run_tile_device": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
+            "owm_xyz_parse": ([vp, ctypes.c_char_p, u64, P(CloudDesc)], ctypes.c_int),
+            "owm_cloud_read": ([vp, P(CloudDesc), dp, dp, dp], ctypes.c_int),
             "owm_batch_run": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp], ctypes.c_int),
             "owm_batch_run_device": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp],
                                      ctypes.c_int),
@@ -215,6 +223,22 @@ class Tree:
             pass
 
 
+class Cloud:
+    """A point cloud parsed on the device (owm_xyz_parse)."""
+
+    def __init__(self, ctx: "Context", desc):
+        self.ctx, self.desc = ctx, desc
+        self.n = desc.n
+        self.bbox = tuple(desc.bbox) if desc.flags & F_HAVE_BBOX else None
+        self.host_lines = desc.host_lines
+
+    def to_numpy(self):
+        x, y, z = (np.empty(self.n, np.float64) for _ in range(3))
+        self.ctx._check(lib().owm_cloud_read(self.ctx.h, ctypes.byref(self.desc), x.ctypes.data,
+                                             y.ctypes.data, z.ctypes.data))
+        return x, y, z
+
+
 class Context:
     """One HIP device, one stream, reusable device arenas."""
 
@@ -299,6 +323,32 @@ class Context:
         return {arr[i].name.decode(): {"ms": arr[i].ms, "calls": arr[i].calls,
                                        "bytes": arr[i].bytes, "items": arr[i].items}
                 for i in range(n.value)}
+
+    # -- reader ------------------------------------------------------------
+    def parse_xyz(self, data):
+        """readXYZfile on the device: `data` is the file's bytes (or a path).
+        Returns a Cloud whose arrays stay in device memory until the next
+        parse on this context."""
+        if isinstance(data, str):
+            with open(data, "rb") as f:
+                data = f.read()
+        c = CloudDesc()
+        self._check(lib().owm_xyz_parse(self.h, data, len(data), ctypes.byref(c)))
+        return Cloud(self, c)
+
+    def run_cloud(self, cloud, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None, cap=None):
+        """run_tile on a parsed cloud; the file header's bbox is used when it
+        had one and no bbox is given (as the reference driver does)."""
+        if bbox is None and cloud.bbox is not None:
+            bbox = cloud.bbox
+        p = make_params(wsize, bsize, overlap, leaf_size, bbox)
+        cap = cloud.n + 1 if cap is None else cap
+        out = self._seeds_out(max(cap, 1))
+        ns = ctypes.c_uint64()
+        d = cloud.desc
+        self._check(lib().owm_run_tile_device(self.h, d.x, d.y, d.z, d.n, ctypes.byref(p),
+                                              out.ctypes.data, cap, ctypes.byref(ns)))
+        return out[:ns.value].copy()
 
     # -- whole pipeline ----------------------------------------------------
     def run_tile(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None,

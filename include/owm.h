@@ -127,6 +127,33 @@ owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_
                                const double *d_z, uint64_t n, const owm_params *params,
                                uint32_t *seeds, uint64_t cap, uint64_t *n_seeds);
 
+/* ---- reader: LiDAR .xyz text -> device-resident cloud --------------------
+ * Replaces readXYZfile (CPP-TBB-CPU/src/optim3_func.cpp:665-722): text holds
+ * the whole file (bytes long, host memory).  A first line holding a single
+ * number starts the reference's 5-number header "N minx maxx miny maxy" (one
+ * per line, as in its *H.xyz files); N then bounds the points read and the
+ * box is returned.  Every other non-blank line is one point: three numbers
+ * separated by blanks, the rest of the line ignored (the reference's
+ * fscanf("%lf %lf %lf") + skip to '\n'); the doubles equal strtod's.  Blank
+ * lines are skipped; a line with fewer than three numbers is OWM_E_ARG.  The
+ * text is parsed on the device; lines outside the exact device path (more
+ * than 19 significant digits, |decimal exponent| > 22 with the mantissa,
+ * inf/nan/hex) are parsed by the host (host_lines counts them).  The cloud
+ * lives in device memory owned by the context, valid until the next
+ * owm_xyz_parse or owm_ctx_destroy; pass it to owm_run_tile_device or the
+ * batch calls (bbox with OWM_F_HAVE_BBOX when flags says so). */
+typedef struct owm_cloud {
+    const double *x, *y, *z; /* device pointers */
+    uint64_t n;
+    double bbox[4];          /* header box: minx, maxx, miny, maxy */
+    uint32_t flags;          /* OWM_F_HAVE_BBOX when the file had a header */
+    uint64_t host_lines;     /* lines the host parsed */
+} owm_cloud;
+owm_status owm_xyz_parse(owm_ctx *ctx, const char *text, uint64_t bytes, owm_cloud *out);
+/* Copies a parsed cloud to caller-owned host arrays of cloud->n doubles
+ * (no reference counterpart: readXYZfile fills host memory directly). */
+owm_status owm_cloud_read(owm_ctx *ctx, const owm_cloud *cloud, double *x, double *y, double *z);
+
 /* ---- batch: many independent tiles in one launch chain ------------------
  * The tiles are slices [offset, offset+n) of one concatenated SoA cloud.
  * Seeds of tile t are written to seeds[seed_off[t] .. seed_off[t+1]) with
