@@ -141,6 +141,50 @@ def run_batch_bench(ctx, torch, synth, n_tiles, steps):
                                          "sort_bits")} for e in table if e["ms"] > 0]}
 
 
+def run_reader_bench(ctx, synth, x, y, z, reps=3):
+    """The .xyz reader on the C2 cloud written as LiDAR text (two decimals,
+    no header): wall time per parse from host bytes (PCIe upload of the text
+    included) and the device time of its kernels (HIP events)."""
+    text = synth.format_xyz(x, y, z, 2)
+    c = ctx.parse_xyz(text)  # allocations
+    ctx.profile(True, phases=["xyz_parse"])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = ctx.parse_xyz(text)
+    wall = (time.perf_counter() - t0) / reps
+    p = ctx.profile_read()["xyz_parse"]
+    ctx.profile(False)
+    dev_ms = p["ms"] / reps
+    return {"bytes": len(text), "points": int(c.n), "host_lines": int(c.host_lines),
+            "wall_ms": round(wall * 1e3, 2), "mpts_wall": round(c.n / wall / 1e6, 1),
+            "device_ms": round(dev_ms, 3), "launches": p["calls"] // reps,
+            "device_gbs_text": round(len(text) / (dev_ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes": int(p["bytes"] / reps),
+            "roofline_frac": round(p["bytes"] / reps / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_unit": "text read once + 24 B per point written"}, text, c
+
+
+def reader_cpu(text, cloud, sample_bytes=100_000_000):
+    """The reference reader's loop (the oracle's strtod restatement, one
+    thread) on the first ~100 MB of the same text, and the device's values
+    against it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    s = text[:sample_bytes]
+    s = s[:s.rindex(b"\n") + 1]
+    t0 = time.perf_counter()
+    ox, oy, oz = pyoracle.parse_xyz(s, s.count(b"\n") + 1)
+    dt = time.perf_counter() - t0
+    gx, gy, gz = cloud.to_numpy()
+    k = len(ox)
+    same = all(np.array_equal(a[:k].view(np.uint64), b.view(np.uint64))
+               for a, b in ((gx, ox), (gy, oy), (gz, oz)))
+    return {"mpts": round(k / dt / 1e6, 2), "cores": 1, "kind": "port",
+            "sample": f"first {len(s)} bytes ({k} points) of the same text, strtod per number",
+            "bit_equal_to_device": bool(same)}
+
+
 def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
     """Poisson request streams against the serving loop: N log-uniform
     10k-8M points, Wsize 5-20, Bsize 10-30, Overlap 0.5-0.9; one request in
@@ -204,6 +248,7 @@ def main():
     ap.add_argument("--batch-tiles", type=int, default=512)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-reader", action="store_true", help="skip the .xyz reader leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--batch-only", action="store_true",
@@ -315,6 +360,9 @@ def main():
     if not args.no_batch:
         log("batch leg")
         line["batch"] = run_batch_bench(ctx, torch, synth, args.batch_tiles, max(1, args.steps // 4))
+    if not args.no_reader:
+        log("reader leg")
+        line["reader"], reader_text, reader_cloud = run_reader_bench(ctx, synth, x, y, z)
     if not args.no_serve:
         log("serve leg")
         rates = [float(r) for r in args.serve_rates.split(",") if r]
@@ -324,6 +372,8 @@ def main():
         cb, parity = cpu_baseline(x, y, z, seeds, args.cpu_reps)
         line["cpu_baseline"] = cb
         line["parity_vs_oracle"] = parity
+        if not args.no_reader:
+            line["reader"]["cpu_baseline"] = reader_cpu(reader_text, reader_cloud)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -81,7 +81,7 @@ struct IoLayout {
 static const char *PHASE_NAMES[PH_COUNT] = {
     "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "agg_levels", "stage1",
     "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
-    "stage1_summaries"};
+    "stage1_summaries", "xyz_parse"};
 
 const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
 

@@ -191,7 +191,7 @@ struct ServeState;
 
 enum Phase {
     PH_BBOX, PH_MORTON, PH_SORT_TREE, PH_LEAVES, PH_KARRAS, PH_LEVELS, PH_STAGE1,
-    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_COUNT
+    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_XYZ, PH_COUNT
 };
 
 const char *phase_name(int ph);

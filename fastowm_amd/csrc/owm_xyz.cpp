@@ -100,7 +100,10 @@ owm_status owm_xyz_parse(owm_ctx *ctx, const char *text, uint64_t bytes, owm_clo
         if (len)
             HIP_TRY(hipMemcpyAsync(dt, data, len, hipMemcpyHostToDevice, s), "text H2D");
         HIP_TRY(hipMemsetAsync(dt + len, 0, padded - len, s), "text padding");
+        // the device part is timed as one phase: index passes and the parse
+        int pi = ctx->prof.begin(s, PH_XYZ);
         launch_xyz_count(s, dt, body, cnt, coff);
+        ctx->prof.end(s, pi, len, len, 2);
         uint64_t nnl = 0;
         HIP_TRY(hipMemcpyAsync(&nnl, coff + nch, 8, hipMemcpyDeviceToHost, s), "newline count");
         HIP_TRY(hipStreamSynchronize(s), "newline count");
@@ -128,9 +131,12 @@ owm_status owm_xyz_parse(owm_ctx *ctx, const char *text, uint64_t bytes, owm_clo
         uint64_t *lb = (uint64_t *)(dw + w_nl + 3 * w_xyz);
         uint64_t *lh = (uint64_t *)(dw + w_nl + 3 * w_xyz + w_list);
         uint64_t *ls = (uint64_t *)(dw + w_nl + 3 * w_xyz + 2 * w_list);
-        launch_xyz_newlines(s, dt, body, coff, nlpos);
         launch_fill_u32(s, counts, 2, 0u);
+        pi = ctx->prof.begin(s, PH_XYZ);
+        launch_xyz_newlines(s, dt, body, coff, nlpos);
         launch_xyz_parse(s, dt, len, nlpos, nnl, nlines, px, py, pz, lb, lh, ls, cap, counts);
+        // algorithmic bytes: the text read once and 24 bytes per point written
+        ctx->prof.end(s, pi, 24ull * nlines, nlines, 2);
         uint32_t hc[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(hc, counts, 8, hipMemcpyDeviceToHost, s), "reader counts");
         HIP_TRY(hipStreamSynchronize(s), "reader parse");

@@ -22,6 +22,9 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
+
+#include <omp.h>
 
 typedef struct {
     uint64_t seed;
@@ -198,4 +201,51 @@ int owm_synth_write_xyz(const char *path, const double *x, const double *y, cons
 uint64_t owm_synth_ctr(uint64_t seed, uint64_t stream, uint64_t tag, uint64_t i, uint32_t k)
 {
     return ctr(seed, stream, tag, i, k);
+}
+
+/* Format points as .xyz text lines "x y z\n" with `decimals` fixed decimals
+ * (the layout of LiDAR exports); parallel over chunks.  Returns the bytes
+ * written, 0 if cap is too small (80 bytes per point always suffice). */
+uint64_t owm_synth_format_xyz(const double *x, const double *y, const double *z, uint64_t n,
+                              int decimals, char *out, uint64_t cap)
+{
+    const int nt = omp_get_max_threads();
+    const uint64_t per = (n + (uint64_t)nt - 1) / (uint64_t)nt;
+    uint64_t *len = (uint64_t *)calloc((size_t)nt + 1, sizeof(uint64_t));
+    char **buf = (char **)calloc((size_t)nt, sizeof(char *));
+    if (!len || !buf)
+        return 0;
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = (uint64_t)t * per, b = a + per < n ? a + per : n;
+        if (a < b) {
+            char *w = (char *)malloc((size_t)((b - a) * 80 + 1));
+            uint64_t o = 0;
+            if (w)
+                for (uint64_t i = a; i < b; ++i)
+                    o += (uint64_t)snprintf(w + o, 80, "%.*f %.*f %.*f\n", decimals, x[i], decimals,
+                                            y[i], decimals, z[i]);
+            buf[t] = w;
+            len[t + 1] = w ? o : 0;
+        }
+    }
+    uint64_t total = 0;
+    for (int t = 0; t < nt; ++t)
+        total += len[t + 1];
+    if (total <= cap) {
+        uint64_t o = 0;
+        for (int t = 0; t < nt; ++t) {
+            if (buf[t])
+                memcpy(out + o, buf[t], (size_t)len[t + 1]);
+            o += len[t + 1];
+        }
+    } else {
+        total = 0;
+    }
+    for (int t = 0; t < nt; ++t)
+        free(buf[t]);
+    free(buf);
+    free(len);
+    return total;
 }

@@ -123,6 +123,22 @@ def bbox(x: np.ndarray, y: np.ndarray):
     return tuple(float(v) for v in out)
 
 
+def format_xyz(x, y, z, decimals=2) -> bytes:
+    """The points as .xyz text lines with fixed decimals (no header)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    cap = len(x) * 80 + 1
+    buf = ctypes.create_string_buffer(cap)
+    lib = _lib()
+    lib.owm_synth_format_xyz.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint64, ctypes.c_int,
+                                                                 ctypes.c_void_p, ctypes.c_uint64]
+    lib.owm_synth_format_xyz.restype = ctypes.c_uint64
+    n = lib.owm_synth_format_xyz(x.ctypes.data, y.ctypes.data, z.ctypes.data, len(x), decimals,
+                                 ctypes.addressof(buf), cap)
+    return buf.raw[:n]
+
+
 def write_xyz(path: str, x, y, z, box=None):
     """Write a headed .xyz file ("N minx maxx miny maxy" then "x y z" lines, %.17g)."""
     box = bbox(x, y) if box is None else box

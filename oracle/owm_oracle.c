@@ -715,3 +715,42 @@ void oracle_set_threads(int n)
 }
 
 int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+/* readXYZfile's point loop (src/optim3_func.cpp:709-716): three numbers per
+ * line read as fscanf("%lf") reads them (strtod), the rest of the line
+ * skipped; blank lines skipped.  Returns the points read (at most cap), or
+ * -1 for a line with fewer than three numbers.  Single-threaded, like the
+ * reference. */
+int64_t oracle_parse_xyz(const char *text, uint64_t bytes, double *x, double *y, double *z,
+                         uint64_t cap)
+{
+    const char *p = text, *end = text + bytes;
+    uint64_t n = 0;
+    while (p < end && n < cap) {
+        const char *eol = memchr(p, '\n', (size_t)(end - p));
+        const char *e = eol ? eol : end;
+        const char *q = p;
+        while (q < e && (*q == ' ' || *q == '\t' || *q == '\r' || *q == '\v' || *q == '\f'))
+            ++q;
+        if (q < e) {
+            char buf[256];
+            size_t len = (size_t)(e - p) < sizeof buf - 1 ? (size_t)(e - p) : sizeof buf - 1;
+            memcpy(buf, p, len);
+            buf[len] = 0;
+            char *r = buf, *s2;
+            double v[3];
+            for (int k = 0; k < 3; ++k) {
+                v[k] = strtod(r, &s2);
+                if (s2 == r)
+                    return -1;
+                r = s2;
+            }
+            x[n] = v[0];
+            y[n] = v[1];
+            z[n] = v[2];
+            ++n;
+        }
+        p = eol ? eol + 1 : end;
+    }
+    return (int64_t)n;
+}

@@ -75,4 +75,9 @@ int oracle_get_threads(void);
 #ifdef __cplusplus
 }
 #endif
+/* The reference reader's point loop on a text held in memory (strtod per
+ * number, rest of line skipped, blank lines skipped); -1 on a short line. */
+int64_t oracle_parse_xyz(const char *text, uint64_t bytes, double *x, double *y, double *z,
+                         uint64_t cap);
+
 #endif

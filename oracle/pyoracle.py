@@ -68,6 +68,8 @@ def lib():
         L.oracle_stage1_brute.argtypes = [dp, dp, dp, ctypes.c_uint64, pp, up]
         L.oracle_stage3_brute.argtypes = [dp, dp, dp, ctypes.c_uint64, pp, up, ctypes.c_uint32, up]
         L.oracle_stage3_brute.restype = ctypes.c_uint32
+        L.oracle_parse_xyz.argtypes = [ctypes.c_char_p, ctypes.c_uint64, dp, dp, dp, ctypes.c_uint64]
+        L.oracle_parse_xyz.restype = ctypes.c_int64
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _LIB = L
@@ -187,3 +189,12 @@ def seeds_from_stages(x, y, z, wsize, bsize, overlap, leaf_size=32, box=None, br
     llp = stage2(p, mins)
     adds = stage3_brute(x, y, z, p, llp) if brute else tree.stage3(llp)
     return np.sort(np.concatenate([llp, adds]).astype(np.uint32)), mins, llp, adds, p
+
+
+def parse_xyz(text: bytes, cap: int):
+    """The reference reader's point loop over a text (strtod per number)."""
+    x, y, z = (np.empty(cap, np.float64) for _ in range(3))
+    n = lib().oracle_parse_xyz(text, len(text), _dp(x), _dp(y), _dp(z), cap)
+    if n < 0:
+        raise ValueError("a line holds fewer than three numbers")
+    return x[:n], y[:n], z[:n]
