@@ -9,8 +9,9 @@
 // Input: .xyz text, "x y z" per line; an optional 5-number header
 // "N minx maxx miny maxy" (the reference's *H.xyz convention,
 // src/optim3_func.cpp:681-700) gives the bbox used for the OWM parameters.
-// The timed region is the pipeline only (tree + stages 1-3), like the
-// reference's (owm_optim3.cpp:77-89,145-206); file reading is not timed.
+// The file is parsed on the device (owm_xyz_parse, its time printed apart)
+// and stays in HBM; the timed region is the pipeline only (tree + stages 1-3),
+// like the reference's (owm_optim3.cpp:77-89,145-206).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -22,63 +23,18 @@
 
 namespace {
 
-bool read_xyz(const std::string &path, std::vector<double> &x, std::vector<double> &y,
-              std::vector<double> &z, bool &have_box, double box[4])
+bool read_file(const std::string &path, std::vector<char> &buf)
 {
     FILE *f = fopen(path.c_str(), "rb");
     if (!f)
         return false;
-    std::vector<char> buf;
     fseek(f, 0, SEEK_END);
     long sz = ftell(f);
     fseek(f, 0, SEEK_SET);
-    buf.resize((size_t)sz + 1);
-    size_t got = fread(buf.data(), 1, (size_t)sz, f);
+    buf.resize(sz > 0 ? (size_t)sz : 0);
+    size_t got = sz > 0 ? fread(buf.data(), 1, (size_t)sz, f) : 0;
     fclose(f);
-    buf[got] = 0;
-    char *p = buf.data();
-    char *end = p + got;
-    // header: first line holds a single number
-    have_box = false;
-    {
-        char *eol = (char *)memchr(p, '\n', (size_t)(end - p));
-        if (eol) {
-            char *q;
-            strtod(p, &q);
-            while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r'))
-                ++q;
-            if (q == eol) {
-                unsigned long long n = strtoull(p, &q, 10);
-                p = eol + 1;
-                double h[4];
-                for (int k = 0; k < 4; ++k)
-                    h[k] = strtod(p, &p);
-                box[0] = h[0];  // minx
-                box[1] = h[1];  // maxx
-                box[2] = h[2];  // miny
-                box[3] = h[3];  // maxy
-                have_box = true;
-                x.reserve(n);
-                y.reserve(n);
-                z.reserve(n);
-            }
-        }
-    }
-    while (p < end) {
-        char *q;
-        double a = strtod(p, &q);
-        if (q == p) {
-            ++p;
-            continue;
-        }
-        double b = strtod(q, &q);
-        double c = strtod(q, &q);
-        x.push_back(a);
-        y.push_back(b);
-        z.push_back(c);
-        char *eol = (char *)memchr(q, '\n', (size_t)(end - q));
-        p = eol ? eol + 1 : end;
-    }
+    buf.resize(got);
     return true;
 }
 
@@ -128,35 +84,45 @@ int main(int argc, char **argv)
             return 2;
         }
     }
-    std::vector<double> x, y, z;
-    double box[4];
-    bool have_box = false;
     std::string path = input + ".xyz";
-    if (!read_xyz(path, x, y, z, have_box, box)) {
-        fprintf(stderr, "Unable to read %s\n", path.c_str());
+    std::vector<char> text;
+    if (!read_file(path, text)) {
+        fprintf(stderr, "Unable to open %s\n", path.c_str());
         return 1;
     }
-    if (have_box) {
-        P.flags |= OWM_F_HAVE_BBOX;
-        memcpy(P.bbox, box, sizeof box);
-    }
-    printf("Input: %s (%zu points) on device %d\n", path.c_str(), x.size(), device);
     owm_ctx *ctx = nullptr;
     owm_status st = owm_ctx_create(device, 0, &ctx);
     if (st != OWM_OK) {
         fprintf(stderr, "owm_ctx_create: %s\n", owm_status_str(st));
         return 1;
     }
-    std::vector<uint32_t> seeds(x.size() + 1);
+    // readXYZfile on the device: the cloud stays in HBM for every loop
+    owm_cloud cloud;
+    auto r0 = std::chrono::steady_clock::now();
+    st = owm_xyz_parse(ctx, text.data(), text.size(), &cloud);
+    auto r1 = std::chrono::steady_clock::now();
+    if (st != OWM_OK) {
+        fprintf(stderr, "Unable to read %s: %s\n", path.c_str(), owm_last_error(ctx));
+        owm_ctx_destroy(ctx);
+        return 1;
+    }
+    if (cloud.flags & OWM_F_HAVE_BBOX) {
+        P.flags |= OWM_F_HAVE_BBOX;
+        memcpy(P.bbox, cloud.bbox, sizeof P.bbox);
+    }
+    const uint64_t n = cloud.n;
+    printf("Input: %s (%llu points) on device %d, read in %.6f s\n", path.c_str(),
+           (unsigned long long)n, device, std::chrono::duration<double>(r1 - r0).count());
+    std::vector<uint32_t> seeds(n + 1);
     uint64_t ns = 0;
     std::vector<double> times;
     for (int r = 0; r < loops; ++r) {
         auto t0 = std::chrono::steady_clock::now();
-        st = owm_run_tile(ctx, x.data(), y.data(), z.data(), x.size(), &P, seeds.data(),
-                          seeds.size(), &ns);
+        st = owm_run_tile_device(ctx, cloud.x, cloud.y, cloud.z, n, &P, seeds.data(),
+                                 seeds.size(), &ns);
         auto t1 = std::chrono::steady_clock::now();
         if (st != OWM_OK) {
-            fprintf(stderr, "owm_run_tile: %s: %s\n", owm_status_str(st), owm_last_error(ctx));
+            fprintf(stderr, "owm_run_tile_device: %s: %s\n", owm_status_str(st), owm_last_error(ctx));
             owm_ctx_destroy(ctx);
             return 1;
         }
@@ -164,14 +130,14 @@ int main(int argc, char **argv)
         times.push_back(s);
         printf("TOTAL time elapsed:     %.6f s\n", s);
         printf("Output ground seed-point cloud with %llu points, %llu fewer points than input cloud\n",
-               (unsigned long long)ns, (unsigned long long)(x.size() - ns));
+               (unsigned long long)ns, (unsigned long long)(n - ns));
     }
     double avg = 0;
     for (double t : times)
         avg += t;
     avg /= times.size();
     printf("Average OWM: %.6lf\n", avg);
-    printf("Mpoints/s (including host transfers): %.1f\n", x.size() / avg / 1e6);
+    printf("Mpoints/s (cloud resident in HBM): %.1f\n", n / avg / 1e6);
     if (!results.empty()) {
         FILE *rf = fopen(results.c_str(), "a+");
         if (rf) {
@@ -183,7 +149,10 @@ int main(int argc, char **argv)
         }
     }
     if (!out.empty()) {
-        FILE *of = fopen(out.c_str(), "w");
+        std::vector<double> x(n), y(n), z(n);
+        FILE *of = owm_cloud_read(ctx, &cloud, x.data(), y.data(), z.data()) == OWM_OK
+                       ? fopen(out.c_str(), "w")
+                       : nullptr;
         if (of) {
             for (uint64_t i = 0; i < ns; ++i)
                 fprintf(of, "%.2f %.2f %.2f\n", x[seeds[i]], y[seeds[i]], z[seeds[i]]);
