@@ -3,23 +3,25 @@
 // The reference reads its clouds with fscanf("%lf %lf %lf") and skips the
 // rest of each line (readXYZfile, CPP-TBB-CPU/src/optim3_func.cpp:665-722).
 // Here the text of the data section sits in device memory: one pass counts
-// the newlines of every 16 KB chunk, a scan turns the counts into line
-// numbers, a second pass writes every newline's position, and one thread per
-// line parses its three numbers.  A number is converted exactly when its
-// decimal mantissa M (up to 19 digits) and exponent q satisfy M <= 2^53 and
-// |q| <= 22: M and 10^|q| are then exact doubles and one IEEE multiply or
-// divide gives the correctly rounded value (Clinger, "How to read floating
-// point numbers accurately", PLDI 1990), i.e. the double strtod and fscanf
-// return.  Any other line (more digits, large exponents, inf/nan, hex, odd
-// separators) is listed for the host, which parses it with strtod; blank lines
-// are listed too and dropped by a compaction.
+// the newlines of every 64 KB chunk, a scan turns the counts into line
+// numbers, a second pass writes every newline's position, and a wave takes 64
+// lines at a time, copies their text to LDS and parses one line per lane.  A
+// number is converted exactly when its decimal mantissa M (up to 19 digits)
+// and exponent q satisfy M <= 2^53 and |q| <= 22: M and 10^|q| are then exact
+// doubles and one IEEE multiply or divide gives the correctly rounded value
+// (Clinger, "How to read floating point numbers accurately", PLDI 1990),
+// i.e. the double strtod and fscanf return.  Any other line (more digits,
+// large exponents, inf/nan, hex, odd separators) is listed for the host,
+// which parses it with strtod; blank lines are listed too and dropped by a
+// compaction.
 #include "owm_kernels.h"
 
 namespace owm {
 
 constexpr int XY_TB = 256;
-constexpr uint32_t XY_PER_THREAD = 64;                // bytes per thread and chunk
-constexpr uint32_t XY_CHUNK = XY_TB * XY_PER_THREAD;  // 16 KB
+constexpr uint32_t XY_PER_THREAD = 64;                    // bytes per thread and round
+constexpr uint32_t XY_ROUNDS = 4;                         // rounds per chunk
+constexpr uint32_t XY_CHUNK = XY_TB * XY_PER_THREAD * XY_ROUNDS;  // 64 KB
 
 // 10^0 .. 10^22, every one an exact double
 __constant__ double XY_P10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
@@ -56,17 +58,21 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *s_w)
     return t;
 }
 
-// Newlines per 16 KB chunk (the buffer is zero-padded to whole chunks).
+// Newlines per 64 KB chunk (the buffer is zero-padded to whole chunks).
 __global__ __launch_bounds__(XY_TB) void k_xyz_count(const uint4 *__restrict__ text,
                                                      uint64_t nchunks, uint32_t *chunk_cnt)
 {
     __shared__ uint32_t s_w[XY_TB / 64];
     for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const uint4 *p = text + (c * XY_CHUNK + threadIdx.x * XY_PER_THREAD) / 16;
         uint32_t n = 0;
 #pragma unroll
-        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j)
-            n += nl_count16(p[j]);
+        for (uint32_t r = 0; r < XY_ROUNDS; ++r) {
+            const uint4 *p =
+                text + (c * XY_CHUNK + (r * XY_TB + threadIdx.x) * XY_PER_THREAD) / 16;
+#pragma unroll
+            for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j)
+                n += nl_count16(p[j]);
+        }
         n = block_sum(n, s_w);
         if (threadIdx.x == 0)
             chunk_cnt[c] = n;
@@ -81,6 +87,7 @@ __global__ __launch_bounds__(1024) void k_xyz_scan(const uint32_t *__restrict__ 
     const uint32_t t = threadIdx.x;
     const uint64_t per = (n + 1023) / 1024, a = t * per, b = a + per < n ? a + per : n;
     uint64_t sum = 0;
+#pragma unroll 8
     for (uint64_t i = a; i < b; ++i)
         sum += in[i];
     s_part[t] = sum;
@@ -108,52 +115,68 @@ __global__ __launch_bounds__(XY_TB) void k_xyz_newlines(const uint4 *__restrict_
 {
     __shared__ uint32_t s_c[XY_TB];
     for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const uint64_t base = c * XY_CHUNK + threadIdx.x * XY_PER_THREAD;
-        const uint4 *p = text + base / 16;
-        uint4 w[XY_PER_THREAD / 16];
-        uint32_t n = 0;
+        uint64_t run = chunk_off[c];  // newlines before this round
+        for (uint32_t r = 0; r < XY_ROUNDS; ++r) {
+            const uint64_t base = c * XY_CHUNK + (r * XY_TB + threadIdx.x) * XY_PER_THREAD;
+            const uint4 *p = text + base / 16;
+            uint4 w[XY_PER_THREAD / 16];
+            uint32_t n = 0;
 #pragma unroll
-        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
-            w[j] = p[j];
-            n += nl_count16(w[j]);
-        }
-        s_c[threadIdx.x] = n;
-        __syncthreads();
-        // exclusive prefix over the block's threads (a simple LDS scan)
-        for (uint32_t o = 1; o < XY_TB; o <<= 1) {
-            const uint32_t v = threadIdx.x >= o ? s_c[threadIdx.x - o] : 0u;
+            for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
+                w[j] = p[j];
+                n += nl_count16(w[j]);
+            }
+            s_c[threadIdx.x] = n;
             __syncthreads();
-            s_c[threadIdx.x] += v;
+            // inclusive prefix over the block's threads (a simple LDS scan)
+            for (uint32_t o = 1; o < XY_TB; o <<= 1) {
+                const uint32_t v = threadIdx.x >= o ? s_c[threadIdx.x - o] : 0u;
+                __syncthreads();
+                s_c[threadIdx.x] += v;
+                __syncthreads();
+            }
+            uint64_t k = run + s_c[threadIdx.x] - n;
+            run += s_c[XY_TB - 1];
             __syncthreads();
-        }
-        uint64_t k = chunk_off[c] + s_c[threadIdx.x] - n;
-        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
-            const uint32_t v[4] = {w[j].x, w[j].y, w[j].z, w[j].w};
+            for (int j = 0; j < (int)(XY_PER_THREAD / 16); ++j) {
+                const uint32_t v[4] = {w[j].x, w[j].y, w[j].z, w[j].w};
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    if (((v[q] >> (8 * b)) & 255u) == 10u)
-                        nlpos[k++] = base + 16 * j + 4 * q + b;
+                    for (int b = 0; b < 4; ++b)
+                        if (((v[q] >> (8 * b)) & 255u) == 10u)
+                            nlpos[k++] = base + 16 * j + 4 * q + b;
+            }
         }
     }
 }
 
-// Byte reader over the 8-byte words of the text (the buffer is padded, so
-// the word holding the last byte is always in bounds).
-struct TextCursor {
+// Byte readers over 8-byte words of the text: from global memory (the buffer
+// is padded, so the word holding the last byte is always in bounds) or from
+// a wave's LDS copy of its lines (words from base_w on).
+struct GlobalWords {
     const unsigned long long *w;
-    uint64_t cached = ~0ull;
+    __device__ __forceinline__ unsigned long long load(uint64_t wi) const { return w[wi]; }
+};
+struct LdsWords {  // positions relative to the wave's copy
+    const unsigned long long *w;
+    __device__ __forceinline__ unsigned long long load(uint32_t wi) const { return w[wi]; }
+};
+
+// Pos: the position type (64-bit in the whole text, 32-bit in an LDS copy)
+template <typename Src, typename Pos>
+struct TextCursor {
+    Src src;
+    Pos cached = ~(Pos)0;
     unsigned long long word = 0;
-    __device__ __forceinline__ uint32_t at(uint64_t p)
+    __device__ __forceinline__ uint32_t at(Pos p)
     {
         if ((p >> 3) != cached) {
             cached = p >> 3;
-            word = w[cached];
+            word = src.load(cached);
         }
-        return (uint32_t)(word >> (8 * (p & 7))) & 255u;
+        return (uint32_t)(word >> (8 * (uint32_t)(p & 7))) & 255u;
     }
 };
 
@@ -164,7 +187,8 @@ __device__ __forceinline__ bool xyz_space(uint32_t c)
 
 // One number at p (leading blanks skipped); false: not one the device
 // converts exactly (the host then parses the line).
-__device__ __forceinline__ bool xyz_number(TextCursor &T, uint64_t &p, uint64_t e, double &out)
+template <typename Cur, typename Pos>
+__device__ __forceinline__ bool xyz_number(Cur &T, Pos &p, Pos e, double &out)
 {
     while (p < e && xyz_space(T.at(p)))
         ++p;
@@ -238,45 +262,92 @@ __device__ __forceinline__ bool xyz_number(TextCursor &T, uint64_t &p, uint64_t 
     return true;
 }
 
-// One thread per line: parsed, blank (listed) or left to the host (listed
-// with its start); lists hold cap entries, counts[0..1] count past it.
-__global__ __launch_bounds__(XY_TB) void k_xyz_parse(const unsigned long long *__restrict__ text,
-                                                     uint64_t bytes,
-                                                     const uint64_t *__restrict__ nlpos,
-                                                     uint64_t nnl, uint64_t nlines, double *x,
-                                                     double *y, double *z, uint64_t *blank,
-                                                     uint64_t *host, uint64_t *host_start,
-                                                     uint32_t cap, uint32_t *counts)
+struct ParseOut {
+    double *x, *y, *z;
+    uint64_t *blank, *host, *host_start;
+    uint32_t cap;
+    uint32_t *counts;
+};
+
+// Line i = [s, e): a point, a blank line (listed) or a line left to the host
+// (listed with its start); lists hold cap entries, counts[0..1] count past it.
+template <typename Src, typename Pos>
+__device__ __forceinline__ void xyz_line(Src src, uint64_t i, uint64_t s, Pos ps, Pos e,
+                                         const ParseOut &o)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * XY_TB + threadIdx.x; i < nlines;
-         i += (uint64_t)gridDim.x * XY_TB) {
-        const uint64_t s = i ? nlpos[i - 1] + 1 : 0, e = i < nnl ? nlpos[i] : bytes;
-        TextCursor T{text};
-        uint64_t p = s;
-        while (p < e && xyz_space(T.at(p)))
-            ++p;
-        if (p >= e) {
-            const uint32_t slot = atomicAdd(&counts[0], 1u);
-            if (slot < cap)
-                blank[slot] = i;
-            continue;
-        }
-        double v[3];
-        bool ok = true;
+    TextCursor<Src, Pos> T{src};
+    Pos p = ps;
+    while (p < e && xyz_space(T.at(p)))
+        ++p;
+    if (p >= e) {
+        const uint32_t slot = atomicAdd(&o.counts[0], 1u);
+        if (slot < o.cap)
+            o.blank[slot] = i;
+        return;
+    }
+    double v[3];
+    bool ok = true;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            ok = ok && xyz_number(T, p, e, v[k]);
-        if (!ok) {
-            const uint32_t slot = atomicAdd(&counts[1], 1u);
-            if (slot < cap) {
-                host[slot] = i;
-                host_start[slot] = s;
-            }
-            continue;
+    for (int k = 0; k < 3; ++k)
+        ok = ok && xyz_number(T, p, e, v[k]);
+    if (!ok) {
+        const uint32_t slot = atomicAdd(&o.counts[1], 1u);
+        if (slot < o.cap) {
+            o.host[slot] = i;
+            o.host_start[slot] = s;
         }
-        x[i] = v[0];
-        y[i] = v[1];
-        z[i] = v[2];
+        return;
+    }
+    o.x[i] = v[0];
+    o.y[i] = v[1];
+    o.z[i] = v[2];
+}
+
+constexpr uint32_t XY_WBUF = 4096;  // LDS bytes of text per wave
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// A wave takes 64 consecutive lines: when their text fits XY_WBUF bytes it is
+// copied to LDS with 16-byte loads (the whole wave, coalesced) and every lane
+// parses its line from there; longer lines are read from global memory.
+__global__ __launch_bounds__(XY_TB) void k_xyz_parse(const uint4 *__restrict__ text, uint64_t bytes,
+                                                     const uint64_t *__restrict__ nlpos,
+                                                     uint64_t nnl, uint64_t nlines, ParseOut o)
+{
+    __shared__ uint4 s_txt[XY_TB / 64][XY_WBUF / 16];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * XY_TB;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * (XY_TB / 64) + w) * 64; i0 < nlines; i0 += stride) {
+        const uint64_t i = i0 + lane;
+        const bool valid = i < nlines;
+        uint64_t s = 0, e = 0;
+        if (valid) {
+            s = i ? nlpos[i - 1] + 1 : 0;
+            e = i < nnl ? nlpos[i] : bytes;
+        }
+        const uint64_t last = nlines - 1 - i0 < 63 ? nlines - 1 - i0 : 63;
+        const uint64_t base = shfl_u64(s, 0) & ~15ull, span = shfl_u64(e, (int)last) - base;
+        if (span <= XY_WBUF) {
+            for (uint64_t off = lane * 16; off < span; off += 64 * 16)
+                s_txt[w][off >> 4] = text[(base + off) >> 4];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (valid)
+                xyz_line(LdsWords{(const unsigned long long *)s_txt[w]}, i, s,
+                         (uint32_t)(s - base), (uint32_t)(e - base), o);
+            // the wave's next lines overwrite the buffer
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else if (valid) {
+            xyz_line(GlobalWords{(const unsigned long long *)text}, i, s, s, e, o);
+        }
     }
 }
 
@@ -351,9 +422,9 @@ void launch_xyz_parse(hipStream_t s, const void *text, uint64_t bytes, const uin
 {
     if (nlines == 0)
         return;
-    hipLaunchKernelGGL(k_xyz_parse, dim3(xy_grid(nlines, XY_TB, 65536)), dim3(XY_TB), 0, s,
-                       (const unsigned long long *)text, bytes, nlpos, nnl, nlines, x, y, z, blank,
-                       host, host_start, cap, counts);
+    const ParseOut o{x, y, z, blank, host, host_start, cap, counts};
+    hipLaunchKernelGGL(k_xyz_parse, dim3(xy_grid(nlines, XY_TB, 16384)), dim3(XY_TB), 0, s,
+                       (const uint4 *)text, bytes, nlpos, nnl, nlines, o);
 }
 
 void launch_xyz_scatter(hipStream_t s, const uint64_t *line, const double *v, uint64_t n, double *x,
