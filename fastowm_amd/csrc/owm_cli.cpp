@@ -4,7 +4,8 @@
 //   -i <input without .xyz>  -W <Wsize>  -B <Bsize>  -O <Overlap>  -s <maxNumber>
 //   -l <loops>  --results <csv>  (-n, -r and -L are accepted and ignored: the
 //   GPU path has no thread count, minRadius policy or parallel tree level)
-// plus --device <id> and -o <output .xyz of the seeds>.
+// plus --device <id>, -o <output .xyz of the seeds> and --gold <file> (default
+// <input>_salidaGOLD.xyz, as the reference: the results CSV's Correctness).
 //
 // Input: .xyz text, "x y z" per line; an optional 5-number header
 // "N minx maxx miny maxy" (the reference's *H.xyz convention,
@@ -49,7 +50,7 @@ void usage()
 
 int main(int argc, char **argv)
 {
-    std::string input = "data/INAER_2011_Alcoy", results = "o3_memoization.csv", out;
+    std::string input = "data/INAER_2011_Alcoy", results = "o3_memoization.csv", out, gold;
     owm_params P{};
     P.wsize = 10;
     P.bsize = 20;
@@ -73,6 +74,7 @@ int main(int argc, char **argv)
         else if (a == "-l" || a == "--loop") loops = atoi(val());
         else if (a == "--results") results = val();
         else if (a == "-o") out = val();
+        else if (a == "--gold") gold = val();
         else if (a == "--device") device = atoi(val());
         else if (a == "-n" || a == "-r" || a == "-L" || a == "--num_threads" || a == "--radius" ||
                  a == "--level") (void)val();
@@ -138,21 +140,76 @@ int main(int argc, char **argv)
     avg /= times.size();
     printf("Average OWM: %.6lf\n", avg);
     printf("Mpoints/s (cloud resident in HBM): %.1f\n", n / avg / 1e6);
+    // host copies of the cloud, for the seed file and the GOLD check
+    const std::string gold_path = gold.empty() ? input + "_salidaGOLD.xyz" : gold;
+    std::vector<char> gold_text;
+    const bool have_gold = read_file(gold_path, gold_text);
+    std::vector<double> x, y, z;
+    if (!out.empty() || have_gold) {
+        x.resize(n);
+        y.resize(n);
+        z.resize(n);
+        if (owm_cloud_read(ctx, &cloud, x.data(), y.data(), z.data()) != OWM_OK) {
+            fprintf(stderr, "owm_cloud_read: %s\n", owm_last_error(ctx));
+            owm_ctx_destroy(ctx);
+            return 1;
+        }
+    }
+    // the reference's check_results against <input>_salidaGOLD.xyz (or --gold)
+    double correctness = -1.0;
+    if (have_gold) {
+        std::vector<double> gx, gy, gz, sx, sy, sz;
+        gold_text.push_back(0);
+        for (const char *p = gold_text.data(); *p;) {
+            char *q;
+            const double a = strtod(p, &q);
+            const char *eol = strchr(p, '\n');
+            if (q != p) {
+                const double b = strtod(q, &q), c = strtod(q, &q);
+                gx.push_back(a);
+                gy.push_back(b);
+                gz.push_back(c);
+            }
+            p = eol ? eol + 1 : p + strlen(p);
+        }
+        for (uint64_t i = 0; i < ns; ++i) {
+            sx.push_back(x[seeds[i]]);
+            sy.push_back(y[seeds[i]]);
+            sz.push_back(z[seeds[i]]);
+        }
+        double bb[4] = {0, 0, 0, 0};
+        if (!(P.flags & OWM_F_HAVE_BBOX) && n) {
+            bb[0] = bb[1] = x[0];
+            bb[2] = bb[3] = y[0];
+            for (uint64_t i = 1; i < n; ++i) {
+                bb[0] = x[i] < bb[0] ? x[i] : bb[0];
+                bb[1] = x[i] > bb[1] ? x[i] : bb[1];
+                bb[2] = y[i] < bb[2] ? y[i] : bb[2];
+                bb[3] = y[i] > bb[3] ? y[i] : bb[3];
+            }
+        }
+        owm_derived d;
+        uint64_t nc = 0;
+        if (owm_derive(&P, n, (P.flags & OWM_F_HAVE_BBOX) ? nullptr : bb, &d) == OWM_OK &&
+            owm_gold_check(sx.data(), sy.data(), sz.data(), sx.size(), gx.data(), gy.data(),
+                           gz.data(), gx.size(), d.displace, &nc, &correctness) == OWM_OK)
+            printf("%llu points correct; %.2f%%\n", (unsigned long long)nc, correctness);
+        else
+            printf("Unable to check results\n");
+    }
     if (!results.empty()) {
         FILE *rf = fopen(results.c_str(), "a+");
         if (rf) {
             fseek(rf, 0, SEEK_END);
             if (ftell(rf) == 0)
                 fprintf(rf, "ExecName InputFile NumThreads MinRadius MaxNumber Level TreeTime OwmTime Correctness\n");
-            fprintf(rf, "%s %s 0 0 %u 0 %.6f %.6f -1\n", argv[0], path.c_str(), P.leaf_size, 0.0, avg);
+            fprintf(rf, "%s %s 0 0 %u 0 %.6f %.6f %.6f\n", argv[0], path.c_str(), P.leaf_size, 0.0,
+                    avg, correctness);
             fclose(rf);
         }
     }
     if (!out.empty()) {
-        std::vector<double> x(n), y(n), z(n);
-        FILE *of = owm_cloud_read(ctx, &cloud, x.data(), y.data(), z.data()) == OWM_OK
-                       ? fopen(out.c_str(), "w")
-                       : nullptr;
+        FILE *of = fopen(out.c_str(), "w");
         if (of) {
             for (uint64_t i = 0; i < ns; ++i)
                 fprintf(of, "%.2f %.2f %.2f\n", x[seeds[i]], y[seeds[i]], z[seeds[i]]);

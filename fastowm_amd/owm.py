@@ -110,6 +110,8 @@ def lib():
             "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_run_tile_device": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_xyz_parse": ([vp, ctypes.c_char_p, u64, P(CloudDesc)], ctypes.c_int),
+            "owm_gold_check": ([dp, dp, dp, u64, dp, dp, dp, u64, ctypes.c_double, P(u64),
+                                P(ctypes.c_double)], ctypes.c_int),
             "owm_cloud_read": ([vp, P(CloudDesc), dp, dp, dp], ctypes.c_int),
             "owm_batch_run": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp], ctypes.c_int),
             "owm_batch_run_device": ([vp, dp, dp, dp, P(TileDesc), u32, u32, dp, u64, dp],
@@ -154,6 +156,21 @@ def make_params(wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None) -> Para
         for k in range(4):
             p.bbox[k] = float(bbox[k])
     return p
+
+
+def gold_check(seeds_xyz, gold_xyz, displace):
+    """check_results of the reference: (correct seeds, percent of the GOLD
+    points); seeds_xyz and gold_xyz are (x, y, z) array triples."""
+    sx, sy, sz = (np.ascontiguousarray(a, np.float64) for a in seeds_xyz)
+    gx, gy, gz = (np.ascontiguousarray(a, np.float64) for a in gold_xyz)
+    n = ctypes.c_uint64()
+    pct = ctypes.c_double()
+    st = lib().owm_gold_check(sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, len(sx),
+                              gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, len(gx),
+                              float(displace), ctypes.byref(n), ctypes.byref(pct))
+    if st:
+        raise OWMError(st, "gold_check")
+    return n.value, pct.value
 
 
 def query_seed_count(wsize, bsize, overlap, n, bbox=None) -> int:

@@ -154,6 +154,15 @@ owm_status owm_xyz_parse(owm_ctx *ctx, const char *text, uint64_t bytes, owm_clo
  * (no reference counterpart: readXYZfile fills host memory directly). */
 owm_status owm_cloud_read(owm_ctx *ctx, const owm_cloud *cloud, double *x, double *y, double *z);
 
+/* ---- GOLD comparison (host only) -----------------------------------------
+ * The reference's check_results (CPP-TBB-CPU/src/optim3_func.cpp:725-794): a
+ * seed (sx, sy, sz) is correct when a GOLD point lies strictly inside the
+ * square of half side (float)Displace * 0.5 around it with |z - gz| < 0.01;
+ * *percent = correct seeds / n_gold * 100 (the driver's Correctness column). */
+owm_status owm_gold_check(const double *sx, const double *sy, const double *sz, uint64_t n_seeds,
+                          const double *gx, const double *gy, const double *gz, uint64_t n_gold,
+                          double displace, uint64_t *n_correct, double *percent);
+
 /* ---- batch: many independent tiles in one launch chain ------------------
  * The tiles are slices [offset, offset+n) of one concatenated SoA cloud.
  * Seeds of tile t are written to seeds[seed_off[t] .. seed_off[t+1]) with

@@ -97,3 +97,40 @@ def test_python_api_has_reference_names():
     from fastowm_amd import owm
     for name in ("parallel_qtree", "stage1", "stage2", "stage3", "run_tile", "batch_run"):
         assert hasattr(owm.Context, name)
+
+
+def _gold_brute(seeds, gold, displace):
+    """check_results of the reference, restated literally: every seed against
+    every GOLD point (src/optim3_func.cpp:725-794)."""
+    r = float(np.float32(np.float32(displace) * 0.5))
+    sx, sy, sz = seeds
+    gx, gy, gz = gold
+    count = 0
+    for x, y, z in zip(sx, sy, sz):
+        inside = (gx > x - r) & (gy > y - r) & (gx < x + r) & (gy < y + r)
+        if np.any(inside & (np.abs(z - gz) < 0.01)):
+            count += 1
+    return count, count / len(gx) * 100.0
+
+
+@pytest.mark.parametrize("seed,displace", [(0, 2.4), (1, 1.2), (2, 3.0), (3, 0.6)])
+def test_gold_check_equals_the_reference_rule(seed, displace):
+    from fastowm_amd import owm
+    rng = np.random.default_rng(seed)
+    n = 3000
+    gx = 5e5 + rng.uniform(0, 200, n)
+    gy = 4.7e6 + rng.uniform(0, 200, n)
+    gz = rng.uniform(0, 5, n).round(2)
+    # seeds: some near GOLD points (inside / on / past the box edge, z within
+    # and past the tolerance), some anywhere
+    r = float(np.float32(np.float32(displace) * 0.5))
+    k = rng.integers(0, n, 1500)
+    dx = rng.choice([0.0, 0.3 * r, r * (1 - 1e-12), r, r * (1 + 1e-9), -0.7 * r, -r], 1500)
+    dz = rng.choice([0.0, 0.005, 0.00999, 0.01, 0.02, -0.004], 1500)
+    sx = np.concatenate([gx[k] + dx, 5e5 + rng.uniform(0, 200, 500)])
+    sy = np.concatenate([gy[k] - dx[::-1], 4.7e6 + rng.uniform(0, 200, 500)])
+    sz = np.concatenate([gz[k] + dz, rng.uniform(0, 5, 500)])
+    want = _gold_brute((sx, sy, sz), (gx, gy, gz), displace)
+    got = owm.gold_check((sx, sy, sz), (gx, gy, gz), displace)
+    assert got[0] == want[0] and got[1] == pytest.approx(want[1], rel=0, abs=1e-12)
+    assert 0 < got[0] < len(sx)
