@@ -241,15 +241,22 @@ class Tree:
 
 
 class Cloud:
-    """A point cloud parsed on the device (owm_xyz_parse)."""
+    """A point cloud parsed on the device (owm_xyz_parse).  Its arrays live
+    in the context's reader buffers: the next parse on the context, or
+    closing it, retires the cloud (using it then raises)."""
 
-    def __init__(self, ctx: "Context", desc):
-        self.ctx, self.desc = ctx, desc
+    def __init__(self, ctx: "Context", desc, generation: int):
+        self.ctx, self.desc, self.generation = ctx, desc, generation
         self.n = desc.n
         self.bbox = tuple(desc.bbox) if desc.flags & F_HAVE_BBOX else None
         self.host_lines = desc.host_lines
 
+    def check(self):
+        if not getattr(self.ctx, "h", None) or self.ctx._clouds != self.generation:
+            raise OWMError(1, "this cloud was retired by a later parse or by closing its context")
+
     def to_numpy(self):
+        self.check()
         x, y, z = (np.empty(self.n, np.float64) for _ in range(3))
         self.ctx._check(lib().owm_cloud_read(self.ctx.h, ctypes.byref(self.desc), x.ctypes.data,
                                              y.ctypes.data, z.ctypes.data))
@@ -268,6 +275,7 @@ class Context:
         self._trees = weakref.WeakSet()
         self._seed_buf = np.empty(0, np.uint32)
         self._seed_mem = None
+        self._clouds = 0  # parses so far (a Cloud is valid for the latest only)
 
     def _seeds_out(self, cap):
         """Reusable page-locked host buffer the library copies seeds into
@@ -350,12 +358,14 @@ class Context:
             with open(data, "rb") as f:
                 data = f.read()
         c = CloudDesc()
+        self._clouds += 1  # earlier clouds' memory is reused from here on
         self._check(lib().owm_xyz_parse(self.h, data, len(data), ctypes.byref(c)))
-        return Cloud(self, c)
+        return Cloud(self, c, self._clouds)
 
     def run_cloud(self, cloud, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None, cap=None):
         """run_tile on a parsed cloud; the file header's bbox is used when it
         had one and no bbox is given (as the reference driver does)."""
+        cloud.check()
         if bbox is None and cloud.bbox is not None:
             bbox = cloud.bbox
         p = make_params(wsize, bsize, overlap, leaf_size, bbox)

@@ -123,6 +123,17 @@ def test_every_line_on_the_host(ctx):
         del os.environ["OWM_XYZ_LIST_CAP"]
 
 
+def test_retired_cloud_raises(ctx):
+    say("retired cloud")
+    a = ctx.parse_xyz(b"1 2 3\n4 5 6\n")
+    b = ctx.parse_xyz(b"7 8 9\n")
+    with pytest.raises(owm.OWMError):
+        a.to_numpy()
+    with pytest.raises(owm.OWMError):
+        ctx.run_cloud(a)
+    assert same_bits(b.to_numpy()[2], np.array([9.0]))
+
+
 def test_malformed_line(ctx):
     say("two numbers on a line")
     with pytest.raises(owm.OWMError) as e:
