@@ -754,3 +754,206 @@ int64_t oracle_parse_xyz(const char *text, uint64_t bytes, double *x, double *y,
     }
     return (int64_t)n;
 }
+
+/* ---- quadtree leaf statistics (the reference's owm_stats) ----------------
+ * A literal sequential restatement of parallel_qtree
+ * (CPP-TBB-CPU/src/optim3_func.cpp:218-253) for both node policies, walked
+ * the way owm_stats walks it (CPP-TBB-CPU/include/stats_func.hpp:31-56):
+ *   - tree_phase1 (:126-139) creates every node of depths 0..level-1 with its
+ *     four children; the depth-`level` nodes collect their points
+ *     (insertPointConcurrent, :142-154), then each gets createQuadrants and
+ *     its points are inserted into those children in index order (:243-248);
+ *   - maxNumber (statsmaxnum, -DMAXNUMBER): insertPointMaxNum (:160-184), a
+ *     full leaf (maxNumber points) splits and re-inserts its points when the
+ *     next one arrives;
+ *   - minRadius (stats): insertPoint (:103-123), a leaf whose radius * 0.5
+ *     exceeds minRadius splits before it stores a point.
+ * Child centres and radii follow createQuadrants (:59-73).  The reference
+ * recurses without bound when more than maxNumber points coincide; here a
+ * split below depth max_depth returns -2 instead. */
+typedef struct {
+    pool P;
+    const double *x, *y;
+    int policy; /* 0 maxNumber, 1 minRadius */
+    uint32_t maxnum;
+    float minr;
+    int max_depth;
+    int err;
+} qsbuild;
+
+static void qs_insert(qsbuild *B, int32_t node, int depth, uint32_t p);
+
+static void qs_split(qsbuild *B, int32_t node, int depth)
+{
+    if (depth + 1 > B->max_depth) {
+        B->err = 1;
+        return;
+    }
+    create_quadrants(&B->P, node);
+}
+
+static void qs_insert(qsbuild *B, int32_t node, int depth, uint32_t p)
+{
+    if (B->err)
+        return;
+    onode *nd = &B->P.nodes[node];
+    if (nd->child >= 0) { /* internal: descend */
+        qs_insert(B, nd->child + quadrant(nd, B->x[p], B->y[p]), depth + 1, p);
+        return;
+    }
+    if (B->policy == 0) {
+        if (nd->npts < B->maxnum) {
+            push_pt(nd, p);
+            return;
+        }
+        uint32_t *old = nd->pts;
+        uint32_t nold = nd->npts;
+        nd->pts = NULL;
+        nd->npts = nd->cap = 0;
+        qs_split(B, node, depth);
+        if (B->err) {
+            free(old);
+            return;
+        }
+        for (uint32_t k = 0; k < nold; ++k) {
+            const onode *cur = &B->P.nodes[node];
+            qs_insert(B, cur->child + quadrant(cur, B->x[old[k]], B->y[old[k]]), depth + 1, old[k]);
+        }
+        free(old);
+        const onode *cur = &B->P.nodes[node];
+        qs_insert(B, cur->child + quadrant(cur, B->x[p], B->y[p]), depth + 1, p);
+    } else {
+        if ((double)nd->r * 0.5 > (double)B->minr) {
+            qs_split(B, node, depth);
+            if (B->err)
+                return;
+            const onode *cur = &B->P.nodes[node];
+            qs_insert(B, cur->child + quadrant(cur, B->x[p], B->y[p]), depth + 1, p);
+        } else {
+            push_pt(nd, p);
+        }
+    }
+}
+
+typedef struct {
+    uint32_t level, count;
+} qsleaf;
+
+static int cmp_qsleaf(const void *a, const void *b)
+{
+    const qsleaf *u = (const qsleaf *)a, *v = (const qsleaf *)b;
+    if (u->level != v->level)
+        return u->level < v->level ? -1 : 1;
+    return u->count < v->count ? -1 : u->count > v->count;
+}
+
+int64_t oracle_qtree_stats(const double *x, const double *y, uint64_t n, double minx, double maxx,
+                           double miny, double maxy, int policy, uint32_t max_number,
+                           float min_radius, int level, int max_depth, uint32_t *cls_level,
+                           uint32_t *cls_count, uint64_t *cls_leaves, uint64_t cap)
+{
+    if ((policy == 0 && max_number == 0) || level < 0 || level > max_depth)
+        return -1;
+    qsbuild B;
+    memset(&B, 0, sizeof B);
+    B.x = x;
+    B.y = y;
+    B.policy = policy;
+    B.maxnum = max_number;
+    B.minr = min_radius;
+    B.max_depth = max_depth;
+    /* getRadius / getCenter: src/optim3_func.cpp:6-26 */
+    double rx = (maxx - minx) / 2.0, ry = (maxy - miny) / 2.0;
+    float maxr = (float)(rx >= ry ? rx : ry);
+    int32_t root = pool_new(&B.P, minx + rx, miny + ry, maxr);
+    /* tree_phase1: the depth-`level` nodes, in creation order */
+    int64_t nwork = 1;
+    for (int l = 0; l < level; ++l)
+        nwork *= 4;
+    int32_t *work = (int32_t *)malloc(sizeof(int32_t) * (size_t)nwork);
+    int32_t nw = 0;
+    build_top(&B.P, root, 0, level, work, &nw);
+    /* insertPointConcurrent: the points of every depth-`level` node */
+    int32_t *slot = (int32_t *)malloc(sizeof(int32_t) * (size_t)B.P.n);
+    for (int32_t i = 0; i < B.P.n; ++i)
+        slot[i] = -1;
+    for (int32_t w = 0; w < nw; ++w)
+        slot[work[w]] = w;
+    uint32_t *sub = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint64_t *off = (uint64_t *)calloc((size_t)nw + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        int32_t nd = root;
+        while (B.P.nodes[nd].child >= 0)
+            nd = B.P.nodes[nd].child + quadrant(&B.P.nodes[nd], x[i], y[i]);
+        sub[i] = (uint32_t)slot[nd];
+        off[sub[i] + 1]++;
+    }
+    for (int32_t w = 0; w < nw; ++w)
+        off[w + 1] += off[w];
+    uint32_t *ord = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)nw + 1));
+    memcpy(cur, off, sizeof(uint64_t) * ((size_t)nw + 1));
+    for (uint64_t i = 0; i < n; ++i)
+        ord[cur[sub[i]]++] = (uint32_t)i;
+    free(cur);
+    free(sub);
+    /* second phase: createQuadrants on each, then its points in order */
+    for (int32_t w = 0; w < nw && !B.err; ++w) {
+        qs_split(&B, work[w], level);
+        for (uint64_t k = off[w]; k < off[w + 1] && !B.err; ++k) {
+            const onode *t = &B.P.nodes[work[w]];
+            qs_insert(&B, t->child + quadrant(t, x[ord[k]], y[ord[k]]), level + 1, ord[k]);
+        }
+    }
+    free(ord);
+    free(off);
+    free(slot);
+    free(work);
+    int64_t ret = -2;
+    if (!B.err) {
+        /* leaves with their depth: iterative walk from the root */
+        int32_t *stk = (int32_t *)malloc(sizeof(int32_t) * (size_t)B.P.n);
+        uint32_t *dep = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)B.P.n);
+        qsleaf *lv = (qsleaf *)malloc(sizeof(qsleaf) * (size_t)B.P.n);
+        int64_t sp = 0, nl = 0;
+        stk[sp] = root;
+        dep[sp++] = 0;
+        while (sp) {
+            --sp;
+            const int32_t nd = stk[sp];
+            const uint32_t d = dep[sp];
+            const onode *o = &B.P.nodes[nd];
+            if (o->child < 0) {
+                lv[nl].level = d;
+                lv[nl++].count = o->npts;
+            } else {
+                for (int i = 0; i < 4; ++i) {
+                    stk[sp] = o->child + i;
+                    dep[sp++] = d + 1;
+                }
+            }
+        }
+        qsort(lv, (size_t)nl, sizeof(qsleaf), cmp_qsleaf);
+        int64_t nc = 0;
+        for (int64_t i = 0; i < nl;) {
+            int64_t j = i + 1;
+            while (j < nl && lv[j].level == lv[i].level && lv[j].count == lv[i].count)
+                ++j;
+            if ((uint64_t)nc < cap) {
+                cls_level[nc] = lv[i].level;
+                cls_count[nc] = lv[i].count;
+                cls_leaves[nc] = (uint64_t)(j - i);
+            }
+            ++nc;
+            i = j;
+        }
+        ret = nc;
+        free(stk);
+        free(dep);
+        free(lv);
+    }
+    for (int32_t i = 0; i < B.P.n; ++i)
+        free(B.P.nodes[i].pts);
+    free(B.P.nodes);
+    return ret;
+}

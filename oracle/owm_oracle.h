@@ -80,4 +80,15 @@ int oracle_get_threads(void);
 int64_t oracle_parse_xyz(const char *text, uint64_t bytes, double *x, double *y, double *z,
                          uint64_t cap);
 
+/* Leaf statistics of the reference's parallel_qtree (owm_stats): policy 0 =
+ * maxNumber (statsmaxnum), 1 = minRadius (stats); the box is the cloud's
+ * (minx, maxx, miny, maxy) as readXYZfile gives it.  Writes the distinct
+ * (depth, points) pairs of the leaves, ascending, with their leaf counts, up to
+ * cap entries, and returns how many there are; -1 for bad arguments, -2 when a
+ * split would go below depth max_depth. */
+int64_t oracle_qtree_stats(const double *x, const double *y, uint64_t n, double minx, double maxx,
+                           double miny, double maxy, int policy, uint32_t max_number,
+                           float min_radius, int level, int max_depth, uint32_t *cls_level,
+                           uint32_t *cls_count, uint64_t *cls_leaves, uint64_t cap);
+
 #endif

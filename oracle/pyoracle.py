@@ -70,6 +70,12 @@ def lib():
         L.oracle_stage3_brute.restype = ctypes.c_uint32
         L.oracle_parse_xyz.argtypes = [ctypes.c_char_p, ctypes.c_uint64, dp, dp, dp, ctypes.c_uint64]
         L.oracle_parse_xyz.restype = ctypes.c_int64
+        L.oracle_qtree_stats.argtypes = [dp, dp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_uint32, ctypes.c_float, ctypes.c_int,
+                                         ctypes.c_int, up, up,
+                                         ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+        L.oracle_qtree_stats.restype = ctypes.c_int64
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _LIB = L
@@ -198,3 +204,31 @@ def parse_xyz(text: bytes, cap: int):
     if n < 0:
         raise ValueError("a line holds fewer than three numbers")
     return x[:n], y[:n], z[:n]
+
+
+QT_MAXNUMBER, QT_MINRADIUS = 0, 1
+
+
+def qtree_stats(x, y, box, policy=QT_MAXNUMBER, max_number=32, min_radius=0.1, level=5,
+                max_depth=64):
+    """Leaf classes of the reference's parallel_qtree as owm_stats walks it:
+    a list of (depth, points, leaves), ascending; None when a split would go
+    below max_depth."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    cap = 4096
+    while True:
+        lv = np.empty(cap, np.uint32)
+        ct = np.empty(cap, np.uint32)
+        nl = np.empty(cap, np.uint64)
+        r = lib().oracle_qtree_stats(_dp(x), _dp(y), len(x), box[0], box[1], box[2], box[3],
+                                     policy, max_number, min_radius, level, max_depth,
+                                     _up(lv), _up(ct),
+                                     nl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap)
+        if r == -2:
+            return None
+        if r < 0:
+            raise ValueError("oracle_qtree_stats: bad arguments")
+        if r <= cap:
+            return [(int(lv[i]), int(ct[i]), int(nl[i])) for i in range(r)]
+        cap = int(r)
