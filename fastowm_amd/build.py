@@ -23,7 +23,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-function", "-Wno-unused-variable"]
 HIP_SOURCES = ["sort.hip", "tree.hip", "stage1.hip", "stages.hip", "xyz.hip", "owm_host.cpp",
-               "owm_capi.cpp", "owm_serve.cpp", "owm_tune.cpp", "owm_xyz.cpp", "owm_gold.cpp"]
+               "owm_capi.cpp", "owm_serve.cpp", "owm_tune.cpp", "owm_xyz.cpp", "owm_gold.cpp",
+               "qstats.hip", "owm_qstats.cpp"]
 
 
 def _run(cmd):

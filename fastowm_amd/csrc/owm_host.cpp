@@ -81,7 +81,7 @@ struct IoLayout {
 static const char *PHASE_NAMES[PH_COUNT] = {
     "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "agg_levels", "stage1",
     "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
-    "stage1_summaries", "xyz_parse"};
+    "stage1_summaries", "xyz_parse", "qtree_stats"};
 
 const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
 
@@ -128,6 +128,14 @@ void Profiler::collect()
         pool.push_back(r.b);
     }
     pending.clear();
+}
+
+void Profiler::add_work(int ph, uint64_t b, uint64_t it)
+{
+    if (!on || !((mask >> ph) & 1u))
+        return;
+    bytes[ph] += b;
+    items[ph] += it;
 }
 
 void Profiler::reset()
@@ -442,10 +450,6 @@ uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg, uint64_t lb_t
            align_up(64, 256) + align_up((uint64_t)passes * tiles * 256 * 4, 256);
 }
 
-struct SortBufs {
-    uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
-};
-
 // Aggregate levels of nleaf leaves: box, count, z key, z index per entry.
 static uint64_t agg_levels_bytes(uint64_t nleaf)
 {
@@ -465,8 +469,7 @@ uint64_t build_scratch_bytes(const TreeLayout &L)
            rec_sets * 3 * align_up(L.np * 8ull, 256) + agg_levels_bytes(L.nleaf);
 }
 
-static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1,
-                           uint64_t lb_tiles = 0)
+SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles)
 {
     SortBufs S;
     uint64_t o = 0;
@@ -488,8 +491,8 @@ static SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg 
 
 // Runs the LSD passes; keys in S.k0 (values implicit iota when iota).  Returns
 // which buffer pair holds the result (0 -> k0/v0, 1 -> k1/v1).
-static int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
-                    const uint32_t *d_n, int passes, bool keys_only, bool iota)
+int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
+             const uint32_t *d_n, int passes, bool keys_only, bool iota)
 {
     // S.hist and S.lookback were zeroed with the chain's other buffers
     launch_sort_hist(s, S.k0, d_n, n_max, passes, S.hist);
@@ -1310,7 +1313,7 @@ uint64_t owm_ctx::room_for(const owm::DevBuf &b) const
 uint64_t owm_ctx::held_bytes() const
 {
     uint64_t b = arena.buf.bytes + small.buf.bytes + upload.bytes + s1_list.bytes;
-    b += xyz_text.bytes + xyz_work.bytes + xyz_cloud.bytes;
+    b += xyz_text.bytes + xyz_work.bytes + xyz_cloud.bytes + qs_work.bytes;
     b += trees_bytes.load();
     return b;
 }

@@ -144,6 +144,16 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
 void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
 uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles);
+// Key/value LSD radix sort (sort.hip): keys in k0 (values in v0, or implicit
+// 0..n-1 with iota); hist, counters and lookback, carved in that order after
+// v1, must start at zero.  run_sort returns which pair holds the result
+// (0: k0/v0, 1: k1/v1) and times the passes as `phase`.
+struct SortBufs {
+    uint32_t *k0, *v0, *k1, *v1, *hist, *counters, *lookback;
+};
+SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg = 1, uint64_t lb_tiles = 0);
+int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max,
+             const uint32_t *d_n, int passes, bool keys_only, bool iota);
 uint64_t build_scratch_bytes(const TreeLayout &L);
 uint64_t table_bytes(uint32_t n_tiles);
 owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const BatchDev &B,
@@ -191,7 +201,7 @@ struct ServeState;
 
 enum Phase {
     PH_BBOX, PH_MORTON, PH_SORT_TREE, PH_LEAVES, PH_KARRAS, PH_LEVELS, PH_STAGE1,
-    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_XYZ, PH_COUNT
+    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_XYZ, PH_QSTATS, PH_COUNT
 };
 
 const char *phase_name(int ph);
@@ -214,6 +224,8 @@ struct Profiler {
     // launches: kernel launches of the phase between begin and end
     void end(hipStream_t s, int idx, uint64_t bytes, uint64_t items, uint32_t launches = 1);
     void collect();  // after the stream has been synchronised
+    // bytes and items of a phase known only after its results are read back
+    void add_work(int ph, uint64_t bytes, uint64_t items);
     void reset();
     ~Profiler();
 };
@@ -246,6 +258,7 @@ struct owm_ctx {
     // .xyz reader: the text and index, parse work (and the cloud when no line
     // was blank), the compacted cloud
     owm::DevBuf xyz_text, xyz_work, xyz_cloud;
+    owm::DevBuf qs_work;  // quadtree statistics: point state, child slots, histograms
     uint64_t s1_need_max = 0;  // largest join that overflowed its first estimate
     owm::Tuner tuner;
     std::atomic<uint64_t> trees_bytes{0};

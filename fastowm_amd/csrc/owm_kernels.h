@@ -124,6 +124,51 @@ void launch_xyz_compact(hipStream_t s, const double *x, const double *y, const d
                         uint64_t nlines, const uint64_t *blank, uint64_t nb, double *ox, double *oy,
                         double *oz);
 
+// ---- quadtree leaf statistics (qstats.hip) ----
+// The reference's parallel_qtree (maxNumber or minRadius policy) resolved one
+// depth at a time: open nodes (split nodes) of depth d-1 own child slots
+// 4 * open + quadrant at depth d.
+constexpr int QS_MAX_DEPTH = 64;
+constexpr uint32_t QS_HB = 1024;  // exact bins of leaf point counts per depth
+constexpr int QS_KEY_LEVELS = 16; // quadrants of depths 1..16 in a point's 32-bit path key
+// Root centre and, per depth k < QS_KEY_LEVELS, the offset from a depth-k
+// centre to its children's (createQuadrants: radius * 0.5f, as a double).
+struct QsPath {
+    double cx, cy;
+    double off[QS_KEY_LEVELS];
+};
+struct QsDev {
+    const double *x, *y;
+    uint64_t n;
+    // points in path-key order: key (null: none) and the index of the point
+    // at each position (null: identity)
+    const uint32_t *key, *perm;
+    uint32_t *st;     // per point: child slot at the last routed depth, OWM_NONE once in a leaf
+    uint32_t *cnt;    // per child slot: points routed into it (zero outside its classify)
+    uint32_t *remap;  // per child slot: open-node index at its depth, OWM_NONE for a leaf
+    double2 *cen[2];  // open-node centres, by depth parity
+    uint32_t *nopen;  // [QS_MAX_DEPTH + 1] open nodes per depth (capped by open_cap)
+    unsigned long long *hist;  // [(QS_MAX_DEPTH + 1) * QS_HB] leaves per (depth, points)
+    uint2 *ovf;       // (depth, points) of every leaf holding >= QS_HB points
+    uint32_t *novf;   // entries written to ovf
+    uint32_t *flags;  // bit 0: ovf full, bit 1: more open nodes than open_cap
+    uint32_t ovf_cap, open_cap;
+    uint32_t policy;  // 0 maxNumber, 1 minRadius
+    uint32_t max_number, level;
+    float min_radius;
+};
+// Root at depth 0 (open node 0), every point in slot 0, whose remap is 0.
+// cnt, nopen, hist, novf and flags must be zero.
+void launch_qs_root(hipStream_t s, const QsDev &D, double cx, double cy);
+// key[i]: the quadrants point i takes at depths 1..16, two bits each, depth 1
+// highest (quadrantIdx against the createQuadrants centres).
+void launch_qs_path(hipStream_t s, const double *x, const double *y, uint64_t n, const QsPath &P,
+                    uint32_t *key);
+// Depth d >= 1: route the points of the open depth-(d-1) nodes to their child
+// slots, then classify the slots (rd = the depth-d radius, off = the centre
+// offset of createQuadrants at depth d-1).
+void launch_qs_level(hipStream_t s, const QsDev &D, int d, float rd, double off);
+
 // ---- stages (stages.hip) ----
 struct Stage1Cfg {
     uint32_t bx, by;  // window block shape

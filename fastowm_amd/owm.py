@@ -77,6 +77,54 @@ class TuneEntry(ctypes.Structure):
 
 TUNE_CLASSES = 6
 
+QT_MAXNUMBER, QT_MINRADIUS = 0, 1
+
+
+class QtreeParams(ctypes.Structure):
+    _fields_ = [("policy", ctypes.c_uint32), ("max_number", ctypes.c_uint32),
+                ("min_radius", ctypes.c_float), ("level", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("bbox", ctypes.c_double * 4)]
+
+
+class LeafBin(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_uint32), ("points", ctypes.c_uint32),
+                ("leaves", ctypes.c_uint64)]
+
+
+class QtreeSummary(ctypes.Structure):
+    _fields_ = [("bbox", ctypes.c_double * 4), ("max_radius", ctypes.c_float),
+                ("depth", ctypes.c_uint32), ("leaves", ctypes.c_uint64),
+                ("n_bins", ctypes.c_uint64)]
+
+
+def qtree_params(policy=QT_MAXNUMBER, max_number=32, min_radius=0.1, level=5, bbox=None):
+    """owm_qtree_params; policy is QT_MAXNUMBER / QT_MINRADIUS or the
+    reference binary's name, "statsmaxnum" / "stats"."""
+    if isinstance(policy, str):
+        policy = {"statsmaxnum": QT_MAXNUMBER, "maxnumber": QT_MAXNUMBER,
+                  "stats": QT_MINRADIUS, "minradius": QT_MINRADIUS}[policy.lower()]
+    q = QtreeParams(policy, max_number, min_radius, level, 0)
+    if bbox is not None:
+        q.flags = F_HAVE_BBOX
+        for k in range(4):
+            q.bbox[k] = bbox[k]
+    return q
+
+
+def write_qtree_stats(path, classes, summary, params, n_points):
+    """Appends the statistics tool's four reports (owm_qtree_stats_write);
+    classes and summary as Context.qtree_stats returns them."""
+    arr = (LeafBin * max(len(classes), 1))(*[LeafBin(*c) for c in classes])
+    sm = QtreeSummary()
+    for k in range(4):
+        sm.bbox[k] = summary["bbox"][k]
+    sm.max_radius = summary["max_radius"]
+    sm.depth, sm.leaves, sm.n_bins = summary["depth"], summary["leaves"], len(classes)
+    st = lib().owm_qtree_stats_write(str(path).encode(), ctypes.byref(params), ctypes.byref(sm),
+                                     arr, len(classes), n_points)
+    if st != 0:
+        raise OWMError(st, f"owm_qtree_stats_write({path}) failed")
+
 
 _lib = None
 
@@ -138,6 +186,10 @@ def lib():
                               P(u32)], ctypes.c_int),
             "owm_tune_get": ([vp, P(TuneEntry), u32, P(u32)], ctypes.c_int),
             "owm_tune_set": ([vp, P(TuneEntry), u32], ctypes.c_int),
+            "owm_qtree_stats": ([vp, dp, dp, u64, ctypes.c_int, P(QtreeParams), P(LeafBin),
+                                 u64, P(QtreeSummary)], ctypes.c_int),
+            "owm_qtree_stats_write": ([ctypes.c_char_p, P(QtreeParams), P(QtreeSummary),
+                                       P(LeafBin), u64, u64], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -376,6 +428,40 @@ class Context:
         self._check(lib().owm_run_tile_device(self.h, d.x, d.y, d.z, d.n, ctypes.byref(p),
                                               out.ctypes.data, cap, ctypes.byref(ns)))
         return out[:ns.value].copy()
+
+    def qtree_stats(self, x, y=None, policy=QT_MAXNUMBER, max_number=32, min_radius=0.1,
+                    level=5, bbox=None):
+        """Leaf statistics of the reference's quadtree (owm_stats): a list of
+        (level, points, leaves), ascending, and a summary dict.  x/y are numpy
+        arrays or float64 torch tensors on the GPU, or x is a parsed Cloud
+        (whose header box is used when it had one and bbox is None)."""
+        if isinstance(x, Cloud):
+            x.check()
+            if bbox is None and x.bbox is not None:
+                bbox = x.bbox
+            d = x.desc
+            dev, px, py, n = 1, d.x, d.y, d.n
+        elif _is_device(x):
+            dev, px, py, n = 1, _dev_ptr(x), _dev_ptr(y), len(x)
+        else:
+            (xa, px), (ya, py) = _host(x), _host(y)
+            dev, n = 0, len(xa)
+        q = qtree_params(policy, max_number, min_radius, level, bbox)
+        sm = QtreeSummary()
+        cap = 4096
+        while True:
+            arr = (LeafBin * cap)()
+            st = lib().owm_qtree_stats(self.h, px, py, n, dev, ctypes.byref(q), arr, cap,
+                                       ctypes.byref(sm))
+            if st == 5 and sm.n_bins > cap:  # OWM_E_CAPACITY
+                cap = int(sm.n_bins)
+                continue
+            self._check(st)
+            break
+        classes = [(arr[k].level, arr[k].points, arr[k].leaves) for k in range(sm.n_bins)]
+        summary = {"bbox": tuple(sm.bbox), "max_radius": sm.max_radius, "depth": sm.depth,
+                   "leaves": sm.leaves, "params": q, "n": n}
+        return classes, summary
 
     # -- whole pipeline ----------------------------------------------------
     def run_tile(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None,

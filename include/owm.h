@@ -163,6 +163,53 @@ owm_status owm_gold_check(const double *sx, const double *sy, const double *sz, 
                           const double *gx, const double *gy, const double *gz, uint64_t n_gold,
                           double displace, uint64_t *n_correct, double *percent);
 
+/* ---- quadtree leaf statistics (owm_stats) ---------------------------------
+ * The leaves of the reference's parallel_qtree (CPP-TBB-CPU/src/optim3_func.cpp:
+ * 218-253) as the statistics tool walks them (CPP-TBB-CPU/application/
+ * owm_stats.cpp:60-136, include/stats_func.hpp:6-56): `statsmaxnum` builds it
+ * with the maxNumber policy, `stats` with minRadius.  Computed on the device
+ * without building the tree; the result is the distinct (depth, points)
+ * pairs of the leaves with how many leaves have each, from which every
+ * histogram of the tool follows (points per leaf, radius, density, level). */
+#define OWM_QT_MAXNUMBER 0u /* split a node holding more than max_number points */
+#define OWM_QT_MINRADIUS 1u /* split a node holding a point while radius * 0.5 > min_radius */
+#define OWM_QT_MAX_LEVEL 10u
+#define OWM_QT_MAX_DEPTH 64u
+typedef struct owm_qtree_params {
+    uint32_t policy;     /* OWM_QT_MAXNUMBER or OWM_QT_MINRADIUS */
+    uint32_t max_number; /* -s maxNumber (>= 1 under OWM_QT_MAXNUMBER) */
+    float min_radius;    /* -r minRadius */
+    uint32_t level;      /* -L: depths 0..level are split whatever they hold (<= OWM_QT_MAX_LEVEL) */
+    uint32_t flags;      /* OWM_F_HAVE_BBOX: bbox is the header box; else the data's */
+    double bbox[4];      /* minx, maxx, miny, maxy */
+} owm_qtree_params;
+typedef struct owm_leaf_bin {
+    uint32_t level;  /* depth of the leaves (root = 0) */
+    uint32_t points; /* points each holds */
+    uint64_t leaves; /* leaves with this depth and count */
+} owm_leaf_bin;
+typedef struct owm_qtree_summary {
+    double bbox[4];     /* the box the root was centred on */
+    float max_radius;   /* the root's radius (getRadius, a float) */
+    uint32_t depth;     /* deepest leaf */
+    uint64_t leaves;    /* leaf nodes */
+    uint64_t n_bins; /* distinct (level, points) pairs */
+} owm_qtree_summary;
+/* bins[0 .. min(cap, n_bins)) in ascending (level, points) order;
+ * OWM_E_CAPACITY when cap < out->n_bins (out is filled).  OWM_E_ARG when a
+ * node would split below OWM_QT_MAX_DEPTH (more than max_number points at one
+ * spot: the reference recursion never ends there) or n >= 2^30 under
+ * OWM_QT_MINRADIUS.  x, y: device pointers when device_ptrs != 0. */
+owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint64_t n,
+                           int device_ptrs, const owm_qtree_params *params,
+                           owm_leaf_bin *bins, uint64_t cap, owm_qtree_summary *out);
+/* Appends the tool's four reports to the text file `path` as save_histogram,
+ * save_areas, save_density and save_levels do (stats_func.hpp:120-366; the
+ * average density is n / (round2d(width) * round2d(height))). */
+owm_status owm_qtree_stats_write(const char *path, const owm_qtree_params *params,
+                                 const owm_qtree_summary *summary, const owm_leaf_bin *bins,
+                                 uint64_t n_bins, uint64_t n_points);
+
 /* ---- batch: many independent tiles in one launch chain ------------------
  * The tiles are slices [offset, offset+n) of one concatenated SoA cloud.
  * Seeds of tile t are written to seeds[seed_off[t] .. seed_off[t+1]) with

@@ -149,3 +149,69 @@ def test_edge_cases():
     assert level_sync(p, p, box, O.QT_MAXNUMBER, max_number=8, level=0) is None
     got = O.qtree_stats(p, p, box, O.QT_MAXNUMBER, max_number=9, level=0)
     assert got == [(1, 0, 3), (1, 9, 1)]
+
+
+# ---- the text reports (owm_qtree_stats_write, host only) ------------------
+
+def c_round2d(v):
+    import math
+    w = v * 100.0
+    return (math.floor(w + 0.5) if w >= 0 else math.ceil(w - 0.5)) / 100.0
+
+
+def expected_reports(path, classes, policy, max_number, min_radius, box, n):
+    """save_histogram, save_areas, save_density and save_levels
+    (CPP-TBB-CPU/include/stats_func.hpp:120-366) over the leaves the classes
+    describe; numbers as an ostream prints them (%g, 6 digits)."""
+    rx, ry = (box[1] - box[0]) / 2.0, (box[3] - box[2]) / 2.0
+    r0 = np.float32(rx if rx >= ry else ry)
+
+    def radius(d):
+        r = r0
+        for _ in range(d):
+            r = np.float32(float(r) * 0.5)
+        return r
+
+    pts, lev, rad, den = Counter(), Counter(), Counter(), Counter()
+    for d, c, k in classes:
+        r = radius(d)
+        pts[c] += k
+        lev[d] += k
+        rad[float(r)] += k
+        den[float(np.float32(c) / (np.float32(4) * r * r))] += k
+    g = lambda v: "%g" % v  # noqa: E731
+    nleaf = sum(pts.values())
+    dens = n / (c_round2d(box[1] - box[0]) * c_round2d(box[3] - box[2]))
+    s = [f"HISTOGRAM: {path}", f"Number of leaf nodes: {nleaf}"]
+    s += [f"MinRadius: {g(float(np.float32(min_radius)))} m" if policy == O.QT_MINRADIUS
+          else f"MaxNumber: {max_number} pts", ""]
+    s += ["Observed: ", f"Average number of points/leaf-node: {sum(c * k for c, k in pts.items()) // nleaf}",
+          f"__min: {min([999] + list(pts))}", f"__max: {max([0] + list(pts))}", "",
+          "Estimation (assuming equal distribution of points): ", f"Average density: {g(dens)}", "",
+          "x_i n_i x_i*n_i", ""]
+    s += [f"{c} {k} {c * k}" for c, k in sorted(pts.items())] + ["", ""]
+    s += ["Radius HISTOGRAM:", f"Number of leaf-nodes: {nleaf}", "", "r_i n_i a_i", ""]
+    s += [f"{g(r)} {k} {g(4 * r * r)}" for r, k in sorted(rad.items())] + ["", ""]
+    s += ["Density HISTOGRAM:", f"Number of leaf-nodes: {nleaf}", "", "d_i n_i", ""]
+    s += [f"{g(v)} {k}" for v, k in sorted(den.items())] + ["", ""]
+    s += ["Levels HISTOGRAM:", f"Number of leaf-nodes: {nleaf}", "", "level_i n_i", ""]
+    s += [f"{d} {k}" for d, k in sorted(lev.items())] + ["", ""]
+    return "\n".join(s) + "\n"
+
+
+@pytest.mark.parametrize("policy,kw", [(O.QT_MAXNUMBER, {"max_number": 32}),
+                                       (O.QT_MINRADIUS, {"min_radius": 0.3})])
+def test_reports_text(tmp_path, policy, kw):
+    from fastowm_amd import owm, synth
+    x, y, _ = tile_for({"cluster_frac": 0.3, "n_clusters": 4}, 40_000).generate()
+    box = synth.bbox(x, y)
+    classes = O.qtree_stats(x, y, box, policy, level=3, **kw)
+    rx, ry = (box[1] - box[0]) / 2.0, (box[3] - box[2]) / 2.0
+    summary = {"bbox": box, "max_radius": float(np.float32(rx if rx >= ry else ry)),
+               "depth": max(c[0] for c in classes), "leaves": sum(c[2] for c in classes)}
+    q = owm.qtree_params(policy, level=3, bbox=box, **kw)
+    path = tmp_path / "tbb_histogram.csv"
+    owm.write_qtree_stats(path, classes, summary, q, len(x))
+    want = expected_reports(str(path), classes, policy, kw.get("max_number", 32),
+                            kw.get("min_radius", 0.1), box, len(x))
+    assert path.read_text() == want
