@@ -185,6 +185,45 @@ def reader_cpu(text, cloud, sample_bytes=100_000_000):
             "bit_equal_to_device": bool(same)}
 
 
+def run_qstats_bench(ctx, synth, dx, dy, x, y, reps=5):
+    """Quadtree leaf statistics (the reference's statsmaxnum -s 32 -L 5 and
+    stats -r 0.1 -L 5) of the resident C2 cloud: device time per call (HIP
+    events over the path keys, their sort and the depth loop) and its
+    algorithmic bytes, then the oracle on a bounded sample of the same cloud:
+    its classes must equal the device's on that sample, and its time is the
+    CPU baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    box = synth.bbox(x, y)
+    out = {}
+    for name, pol, kw in (("statsmaxnum", 0, {"max_number": 32}), ("stats", 1, {"min_radius": 0.1})):
+        ctx.qtree_stats(dx, dy, policy=pol, level=5, bbox=box, **kw)  # allocations
+        ctx.profile(True, phases=["qtree_stats"])
+        for _ in range(reps):
+            cls, sm = ctx.qtree_stats(dx, dy, policy=pol, level=5, bbox=box, **kw)
+        p = ctx.profile_read()["qtree_stats"]
+        ctx.profile(False)
+        ms = p["ms"] / reps
+        # oracle on the first 2M points (a few seconds of one core)
+        k = 2_000_000
+        sub, _ = ctx.qtree_stats(dx[:k].contiguous(), dy[:k].contiguous(), policy=pol, level=5,
+                                 bbox=box, **kw)
+        t0 = time.perf_counter()
+        want = pyoracle.qtree_stats(x[:k], y[:k], box, pol, level=5, **kw)
+        dt = time.perf_counter() - t0
+        out[name] = {"params": dict(kw, level=5), "points": len(x), "depth": sm["depth"],
+                     "leaves": sm["leaves"], "bins": len(cls), "device_ms": round(ms, 3),
+                     "mpts": round(len(x) / (ms * 1e-3) / 1e6, 1), "launches": p["calls"] // reps,
+                     "algorithmic_bytes": int(p["bytes"] / reps),
+                     "gbs": round(p["bytes"] / reps / (ms * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(p["bytes"] / reps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "cpu_baseline": {"mpts": round(k / dt / 1e6, 2), "cores": 1, "kind": "port",
+                                      "sample": f"first {k} points of the same cloud, the oracle's "
+                                                "sequential parallel_qtree restatement"},
+                     "equal_to_oracle_on_sample": sub == want}
+    return out
+
+
 def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
     """Poisson request streams against the serving loop: N log-uniform
     10k-8M points, Wsize 5-20, Bsize 10-30, Overlap 0.5-0.9; one request in
@@ -249,6 +288,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-reader", action="store_true", help="skip the .xyz reader leg")
+    ap.add_argument("--no-qstats", action="store_true", help="skip the quadtree statistics leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--batch-only", action="store_true",
@@ -363,6 +403,9 @@ def main():
     if not args.no_reader:
         log("reader leg")
         line["reader"], reader_text, reader_cloud = run_reader_bench(ctx, synth, x, y, z)
+    if not args.no_qstats:
+        log("quadtree statistics leg")
+        line["qtree_stats"] = run_qstats_bench(ctx, synth, dx, dy, x, y)
     if not args.no_serve:
         log("serve leg")
         rates = [float(r) for r in args.serve_rates.split(",") if r]

@@ -2,10 +2,15 @@
 //
 // Mirrors the reference CPU driver's flags (CPP-TBB-CPU/application/owm_optim3.cpp:14-40):
 //   -i <input without .xyz>  -W <Wsize>  -B <Bsize>  -O <Overlap>  -s <maxNumber>
-//   -l <loops>  --results <csv>  (-n, -r and -L are accepted and ignored: the
-//   GPU path has no thread count, minRadius policy or parallel tree level)
+//   -l <loops>  --results <csv>  (-n is accepted and ignored: the GPU path
+//   has no thread count; -r and -L only matter to --stats)
 // plus --device <id>, -o <output .xyz of the seeds> and --gold <file> (default
 // <input>_salidaGOLD.xyz, as the reference: the results CSV's Correctness).
+//
+// --stats maxnum|radius runs the reference's statistics tool instead
+// (CPP-TBB-CPU/application/owm_stats.cpp: `statsmaxnum` with -s maxNumber,
+// `stats` with -r minRadius, -L level for both): the quadtree's leaf
+// histograms are appended to --results (default tbb_histogram.csv).
 //
 // Input: .xyz text, "x y z" per line; an optional 5-number header
 // "N minx maxx miny maxy" (the reference's *H.xyz convention,
@@ -43,14 +48,63 @@ void usage()
 {
     fprintf(stderr,
             "usage: owm -i <input without .xyz> [-W 10] [-B 20] [-O 0.80] [-s 32] [-l 1]\n"
-            "           [--results o3_memoization.csv] [-o seeds.xyz] [--device 0]\n");
+            "           [--results o3_memoization.csv] [-o seeds.xyz] [--device 0]\n"
+            "       owm --stats maxnum|radius -i <input without .xyz> [-s 32] [-r 0.1] [-L 5]\n"
+            "           [--results tbb_histogram.csv] [--device 0]\n");
+}
+
+// The statistics tool on a parsed cloud (owm_stats.cpp:60-136).
+int run_stats(owm_ctx *ctx, const owm_cloud &cloud, const std::string &mode, uint32_t max_number,
+              float min_radius, uint32_t level, const std::string &results)
+{
+    owm_qtree_params q{};
+    q.policy = mode == "radius" ? OWM_QT_MINRADIUS : OWM_QT_MAXNUMBER;
+    q.max_number = max_number;
+    q.min_radius = min_radius;
+    q.level = level;
+    if (cloud.flags & OWM_F_HAVE_BBOX) {
+        q.flags = OWM_F_HAVE_BBOX;
+        memcpy(q.bbox, cloud.bbox, sizeof q.bbox);
+    }
+    std::vector<owm_leaf_bin> bins(4096);
+    owm_qtree_summary sm;
+    auto t0 = std::chrono::steady_clock::now();
+    owm_status st = owm_qtree_stats(ctx, cloud.x, cloud.y, cloud.n, 1, &q, bins.data(), bins.size(), &sm);
+    if (st == OWM_E_CAPACITY) {
+        bins.resize(sm.n_bins);
+        st = owm_qtree_stats(ctx, cloud.x, cloud.y, cloud.n, 1, &q, bins.data(), bins.size(), &sm);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (st != OWM_OK) {
+        fprintf(stderr, "owm_qtree_stats: %s: %s\n", owm_status_str(st), owm_last_error(ctx));
+        return 1;
+    }
+    const double rx = (sm.bbox[1] - sm.bbox[0]) / 2.0, ry = (sm.bbox[3] - sm.bbox[2]) / 2.0;
+    printf("QTREE PARAMETERS:\n");
+    printf("MaxRadius:  %.3f\n", sm.max_radius);
+    printf("Center:     %.2f , %.2f\n", sm.bbox[0] + rx, sm.bbox[2] + ry);
+    printf("Radius:     %.2f , %.2f\n", rx, ry);
+    if (q.policy == OWM_QT_MINRADIUS)
+        printf("Building quadtree with minRadius: %g\n", min_radius);
+    else
+        printf("Building quadtree with maxNumber: %u\n", max_number);
+    printf("Leaf nodes: %llu (depth %u), statistics in %.6f s\n", (unsigned long long)sm.leaves,
+           sm.depth, std::chrono::duration<double>(t1 - t0).count());
+    st = owm_qtree_stats_write(results.c_str(), &q, &sm, bins.data(), sm.n_bins, cloud.n);
+    if (st != OWM_OK) {
+        fprintf(stderr, "cannot write %s\n", results.c_str());
+        return 1;
+    }
+    return 0;
 }
 
 }  // namespace
 
 int main(int argc, char **argv)
 {
-    std::string input = "data/INAER_2011_Alcoy", results = "o3_memoization.csv", out, gold;
+    std::string input = "data/INAER_2011_Alcoy", results, out, gold, stats;
+    float min_radius = 0.1f;
+    uint32_t level = 5;
     owm_params P{};
     P.wsize = 10;
     P.bsize = 20;
@@ -76,8 +130,10 @@ int main(int argc, char **argv)
         else if (a == "-o") out = val();
         else if (a == "--gold") gold = val();
         else if (a == "--device") device = atoi(val());
-        else if (a == "-n" || a == "-r" || a == "-L" || a == "--num_threads" || a == "--radius" ||
-                 a == "--level") (void)val();
+        else if (a == "--stats") stats = val();
+        else if (a == "-r" || a == "--radius") min_radius = strtof(val(), nullptr);
+        else if (a == "-L" || a == "--level") level = (uint32_t)strtoul(val(), nullptr, 10);
+        else if (a == "-n" || a == "--num_threads") (void)val();
         else if (a == "-h" || a == "--help") {
             usage();
             return 0;
@@ -86,6 +142,12 @@ int main(int argc, char **argv)
             return 2;
         }
     }
+    if (!stats.empty() && stats != "maxnum" && stats != "radius") {
+        usage();
+        return 2;
+    }
+    if (results.empty())
+        results = stats.empty() ? "o3_memoization.csv" : "tbb_histogram.csv";
     std::string path = input + ".xyz";
     std::vector<char> text;
     if (!read_file(path, text)) {
@@ -115,6 +177,11 @@ int main(int argc, char **argv)
     const uint64_t n = cloud.n;
     printf("Input: %s (%llu points) on device %d, read in %.6f s\n", path.c_str(),
            (unsigned long long)n, device, std::chrono::duration<double>(r1 - r0).count());
+    if (!stats.empty()) {
+        const int rc = run_stats(ctx, cloud, stats, P.leaf_size, min_radius, level, results);
+        owm_ctx_destroy(ctx);
+        return rc;
+    }
     std::vector<uint32_t> seeds(n + 1);
     uint64_t ns = 0;
     std::vector<double> times;
