@@ -143,7 +143,8 @@ struct QsDev {
     // points in path-key order: key (null: none) and the index of the point
     // at each position (null: identity)
     const uint32_t *key, *perm;
-    uint32_t *st;     // per point: child slot at the last routed depth, OWM_NONE once in a leaf
+    uint2 *rng[2];    // open-node key ranges [first, end) down to depth QS_KEY_LEVELS, by parity
+    uint32_t *st;     // below QS_KEY_LEVELS, per point: child slot at the last routed depth (OWM_NONE: in a leaf)
     uint32_t *cnt;    // per child slot: points routed into it (zero outside its classify)
     uint32_t *remap;  // per child slot: open-node index at its depth, OWM_NONE for a leaf
     double2 *cen[2];  // open-node centres, by depth parity
@@ -157,16 +158,21 @@ struct QsDev {
     uint32_t max_number, level;
     float min_radius;
 };
-// Root at depth 0 (open node 0), every point in slot 0, whose remap is 0.
-// cnt, nopen, hist, novf and flags must be zero.
+// Root at depth 0: open node 0 with every point's key range.  cnt, nopen,
+// hist, novf and flags must be zero.
 void launch_qs_root(hipStream_t s, const QsDev &D, double cx, double cy);
+// Depth d <= QS_KEY_LEVELS from the open depth-(d-1) nodes' key ranges (key
+// sorted); rd, off as for launch_qs_level.
+void launch_qs_split(hipStream_t s, const QsDev &D, int d, float rd, double off);
+// Hands the points of the nodes open at depth QS_KEY_LEVELS to launch_qs_level.
+void launch_qs_deep_start(hipStream_t s, const QsDev &D);
 // key[i]: the quadrants point i takes at depths 1..16, two bits each, depth 1
 // highest (quadrantIdx against the createQuadrants centres).
 void launch_qs_path(hipStream_t s, const double *x, const double *y, uint64_t n, const QsPath &P,
                     uint32_t *key);
-// Depth d >= 1: route the points of the open depth-(d-1) nodes to their child
-// slots, then classify the slots (rd = the depth-d radius, off = the centre
-// offset of createQuadrants at depth d-1).
+// Depth d > QS_KEY_LEVELS: route the points of the open depth-(d-1) nodes to
+// their child slots, then classify the slots (rd = the depth-d radius, off =
+// the centre offset of createQuadrants at depth d-1).
 void launch_qs_level(hipStream_t s, const QsDev &D, int d, float rd, double off);
 
 // ---- stages (stages.hip) ----

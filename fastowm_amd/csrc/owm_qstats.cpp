@@ -77,13 +77,14 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         // layout: [x y] | st | cnt | remap | cen0 | cen1 | nopen | hist | ovf | novf flags | bbox
         const uint64_t w_xy = device_ptrs ? 0 : al256(n * 8);
         const uint64_t w_st = al256(std::max<uint64_t>(n, 1) * 4), w_sl = al256(slots * 4);
-        const uint64_t w_cen = al256(open_cap * 16), w_nopen = al256((QS_MAX_DEPTH + 1) * 4);
+        const uint64_t w_cen = al256(open_cap * 16), w_rng = al256(open_cap * 8);
+        const uint64_t w_nopen = al256((QS_MAX_DEPTH + 1) * 4);
         const uint64_t w_hist = al256(hist_words * 8), w_ovf = al256(ovf_cap * 8);
         const uint64_t w_bb = need_bbox ? al256(bbox_key_words(1) * 8) + 256 : 0;
         const int passes = 2 * QS_KEY_LEVELS / 8;
         const uint64_t w_sort = al256(sort_scratch_bytes(std::max<uint64_t>(n, 1), passes, 1, 0));
-        const uint64_t total = 2 * w_xy + w_st + 2 * w_sl + 2 * w_cen + w_nopen + w_hist + w_ovf +
-                               256 + w_sort + w_bb;
+        const uint64_t total = 2 * w_xy + w_st + 2 * w_sl + 2 * w_cen + 2 * w_rng + w_nopen +
+                               w_hist + w_ovf + 256 + w_sort + w_bb;
         HIP_TRY(hipStreamSynchronize(s), "sync before the statistics");
         if (ctx->qs_work.bytes < total) {
             if (ctx->budget && ctx->held_bytes() - ctx->qs_work.bytes + total > ctx->budget) {
@@ -114,6 +115,8 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         D.remap = (uint32_t *)take(w_sl);
         D.cen[0] = (double2 *)take(w_cen);
         D.cen[1] = (double2 *)take(w_cen);
+        D.rng[0] = (uint2 *)take(w_rng);
+        D.rng[1] = (uint2 *)take(w_rng);
         uint8_t *zero_from = w;  // nopen, hist, novf and flags start at zero
         D.nopen = (uint32_t *)take(w_nopen);
         D.hist = (unsigned long long *)take(w_hist);
@@ -178,7 +181,6 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         const double cx = box[0] + rx, cy = box[2] + ry;
 
         int pi = ctx->prof.begin(s, PH_QSTATS);
-        launch_fill_u32(s, D.st, n, 0u);
         launch_fill_u32(s, D.cnt, slots, 0u);
         launch_fill_u32(s, (uint32_t *)zero_from, (uint64_t)(zero_end - zero_from) / 4, 0u);
         launch_qs_root(s, D, cx, cy);
@@ -190,7 +192,9 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             P.off[k] = (double)(depth_radius(r0, k) * 0.5f);
         launch_qs_path(s, dx, dy, n, P, S.k0);
         launch_fill_u32(s, S.hist, (uint64_t)(sort_base + w_sort - (uint8_t *)S.hist) / 4, 0u);
-        ctx->prof.end(s, pi, 0, 0, 6);
+        ctx->prof.end(s, pi, 0, 0, 5);
+        D.key = S.k0;
+        D.perm = S.v0;
         if (n) {
             const int which = run_sort(ctx, PH_QSTATS, s, S, (uint32_t)n, nullptr, passes, false, true);
             D.key = which ? S.k1 : S.k0;
@@ -201,20 +205,33 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             set_err(ctx, "pinned host allocation failed");
             return OWM_E_NOMEM;
         }
-        // depths in groups of 6, then one readback of the open-node counts
+        // depths in groups of up to 6 (the key-range depths and the point
+        // depths apart), then one readback of the open-node counts
         int d = 1, last = 0;
         float r = r0;
         while (true) {
-            const int upto = std::min(QS_MAX_DEPTH, d + 5);
-            const int d0 = d;
+            int upto = std::min(QS_MAX_DEPTH, d + 5);
+            if (d <= QS_KEY_LEVELS && upto > QS_KEY_LEVELS)
+                upto = QS_KEY_LEVELS;
+            uint32_t launches = 0;
             pi = ctx->prof.begin(s, PH_QSTATS);
             for (; d <= upto; ++d) {
                 const float rd = (float)((double)r * 0.5);
                 const double off = (double)(r * 0.5f);  // createQuadrants' radius * 0.5f
-                launch_qs_level(s, D, d, rd, off);
+                if (d <= QS_KEY_LEVELS) {
+                    launch_qs_split(s, D, d, rd, off);
+                    launches += 1;
+                } else {
+                    if (d == QS_KEY_LEVELS + 1) {
+                        launch_qs_deep_start(s, D);
+                        launches += 2;
+                    }
+                    launch_qs_level(s, D, d, rd, off);
+                    launches += 2;
+                }
                 r = rd;
             }
-            ctx->prof.end(s, pi, 0, 0, 2 * (uint32_t)(upto - d0 + 1));
+            ctx->prof.end(s, pi, 0, 0, launches);
             HIP_TRY(hipMemcpyAsync(hn, D.nopen, 4 * (QS_MAX_DEPTH + 1), hipMemcpyDeviceToHost, s),
                     "open nodes D2H");
             HIP_TRY(hipMemcpyAsync(hn + QS_MAX_DEPTH + 1, D.novf, 8, hipMemcpyDeviceToHost, s),
@@ -259,25 +276,35 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             for (auto it = big.lower_bound({(uint32_t)dd, 0}); it != big.end() && it->first.first == (uint32_t)dd; ++it)
                 cls.push_back({it->first.first, it->first.second, it->second});
         }
-        uint64_t leaves = 0, settled = 0, routed = 0, slot_total = 0;
+        uint64_t leaves = 0, settled = 0, routed = 0;
         for (const auto &c : cls) {
             leaves += c.leaves;
             settled += (uint64_t)c.points * c.leaves;
-            routed += (uint64_t)c.points * c.leaves * c.level;  // each point routed once per depth
+            if (c.level > QS_KEY_LEVELS)  // routed once per depth below the key depths
+                routed += (uint64_t)c.points * c.leaves * (c.level - QS_KEY_LEVELS);
         }
         if (settled != n) {
             set_err(ctx, "leaves hold %llu points of %llu", (unsigned long long)settled,
                     (unsigned long long)n);
             return OWM_E_INTERNAL;
         }
-        for (int dd = 1; dd <= last; ++dd)
-            slot_total += 4ull * nopen[dd - 1];
         // algorithmic bytes besides the sort's own: the path keys (x, y read,
-        // key written); per depth every point's state (4); per routed point its
-        // key, its node (remap) and its new state (12); per child slot the
-        // count (read and cleared) and the remap (12); per open node a centre
-        ctx->prof.add_work(PH_QSTATS, 20 * n + 4 * n * (uint64_t)last + 12 * routed +
-                                           12 * slot_total + 16 * (slot_total / 4), n);
+        // key written); per open node of a key depth its range and centre read
+        // (24) and the keys at its three child boundaries (12), per new open
+        // node its range and centre written (24); below the key depths, the
+        // state fill and per depth every point's state (4 + 4), per routed
+        // point its coordinates, node and new state (24), per child slot its
+        // count (read and cleared) and remap (12), per new open node a centre
+        uint64_t alg = 20 * n;
+        for (int dd = 1; dd <= last; ++dd) {
+            if (dd <= QS_KEY_LEVELS)
+                alg += 36ull * nopen[dd - 1] + 24ull * nopen[dd];
+            else
+                alg += 4 * n + 12 * 4ull * nopen[dd - 1] + 16ull * nopen[dd];
+        }
+        if (last > QS_KEY_LEVELS)
+            alg += 4 * n + 24 * routed;
+        ctx->prof.add_work(PH_QSTATS, alg, n);
         memcpy(out->bbox, box, sizeof box);
         out->max_radius = r0;
         out->depth = (uint32_t)last;

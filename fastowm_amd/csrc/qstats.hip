@@ -7,18 +7,18 @@
 // (insertPointMaxNum, :160-184) a node is split exactly when more than
 // maxNumber points reach it, and under the minRadius policy (insertPoint,
 // :103-123) when at least one point reaches it and radius * 0.5 > minRadius.
-// So no pointer tree is built: depth by depth, every point still in a split
-// node is routed to its child (quadrantIdx, :91-100, against the child
-// centres of createQuadrants, :59-73), the children's points are counted, and
-// each child is either split (it gets an open-node index and a centre) or a
-// leaf (its (depth, points) pair is counted).  One u32 of state per point;
-// the points are first sorted by their path key (their quadrants at depths
-// 1..16).  Down to depth 16 a point's quadrant is read from its key, and as a
-// node's points are then contiguous, the first point of each child counts the
-// child's run by a galloping search and stores it: no atomics, which at the
-// top depths would all land on the same few counters.  Below depth 16 the
-// quadrant comes from the coordinates against the node's centre and runs of
-// equal children within a wave are added atomically, once per run.
+// So no pointer tree is built.  Every point gets a path key, its quadrants
+// (quadrantIdx, :91-100, against the child centres of createQuadrants,
+// :59-73) at depths 1..16, two bits a depth, and the keys are sorted with the
+// point index.  Down to depth 16 a node's points are then one contiguous key
+// range: depth by depth, each split ("open") node's range is cut into its four
+// children's by binary searches on the keys, and each child is split (it gets
+// an open-node index, a range and a centre) or a leaf (its (depth, points)
+// pair is counted).  That work scales with the nodes, not the points.  Nodes
+// still open at depth 16 (more than maxNumber points within 1/2^16 of the box)
+// go on point by point: each point in one holds its node, is routed to a child
+// against the node's centre, the children's points are counted with atomics
+// (runs of equal children within a wave add once) and classified as above.
 #include "owm_kernels.h"
 
 namespace owm {
@@ -47,13 +47,146 @@ __global__ __launch_bounds__(QS_TB) void k_qs_path(const double *__restrict__ x,
 
 __global__ void k_qs_root(QsDev D, double cx, double cy)
 {
-    D.remap[0] = 0;
     D.nopen[0] = 1;
     D.cen[0][0] = make_double2(cx, cy);
+    D.rng[0][0] = make_uint2(0u, (uint32_t)D.n);
 }
 
-// Depth d: each point of an open depth-(d-1) node moves to child slot
-// 4 * node + quadrant; points whose node became a leaf are marked settled.
+// Leaves of one block: counts below QS_HB in LDS bins, larger ones listed.
+__device__ __forceinline__ void qs_leaf(QsDev &D, uint32_t *s_hist, int d, uint32_t c)
+{
+    if (c < QS_HB) {
+        atomicAdd(&s_hist[c], 1u);
+    } else {
+        const uint32_t o = atomicAdd(D.novf, 1u);
+        if (o < D.ovf_cap)
+            D.ovf[o] = make_uint2((uint32_t)d, c);
+        else
+            atomicOr(D.flags, 1u);
+    }
+}
+
+__device__ __forceinline__ void qs_flush(QsDev &D, uint32_t *s_hist, int d)
+{
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < QS_HB; k += QS_TB)
+        if (s_hist[k])
+            atomicAdd(&D.hist[(uint64_t)d * QS_HB + k], (unsigned long long)s_hist[k]);
+}
+
+__device__ __forceinline__ bool qs_split_rule(const QsDev &D, bool forced, bool radius_split,
+                                              uint32_t c)
+{
+    return forced || (D.policy == 0 ? c > D.max_number : (c >= 1 && radius_split));
+}
+
+// Depth d <= QS_KEY_LEVELS: one thread per open depth-(d-1) node cuts its key
+// range into its children's and classifies them; new open nodes get their
+// indices from one atomic per block.
+__global__ __launch_bounds__(QS_TB) void k_qs_split(QsDev D, int d, float rd, double off)
+{
+    __shared__ uint32_t s_hist[QS_HB];
+    __shared__ uint32_t s_scan[QS_TB / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t nprev = min(D.nopen[d - 1], D.open_cap);
+    if (blockIdx.x * QS_TB >= nprev)
+        return;
+    for (uint32_t k = threadIdx.x; k < QS_HB; k += QS_TB)
+        s_hist[k] = 0;
+    const uint2 *pr = D.rng[(d - 1) & 1];
+    const double2 *pc = D.cen[(d - 1) & 1];
+    uint2 *nr = D.rng[d & 1];
+    double2 *nc = D.cen[d & 1];
+    const bool forced = d <= (int)D.level;
+    const bool radius_split = (double)rd * 0.5 > (double)D.min_radius;
+    const int sh = 2 * (QS_KEY_LEVELS - d);
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t base = blockIdx.x * QS_TB; base < nprev; base += gridDim.x * QS_TB) {
+        const uint32_t o = base + threadIdx.x;
+        uint32_t b[5] = {0, 0, 0, 0, 0};
+        uint32_t nopen = 0;
+        bool op[4] = {false, false, false, false};
+        if (o < nprev) {
+            const uint2 r = pr[o];
+            b[0] = r.x;
+            b[4] = r.y;
+            // keys in [r.x, r.y) share the depth-(d-1) prefix; their depth-d
+            // quadrant (key >> sh) & 3 ascends: b[q] = first with quadrant >= q
+            for (uint32_t q = 1; q < 4; ++q) {
+                uint32_t lo = b[q - 1], hi = r.y;
+                while (lo < hi) {
+                    const uint32_t mid = lo + (hi - lo) / 2;
+                    if (((D.key[mid] >> sh) & 3u) < q)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                b[q] = lo;
+            }
+            for (int q = 0; q < 4; ++q) {
+                op[q] = qs_split_rule(D, forced, radius_split, b[q + 1] - b[q]);
+                nopen += op[q] ? 1u : 0u;
+            }
+        }
+        // block-wide exclusive scan of the open children
+        uint32_t incl = nopen;
+#pragma unroll
+        for (int k = 1; k < 64; k <<= 1) {
+            const uint32_t v = __shfl_up(incl, k, 64);
+            if (lane >= (uint32_t)k)
+                incl += v;
+        }
+        __syncthreads();  // s_scan / s_base of the previous round are consumed
+        if (lane == 63)
+            s_scan[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < QS_TB / 64; ++w) {
+                const uint32_t v = s_scan[w];
+                s_scan[w] = t;
+                t += v;
+            }
+            s_base = t ? atomicAdd(&D.nopen[d], t) : 0u;
+        }
+        __syncthreads();
+        uint32_t idx = s_base + s_scan[wv] + incl - nopen;
+        if (o < nprev) {
+            const double2 p = pc[o];
+            for (int q = 0; q < 4; ++q) {
+                if (!op[q]) {
+                    qs_leaf(D, s_hist, d, b[q + 1] - b[q]);
+                    continue;
+                }
+                if (idx < D.open_cap) {
+                    nr[idx] = make_uint2(b[q], b[q + 1]);
+                    nc[idx] = make_double2(p.x + ((q & 2) ? off : -off), p.y + ((q & 1) ? off : -off));
+                } else {
+                    atomicOr(D.flags, 2u);
+                }
+                ++idx;
+            }
+        }
+    }
+    qs_flush(D, s_hist, d);
+}
+
+// Depth QS_KEY_LEVELS still has open nodes: every point of one holds its
+// index (st was filled with OWM_NONE), and remap is the identity on them.
+__global__ __launch_bounds__(QS_TB) void k_qs_deep_start(QsDev D)
+{
+    const uint32_t nopen = min(D.nopen[QS_KEY_LEVELS], D.open_cap);
+    const uint2 *r = D.rng[QS_KEY_LEVELS & 1];
+    for (uint32_t o = blockIdx.x * QS_TB + threadIdx.x; o < nopen; o += gridDim.x * QS_TB) {
+        D.remap[o] = o;
+        for (uint32_t i = r[o].x; i < r[o].y; ++i)
+            D.st[i] = o;
+    }
+}
+
+// Depth d > QS_KEY_LEVELS: each point of an open depth-(d-1) node moves to
+// child slot 4 * node + quadrant; points whose node became a leaf are marked
+// settled.  Points are in key order (perm gives each one's index).
 __global__ __launch_bounds__(QS_TB) void k_qs_route(QsDev D, int d)
 {
     const uint32_t nprev = min(D.nopen[d - 1], D.open_cap);
@@ -72,44 +205,13 @@ __global__ __launch_bounds__(QS_TB) void k_qs_route(QsDev D, int d)
                 if (node == OWM_NONE) {
                     D.st[i] = OWM_NONE;
                 } else {
-                    uint32_t q;
-                    if (D.key && d <= QS_KEY_LEVELS) {
-                        q = (D.key[i] >> (2 * (QS_KEY_LEVELS - d))) & 3u;
-                    } else {
-                        const uint64_t pi = D.perm ? D.perm[i] : i;
-                        const double2 c = cen[node];
-                        q = (D.x[pi] >= c.x ? 2u : 0u) | (D.y[pi] >= c.y ? 1u : 0u);
-                    }
+                    const uint64_t pi = D.perm ? D.perm[i] : i;
+                    const double2 c = cen[node];
+                    const uint32_t q = (D.x[pi] >= c.x ? 2u : 0u) | (D.y[pi] >= c.y ? 1u : 0u);
                     child = 4u * node + q;
                     D.st[i] = child;
                 }
             }
-        }
-        if (D.key && d <= QS_KEY_LEVELS) {
-            // the first point of a child's run (in key order) counts the run
-            if (child != OWM_NONE) {
-                const int sh = 2 * (QS_KEY_LEVELS - d);
-                const uint32_t pre = D.key[i] >> sh;
-                if (i == 0 || (D.key[i - 1] >> sh) != pre) {
-                    uint64_t lo = i, hi = i + 1, step = 1;  // key[lo] in the run; hi unknown
-                    while (hi < D.n && (D.key[hi] >> sh) == pre) {
-                        lo = hi;
-                        step <<= 1;
-                        hi = lo + step;
-                    }
-                    if (hi > D.n)
-                        hi = D.n;
-                    while (hi - lo > 1) {  // key[lo] in the run, key[hi] past it (or hi = n)
-                        const uint64_t mid = lo + (hi - lo) / 2;
-                        if ((D.key[mid] >> sh) == pre)
-                            lo = mid;
-                        else
-                            hi = mid;
-                    }
-                    D.cnt[child] = (uint32_t)(hi - i);
-                }
-            }
-            continue;
         }
         // a run of lanes with the same child adds its length once
         const uint32_t prev = __shfl_up(child, 1, 64);
@@ -147,7 +249,7 @@ __global__ __launch_bounds__(QS_TB) void k_qs_classify(QsDev D, int d, float rd,
         if (valid) {
             c = D.cnt[j];
             D.cnt[j] = 0;
-            open = forced || (D.policy == 0 ? c > D.max_number : (c >= 1 && radius_split));
+            open = qs_split_rule(D, forced, radius_split, c);
         }
         const uint64_t m = __ballot(valid && open);
         uint32_t first = 0;
@@ -171,21 +273,10 @@ __global__ __launch_bounds__(QS_TB) void k_qs_classify(QsDev D, int d, float rd,
             }
         } else {
             D.remap[j] = OWM_NONE;
-            if (c < QS_HB) {
-                atomicAdd(&s_hist[c], 1u);
-            } else {
-                const uint32_t o = atomicAdd(D.novf, 1u);
-                if (o < D.ovf_cap)
-                    D.ovf[o] = make_uint2((uint32_t)d, c);
-                else
-                    atomicOr(D.flags, 1u);
-            }
+            qs_leaf(D, s_hist, d, c);
         }
     }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < QS_HB; k += QS_TB)
-        if (s_hist[k])
-            atomicAdd(&D.hist[(uint64_t)d * QS_HB + k], (unsigned long long)s_hist[k]);
+    qs_flush(D, s_hist, d);
 }
 
 void launch_qs_root(hipStream_t s, const QsDev &D, double cx, double cy)
@@ -201,6 +292,21 @@ void launch_qs_path(hipStream_t s, const double *x, const double *y, uint64_t n,
     uint64_t b = (n + QS_TB - 1) / QS_TB;
     b = b > 16384 ? 16384 : b;
     hipLaunchKernelGGL(k_qs_path, dim3((unsigned)b), dim3(QS_TB), 0, s, x, y, n, P, key);
+}
+
+void launch_qs_split(hipStream_t s, const QsDev &D, int d, float rd, double off)
+{
+    uint64_t b = ((uint64_t)D.open_cap + QS_TB - 1) / QS_TB;
+    b = b < 1 ? 1 : (b > 4096 ? 4096 : b);
+    hipLaunchKernelGGL(k_qs_split, dim3((unsigned)b), dim3(QS_TB), 0, s, D, d, rd, off);
+}
+
+void launch_qs_deep_start(hipStream_t s, const QsDev &D)
+{
+    launch_fill_u32(s, D.st, D.n, OWM_NONE);
+    uint64_t b = ((uint64_t)D.open_cap + QS_TB - 1) / QS_TB;
+    b = b < 1 ? 1 : (b > 4096 ? 4096 : b);
+    hipLaunchKernelGGL(k_qs_deep_start, dim3((unsigned)b), dim3(QS_TB), 0, s, D);
 }
 
 void launch_qs_level(hipStream_t s, const QsDev &D, int d, float rd, double off)

@@ -154,3 +154,24 @@ def test_edge_cases(ctx):
     # the context still works after the errors
     x, y, _ = tile_for({"seed": 12}, 30_000).generate()
     both(ctx, x, y, synth.bbox(x, y), owm.QT_MINRADIUS, min_radius=0.5, level=1)
+
+
+def deep_cloud():
+    """2000 points over a 1 km box and a 12 x 12 lattice 0.1 mm apart: its
+    nodes stay split below depth 16 (nodes 1.5 cm wide there)."""
+    rng = np.random.default_rng(5)
+    g = np.arange(12) * 1e-4
+    X, Y = np.meshgrid(500.123 + g, 250.456 + g, indexing="ij")
+    x = np.concatenate([rng.uniform(0, 1000, 2000), X.ravel()])
+    y = np.concatenate([rng.uniform(0, 1000, 2000), Y.ravel()])
+    return x, y, (0.0, 1000.0, 0.0, 1000.0)
+
+
+def test_leaves_below_the_key_depths(ctx):
+    say("deep")
+    x, y, box = deep_cloud()
+    for maxnum, level in ((4, 0), (1, 3), (20, 5)):
+        got = both(ctx, x, y, box, owm.QT_MAXNUMBER, max_number=maxnum, level=level)
+        assert max(d for d, _, _ in got) > 16
+    got = both(ctx, x, y, box, owm.QT_MINRADIUS, min_radius=1e-5, level=2)
+    assert max(d for d, _, _ in got) > 16

@@ -215,3 +215,18 @@ def test_reports_text(tmp_path, policy, kw):
     want = expected_reports(str(path), classes, policy, kw.get("max_number", 32),
                             kw.get("min_radius", 0.1), box, len(x))
     assert path.read_text() == want
+
+
+def test_leaves_below_depth_16():
+    rng = np.random.default_rng(5)
+    g = np.arange(12) * 1e-4
+    X, Y = np.meshgrid(500.123 + g, 250.456 + g, indexing="ij")
+    x = np.concatenate([rng.uniform(0, 1000, 2000), X.ravel()])
+    y = np.concatenate([rng.uniform(0, 1000, 2000), Y.ravel()])
+    box = (0.0, 1000.0, 0.0, 1000.0)
+    for maxnum, level in ((4, 0), (1, 3)):
+        got = O.qtree_stats(x, y, box, O.QT_MAXNUMBER, max_number=maxnum, level=level)
+        assert got == level_sync(x, y, box, O.QT_MAXNUMBER, max_number=maxnum, level=level)
+        assert max(d for d, _, _ in got) > 16
+    got = O.qtree_stats(x, y, box, O.QT_MINRADIUS, min_radius=1e-5, level=2)
+    assert got == level_sync(x, y, box, O.QT_MINRADIUS, min_radius=1e-5, level=2)
