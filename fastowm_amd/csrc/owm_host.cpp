@@ -509,8 +509,9 @@ int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max
     }
     // algorithmic bytes per key and pass: 12 for the window minima (with the
     // run-length pass, 44 per window over three passes), 8 for the seed merge
-    // (keys only)
-    ctx->prof.end(s, pi, (phase == PH_SORT_WIN ? 12ull : 8ull) * n_max * passes,
+    // (keys only), 16 for key/value pairs (12 in a first pass with iota values)
+    const uint64_t per = phase == PH_SORT_WIN ? 12ull : (keys_only ? 8ull : 16ull);
+    ctx->prof.end(s, pi, per * n_max * passes - (iota && !keys_only ? 4ull * n_max : 0ull),
                   (uint64_t)n_max * passes, (uint32_t)passes);
     return (passes & 1) ? 1 : 0;
 }

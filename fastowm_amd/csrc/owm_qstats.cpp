@@ -193,12 +193,15 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         launch_qs_path(s, dx, dy, n, P, S.k0);
         launch_fill_u32(s, S.hist, (uint64_t)(sort_base + w_sort - (uint8_t *)S.hist) / 4, 0u);
         ctx->prof.end(s, pi, 0, 0, 5);
+        // the keys alone: which point sits where is needed only below the key
+        // depths, and then the sort is repeated with the index as value (LSD
+        // passes are stable, so the key order, hence every node's range, is
+        // the same)
         D.key = S.k0;
-        D.perm = S.v0;
+        D.perm = nullptr;
         if (n) {
-            const int which = run_sort(ctx, PH_QSTATS, s, S, (uint32_t)n, nullptr, passes, false, true);
+            const int which = run_sort(ctx, PH_QSTATS, s, S, (uint32_t)n, nullptr, passes, true, false);
             D.key = which ? S.k1 : S.k0;
-            D.perm = which ? S.v1 : S.v0;
         }
         uint32_t *hn = (uint32_t *)ctx->pinned(ctx->h_io, ctx->h_io_cap, 4 * (QS_MAX_DEPTH + 4));
         if (!hn) {
@@ -210,6 +213,17 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         int d = 1, last = 0;
         float r = r0;
         while (true) {
+            if (d == QS_KEY_LEVELS + 1) {
+                // nodes are open at the last key depth: sort again with values
+                pi = ctx->prof.begin(s, PH_QSTATS);
+                launch_qs_path(s, dx, dy, n, P, S.k0);
+                launch_fill_u32(s, S.hist, (uint64_t)(sort_base + w_sort - (uint8_t *)S.hist) / 4, 0u);
+                ctx->prof.end(s, pi, 0, 0, 2);
+                const int which = run_sort(ctx, PH_QSTATS, s, S, (uint32_t)n, nullptr, passes, false, true);
+                D.key = nullptr;  // not read below the key depths
+                D.perm = which ? S.v1 : S.v0;
+                ctx->prof.add_work(PH_QSTATS, 20 * n, 0);
+            }
             int upto = std::min(QS_MAX_DEPTH, d + 5);
             if (d <= QS_KEY_LEVELS && upto > QS_KEY_LEVELS)
                 upto = QS_KEY_LEVELS;
