@@ -159,9 +159,13 @@ struct GlobalWords {
     const unsigned long long *w;
     __device__ __forceinline__ unsigned long long load(uint64_t wi) const { return w[wi]; }
 };
-struct LdsWords {  // positions relative to the wave's copy
+constexpr uint32_t XY_WBUF = 4096;  // LDS bytes of text per wave
+struct LdsWords {  // positions relative to the wave's copy (zero past its buffer)
     const unsigned long long *w;
-    __device__ __forceinline__ unsigned long long load(uint32_t wi) const { return w[wi]; }
+    __device__ __forceinline__ unsigned long long load(uint32_t wi) const
+    {
+        return wi < XY_WBUF / 8 ? w[wi] : 0ull;
+    }
 };
 
 // Pos: the position type (64-bit in the whole text, 32-bit in an LDS copy)
@@ -178,7 +182,65 @@ struct TextCursor {
         }
         return (uint32_t)(word >> (8 * (uint32_t)(p & 7))) & 255u;
     }
+    // bytes p .. p+7, text[p] in the low byte
+    __device__ __forceinline__ unsigned long long word8(Pos p)
+    {
+        const uint32_t sh = 8 * (uint32_t)(p & 7);
+        const unsigned long long w0 = src.load(p >> 3);
+        if (sh == 0)
+            return w0;
+        return (w0 >> sh) | (src.load((p >> 3) + 1) << (64 - sh));
+    }
 };
+
+__constant__ unsigned long long XY_P10U[9] = {1ull,      10ull,      100ull,      1000ull,    10000ull,
+                                              100000ull, 1000000ull, 10000000ull, 100000000ull};
+
+// The digit run at p (up to e), eight bytes a step: a byte is a digit when
+// its high nibble is 3 and adding 6 keeps it 3 (a carry from a lower byte
+// only reaches bytes past the first non-digit); the run's digits, most
+// significant in the low byte, are shifted up so that the missing ones read
+// as leading zeros and converted with three multiplies (pairs, quads, eight:
+// the method of Lemire's and simdjson's eight-digit parsing).  m and sig
+// follow the byte loop exactly: digits before the first non-zero one are not
+// significant, and more than 19 significant digits return false.
+template <typename Cur, typename Pos>
+__device__ __forceinline__ bool xyz_digits(Cur &T, Pos &p, Pos e, unsigned long long &m, int &sig,
+                                           int &count)
+{
+    while (p < e) {
+        const unsigned long long w = T.word8(p);
+        const unsigned long long bad = ((w & 0xF0F0F0F0F0F0F0F0ull) ^ 0x3030303030303030ull) |
+                                       (((w + 0x0606060606060606ull) & 0xF0F0F0F0F0F0F0F0ull) ^
+                                        0x3030303030303030ull);
+        uint32_t n = bad ? (uint32_t)__builtin_ctzll(bad) >> 3 : 8u;
+        if ((Pos)n > e - p)
+            n = (uint32_t)(e - p);
+        if (n == 0)
+            break;
+        const unsigned long long x = w - 0x3030303030303030ull;  // digit values in bytes < n
+        const unsigned long long low = n == 8 ? ~0ull : (1ull << (8 * n)) - 1ull;
+        if (m == 0) {
+            const unsigned long long nz = (x + 0x7F7F7F7F7F7F7F7Full) & 0x8080808080808080ull & low;
+            if (nz)
+                sig += (int)n - (int)(__builtin_ctzll(nz) >> 3);
+        } else {
+            sig += (int)n;
+        }
+        if (sig > 19)
+            return false;
+        unsigned long long v = (x & low) << (8 * (8 - n));
+        v = (v * 2561ull) >> 8;
+        v = ((v & 0x00FF00FF00FF00FFull) * 6553601ull) >> 16;
+        v = ((v & 0x0000FFFF0000FFFFull) * 42949672960001ull) >> 32;
+        m = m * XY_P10U[n] + v;
+        count += (int)n;
+        p += n;
+        if (n < 8)
+            break;
+    }
+    return true;
+}
 
 __device__ __forceinline__ bool xyz_space(uint32_t c)
 {
@@ -202,27 +264,13 @@ __device__ __forceinline__ bool xyz_number(Cur &T, Pos &p, Pos e, double &out)
     }
     unsigned long long m = 0;
     int sig = 0, frac = 0, digits = 0;
-    while (p < e && (c = T.at(p)) >= '0' && c <= '9') {
-        if (m != 0 || c != '0') {
-            if (++sig > 19)
-                return false;
-            m = m * 10 + (c - '0');
-        }
-        ++digits;
-        ++p;
-    }
+    if (!xyz_digits(T, p, e, m, sig, digits))
+        return false;
     if (p < e && T.at(p) == '.') {
         ++p;
-        while (p < e && (c = T.at(p)) >= '0' && c <= '9') {
-            if (m != 0 || c != '0') {
-                if (++sig > 19)
-                    return false;
-                m = m * 10 + (c - '0');
-            }
-            ++frac;
-            ++digits;
-            ++p;
-        }
+        if (!xyz_digits(T, p, e, m, sig, frac))
+            return false;
+        digits += frac;
     }
     if (digits == 0)
         return false;
@@ -302,8 +350,6 @@ __device__ __forceinline__ void xyz_line(Src src, uint64_t i, uint64_t s, Pos ps
     o.y[i] = v[1];
     o.z[i] = v[2];
 }
-
-constexpr uint32_t XY_WBUF = 4096;  // LDS bytes of text per wave
 
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
 {
