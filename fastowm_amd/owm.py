@@ -111,6 +111,26 @@ def qtree_params(policy=QT_MAXNUMBER, max_number=32, min_radius=0.1, level=5, bb
     return q
 
 
+def qtree_histograms(classes, summary):
+    """The statistics tool's four histograms from the leaf classes, as its
+    makeHistogram calls build them (CPP-TBB-CPU/include/stats_func.hpp:58-118,
+    application/owm_stats.cpp:98-133): (value, leaves) pairs in ascending
+    value order for points per leaf, leaf radius (the root's float radius
+    halved per depth), leaf density (points / (4 r r) in float arithmetic, as
+    getDensity) and leaf depth."""
+    r0 = np.float32(summary["max_radius"])
+    pts, rad, den, lev = {}, {}, {}, {}
+    for d, c, k in classes:
+        r = r0
+        for _ in range(d):
+            r = np.float32(float(r) * 0.5)
+        dens = float(np.float32(c) / (np.float32(4) * r * r))
+        for h, v in ((pts, c), (rad, float(r)), (den, dens), (lev, d)):
+            h[v] = h.get(v, 0) + k
+    return {name: sorted(h.items()) for name, h in
+            (("points", pts), ("radius", rad), ("density", den), ("levels", lev))}
+
+
 def write_qtree_stats(path, classes, summary, params, n_points):
     """Appends the statistics tool's four reports (owm_qtree_stats_write);
     classes and summary as Context.qtree_stats returns them."""

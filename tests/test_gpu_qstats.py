@@ -175,3 +175,17 @@ def test_leaves_below_the_key_depths(ctx):
         assert max(d for d, _, _ in got) > 16
     got = both(ctx, x, y, box, owm.QT_MINRADIUS, min_radius=1e-5, level=2)
     assert max(d for d, _, _ in got) > 16
+
+
+def test_non_finite_coordinates(ctx):
+    say("non-finite")
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 100, 3000)
+    y = rng.uniform(0, 100, 3000)
+    x[:5] = np.nan
+    y[5:9] = np.inf
+    x[9:12] = -np.inf
+    y[12] = np.nan
+    box = (0.0, 100.0, 0.0, 100.0)
+    both(ctx, x, y, box, owm.QT_MAXNUMBER, max_number=8, level=2)
+    both(ctx, x, y, box, owm.QT_MINRADIUS, min_radius=0.5, level=2)

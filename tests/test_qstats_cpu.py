@@ -230,3 +230,50 @@ def test_leaves_below_depth_16():
         assert max(d for d, _, _ in got) > 16
     got = O.qtree_stats(x, y, box, O.QT_MINRADIUS, min_radius=1e-5, level=2)
     assert got == level_sync(x, y, box, O.QT_MINRADIUS, min_radius=1e-5, level=2)
+
+
+def test_non_finite_coordinates():
+    # quadrantIdx compares with >=: NaN goes to quadrant 0 at every depth,
+    # +inf to the upper and -inf to the lower half
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 100, 3000)
+    y = rng.uniform(0, 100, 3000)
+    x[:5] = np.nan
+    y[5:9] = np.inf
+    x[9:12] = -np.inf
+    y[12] = np.nan
+    box = (0.0, 100.0, 0.0, 100.0)
+    for pol, kw in ((O.QT_MAXNUMBER, {"max_number": 8}), (O.QT_MINRADIUS, {"min_radius": 0.5})):
+        got = O.qtree_stats(x, y, box, pol, level=2, **kw)
+        assert got == level_sync(x, y, box, pol, level=2, **kw)
+        assert sum(c * k for _, c, k in got) == len(x)
+
+
+def test_histograms_match_the_written_reports(tmp_path):
+    from fastowm_amd import owm, synth
+    x, y, _ = tile_for({"seed": 6}, 30_000).generate()
+    box = synth.bbox(x, y)
+    classes = O.qtree_stats(x, y, box, O.QT_MINRADIUS, min_radius=0.2, level=4)
+    rx, ry = (box[1] - box[0]) / 2.0, (box[3] - box[2]) / 2.0
+    summary = {"bbox": box, "max_radius": float(np.float32(rx if rx >= ry else ry)),
+               "depth": 0, "leaves": 0}
+    h = owm.qtree_histograms(classes, summary)
+    path = tmp_path / "h.csv"
+    owm.write_qtree_stats(path, classes, summary, owm.qtree_params(O.QT_MINRADIUS, min_radius=0.2,
+                                                                   level=4, bbox=box), len(x))
+    # each report's rows, after its column header line and a blank line
+    text = path.read_text().split("\n")
+    tables = {}
+    for head, name in (("x_i n_i x_i*n_i", "points"), ("r_i n_i a_i", "radius"),
+                       ("d_i n_i", "density"), ("level_i n_i", "levels")):
+        i = text.index(head) + 2
+        rows = []
+        while text[i]:
+            rows.append(text[i].split())
+            i += 1
+        tables[name] = rows
+    assert [[str(v), str(k)] for v, k in h["points"]] == [r[:2] for r in tables["points"]]
+    assert [["%g" % v, str(k)] for v, k in h["radius"]] == [r[:2] for r in tables["radius"]]
+    assert [["%g" % v, str(k)] for v, k in h["density"]] == tables["density"]
+    assert [[str(v), str(k)] for v, k in h["levels"]] == tables["levels"]
+    assert sum(k for _, k in h["levels"]) == sum(k for _, _, k in classes)
