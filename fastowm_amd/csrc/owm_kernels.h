@@ -130,7 +130,10 @@ void launch_xyz_compact(hipStream_t s, const double *x, const double *y, const d
 // 4 * open + quadrant at depth d.
 constexpr int QS_MAX_DEPTH = 64;
 constexpr uint32_t QS_HB = 1024;  // exact bins of leaf point counts per depth
-constexpr int QS_KEY_LEVELS = 16; // quadrants of depths 1..16 in a point's 32-bit path key
+// A point's 32-bit path key holds its quadrants at depths 1..16: the sorted
+// depths 1..S (S = QsDev.sort_levels, 12 or 16) in bits 0..2S-1 (depth 1
+// highest), depths S+1..16 in the bits above (depth S+1 highest).
+constexpr int QS_KEY_LEVELS = 16;
 // Root centre and, per depth k < QS_KEY_LEVELS, the offset from a depth-k
 // centre to its children's (createQuadrants: radius * 0.5f, as a double).
 struct QsPath {
@@ -143,8 +146,8 @@ struct QsDev {
     // points in path-key order: key (null: none) and the index of the point
     // at each position (null: identity)
     const uint32_t *key, *perm;
-    uint2 *rng[2];    // open-node key ranges [first, end) down to depth QS_KEY_LEVELS, by parity
-    uint32_t *st;     // below QS_KEY_LEVELS, per point: child slot at the last routed depth (OWM_NONE: in a leaf)
+    uint2 *rng[2];    // open-node key ranges [first, end) down to depth sort_levels, by parity
+    uint32_t *st;     // below sort_levels, per point: child slot at the last routed depth (OWM_NONE: in a leaf)
     uint32_t *cnt;    // per child slot: points routed into it (zero outside its classify)
     uint32_t *remap;  // per child slot: open-node index at its depth, OWM_NONE for a leaf
     double2 *cen[2];  // open-node centres, by depth parity
@@ -155,24 +158,26 @@ struct QsDev {
     uint32_t *flags;  // bit 0: ovf full, bit 1: more open nodes than open_cap
     uint32_t ovf_cap, open_cap;
     uint32_t policy;  // 0 maxNumber, 1 minRadius
+    int sort_levels;  // depths sorted in the keys (12 or 16)
     uint32_t max_number, level;
     float min_radius;
 };
 // Root at depth 0: open node 0 with every point's key range.  cnt, nopen,
 // hist, novf and flags must be zero.
 void launch_qs_root(hipStream_t s, const QsDev &D, double cx, double cy);
-// Depth d <= QS_KEY_LEVELS from the open depth-(d-1) nodes' key ranges (key
-// sorted); rd, off as for launch_qs_level.
+// Depth d <= sort_levels from the open depth-(d-1) nodes' key ranges (keys
+// sorted by their sorted depths); rd, off as for launch_qs_level.
 void launch_qs_split(hipStream_t s, const QsDev &D, int d, float rd, double off);
-// Hands the points of the nodes open at depth QS_KEY_LEVELS to launch_qs_level.
+// Hands the points of the nodes open at depth sort_levels to launch_qs_level.
 void launch_qs_deep_start(hipStream_t s, const QsDev &D);
-// key[i]: the quadrants point i takes at depths 1..16, two bits each, depth 1
-// highest (quadrantIdx against the createQuadrants centres).
+// key[i]: the quadrants point i takes at depths 1..16 (quadrantIdx against
+// the createQuadrants centres), laid out as above.
 void launch_qs_path(hipStream_t s, const double *x, const double *y, uint64_t n, const QsPath &P,
-                    uint32_t *key);
-// Depth d > QS_KEY_LEVELS: route the points of the open depth-(d-1) nodes to
-// their child slots, then classify the slots (rd = the depth-d radius, off =
-// the centre offset of createQuadrants at depth d-1).
+                    int sort_levels, uint32_t *key);
+// Depth d > sort_levels: route the points of the open depth-(d-1) nodes to
+// their child slots (by their key down to depth 16, by their coordinates
+// below), then classify the slots (rd = the depth-d radius, off = the centre
+// offset of createQuadrants at depth d-1).
 void launch_qs_level(hipStream_t s, const QsDev &D, int d, float rd, double off);
 
 // ---- stages (stages.hip) ----

@@ -81,8 +81,8 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         const uint64_t w_nopen = al256((QS_MAX_DEPTH + 1) * 4);
         const uint64_t w_hist = al256(hist_words * 8), w_ovf = al256(ovf_cap * 8);
         const uint64_t w_bb = need_bbox ? al256(bbox_key_words(1) * 8) + 256 : 0;
-        const int passes = 2 * QS_KEY_LEVELS / 8;
-        const uint64_t w_sort = al256(sort_scratch_bytes(std::max<uint64_t>(n, 1), passes, 1, 0));
+        // sort scratch for up to 16 sorted depths (4 passes of one key byte)
+        const uint64_t w_sort = al256(sort_scratch_bytes(std::max<uint64_t>(n, 1), 4, 1, 0));
         const uint64_t total = 2 * w_xy + w_st + 2 * w_sl + 2 * w_cen + 2 * w_rng + w_nopen +
                                w_hist + w_ovf + 256 + w_sort + w_bb;
         HIP_TRY(hipStreamSynchronize(s), "sync before the statistics");
@@ -125,7 +125,7 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         D.flags = D.novf + 1;
         uint8_t *const zero_end = w;
         uint8_t *const sort_base = take(w_sort);
-        SortBufs S = carve_sort(sort_base, std::max<uint64_t>(n, 1), passes);
+        SortBufs S = carve_sort(sort_base, std::max<uint64_t>(n, 1), 4);
         D.ovf_cap = (uint32_t)ovf_cap;
         D.open_cap = (uint32_t)open_cap;
         D.policy = p->policy;
@@ -179,6 +179,23 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         const double rx = (box[1] - box[0]) / 2.0, ry = (box[3] - box[2]) / 2.0;
         const float r0 = (float)(rx >= ry ? rx : ry);
         const double cx = box[0] + rx, cy = box[2] + ry;
+        // depths sorted in the keys: 12, or all 16 when leaves are expected
+        // below depth 12 (minRadius: every occupied node splits down to the
+        // radius limit, known here; maxNumber: about log4(n / maxNumber) + 1,
+        // as for uniform points).  Depths past the sorted ones go point by point.
+        int sl = 12;
+        if (p->policy == OWM_QT_MINRADIUS) {
+            int dleaf = 0;  // occupied nodes split at depths 0 .. dleaf-1
+            for (float rr = r0; dleaf < QS_MAX_DEPTH && (double)rr * 0.5 > (double)p->min_radius;
+                 rr = (float)((double)rr * 0.5))
+                ++dleaf;
+            sl = dleaf > 12 ? 16 : 12;
+        } else if (n > 0) {
+            const double est = std::log((double)n / ((double)p->max_number + 1.0)) / std::log(4.0) + 1.0;
+            sl = est > 12.0 ? 16 : 12;
+        }
+        D.sort_levels = sl;
+        const int passes = sl / 4;
 
         int pi = ctx->prof.begin(s, PH_QSTATS);
         launch_fill_u32(s, D.cnt, slots, 0u);
@@ -190,7 +207,7 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         P.cy = cy;
         for (int k = 0; k < QS_KEY_LEVELS; ++k)
             P.off[k] = (double)(depth_radius(r0, k) * 0.5f);
-        launch_qs_path(s, dx, dy, n, P, S.k0);
+        launch_qs_path(s, dx, dy, n, P, sl, S.k0);
         launch_fill_u32(s, S.hist, (uint64_t)(sort_base + w_sort - (uint8_t *)S.hist) / 4, 0u);
         ctx->prof.end(s, pi, 0, 0, 5);
         // the keys alone: which point sits where is needed only below the key
@@ -216,7 +233,7 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             if (d == QS_KEY_LEVELS + 1) {
                 // nodes are open at the last key depth: sort again with values
                 pi = ctx->prof.begin(s, PH_QSTATS);
-                launch_qs_path(s, dx, dy, n, P, S.k0);
+                launch_qs_path(s, dx, dy, n, P, sl, S.k0);
                 launch_fill_u32(s, S.hist, (uint64_t)(sort_base + w_sort - (uint8_t *)S.hist) / 4, 0u);
                 ctx->prof.end(s, pi, 0, 0, 2);
                 const int which = run_sort(ctx, PH_QSTATS, s, S, (uint32_t)n, nullptr, passes, false, true);
@@ -225,6 +242,8 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
                 ctx->prof.add_work(PH_QSTATS, 20 * n, 0);
             }
             int upto = std::min(QS_MAX_DEPTH, d + 5);
+            if (d <= sl && upto > sl)
+                upto = sl;
             if (d <= QS_KEY_LEVELS && upto > QS_KEY_LEVELS)
                 upto = QS_KEY_LEVELS;
             uint32_t launches = 0;
@@ -232,11 +251,11 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             for (; d <= upto; ++d) {
                 const float rd = (float)((double)r * 0.5);
                 const double off = (double)(r * 0.5f);  // createQuadrants' radius * 0.5f
-                if (d <= QS_KEY_LEVELS) {
+                if (d <= sl) {
                     launch_qs_split(s, D, d, rd, off);
                     launches += 1;
                 } else {
-                    if (d == QS_KEY_LEVELS + 1) {
+                    if (d == sl + 1) {
                         launch_qs_deep_start(s, D);
                         launches += 2;
                     }
@@ -294,8 +313,8 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         for (const auto &c : cls) {
             leaves += c.leaves;
             settled += (uint64_t)c.points * c.leaves;
-            if (c.level > QS_KEY_LEVELS)  // routed once per depth below the key depths
-                routed += (uint64_t)c.points * c.leaves * (c.level - QS_KEY_LEVELS);
+            if (c.level > sl)  // routed once per depth below the sorted depths
+                routed += (uint64_t)c.points * c.leaves * (c.level - sl);
         }
         if (settled != n) {
             set_err(ctx, "leaves hold %llu points of %llu", (unsigned long long)settled,
@@ -303,21 +322,22 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             return OWM_E_INTERNAL;
         }
         // algorithmic bytes besides the sort's own: the path keys (x, y read,
-        // key written); per open node of a key depth its range and centre read
-        // (24) and the keys at its three child boundaries (12), per new open
-        // node its range and centre written (24); below the key depths, the
-        // state fill and per depth every point's state (4 + 4), per routed
-        // point its coordinates, node and new state (24), per child slot its
-        // count (read and cleared) and remap (12), per new open node a centre
+        // key written); per open node of a sorted depth its range and centre
+        // read (24) and the keys at its three child boundaries (12), per new
+        // open node its range and centre written (24); below the sorted
+        // depths, the state fill and per depth every point's state (4 + 4),
+        // per routed point its key or coordinates, node and new state (at
+        // least 12), per child slot its count (read and cleared) and remap
+        // (12), per new open node a centre
         uint64_t alg = 20 * n;
         for (int dd = 1; dd <= last; ++dd) {
-            if (dd <= QS_KEY_LEVELS)
+            if (dd <= sl)
                 alg += 36ull * nopen[dd - 1] + 24ull * nopen[dd];
             else
                 alg += 4 * n + 12 * 4ull * nopen[dd - 1] + 16ull * nopen[dd];
         }
-        if (last > QS_KEY_LEVELS)
-            alg += 4 * n + 24 * routed;
+        if (last > sl)
+            alg += 4 * n + 12 * routed;
         ctx->prof.add_work(PH_QSTATS, alg, n);
         memcpy(out->bbox, box, sizeof box);
         out->max_radius = r0;

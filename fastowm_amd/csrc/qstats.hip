@@ -9,16 +9,19 @@
 // :103-123) when at least one point reaches it and radius * 0.5 > minRadius.
 // So no pointer tree is built.  Every point gets a path key, its quadrants
 // (quadrantIdx, :91-100, against the child centres of createQuadrants,
-// :59-73) at depths 1..16, two bits a depth, and the keys are sorted with the
-// point index.  Down to depth 16 a node's points are then one contiguous key
-// range: depth by depth, each split ("open") node's range is cut into its four
-// children's by binary searches on the keys, and each child is split (it gets
-// an open-node index, a range and a centre) or a leaf (its (depth, points)
-// pair is counted).  That work scales with the nodes, not the points.  Nodes
-// still open at depth 16 (more than maxNumber points within 1/2^16 of the box)
-// go on point by point: each point in one holds its node, is routed to a child
-// against the node's centre, the children's points are counted with atomics
-// (runs of equal children within a wave add once) and classified as above.
+// :59-73) at depths 1..16, two bits a depth, and the keys are sorted by the
+// quadrants of depths 1..S (S = 12, or 16 when the tree is expected to go
+// deeper than 12).  Down to depth S a node's points are then one
+// contiguous key range: depth by depth, each split ("open") node's range is
+// cut into its four children's by binary searches on the keys, and each
+// child is split (it gets an open-node index, a range and a centre) or a leaf
+// (its (depth, points) pair is counted).  That work scales with the nodes,
+// not the points.  Nodes still open at depth S go on point by point: each
+// point in one holds its node and is routed to a child by its key's next
+// quadrant (below depth 16 by its coordinates against the node's centre, the
+// index of the point at each key position then coming from a second sort with
+// values), the children's points are counted with atomics (runs of equal
+// children within a wave add once) and classified as above.
 #include "owm_kernels.h"
 
 namespace owm {
@@ -27,7 +30,7 @@ constexpr int QS_TB = 256;
 
 __global__ __launch_bounds__(QS_TB) void k_qs_path(const double *__restrict__ x,
                                                     const double *__restrict__ y, uint64_t n,
-                                                    QsPath P, uint32_t *__restrict__ key)
+                                                    QsPath P, int sl, uint32_t *__restrict__ key)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * QS_TB + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * QS_TB) {
@@ -41,7 +44,8 @@ __global__ __launch_bounds__(QS_TB) void k_qs_path(const double *__restrict__ x,
             cx += qx ? P.off[l] : -P.off[l];
             cy += qy ? P.off[l] : -P.off[l];
         }
-        key[i] = k;
+        // depths 1..sl to bits 0..2sl-1, the deeper ones above
+        key[i] = sl >= QS_KEY_LEVELS ? k : (k >> (2 * (QS_KEY_LEVELS - sl))) | (k << (2 * sl));
     }
 }
 
@@ -80,7 +84,13 @@ __device__ __forceinline__ bool qs_split_rule(const QsDev &D, bool forced, bool 
     return forced || (D.policy == 0 ? c > D.max_number : (c >= 1 && radius_split));
 }
 
-// Depth d <= QS_KEY_LEVELS: one thread per open depth-(d-1) node cuts its key
+// The quadrant a path key gives at depth d (1..16), sl depths sorted.
+__device__ __forceinline__ uint32_t qs_quadrant(uint32_t key, int d, int sl)
+{
+    return d <= sl ? (key >> (2 * (sl - d))) & 3u : (key >> (2 * sl + 2 * (QS_KEY_LEVELS - d))) & 3u;
+}
+
+// Depth d <= sort_levels: one thread per open depth-(d-1) node cuts its key
 // range into its children's and classifies them; new open nodes get their
 // indices from one atomic per block.
 __global__ __launch_bounds__(QS_TB) void k_qs_split(QsDev D, int d, float rd, double off)
@@ -99,7 +109,7 @@ __global__ __launch_bounds__(QS_TB) void k_qs_split(QsDev D, int d, float rd, do
     double2 *nc = D.cen[d & 1];
     const bool forced = d <= (int)D.level;
     const bool radius_split = (double)rd * 0.5 > (double)D.min_radius;
-    const int sh = 2 * (QS_KEY_LEVELS - d);
+    const int sh = 2 * (D.sort_levels - d);
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (uint32_t base = blockIdx.x * QS_TB; base < nprev; base += gridDim.x * QS_TB) {
         const uint32_t o = base + threadIdx.x;
@@ -171,12 +181,12 @@ __global__ __launch_bounds__(QS_TB) void k_qs_split(QsDev D, int d, float rd, do
     qs_flush(D, s_hist, d);
 }
 
-// Depth QS_KEY_LEVELS still has open nodes: every point of one holds its
+// Depth sort_levels still has open nodes: every point of one holds its
 // index (st was filled with OWM_NONE), and remap is the identity on them.
 __global__ __launch_bounds__(QS_TB) void k_qs_deep_start(QsDev D)
 {
-    const uint32_t nopen = min(D.nopen[QS_KEY_LEVELS], D.open_cap);
-    const uint2 *r = D.rng[QS_KEY_LEVELS & 1];
+    const uint32_t nopen = min(D.nopen[D.sort_levels], D.open_cap);
+    const uint2 *r = D.rng[D.sort_levels & 1];
     for (uint32_t o = blockIdx.x * QS_TB + threadIdx.x; o < nopen; o += gridDim.x * QS_TB) {
         D.remap[o] = o;
         for (uint32_t i = r[o].x; i < r[o].y; ++i)
@@ -184,9 +194,10 @@ __global__ __launch_bounds__(QS_TB) void k_qs_deep_start(QsDev D)
     }
 }
 
-// Depth d > QS_KEY_LEVELS: each point of an open depth-(d-1) node moves to
+// Depth d > sort_levels: each point of an open depth-(d-1) node moves to
 // child slot 4 * node + quadrant; points whose node became a leaf are marked
-// settled.  Points are in key order (perm gives each one's index).
+// settled.  Points are in key order; below depth 16, perm gives each one's
+// index.
 __global__ __launch_bounds__(QS_TB) void k_qs_route(QsDev D, int d)
 {
     const uint32_t nprev = min(D.nopen[d - 1], D.open_cap);
@@ -205,9 +216,14 @@ __global__ __launch_bounds__(QS_TB) void k_qs_route(QsDev D, int d)
                 if (node == OWM_NONE) {
                     D.st[i] = OWM_NONE;
                 } else {
-                    const uint64_t pi = D.perm ? D.perm[i] : i;
-                    const double2 c = cen[node];
-                    const uint32_t q = (D.x[pi] >= c.x ? 2u : 0u) | (D.y[pi] >= c.y ? 1u : 0u);
+                    uint32_t q;
+                    if (D.key && d <= QS_KEY_LEVELS) {
+                        q = qs_quadrant(D.key[i], d, D.sort_levels);
+                    } else {
+                        const uint64_t pi = D.perm ? D.perm[i] : i;
+                        const double2 c = cen[node];
+                        q = (D.x[pi] >= c.x ? 2u : 0u) | (D.y[pi] >= c.y ? 1u : 0u);
+                    }
                     child = 4u * node + q;
                     D.st[i] = child;
                 }
@@ -285,13 +301,13 @@ void launch_qs_root(hipStream_t s, const QsDev &D, double cx, double cy)
 }
 
 void launch_qs_path(hipStream_t s, const double *x, const double *y, uint64_t n, const QsPath &P,
-                    uint32_t *key)
+                    int sort_levels, uint32_t *key)
 {
     if (n == 0)
         return;
     uint64_t b = (n + QS_TB - 1) / QS_TB;
     b = b > 16384 ? 16384 : b;
-    hipLaunchKernelGGL(k_qs_path, dim3((unsigned)b), dim3(QS_TB), 0, s, x, y, n, P, key);
+    hipLaunchKernelGGL(k_qs_path, dim3((unsigned)b), dim3(QS_TB), 0, s, x, y, n, P, sort_levels, key);
 }
 
 void launch_qs_split(hipStream_t s, const QsDev &D, int d, float rd, double off)
