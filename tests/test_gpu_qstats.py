@@ -189,3 +189,27 @@ def test_non_finite_coordinates(ctx):
     box = (0.0, 100.0, 0.0, 100.0)
     both(ctx, x, y, box, owm.QT_MAXNUMBER, max_number=8, level=2)
     both(ctx, x, y, box, owm.QT_MINRADIUS, min_radius=0.5, level=2)
+
+
+def test_full_size_invariants(ctx):
+    """C2 (20M points): properties that hold at any size -- every point in one
+    leaf, leaves = 1 + 3 x split nodes, counts within maxNumber, no leaf above
+    the forced depths -- and the same classes for the points in another order."""
+    import torch
+    say("c2 invariants")
+    x, y, _ = synth.config_c2(0).generate()
+    box = synth.bbox(x, y)
+    tx, ty = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    got, sm = ctx.qtree_stats(tx, ty, policy=owm.QT_MAXNUMBER, max_number=32, level=5, bbox=box)
+    leaves = sum(k for _, _, k in got)
+    assert sum(c * k for _, c, k in got) == len(x)
+    assert (leaves - 1) % 3 == 0 and leaves == sm["leaves"]
+    assert all(c <= 32 for _, c, _ in got) and min(d for d, _, _ in got) >= 6
+    perm = torch.randperm(len(x), generator=torch.Generator().manual_seed(1)).cuda()
+    again, _ = ctx.qtree_stats(tx[perm].contiguous(), ty[perm].contiguous(),
+                               policy=owm.QT_MAXNUMBER, max_number=32, level=5, bbox=box)
+    assert again == got
+    got, _ = ctx.qtree_stats(tx, ty, policy=owm.QT_MINRADIUS, min_radius=0.1, level=5, bbox=box)
+    assert sum(c * k for _, c, k in got) == len(x)
+    # minRadius 0.1 on a 2 km box: every occupied node splits down to depth 13
+    assert {d for d, c, _ in got if c > 0} == {13}
