@@ -52,7 +52,7 @@ def test_fixtures_cover_stage3_and_overlap0():
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "..", "oracle", "_ref", "o1qtree")),
-                    reason="reference binary not built here (needs /root/reference)")
+                    reason="reference binary not built here (the reference sources are absent)")
 def test_live_reference_run_matches_fixture():
     import sys
     sys.path.insert(0, os.path.join(HERE, "golden"))
