@@ -134,3 +134,60 @@ def test_gold_check_equals_the_reference_rule(seed, displace):
     got = owm.gold_check((sx, sy, sz), (gx, gy, gz), displace)
     assert got[0] == want[0] and got[1] == pytest.approx(want[1], rel=0, abs=1e-12)
     assert 0 < got[0] < len(sx)
+
+
+C_CALLER = r"""
+#include <stdio.h>
+#include <string.h>
+#include "owm.h"
+
+int main(int argc, char **argv)
+{
+    /* the driver's parameter derivation, as a C caller sizes its buffers */
+    owm_params p;
+    memset(&p, 0, sizeof p);
+    p.wsize = 12; p.bsize = 20; p.overlap = 0.8; p.leaf_size = 32;
+    double box[4] = {0.0, 500.0, 0.0, 500.0};
+    owm_derived d;
+    if (owm_derive(&p, 1000000, box, &d) != OWM_OK) return 1;
+    printf("%u %u %u %u %u\n", d.min_num_points, d.ncols, d.nrows, d.ncolsg, d.nrowsg);
+    /* the statistics tool's reports from a histogram: two leaves of 3 points
+       at depth 1, two empty ones */
+    owm_qtree_params q;
+    memset(&q, 0, sizeof q);
+    q.policy = OWM_QT_MAXNUMBER; q.max_number = 4; q.level = 0;
+    owm_qtree_summary s;
+    memset(&s, 0, sizeof s);
+    memcpy(s.bbox, box, sizeof box);
+    s.max_radius = 250.0f; s.depth = 1; s.leaves = 4; s.n_bins = 2;
+    owm_leaf_bin bins[2] = {{1, 0, 2}, {1, 3, 2}};
+    if (owm_qtree_stats_write(argv[1], &q, &s, bins, 2, 6) != OWM_OK) return 2;
+    printf("%s\n", owm_status_str(OWM_E_CAPACITY));
+    return 0;
+}
+"""
+
+
+def test_plain_c_caller_builds_and_runs(tmp_path):
+    """The header compiles as C and a C program links the library and calls
+    its host-only entry points."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(LIB)
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", str(src), "-I", os.path.join(ROOT, "include"),
+                    "-L", libdir, "-lfastowm", "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    report = tmp_path / "hist.csv"
+    out = subprocess.run([str(exe), str(report)], check=True, capture_output=True, text=True).stdout
+    ref = O.params(12, 20, 0.8, (0.0, 500.0, 0.0, 500.0), 1000000)
+    assert out.split("\n")[0].split() == [str(v) for v in (ref.min_num_points, ref.ncols, ref.nrows,
+                                                            ref.ncolsg, ref.nrowsg)]
+    text = report.read_text()
+    assert "Number of leaf nodes: 4" in text and "MaxNumber: 4 pts" in text
+    assert "\n0 2 0\n3 2 6\n" in text  # x_i n_i x_i*n_i rows
+    assert "\n1 4\n" in text           # level 1: four leaves
