@@ -2,6 +2,7 @@
 // returns a status; none exits the process.
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "owm_host.h"
 
@@ -381,6 +382,70 @@ owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_i
     GUARD_END(ctx)
 }
 
+owm_status owm_tree_read(const owm_tree *tree, owm_tree_view *v)
+{
+    if (!tree || !v || !tree->ctx)
+        return OWM_E_ARG;
+    owm_ctx *ctx = tree->ctx;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    const TileDev &TD = tree->P.td[0];
+    const uint64_t n = tree->P.L.np, nl = tree->P.L.nleaf, nn = tree->P.L.nnode;
+    v->n = n;
+    v->n_leaves = nl;
+    v->n_internal = nn;
+    v->leaf_size = TD.leaf_size;
+    v->key_bits = (uint32_t)tree->P.L.key_bits;
+    v->root = TD.root;
+    v->qmin[0] = TD.qminx;
+    v->qmin[1] = TD.qminy;
+    v->qscale = TD.qscale;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "tree read");
+    auto get = [&](void *dst, const void *src, uint64_t bytes) -> owm_status {
+        if (!dst || !bytes)
+            return OWM_OK;
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "tree read");
+        return OWM_OK;
+    };
+    // boxes are stored (minx, miny, maxx, maxy); the view uses header order
+    auto boxes = [&](double *dst, const double4 *src, uint64_t cnt) -> owm_status {
+        if (!dst || !cnt)
+            return OWM_OK;
+        std::vector<double4> tmp(cnt);
+        owm_status st = get(tmp.data(), src, cnt * sizeof(double4));
+        for (uint64_t i = 0; st == OWM_OK && i < cnt; ++i) {
+            dst[4 * i + 0] = tmp[i].x;
+            dst[4 * i + 1] = tmp[i].z;
+            dst[4 * i + 2] = tmp[i].y;
+            dst[4 * i + 3] = tmp[i].w;
+        }
+        return st;
+    };
+    const TreeOut &T = tree->T;
+    owm_status st = OWM_OK;
+    const struct {
+        void *dst;
+        const void *src;
+        uint64_t bytes;
+    } copies[] = {
+        {v->ids, T.ids, n * 4},          {v->xs, T.xs, n * 8},
+        {v->ys, T.ys, n * 8},            {v->zs, T.zs, n * 8},
+        {v->leaf_key, T.lf_key0, nl * 4}, {v->leaf_start, T.lf_start, nl * 4},
+        {v->leaf_points, T.lf_npts, nl * 4}, {v->leaf_count, T.lf_cnt, nl * 4},
+        {v->leaf_min, T.lf_zid, nl * 4}, {v->node_count, T.nd_cnt, nn * 4},
+        {v->node_min, T.nd_zid, nn * 4}, {v->node_child, T.nd_child, nn * 8},
+    };
+    for (const auto &c : copies)
+        if (st == OWM_OK)
+            st = get(c.dst, c.src, c.bytes);
+    if (st == OWM_OK)
+        st = boxes(v->leaf_bbox, T.lf_box, nl);
+    if (st == OWM_OK)
+        st = boxes(v->node_bbox, T.nd_box, nn);
+    return st;
+    GUARD_END(ctx)
+}
+
 // Stages from..to on a resident tree.
 static owm_status tree_query(owm_ctx *ctx, owm_tree *tree, const owm_params *params, int from,
                              int to, const uint32_t *h_in, uint64_t n_in, Prepared &P, QueryOut &qo,
@@ -442,7 +507,7 @@ static owm_status tree_query_sync(owm_ctx *ctx, owm_tree *tree, const owm_params
 owm_status owm_stage1(owm_ctx *ctx, owm_tree *tree, const owm_params *params, uint32_t *min_ids,
                       uint64_t cap, owm_derived *derived)
 {
-    if (!ctx || !tree || !params)
+    if (!ctx || !tree || !params || (cap && !min_ids))
         return OWM_E_ARG;
     GUARD_BEGIN
     (void)hipSetDevice(ctx->device);
@@ -470,7 +535,8 @@ owm_status owm_stage2(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
                       const uint32_t *min_ids, uint64_t n_ids, uint32_t *llp, uint64_t cap,
                       uint64_t *n_llp)
 {
-    if (!ctx || !tree || !params || !n_llp || (n_ids && !min_ids) || n_ids >= (1ull << 31))
+    if (!ctx || !tree || !params || !n_llp || (n_ids && !min_ids) || (cap && !llp) ||
+        n_ids >= (1ull << 31))
         return OWM_E_ARG;
     GUARD_BEGIN
     (void)hipSetDevice(ctx->device);
@@ -502,7 +568,8 @@ owm_status owm_stage2(owm_ctx *ctx, owm_tree *tree, const owm_params *params,
 owm_status owm_stage3(owm_ctx *ctx, owm_tree *tree, const owm_params *params, const uint32_t *llp,
                       uint64_t n_llp, uint32_t *adds, uint64_t cap, uint64_t *n_adds)
 {
-    if (!ctx || !tree || !params || !n_adds || (n_llp && !llp) || n_llp >= (1ull << 31))
+    if (!ctx || !tree || !params || !n_adds || (n_llp && !llp) || (cap && !adds) ||
+        n_llp >= (1ull << 31))
         return OWM_E_ARG;
     GUARD_BEGIN
     (void)hipSetDevice(ctx->device);

@@ -9,7 +9,10 @@
 // seed counts once either way, so the count is the same.  Host code only.
 #include "owm_host.h"
 
+#include <sys/stat.h>
+
 #include <cmath>
+#include <cstdio>
 #include <unordered_map>
 #include <vector>
 
@@ -69,3 +72,28 @@ owm_status owm_gold_check(const double *sx, const double *sy, const double *sz, 
 }
 
 }  // extern "C"
+
+// The driver's results row, save_time (CPP-TBB-CPU/src/optim3_func.cpp:797-828):
+// a header line when the file does not exist yet, then one space-separated row.
+// The reference streams MinRadius and Correctness in defaultfloat (printf %g
+// at the stream's default precision 6) and the two times in fixed (%.6f).
+extern "C" owm_status owm_save_time(const char *exec_name, const char *results_path,
+                                    const char *input_file, int num_threads, float min_radius,
+                                    int max_number, int level, double tree_time,
+                                    double owm_time, double correctness)
+{
+    if (!results_path || !exec_name || !input_file)
+        return OWM_E_ARG;
+    struct stat sb;
+    const bool fresh = stat(results_path, &sb) != 0;
+    FILE *f = fopen(results_path, "a");
+    if (!f)
+        return OWM_E_ARG;
+    if (fresh)
+        fputs("ExecName InputFile NumThreads MinRadius MaxNumber Level TreeTime OwmTime Correctness\n",
+              f);
+    fprintf(f, "%s %s %d %g %d %d %.6f %.6f %g\n", exec_name, input_file, num_threads,
+            (double)min_radius, max_number, level, tree_time, owm_time, correctness);
+    const bool ok = fclose(f) == 0;
+    return ok ? OWM_OK : OWM_E_ARG;
+}

@@ -189,6 +189,15 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
             for (float rr = r0; dleaf < QS_MAX_DEPTH && (double)rr * 0.5 > (double)p->min_radius;
                  rr = (float)((double)rr * 0.5))
                 ++dleaf;
+            if (dleaf >= QS_MAX_DEPTH && n > 0) {
+                // every occupied node splits while radius * 0.5 > minRadius:
+                // with minRadius <= 0 (or tiny against the box) that goes on
+                // past the deepest level this tool follows
+                set_err(ctx, "minRadius %g: occupied nodes of a %g m root would split below "
+                             "depth %d",
+                        (double)p->min_radius, (double)r0, QS_MAX_DEPTH);
+                return OWM_E_ARG;
+            }
             sl = dleaf > 12 ? 16 : 12;
         } else if (n > 0) {
             const double est = std::log((double)n / ((double)p->max_number + 1.0)) / std::log(4.0) + 1.0;
@@ -276,9 +285,13 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
                 break;  // the depth-`last` nodes are all leaves
             }
             if (upto == QS_MAX_DEPTH) {
-                set_err(ctx, "a node holds more than %u points below depth %d: the reference "
-                             "quadtree never stops splitting there",
-                        p->max_number, QS_MAX_DEPTH);
+                if (p->policy == OWM_QT_MAXNUMBER)
+                    set_err(ctx, "a node holds more than %u points below depth %d: the "
+                                 "reference quadtree never stops splitting there",
+                            p->max_number, QS_MAX_DEPTH);
+                else
+                    set_err(ctx, "minRadius %g: occupied nodes split below depth %d",
+                            (double)p->min_radius, QS_MAX_DEPTH);
                 return OWM_E_ARG;
             }
         }

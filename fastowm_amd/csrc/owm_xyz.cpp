@@ -273,4 +273,39 @@ owm_status owm_cloud_read(owm_ctx *ctx, const owm_cloud *cloud, double *x, doubl
     }
 }
 
+// The reference reader's table of its own header-less clouds
+// (CPP-TBB-CPU/src/optim3_func.cpp:667-680): a file whose name contains one of
+// these names holds that many points over that box.  The values are the
+// reference's (data about its published clouds), kept as it lists them.
+owm_status owm_xyz_known_cloud(const char *path, uint64_t *n_points, double *bbox)
+{
+    if (!path || !n_points || !bbox)
+        return OWM_E_ARG;
+    struct Known {
+        const char *name;
+        uint64_t n;
+        double minx, miny, maxx, maxy;
+    };
+    static const Known table[] = {
+        {"INAER_2011_Alcoy.xyz", 2772832, 715244.96, 4286623.63, 716057.75, 4287447.70},
+        {"INAER_2011_Alcoy_Core.xyz", 20380212, 714947.98, 4286501.93, 716361.06, 4288406.23},
+        {"BABCOCK_2017_Arzua_3B.xyz", 40706503, 568000.00, 4752320.00, 568999.99, 4753319.99},
+        {"V21_group1_densified_point_cloud.xyz", 42384876, 526964.093, 4742610.292, 527664.647,
+         4743115.738},
+        {"V19_group1_densified_point_cloud.xyz", 48024480, 526955.908, 4742586.025, 527686.445,
+         4743124.373},
+        {"sample24.xyz", 7492, 513748.12, 5403124.76, 513869.97, 5403197.20},
+    };
+    for (const Known &k : table)
+        if (strstr(path, k.name)) {
+            *n_points = k.n;
+            bbox[0] = k.minx;
+            bbox[1] = k.maxx;
+            bbox[2] = k.miny;
+            bbox[3] = k.maxy;
+            return OWM_OK;
+        }
+    return OWM_E_NOTFOUND;
+}
+
 }  // extern "C"

@@ -12,8 +12,10 @@ nothing falls back to a CPU path: without the HIP library every call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
 import weakref
 
 import numpy as np
@@ -76,6 +78,17 @@ class TuneEntry(ctypes.Structure):
 
 
 TUNE_CLASSES = 6
+TREE_LEAF = 0x80000000
+
+
+class TreeView(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("n_leaves", ctypes.c_uint64),
+                ("n_internal", ctypes.c_uint64), ("leaf_size", ctypes.c_uint32),
+                ("key_bits", ctypes.c_uint32), ("root", ctypes.c_uint32),
+                ("qmin", ctypes.c_double * 2), ("qscale", ctypes.c_double)] + \
+               [(f, ctypes.c_void_p) for f in ("ids", "xs", "ys", "zs", "leaf_key", "leaf_start",
+                                               "leaf_points", "leaf_count", "leaf_bbox", "leaf_min",
+                                               "node_child", "node_count", "node_bbox", "node_min")]
 
 QT_MAXNUMBER, QT_MINRADIUS = 0, 1
 
@@ -147,6 +160,18 @@ def write_qtree_stats(path, classes, summary, params, n_points):
 
 
 _lib = None
+# Contexts not closed yet.  At interpreter exit they are closed by an atexit
+# hook, which runs before module teardown and before the HIP runtime is torn
+# down; finalizers that run later (sys.is_finalizing()) make no library call.
+_live_contexts = weakref.WeakSet()
+
+
+def _close_live_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
 
 
 def lib():
@@ -163,6 +188,8 @@ def lib():
         except Exception:
             pass
         L = ctypes.CDLL(LIB_PATH)
+        # registered after torch's own exit handlers, so it runs before them
+        atexit.register(_close_live_contexts)
         vp, u64, u32, dp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p
         P = ctypes.POINTER
         sig = {
@@ -188,6 +215,7 @@ def lib():
             "owm_tree_free": ([vp], None),
             "owm_tree_points": ([vp], u64),
             "owm_tree_info": ([vp, P(u64), P(u64), P(u64), dp], ctypes.c_int),
+            "owm_tree_read": ([vp, P(TreeView)], ctypes.c_int),
             "owm_stage1": ([vp, vp, P(Params), dp, u64, P(Derived)], ctypes.c_int),
             "owm_stage2": ([vp, vp, P(Params), dp, u64, dp, u64, P(u64)], ctypes.c_int),
             "owm_stage3": ([vp, vp, P(Params), dp, u64, dp, u64, P(u64)], ctypes.c_int),
@@ -300,12 +328,39 @@ class Tree:
         return {"leaves": nl.value, "internal": ni.value, "root_count": rc.value,
                 "root_bbox": tuple(box)}
 
+    def read(self):
+        """The tree's structure (owm_tree_read): scalars and every array as
+        numpy (bboxes as (k, 4) minx, maxx, miny, maxy; node_child (k, 2))."""
+        v = TreeView()
+        self.ctx._check(lib().owm_tree_read(self.h, ctypes.byref(v)))
+        n, nl, nn = v.n, v.n_leaves, v.n_internal
+        shapes = {"ids": (n, np.uint32), "xs": (n, np.float64), "ys": (n, np.float64),
+                  "zs": (n, np.float64), "leaf_key": (nl, np.uint32),
+                  "leaf_start": (nl, np.uint32), "leaf_points": (nl, np.uint32),
+                  "leaf_count": (nl, np.uint32), "leaf_bbox": (4 * nl, np.float64),
+                  "leaf_min": (nl, np.uint32), "node_child": (2 * nn, np.uint32),
+                  "node_count": (nn, np.uint32), "node_bbox": (4 * nn, np.float64),
+                  "node_min": (nn, np.uint32)}
+        out = {k: np.empty(max(c, 1), t) for k, (c, t) in shapes.items()}
+        for k, a in out.items():
+            setattr(v, k, a.ctypes.data)
+        self.ctx._check(lib().owm_tree_read(self.h, ctypes.byref(v)))
+        res = {k: out[k][:c] for k, (c, _) in shapes.items()}
+        res["leaf_bbox"] = res["leaf_bbox"].reshape(-1, 4)
+        res["node_bbox"] = res["node_bbox"].reshape(-1, 4)
+        res["node_child"] = res["node_child"].reshape(-1, 2)
+        res.update(n=n, n_leaves=nl, n_internal=nn, leaf_size=v.leaf_size, key_bits=v.key_bits,
+                   root=v.root, qmin=(v.qmin[0], v.qmin[1]), qscale=v.qscale)
+        return res
+
     def free(self):
         if self.h and getattr(self.ctx, "h", None):
             lib().owm_tree_free(self.h)
         self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # the exit hook closed the context and its trees already
         try:
             self.free()
         except Exception:
@@ -344,6 +399,7 @@ class Context:
         if st:
             raise OWMError(st, f"cannot create a context on device {device}")
         self.h = h
+        _live_contexts.add(self)
         self._trees = weakref.WeakSet()
         self._seed_buf = np.empty(0, np.uint32)
         self._seed_mem = None
@@ -375,9 +431,10 @@ class Context:
         if getattr(self, "h", None):
             for t in list(self._trees):
                 t.free()
-            lib().owm_ctx_destroy(self.h)
-            self.h = None
+            h, self.h = self.h, None
+            lib().owm_ctx_destroy(h)
             self._free_seed_buf()
+            _live_contexts.discard(self)
 
     def __enter__(self):
         return self
@@ -386,6 +443,8 @@ class Context:
         self.close()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # closed by the exit hook; no HIP call during interpreter teardown
         try:
             self.close()
         except Exception:

@@ -153,6 +153,13 @@ owm_status owm_xyz_parse(owm_ctx *ctx, const char *text, uint64_t bytes, owm_clo
 /* Copies a parsed cloud to caller-owned host arrays of cloud->n doubles
  * (no reference counterpart: readXYZfile fills host memory directly). */
 owm_status owm_cloud_read(owm_ctx *ctx, const owm_cloud *cloud, double *x, double *y, double *z);
+/* readXYZfile's table of the reference's header-less clouds
+ * (CPP-TBB-CPU/src/optim3_func.cpp:667-680): when `path` contains one of their
+ * file names (INAER_2011_Alcoy.xyz, INAER_2011_Alcoy_Core.xyz,
+ * BABCOCK_2017_Arzua_3B.xyz, V21_/V19_group1_densified_point_cloud.xyz,
+ * sample24.xyz), *n_points and bbox (minx, maxx, miny, maxy) get the values
+ * the reference hard-codes for it; else OWM_E_NOTFOUND.  Host only. */
+owm_status owm_xyz_known_cloud(const char *path, uint64_t *n_points, double *bbox);
 
 /* ---- GOLD comparison (host only) -----------------------------------------
  * The reference's check_results (CPP-TBB-CPU/src/optim3_func.cpp:725-794): a
@@ -162,6 +169,16 @@ owm_status owm_cloud_read(owm_ctx *ctx, const owm_cloud *cloud, double *x, doubl
 owm_status owm_gold_check(const double *sx, const double *sy, const double *sz, uint64_t n_seeds,
                           const double *gx, const double *gy, const double *gz, uint64_t n_gold,
                           double displace, uint64_t *n_correct, double *percent);
+
+/* The driver's results file (save_time, CPP-TBB-CPU/src/optim3_func.cpp:797-828,
+ * same argument order): appends "exec input threads minRadius maxNumber level
+ * treeTime owmTime correctness" to results_path (minRadius and correctness as
+ * %g, the times as %.6f), after the header line "ExecName InputFile NumThreads
+ * MinRadius MaxNumber Level TreeTime OwmTime Correctness" when the file does
+ * not exist.  OWM_E_ARG when the file cannot be written.  Host only. */
+owm_status owm_save_time(const char *exec_name, const char *results_path, const char *input_file,
+                         int num_threads, float min_radius, int max_number, int level,
+                         double tree_time, double owm_time, double correctness);
 
 /* ---- quadtree leaf statistics (owm_stats) ---------------------------------
  * The leaves of the reference's parallel_qtree (CPP-TBB-CPU/src/optim3_func.cpp:
@@ -274,6 +291,38 @@ uint64_t owm_tree_points(const owm_tree *tree);
  * point count and bbox (minx, maxx, miny, maxy). */
 owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_internal,
                          uint64_t *root_count, double *root_bbox);
+
+/* The whole structure of a resident tree, for checks in the spirit of the
+ * reference's checkNumPoints (CPP-TBB-CPU/src/optim3_func.cpp:270-283, called
+ * at application/owm_optim3.cpp:90), which walks its quadtree to compare
+ * counts.  owm_tree_read fills the scalars and copies every array whose
+ * pointer is non-null (caller-owned; sizes n, n_leaves, n_internal).  Points
+ * sit in Morton order: a point's sort key is the top key_bits bits of the
+ * 64-bit interleave of qx (even bits) and qy (odd bits), where
+ * q = (uint32)((v - qmin) * qscale) in IEEE double, 0 when not > 0 (or NaN)
+ * and 0xFFFFFFFF from 4294967295 up; leaf j spans tree positions
+ * [leaf_start, leaf_start + leaf_points), leaf_size of them but the last. */
+#define OWM_TREE_LEAF 0x80000000u /* child reference bit: a leaf, not an internal node */
+typedef struct owm_tree_view {
+    uint64_t n, n_leaves, n_internal;
+    uint32_t leaf_size, key_bits;
+    uint32_t root;          /* internal node index, or OWM_TREE_LEAF | leaf index */
+    double qmin[2], qscale; /* Morton quantisation of x and y */
+    uint32_t *ids;          /* [n] input index of the point at each tree position */
+    double *xs, *ys, *zs;   /* [n] its coordinates */
+    uint32_t *leaf_key;     /* [n_leaves] sort key of the leaf's first point */
+    uint32_t *leaf_start;   /* [n_leaves] first tree position */
+    uint32_t *leaf_points;  /* [n_leaves] tree positions the leaf spans */
+    uint32_t *leaf_count;   /* [n_leaves] its points with non-NaN x and y */
+    double *leaf_bbox;      /* [4 n_leaves] minx, maxx, miny, maxy of those */
+    uint32_t *leaf_min;     /* [n_leaves] input index of the lowest point with z < DBL_MAX
+                               (lowest z, then lowest index); 0xFFFFFFFF if none */
+    uint32_t *node_child;   /* [2 n_internal] left and right child references */
+    uint32_t *node_count;   /* [n_internal] as leaf_count, over the node's leaves */
+    double *node_bbox;      /* [4 n_internal] */
+    uint32_t *node_min;     /* [n_internal] */
+} owm_tree_view;
+owm_status owm_tree_read(const owm_tree *tree, owm_tree_view *view);
 
 /* Stage 1: min_ids[jj*ncols + ii] = index of the lowest point of window
  * (ii, jj) if it holds at least minNumPoints points, else 0xFFFFFFFF

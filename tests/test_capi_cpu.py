@@ -163,6 +163,19 @@ int main(int argc, char **argv)
     owm_leaf_bin bins[2] = {{1, 0, 2}, {1, 3, 2}};
     if (owm_qtree_stats_write(argv[1], &q, &s, bins, 2, 6) != OWM_OK) return 2;
     printf("%s\n", owm_status_str(OWM_E_CAPACITY));
+    /* argument guards come before any use of the handles: a missing output
+       buffer with a nonzero capacity is OWM_E_ARG (the handles here are
+       stand-ins that must never be dereferenced) */
+    static char fake_ctx[64], fake_tree[64];
+    owm_ctx *c = (owm_ctx *)fake_ctx;
+    owm_tree *t = (owm_tree *)fake_tree;
+    uint64_t cnt = 0;
+    if (owm_stage1(c, t, &p, NULL, 16, NULL) != OWM_E_ARG) return 3;
+    if (owm_stage2(c, t, &p, NULL, 4, NULL, 4, &cnt) != OWM_E_ARG) return 4;
+    if (owm_stage3(c, t, &p, NULL, 0, NULL, 4, &cnt) != OWM_E_ARG) return 5;
+    if (owm_tree_run(c, t, &p, NULL, 4, &cnt) != OWM_E_ARG) return 6;
+    if (owm_run_tile(c, NULL, NULL, NULL, 10, &p, NULL, 0, &cnt) != OWM_E_ARG) return 7;
+    if (owm_stage1(NULL, t, &p, NULL, 0, NULL) != OWM_E_ARG) return 8;
     return 0;
 }
 """

@@ -151,6 +151,10 @@ def test_edge_cases(ctx):
         ctx.qtree_stats(p, p, policy=owm.QT_MAXNUMBER, level=11, bbox=box)
     with pytest.raises(owm.OWMError):  # no box and no points
         ctx.qtree_stats(e, e, policy=owm.QT_MAXNUMBER)
+    for r in (0.0, -1.0, 1e-30):  # occupied nodes would split past depth 64
+        with pytest.raises(owm.OWMError) as err:
+            ctx.qtree_stats(p, p, policy=owm.QT_MINRADIUS, min_radius=r, level=0, bbox=box)
+        assert err.value.name == "E_ARG" and "minRadius" in str(err.value)
     # the context still works after the errors
     x, y, _ = tile_for({"seed": 12}, 30_000).generate()
     both(ctx, x, y, synth.bbox(x, y), owm.QT_MINRADIUS, min_radius=0.5, level=1)
