@@ -1,6 +1,7 @@
 // C ABI entry points (include/owm.h).  Each catches every C++ exception and
 // returns a status; none exits the process.
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -21,6 +22,37 @@ using namespace owm;
     }
 
 static inline uint64_t al256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+// Device scratch of owm_run_tile_device_noalloc: a file-scope array of the
+// library's code object, of which every device holds its own copy from the
+// time the library's code is loaded on it until the process ends.  Its
+// host-side name is not a device address: the address hipGetSymbolAddress
+// gives on each device (resolved once per device, with that device current,
+// when the first context is created on it) is what host code passes.
+static constexpr uint64_t NOALLOC_SCRATCH_BYTES = 256ull << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static hipError_t g_scratch_err[NOALLOC_MAX_DEVICES];
+static bool g_scratch_done[NOALLOC_MAX_DEVICES];
+
+// With `device` current.
+static void resolve_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device])
+        return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? (uint8_t *)p : nullptr;
+    if (e != hipSuccess)
+        (void)hipGetLastError();
+    g_scratch_done[device] = true;
+}
 
 extern "C" {
 
@@ -56,6 +88,20 @@ owm_status owm_ctx_create(int device, uint64_t hbm_budget_bytes, owm_ctx **out)
         return OWM_E_DEVICE;
     owm_ctx *c = new owm_ctx();
     c->device = device;
+    resolve_scratch(device);
+    // the planning state of owm_run_tile_device_noalloc, sized for one tile
+    // now so that those calls allocate nothing (not even host memory)
+    {
+        owm::Prepared &F = c->fixed_plan;
+        F.td.reserve(1);
+        F.n.reserve(1);
+        F.leaf.reserve(1);
+        F.box.reserve(1);
+        for (auto *v : {&F.L.pt_off, &F.L.leaf_off, &F.L.node_off, &F.L.grp_off, &F.L.sort_units,
+                        &F.Q.win_off, &F.Q.cell_off, &F.Q.blk_off})
+            v->reserve(2);
+        c->err.reserve(1024);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return OWM_E_DEVICE;
@@ -219,6 +265,35 @@ owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_
     GUARD_BEGIN
     (void)hipSetDevice(ctx->device);
     return run_one(ctx, d_x, d_y, d_z, n, params, seeds, cap, n_seeds);
+    GUARD_END(ctx)
+}
+
+owm_status owm_run_tile_device_noalloc(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                       const double *d_z, uint64_t n, const owm_params *params,
+                                       uint32_t *seeds, uint64_t cap, uint64_t *n_seeds)
+{
+    if (!ctx || !params || !n_seeds || (n && (!d_x || !d_y || !d_z)) || (cap && !seeds))
+        return OWM_E_ARG;
+    GUARD_BEGIN
+    (void)hipSetDevice(ctx->device);
+    const int dev = ctx->device;
+    uint8_t *base = dev >= 0 && dev < NOALLOC_MAX_DEVICES ? g_scratch_ptr[dev] : nullptr;
+    if (!base) {
+        set_err(ctx, "no static scratch on device %d: %s", dev,
+                dev >= 0 && dev < NOALLOC_MAX_DEVICES ? hipGetErrorString(g_scratch_err[dev])
+                                                      : "device index too large");
+        return OWM_E_DEVICE;
+    }
+    // host staging of one tile's readbacks and table, on this call's stack
+    alignas(256) uint8_t h_io[2048];
+    alignas(256) uint8_t h_tab[4096];
+    if (io_bytes(1) > sizeof h_io || table_bytes(1) > sizeof h_tab) {
+        set_err(ctx, "host staging of one tile does not fit");
+        return OWM_E_INTERNAL;
+    }
+    uint64_t need = 0;
+    return run_tile_fixed(ctx, d_x, d_y, d_z, n, *params, seeds, cap, n_seeds, base,
+                          NOALLOC_SCRATCH_BYTES, h_io, h_tab, &need);
     GUARD_END(ctx)
 }
 

@@ -160,16 +160,19 @@ DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()
 {
-    if (ptr)
+    if (ptr && !borrowed)
         (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
 hipError_t DevBuf::reserve(uint64_t need, uint64_t cap)
 {
     if (need <= bytes)
         return hipSuccess;
+    if (borrowed)
+        return hipErrorOutOfMemory;  // a borrowed region never grows
     release();
     uint64_t want = std::max(need, std::min(need + need / 4, cap));
     hipError_t e = hipMalloc(&ptr, want);
@@ -639,7 +642,7 @@ uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
 owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const TreeDev &TV,
                          const QueryLayout &Q, uint64_t np, const Stage1Cfg &cfg,
                          uint32_t any_need1, Arena &scratch, QueryOut &out, int from, int to,
-                         const uint32_t *h_in, uint64_t n_in)
+                         const uint32_t *h_in, uint64_t n_in, const RunMem *mem)
 {
     const int bits = std::max(bit_length(np), 1);
     const int passes = (bits + 7) / 8;
@@ -679,8 +682,9 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     z.add(s, SD.lookback, (uint64_t)passes * sort_tiles((uint32_t)nseed_max) * 256);
     z.launch(s);
     if (from <= 1) {
-        const uint32_t s1cap = (uint32_t)std::min<uint64_t>(Q.s1_cap, ctx->s1_list.bytes / 4);
-        uint32_t *list = (uint32_t *)ctx->s1_list.ptr;
+        const uint64_t list_words = mem ? mem->s1_words : ctx->s1_list.bytes / 4;
+        const uint32_t s1cap = (uint32_t)std::min<uint64_t>(Q.s1_cap, list_words);
+        uint32_t *list = mem ? mem->s1_list : (uint32_t *)ctx->s1_list.ptr;
         int pi = ctx->prof.begin(s, PH_S1_JOIN);
         launch_stage1_join(s, B, TV, cfg, s1work, list, s1cap, cnt + 3, cnt + 6);
         // per leaf: bbox + count read by both join passes; per block: count, start
@@ -826,7 +830,7 @@ Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total)
 // Per-tile bbox (caller's, else an exact device reduction), leaf sizes and the
 // tree layout of a batch whose points are in device memory.
 owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const owm_tile *tiles,
-                        uint32_t n_tiles, Prepared &P)
+                        uint32_t n_tiles, Prepared &P, const RunMem *mem)
 {
     owm_status st = check_tiles(ctx, tiles, n_tiles);
     if (st != OWM_OK)
@@ -855,17 +859,28 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
         }
     }
     if (need_bbox) {
-        std::vector<std::array<double, 4>> unit(n_tiles, {0.0, 1.0, 0.0, 1.0});
-        plan_tree(32, P.td, P.n, P.leaf, unit, P.L);
+        // tile offsets first (the boxes are placeholders until the pass below)
+        for (uint32_t t = 0; t < n_tiles; ++t)
+            if (!(tiles[t].params.flags & OWM_F_HAVE_BBOX))
+                P.box[t] = {0.0, 1.0, 0.0, 1.0};
+        plan_tree(32, P.td, P.n, P.leaf, P.box, P.L);
         const uint64_t bk_bytes = bbox_key_words(n_tiles) * 8;
         uint64_t bytes = align_up(bk_bytes, 256) + align_up((n_tiles + 1) * 4ull, 256);
-        st = ctx->small.ensure(ctx, bytes);
-        if (st != OWM_OK)
+        // the bbox pass's scratch: the context's small arena, or the start of
+        // a borrowed region (released again once the boxes are on the host)
+        Arena &SA = mem ? *mem->arena : ctx->small;
+        if (mem)
+            SA.used = 0;
+        else if ((st = ctx->small.ensure(ctx, bytes)) != OWM_OK)
             return st;
-        unsigned long long *bk = (unsigned long long *)ctx->small.take(bk_bytes);
-        uint32_t *ptoff = (uint32_t *)ctx->small.take((n_tiles + 1) * 4ull);
+        unsigned long long *bk = (unsigned long long *)SA.take(bk_bytes);
+        uint32_t *ptoff = (uint32_t *)SA.take((n_tiles + 1) * 4ull);
+        if (!bk || !ptoff) {
+            set_err(ctx, "scratch exhausted (bbox pass)");
+            return OWM_E_BUDGET;
+        }
         const IoLayout io(n_tiles);
-        uint8_t *h = ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
+        uint8_t *h = mem ? mem->h_io : ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
         if (!h) {
             set_err(ctx, "pinned host allocation failed");
             return OWM_E_NOMEM;
@@ -901,6 +916,8 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
                 P.box[t] = {dec(hk[4 * t]), dec(hk[4 * t + 2]), dec(hk[4 * t + 1]),
                             dec(hk[4 * t + 3])};
             }
+        if (mem)
+            SA.used = 0;
     }
     plan_tree((int)ctx->tuner.sort_bits_for(np), P.td, P.n, P.leaf, P.box, P.L);
     return OWM_OK;
@@ -983,22 +1000,28 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
 }
 
 // Uploads the tile table and offset arrays; fills B's pointers.
-owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B)
+owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B,
+                        const RunMem *mem)
 {
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
-    // tiles, then the eight offset arrays, as staged by stage_table
+    // tiles, then the eight offset arrays, as staged by write_table
     uint8_t *d = A.take(table_bytes(n_tiles));
     if (!d) {
         set_err(ctx, "scratch arena exhausted (table)");
         return OWM_E_INTERNAL;
     }
-    if (!ctx->stage_table(P.td, P.L, P.Q)) {
+    const uint8_t *h = nullptr;
+    if (mem) {
+        write_table(mem->h_tab, P.td, P.L, P.Q);
+        h = mem->h_tab;
+    } else if (ctx->stage_table(P.td, P.L, P.Q)) {
+        h = ctx->h_tab;
+    } else {
         set_err(ctx, "pinned host allocation failed");
         return OWM_E_NOMEM;
     }
-    HIP_TRY(hipMemcpyAsync(d, ctx->h_tab, table_bytes(n_tiles), hipMemcpyHostToDevice, s),
-            "upload table");
+    HIP_TRY(hipMemcpyAsync(d, h, table_bytes(n_tiles), hipMemcpyHostToDevice, s), "upload table");
     TileDev *d_tiles = (TileDev *)d;
     uint32_t *d_off[8];
     uint64_t o = align_up((uint64_t)n_tiles * sizeof(TileDev), 256);
@@ -1061,12 +1084,13 @@ owm_status check_guards(owm_ctx *ctx, const uint32_t *hc, uint64_t *s1_needed)
 }
 
 owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
-                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed)
+                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed,
+                      const RunMem *mem)
 {
     hipStream_t s = ctx->stream;
     const uint32_t n_tiles = (uint32_t)P.td.size();
     const IoLayout io(n_tiles);
-    uint8_t *h = ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
+    uint8_t *h = mem ? mem->h_io : ctx->pinned(ctx->h_io, ctx->h_io_cap, io.bytes);
     if (!h) {
         set_err(ctx, "pinned host allocation failed");
         return OWM_E_NOMEM;
@@ -1130,10 +1154,10 @@ void GraphCache::clear()
 // and replayed while its key matches.
 static owm_status enqueue_chain(owm_ctx *ctx, bool graph, const double *dx, const double *dy,
                                 const double *dz, Prepared &P, BatchDev &B, TreeOut &T,
-                                uint32_t *guard, QueryOut &qo)
+                                uint32_t *guard, QueryOut &qo, const RunMem *mem = nullptr)
 {
     hipStream_t s = ctx->stream;
-    Arena &A = ctx->arena;
+    Arena &A = mem ? *mem->arena : ctx->arena;
     auto body = [&]() -> owm_status {
         // build_tree zeroes the guard with its own scratch
         owm_status st = build_tree(ctx, s, P.L, B, dx, dy, dz, T, A, guard);
@@ -1143,7 +1167,7 @@ static owm_status enqueue_chain(owm_ctx *ctx, bool graph, const double *dx, cons
         TV.X = dx;
         TV.Y = dy;
         st = enqueue_query(ctx, s, B, TV, P.Q, P.L.np, P.cfg, P.any_need1 ? 1u : 0u, A, qo, 1, 4,
-                           nullptr, 0);
+                           nullptr, 0, mem);
         if (st != OWM_OK)
             return st;
         launch_add_u32(s, qo.cnt + 3, guard, qo.cnt + 3);
@@ -1260,6 +1284,83 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     }
 }
 
+uint64_t io_bytes(uint32_t n_tiles) { return IoLayout(n_tiles).bytes; }
+
+// One tile inside a borrowed device region with caller-provided host staging:
+// the steps of run_batch (bbox, plan, table, chain, readback) with every
+// buffer carved from [base, base + bytes), so that nothing is allocated:
+// no device or page-locked memory, no events (phase timing is paused), and the
+// planning vectors are the context's, reserved when it was created.
+owm_status run_tile_fixed(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                          uint64_t n, const owm_params &params, uint32_t *seeds, uint64_t cap,
+                          uint64_t *n_seeds, uint8_t *base, uint64_t bytes, uint8_t *h_io,
+                          uint8_t *h_tab, uint64_t *need)
+{
+    *need = 0;
+    Arena A;
+    A.buf.ptr = base;
+    A.buf.bytes = bytes;
+    A.buf.borrowed = true;
+    RunMem mem;
+    mem.arena = &A;
+    mem.h_io = h_io;
+    mem.h_tab = h_tab;
+    owm_tile tile;
+    tile.offset = 0;
+    tile.n = n;
+    tile.params = params;
+    struct Pause {  // phase timing would create events
+        Profiler &p;
+        bool on;
+        explicit Pause(Profiler &q) : p(q), on(q.on) { p.on = false; }
+        ~Pause() { p.on = on; }
+    } pause(ctx->prof);
+    Prepared &P = ctx->fixed_plan;
+    owm_status st = prepare_tree(ctx, dx, dy, &tile, 1, P, &mem);
+    if (st != OWM_OK)
+        return st;
+    st = prepare_query(ctx, &tile.params, sizeof(owm_tile), P.box, P.L.np, P);
+    if (st != OWM_OK)
+        return st;
+    for (int attempt = 0;; ++attempt) {
+        const uint64_t list_bytes = align_up(std::max<uint64_t>(P.Q.s1_cap, 1) * 4, 256);
+        *need = 256 + table_bytes(1) + tree_bytes(P.L) + build_scratch_bytes(P.L) +
+                query_bytes(P.Q, P.L.np, 0) + list_bytes + 8192;
+        if (*need > bytes) {
+            set_err(ctx, "a tile of %llu points needs %llu bytes of scratch; this entry point "
+                         "has %llu",
+                    (unsigned long long)n, (unsigned long long)*need, (unsigned long long)bytes);
+            return OWM_E_BUDGET;
+        }
+        A.used = 0;
+        uint32_t *guard = (uint32_t *)A.take(256);
+        mem.s1_list = (uint32_t *)A.take(list_bytes);
+        mem.s1_words = list_bytes / 4;
+        BatchDev B;
+        st = upload_table(ctx, A, P, B, &mem);
+        if (st != OWM_OK)
+            return st;
+        TreeOut T;
+        carve_tree(A.take(tree_bytes(P.L)), P.L, T);
+        T.pos = nullptr;
+        QueryOut qo;
+        st = enqueue_chain(ctx, false, dx, dy, dz, P, B, T, guard, qo, &mem);
+        if (st != OWM_OK)
+            return st;
+        uint64_t s1_need = 0, off[2] = {0, 0};
+        st = read_seeds(ctx, qo, P, seeds, cap, off, &s1_need, &mem);
+        *n_seeds = off[1];
+        if (st != OWM_OK || s1_need == 0)
+            return st;
+        if (attempt > 0) {
+            set_err(ctx, "stage-1 join list overflow persisted");
+            return OWM_E_INTERNAL;
+        }
+        // the join's exact size is known now: run again with room for it
+        P.Q.s1_cap = s1_need + s1_need / 8 + 1024;
+    }
+}
+
 }  // namespace owm
 
 using owm::TreeLayout;
@@ -1284,15 +1385,10 @@ uint8_t *owm_ctx::pinned(uint8_t *&buf, uint64_t &cap, uint64_t need)
     return buf;
 }
 
-bool owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, const QueryLayout &Q)
+void owm::write_table(uint8_t *h, const std::vector<TileDev> &td, const TreeLayout &L,
+                      const QueryLayout &Q)
 {
     const uint32_t T = (uint32_t)td.size();
-    const uint64_t bytes = owm::table_bytes(T);
-    // the previous upload from this buffer must have completed
-    (void)hipStreamSynchronize(stream);
-    uint8_t *h = pinned(h_tab, h_tab_cap, bytes);
-    if (!h)
-        return false;
     memcpy(h, td.data(), T * sizeof(TileDev));
     const std::vector<uint32_t> *src[8] = {&L.pt_off,   &L.leaf_off, &L.node_off,
                                            &Q.win_off,  &Q.cell_off, &Q.blk_off,
@@ -1300,6 +1396,17 @@ bool owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, c
     uint64_t o = owm::align_up((uint64_t)T * sizeof(TileDev), 256);
     for (int k = 0; k < 8; ++k, o += owm::align_up((T + 1) * 4ull, 256))
         memcpy(h + o, src[k]->data(), (T + 1) * 4);
+}
+
+bool owm_ctx::stage_table(const std::vector<TileDev> &td, const TreeLayout &L, const QueryLayout &Q)
+{
+    const uint64_t bytes = owm::table_bytes((uint32_t)td.size());
+    // the previous upload from this buffer must have completed
+    (void)hipStreamSynchronize(stream);
+    uint8_t *h = pinned(h_tab, h_tab_cap, bytes);
+    if (!h)
+        return false;
+    owm::write_table(h, td, L, Q);
     return true;
 }
 

@@ -21,6 +21,7 @@ namespace owm {
 struct DevBuf {
     void *ptr = nullptr;
     uint64_t bytes = 0;
+    bool borrowed = false;  // memory owned elsewhere: never freed or regrown here
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -123,6 +124,18 @@ struct Tuner {
     uint32_t sort_bits_for(uint64_t n) const { return cfg_for(n).sort_bits; }
 };
 
+// Where a run takes its memory when not from the context's own arenas and
+// pinned buffers (which grow on demand): one borrowed device region that
+// holds everything (bbox pass, table, tree, build and query scratch, stage-1
+// join list) and caller-provided host staging.  Nothing is allocated.
+struct RunMem {
+    Arena *arena = nullptr;      // the borrowed device region
+    uint32_t *s1_list = nullptr; // stage-1 join list (carved from the region)
+    uint64_t s1_words = 0;
+    uint8_t *h_io = nullptr;     // host readbacks (tile offsets, bbox keys, counts, ranges)
+    uint8_t *h_tab = nullptr;    // host staging of the tile table
+};
+
 struct Prepared {
     std::vector<TileDev> td;
     std::vector<uint64_t> n;
@@ -162,15 +175,30 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L, const Ba
 owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const TreeDev &TV,
                          const QueryLayout &Q, uint64_t np, const Stage1Cfg &cfg,
                          uint32_t any_need1, Arena &scratch, QueryOut &out, int from, int to,
-                         const uint32_t *h_in, uint64_t n_in);
+                         const uint32_t *h_in, uint64_t n_in, const RunMem *mem = nullptr);
 owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const owm_tile *tiles,
-                        uint32_t n_tiles, Prepared &P);
+                        uint32_t n_tiles, Prepared &P, const RunMem *mem = nullptr);
 owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride_bytes,
                          const std::vector<std::array<double, 4>> &box, uint64_t np_total,
                          Prepared &P);
-owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B);
+owm_status upload_table(owm_ctx *ctx, Arena &A, const Prepared &P, BatchDev &B,
+                        const RunMem *mem = nullptr);
 owm_status read_seeds(owm_ctx *ctx, const QueryOut &qo, const Prepared &P, uint32_t *seeds,
-                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed);
+                      uint64_t cap, uint64_t *seed_off_host, uint64_t *s1_needed,
+                      const RunMem *mem = nullptr);
+// Host bytes of the readback and table staging of an n_tiles batch.
+uint64_t io_bytes(uint32_t n_tiles);
+// Writes the tile table (tiles, then the eight offset arrays, in the device
+// layout) into host memory h of table_bytes(td.size()) bytes.
+void write_table(uint8_t *h, const std::vector<::TileDev> &td, const TreeLayout &L,
+                 const QueryLayout &Q);
+// One tile entirely inside the device region [base, base + bytes) with the
+// given host staging; allocates nothing.  OWM_E_BUDGET (and *need set) when
+// the tile's work does not fit the region.
+owm_status run_tile_fixed(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
+                          uint64_t n, const owm_params &params, uint32_t *seeds, uint64_t cap,
+                          uint64_t *n_seeds, uint8_t *base, uint64_t bytes, uint8_t *h_io,
+                          uint8_t *h_tab, uint64_t *need);
 // Makes ctx->s1_list hold at least cap entries.
 owm_status ensure_s1_list(owm_ctx *ctx, uint64_t cap);
 // Device-guard flags of a finished query: OWM_OK, or OWM_E_INTERNAL with a
@@ -273,6 +301,7 @@ struct owm_ctx {
     owm::ServeState *serve = nullptr;
     owm::Profiler prof;  // owned; released by owm_serve_stop
     owm::GraphCache graphs;  // captured batch chains (OWM_B_GRAPH)
+    owm::Prepared fixed_plan;  // planning state of run_tile_fixed, reserved at creation
     // copies the tile table into h_tab (false: no pinned memory)
     bool stage_table(const std::vector<::TileDev> &td, const owm::TreeLayout &L,
                      const owm::QueryLayout &Q);

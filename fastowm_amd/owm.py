@@ -204,6 +204,8 @@ def lib():
             "owm_query_seed_count": ([P(Params), u64, dp, P(u64)], ctypes.c_int),
             "owm_run_tile": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_run_tile_device": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)], ctypes.c_int),
+            "owm_run_tile_device_noalloc": ([vp, dp, dp, dp, u64, P(Params), dp, u64, P(u64)],
+                                            ctypes.c_int),
             "owm_xyz_parse": ([vp, ctypes.c_char_p, u64, P(CloudDesc)], ctypes.c_int),
             "owm_gold_check": ([dp, dp, dp, u64, dp, dp, dp, u64, ctypes.c_double, P(u64),
                                 P(ctypes.c_double)], ctypes.c_int),

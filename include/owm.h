@@ -127,6 +127,22 @@ owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_
                                const double *d_z, uint64_t n, const owm_params *params,
                                uint32_t *seeds, uint64_t cap, uint64_t *n_seeds);
 
+/* owm_run_tile_device without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as owm_run_tile_device; every device buffer of the run is carved
+ * from a 256 MiB static scratch array that each device holds for the life of
+ * the process, host staging lives on the call's stack, phase timing is paused,
+ * and nothing is allocated (no hipMalloc/hipFree, no page-locked or host heap
+ * memory, no events).  A tile whose work needs more scratch than that (the
+ * work takes about 105 bytes per point, so up to about 2.5 million points) is
+ * refused with OWM_E_BUDGET and a message giving the bytes needed.  The scratch is shared by every call on a device and
+ * keeps the last call's contents: calls that overlap on one device (from two
+ * threads or two contexts) race and must be serialised by the caller. */
+owm_status owm_run_tile_device_noalloc(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                       const double *d_z, uint64_t n, const owm_params *params,
+                                       uint32_t *seeds, uint64_t cap, uint64_t *n_seeds);
+
 /* ---- reader: LiDAR .xyz text -> device-resident cloud --------------------
  * Replaces readXYZfile (CPP-TBB-CPU/src/optim3_func.cpp:665-722): text holds
  * the whole file (bytes long, host memory).  A first line holding a single
