@@ -235,7 +235,7 @@ static owm_status run_one(owm_ctx *ctx, const double *dx, const double *dy, cons
     t.n = n;
     t.params = *params;
     uint64_t off[2] = {0, 0};
-    owm_status st = run_batch(ctx, dx, dy, dz, &t, 1, 0, seeds, cap, off);
+    owm_status st = run_batch(ctx, dx, dy, dz, &t, 1, B_AUTO_GRAPH, seeds, cap, off);
     if (n_seeds)
         *n_seeds = off[1];
     return st;

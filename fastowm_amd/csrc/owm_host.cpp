@@ -1147,6 +1147,7 @@ void GraphCache::clear()
         if (s.exec)
             (void)hipGraphExecDestroy(s.exec);
     slots.clear();
+    last_key.clear();
 }
 
 // Enqueues (or replays) the device chain of a batch: guard reset, tree build,
@@ -1245,6 +1246,7 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     if (st != OWM_OK)
         return st;
     bool graph = (flags & OWM_B_GRAPH) != 0;
+    const bool auto_graph = (flags & B_AUTO_GRAPH) != 0 && !graph;
     for (int attempt = 0;; ++attempt) {
         const uint64_t need = table_bytes(n_tiles) + tree_bytes(P.L) + build_scratch_bytes(P.L) +
                               query_bytes(P.Q, P.L.np, 0) + 8192;
@@ -1257,6 +1259,15 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
             return st;
         if (ctx->arena.buf.ptr != old_arena)
             ctx->graphs.clear();  // captured chains point into the old arena
+        if (auto_graph) {
+            // a call that repeats the previous one's layout exactly (same
+            // inputs, arena, table and configuration) replays its chain from a
+            // graph, captured on the first repeat; phase timing keeps it eager
+            std::vector<uint8_t> key;
+            graph_key(ctx, dx, dy, dz, P, key);
+            graph = !ctx->prof.on && key == ctx->graphs.last_key;
+            ctx->graphs.last_key.swap(key);
+        }
         Arena &A = ctx->arena;
         uint32_t *guard = (uint32_t *)A.take(256);
         BatchDev B;

@@ -218,6 +218,9 @@ owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what);
 // Copies a host cloud into the context's upload buffer (stream-ordered).
 owm_status upload_cloud(owm_ctx *ctx, const double *x, const double *y, const double *z,
                         uint64_t n, double **dx, double **dy, double **dz);
+// run_batch flag (internal, beside OWM_B_GRAPH): replay from a graph when the
+// call repeats the previous auto-graph call's chain exactly.
+constexpr uint32_t B_AUTO_GRAPH = 1u << 31;
 owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const double *dz,
                      const owm_tile *tiles, uint32_t n_tiles, uint32_t flags, uint32_t *seeds,
                      uint64_t cap, uint64_t *seed_off_host);
@@ -268,6 +271,7 @@ struct GraphCache {
         uint64_t used_after = 0;  // arena offset after the chain's allocations
     };
     std::vector<Slot> slots;
+    std::vector<uint8_t> last_key;  // key of the last auto-graph call (repeat detection)
     void clear();
     ~GraphCache();
 };

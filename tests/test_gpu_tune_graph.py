@@ -129,3 +129,38 @@ def test_many_small_tiles_over_two_million_points():
             want, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
             assert np.array_equal(g, want)
         assert _same(ctx.batch_run(x, y, z, desc, graph=True), got)
+
+
+def test_repeated_single_tile_calls_replay_a_graph():
+    """owm_run_tile* replays the chain from a graph when a call repeats the
+    previous call's layout exactly (captured on the first repeat): repeats of
+    one tile, with other work between them, and host-input calls whose clouds
+    differ but share the upload buffer, size and parameters, all give the
+    oracle's seeds."""
+    torch = pytest.importorskip("torch")
+    t = synth.scaled(synth.config_c2(seed=31), 400_000)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    want, _, _ = O.run(x, y, z, 12, 20, 0.8, 32)
+    other = synth.scaled(synth.config_c1(seed=32), 50_000)
+    ox, oy, oz = other.generate()
+    owant, _, _ = O.run(ox, oy, oz, 10, 15, 0.6, 16)
+    dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
+    with owm.Context(0) as ctx:
+        for rep in range(4):
+            say("device repeat", rep)
+            assert np.array_equal(ctx.run_tile(*dev, 12, 20, 0.8, 32), want)
+            assert np.array_equal(ctx.run_tile(*dev, 12, 20, 0.8, 32), want)
+            if rep % 2:
+                assert np.array_equal(ctx.run_tile(ox, oy, oz, 10, 15, 0.6, 16), owant)
+            torch.cuda.synchronize()
+        # same size, bbox and parameters, other points: the replayed chain
+        # reads the new upload
+        rng = np.random.default_rng(5)
+        perm = rng.permutation(len(x))
+        x2, y2, z2 = x[perm], y[perm], z[perm] + rng.normal(0, 0.5, len(x))
+        want2, _, _ = O.run(x2, y2, z2, 12, 20, 0.8, 32, box)
+        for rep in range(3):
+            say("host repeat", rep)
+            assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 32, bbox=box), want)
+            assert np.array_equal(ctx.run_tile(x2, y2, z2, 12, 20, 0.8, 32, bbox=box), want2)
