@@ -503,8 +503,6 @@ owm_status owm_tree_read(const owm_tree *tree, owm_tree_view *v)
         const void *src;
         uint64_t bytes;
     } copies[] = {
-        {v->ids, T.ids, n * 4},          {v->xs, T.xs, n * 8},
-        {v->ys, T.ys, n * 8},            {v->zs, T.zs, n * 8},
         {v->leaf_key, T.lf_key0, nl * 4}, {v->leaf_start, T.lf_start, nl * 4},
         {v->leaf_points, T.lf_npts, nl * 4}, {v->leaf_count, T.lf_cnt, nl * 4},
         {v->leaf_min, T.lf_zid, nl * 4}, {v->node_count, T.nd_cnt, nn * 4},
@@ -513,6 +511,27 @@ owm_status owm_tree_read(const owm_tree *tree, owm_tree_view *v)
     for (const auto &c : copies)
         if (st == OWM_OK)
             st = get(c.dst, c.src, c.bytes);
+    // the points are (x, y) pairs and (z, id, key) triples on the device
+    if (st == OWM_OK && n && (v->xs || v->ys)) {
+        std::vector<double2> xy(n);
+        st = get(xy.data(), T.pts.xy, n * 16);
+        for (uint64_t i = 0; st == OWM_OK && i < n; ++i) {
+            if (v->xs)
+                v->xs[i] = xy[i].x;
+            if (v->ys)
+                v->ys[i] = xy[i].y;
+        }
+    }
+    if (st == OWM_OK && n && (v->ids || v->zs)) {
+        std::vector<PointZ> zi(n);
+        st = get(zi.data(), T.pts.zi, n * 16);
+        for (uint64_t i = 0; st == OWM_OK && i < n; ++i) {
+            if (v->ids)
+                v->ids[i] = zi[i].id;
+            if (v->zs)
+                v->zs[i] = zi[i].z;
+        }
+    }
     if (st == OWM_OK)
         st = boxes(v->leaf_bbox, T.lf_box, nl);
     if (st == OWM_OK)

@@ -1,7 +1,7 @@
 // Device-side data layout and helpers shared by the OWM kernels and the host
 // orchestration.  Layout in HBM (all tiles of a batch concatenated):
 //   points    x, y, z            fp64 SoA, caller order, tile t = [pt_off, pt_off+n)
-//   sorted    xs, ys, zs, ids    fp64 SoA + u32 original index, Morton order
+//   sorted    xy (double2) + PointZ {z, id, key} (16 B each), Morton order
 //   leaves    box (minx,miny,maxx,maxy) f64x4, cnt, start, npts, zkey (u64), zid,
 //             parent, key0 (Morton key of the first point)
 //   internal  box f64x4, cnt, zkey, zid, child (u32x2, bit 31 = leaf), parent, flag
@@ -13,6 +13,32 @@
 #define OWM_NONE 0xFFFFFFFFu
 #define OWM_LEAF_BIT 0x80000000u
 #define OWM_ZKEY_NONE 0xFFFFFFFFFFFFFFFFull
+
+// One point of the sorted cloud: coordinates, batch index and sort key (the
+// top key_bits of its Morton code within its tile).  Stored as two 16-byte
+// streams, (x, y) and PointZ (z, id, key): every access is one 16-byte load
+// or store per lane, coalesced when lanes take consecutive points, and the
+// tree sort's digit runs are written at about the speed of a contiguous copy
+// (four SoA streams of 8- and 4-byte fields written the same way are about a
+// third slower; lanes reading whole 32-byte records use half of each
+// fetched span per load).
+struct PointRec {
+    double x, y, z;
+    uint32_t id, key;
+};
+struct __align__(16) PointZ {
+    double z;
+    uint32_t id, key;
+};
+struct PointsDev {
+    const double2 *xy;
+    const PointZ *zi;
+};
+struct PointsOut {
+    double2 *xy;
+    PointZ *zi;
+    __host__ __device__ operator PointsDev() const { return PointsDev{xy, zi}; }
+};
 
 // Per-tile description (device table).  Tree fields depend only on the cloud
 // and the leaf size; query fields on the OWM parameters.
@@ -54,8 +80,7 @@ struct BatchDev {
 
 // Tree arrays of a batch.
 struct TreeDev {
-    const double *xs, *ys, *zs;
-    const uint32_t *ids;  // sorted position -> original (batch-global) index
+    PointsDev pts;  // the cloud in Morton order; pts.zi[i].id = batch index
     const uint32_t *pos;  // original index -> sorted position (resident trees)
     const double *X, *Y;  // original cloud when still available (else pos is used)
     const double4 *lf_box;
@@ -126,5 +151,58 @@ __device__ __forceinline__ double owm_gcenter(double init, uint32_t bsize, uint3
 }
 
 __device__ __forceinline__ uint32_t owm_lane() { return __lane_id(); }
+
+// (v - vmin) * scale truncated to 32 bits, 0 when not positive (or NaN).
+__device__ __forceinline__ uint32_t quantise(double v, double vmin, double scale)
+{
+    double q = __dmul_rn(__dsub_rn(v, vmin), scale);
+    if (!(q > 0.0))
+        return 0u;
+    if (q >= 4294967295.0)
+        return 0xFFFFFFFFu;
+    return (uint32_t)q;
+}
+
+// A point's Morton code interleaves its quantised x (even bits) and y (odd
+// bits), 32 bits per axis over the tile's square bbox; the sort key is its top
+// key_bits bits.
+
+// The low 16 bits of v spread to the even bit positions of 32.
+__device__ __forceinline__ uint32_t spread16(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Sort key of a point: the top bits of its Morton code (shift = 64 - key
+// bits, key bits even and at most 32), computed as the interleave of the top
+// key_bits / 2 bits of each quantised coordinate (the same bits, with 32-bit
+// arithmetic).  The Morton pass (histograms) and the first sort pass (which
+// stores it in the point's record) both compute it with this function.
+__device__ __forceinline__ uint32_t sort_key(const TileDev &T, double x, double y, uint32_t shift)
+{
+    const uint32_t drop = 32u - (64u - shift) / 2u;  // low bits of each axis left out
+    const uint32_t hx = (uint32_t)((uint64_t)quantise(x, T.qminx, T.qscale) >> drop);
+    const uint32_t hy = (uint32_t)((uint64_t)quantise(y, T.qminy, T.qscale) >> drop);
+    return spread16(hx) | (spread16(hy) << 1);
+}
+
+// A whole point in two 16-byte loads.
+__device__ __forceinline__ PointRec owm_load_pt(const PointsDev &P, uint32_t i)
+{
+    const double2 xy = P.xy[i];
+    const uint4 b = *reinterpret_cast<const uint4 *>(P.zi + i);
+    PointRec r;
+    r.x = xy.x;
+    r.y = xy.y;
+    r.z = __hiloint2double((int)b.y, (int)b.x);
+    r.id = b.z;
+    r.key = b.w;
+    return r;
+}
 
 #endif

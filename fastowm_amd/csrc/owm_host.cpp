@@ -376,8 +376,8 @@ void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &
 uint64_t tree_bytes(const TreeLayout &L)
 {
     uint64_t b = 0;
-    b += align_up(L.np * 8ull, 256) * 3;                  // xs ys zs
-    b += align_up(L.np * 4ull, 256) * 2;                  // ids pos
+    b += align_up(L.np * 16ull, 256) * 2;                 // pts: xy, (z, id, key)
+    b += align_up(L.np * 4ull, 256);                      // pos
     b += align_up(L.nleaf * 32ull, 256);                  // lf_box
     b += align_up(L.nleaf * 4ull, 256) * 6;               // cnt start npts zid parent key0
     b += align_up(L.nleaf * 8ull, 256);                   // lf_zkey
@@ -396,10 +396,8 @@ void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
         return p;
     };
     const uint64_t nn = std::max<uint64_t>(L.nnode, 1);
-    T.xs = (double *)take(L.np * 8ull);
-    T.ys = (double *)take(L.np * 8ull);
-    T.zs = (double *)take(L.np * 8ull);
-    T.ids = (uint32_t *)take(L.np * 4ull);
+    T.pts.xy = (double2 *)take(L.np * 16ull);
+    T.pts.zi = (PointZ *)take(L.np * 16ull);
     T.pos = (uint32_t *)take(L.np * 4ull);
     T.lf_box = (double4 *)take(L.nleaf * 32ull);
     T.lf_cnt = (uint32_t *)take(L.nleaf * 4ull);
@@ -421,10 +419,7 @@ void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
 TreeDev tree_view(const TreeOut &T)
 {
     TreeDev V;
-    V.xs = T.xs;
-    V.ys = T.ys;
-    V.zs = T.zs;
-    V.ids = T.ids;
+    V.pts = PointsDev{T.pts.xy, T.pts.zi};
     V.pos = T.pos;
     V.lf_box = T.lf_box;
     V.lf_cnt = T.lf_cnt;
@@ -462,14 +457,24 @@ static uint64_t agg_levels_bytes(uint64_t nleaf)
     return align_up(e * 32, 256) + 2 * align_up(e * 4, 256) + align_up(e * 8, 256);
 }
 
-// Device scratch of one tree build: the sort buffers, one set of sorted point
-// records (two when more than two passes), and the aggregate levels.
+// Histograms, tile counters and look-back rows of the tree sort.
+static uint64_t tree_sort_bytes(const TreeLayout &L, int passes)
+{
+    const uint64_t lb_tiles = L.sort_units[L.ntiles] + 1;
+    return align_up((uint64_t)L.ntiles * SORT_SEG_HIST * 4, 256) + align_up(64, 256) +
+           align_up((uint64_t)passes * lb_tiles * 256 * 4, 256) + align_up(lb_tiles * 4, 256);
+}
+
+// Device scratch of one tree build: the tree sort's counters, the point
+// records between its passes (one set for two passes, two for more: the
+// first pass reads the caller's arrays, the last writes the tree's), and the
+// aggregate levels.
 uint64_t build_scratch_bytes(const TreeLayout &L)
 {
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    const int rec_sets = passes > 2 ? 2 : 1;
-    return sort_scratch_bytes(L.np, passes, L.ntiles, L.sort_units[L.ntiles] + 1) +
-           rec_sets * 3 * align_up(L.np * 8ull, 256) + agg_levels_bytes(L.nleaf);
+    const int rec_sets = passes > 2 ? 2 : passes - 1;
+    return tree_sort_bytes(L, passes) + rec_sets * 2 * align_up(L.np * 16ull, 256) +
+           agg_levels_bytes(L.nleaf);
 }
 
 SortBufs carve_sort(uint8_t *base, uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles)
@@ -527,14 +532,12 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     if (L.np == 0)
         return OWM_OK;
     const int passes = (L.tile_bits + L.key_bits + 7) / 8;
-    // record-sort tiles (their size follows the segments, not the key/value sort)
-    const uint64_t lb_tiles = L.sort_units[L.ntiles] + 1;
-    uint8_t *sb = scratch.take(sort_scratch_bytes(L.np, passes, L.ntiles, lb_tiles));
-    if (!sb) {
-        set_err(ctx, "scratch arena exhausted (tree sort)");
-        return OWM_E_INTERNAL;
-    }
-    SortBufs S = carve_sort(sb, L.np, passes, L.ntiles, lb_tiles);
+    // sort tiles of the segments, and one spare look-back row
+    const uint32_t tiles = L.sort_units[L.ntiles];
+    uint32_t *hist = (uint32_t *)scratch.take((uint64_t)L.ntiles * SORT_SEG_HIST * 4);
+    uint32_t *counters = (uint32_t *)scratch.take(64);
+    uint32_t *lookback = (uint32_t *)scratch.take((uint64_t)passes * (tiles + 1) * 256 * 4);
+    uint32_t *tile_seg = (uint32_t *)scratch.take((tiles + 1) * 4ull);
     uint64_t lv_entries = 0;
     AggLevels V = agg_levels_layout(L.nleaf, &lv_entries);
     lv_entries = std::max<uint64_t>(lv_entries, 1);
@@ -542,60 +545,47 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     V.cnt = (uint32_t *)scratch.take(lv_entries * 4);
     V.zid = (uint32_t *)scratch.take(lv_entries * 4);
     V.zk = (unsigned long long *)scratch.take(lv_entries * 8);
-    bool have = V.box && V.cnt && V.zid && V.zk;
-    // record sets between passes: (x, y, z) here, indices in the values buffers
-    RecOut rec[2] = {};
-    for (int k = 0; k < (passes > 2 ? 2 : 1); ++k) {
-        rec[k].x = (double *)scratch.take(L.np * 8ull);
-        rec[k].y = (double *)scratch.take(L.np * 8ull);
-        rec[k].z = (double *)scratch.take(L.np * 8ull);
-        rec[k].id = k ? S.v1 : S.v0;
-        have = have && rec[k].x && rec[k].y && rec[k].z;
+    bool have = hist && counters && lookback && tile_seg && V.box && V.cnt && V.zid && V.zk;
+    // records between passes: the cloud -> rec[0] -> rec[1] -> rec[0] ... -> the tree
+    PointsOut rec[2] = {};
+    for (int k = 0; k < (passes > 2 ? 2 : passes - 1); ++k) {
+        rec[k].xy = (double2 *)scratch.take(L.np * 16ull);
+        rec[k].zi = (PointZ *)scratch.take(L.np * 16ull);
+        have = have && rec[k].xy && rec[k].zi;
     }
     if (!have) {
         set_err(ctx, "scratch arena exhausted (tree build)");
         return OWM_E_INTERNAL;
     }
-    const uint32_t tiles = L.sort_units[L.ntiles];
     ZeroList z;
     z.add(s, guard, 64);
-    z.add(s, S.hist, (uint64_t)L.ntiles * SORT_SEG_HIST);
-    z.add(s, S.lookback, (uint64_t)passes * tiles * 256);
+    z.add(s, hist, (uint64_t)L.ntiles * SORT_SEG_HIST);
+    z.add(s, lookback, (uint64_t)passes * tiles * 256);
     z.launch(s);
     int pi = ctx->prof.begin(s, PH_MORTON);
-    launch_morton(s, B, dx, dy, S.k0, S.hist, passes);
-    // algorithmic bytes per point: read x, y; write a key
-    ctx->prof.end(s, pi, 20ull * L.np, L.np);
+    launch_morton(s, B, dx, dy, hist, passes);
+    // algorithmic bytes per point: x and y read
+    ctx->prof.end(s, pi, 16ull * L.np, L.np);
     SortSegs seg;
     seg.pt = B.pt_off;
     seg.units = B.sort_units;
     seg.nseg = L.ntiles;
     seg.digit_stride = SORT_SEG_HIST;
-    launch_sort_scan_seg(s, S.hist, seg.pt, seg.nseg, passes, S.counters);
-    // passes move (key, x, y, z, index): the cloud -> rec[0] -> rec[1] -> ... -> the tree
-    const RecOut tree_rec{T.xs, T.ys, T.zs, T.ids, T.pos};
-    uint32_t *kin = S.k0, *kout = S.k1;
+    seg.tile_seg = tile_seg;
+    launch_sort_scan_seg(s, hist, seg.pt, seg.units, seg.nseg, passes, counters, tile_seg);
     uint64_t sort_bytes = 0;
     pi = ctx->prof.begin(s, PH_SORT_TREE);
     for (int p = 0; p < passes; ++p) {
-        RecIn in;
-        if (p == 0) {
-            in = RecIn{dx, dy, dz, nullptr};
-        } else {
-            const RecOut &r = rec[(p - 1) & 1];
-            in = RecIn{r.x, r.y, r.z, r.id};
-        }
-        const RecOut &out = p == passes - 1 ? tree_rec : rec[p & 1];
         const bool last = p == passes - 1;
-        launch_sort_records(s, kin, in, kout, out, (uint32_t)L.np, 8 * p,
-                            L.tile_bits + L.key_bits, S.hist + 256 * p,
-                            S.lookback + (uint64_t)p * tiles * 256, S.counters + p, p == 0,
-                            last, seg, tiles);
-        // algorithmic bytes per point and pass: a key and a 28-byte record
-        // read and written, except that the first pass reads no index (it
-        // numbers the points) and the last writes no key
-        sort_bytes += (p == 0 || last ? 60ull : 64ull) * L.np;
-        std::swap(kin, kout);
+        const PointsIn in = p == 0 ? PointsIn{dx, dy, dz, PointsDev{}}
+                                   : PointsIn{nullptr, nullptr, nullptr, rec[(p - 1) & 1]};
+        const PointsOut out = last ? T.pts : rec[p & 1];
+        launch_sort_records(s, B, in, out, last ? T.pos : nullptr, (uint32_t)L.np, 8 * p,
+                            L.tile_bits + L.key_bits, hist + 256 * p,
+                            lookback + (uint64_t)p * tiles * 256, counters + p, p == 0, seg, tiles);
+        // algorithmic bytes per point and pass: a 32-byte record written, and
+        // read (the first pass reads x, y and z, and numbers the points)
+        sort_bytes += (p == 0 ? 56ull : 64ull) * L.np + (last && T.pos ? 4ull * L.np : 0ull);
     }
     ctx->prof.end(s, pi, sort_bytes, (uint64_t)L.np * passes, (uint32_t)passes);
     pi = ctx->prof.begin(s, PH_LEAVES);

@@ -13,6 +13,7 @@ namespace owm {
 struct SortSegs {
     const uint32_t *pt = nullptr;
     const uint32_t *units = nullptr;
+    const uint32_t *tile_seg = nullptr;  // segment of every sort tile (else searched in units)
     uint32_t nseg = 0;
     uint32_t digit_stride = 0;
 };
@@ -20,9 +21,10 @@ uint32_t sort_tiles(uint32_t n);
 // points per tile of an n-point record sort over nseg segments
 uint32_t record_tile_keys(uint32_t n, uint32_t nseg);
 constexpr uint32_t SORT_SEG_HIST = 4 * 256;  // histogram words per segment (all passes)
-// hist[t * SORT_SEG_HIST + p * 256 + d] -> segment base + exclusive offsets
-void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,
-                          int passes, uint32_t *counters);
+// hist[t * SORT_SEG_HIST + p * 256 + d] -> segment base + exclusive offsets;
+// tile_seg[u] = t for the sort tiles u of segment t (units[t] <= u < units[t+1])
+void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, const uint32_t *units,
+                          uint32_t nseg, int passes, uint32_t *counters, uint32_t *tile_seg);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 // Zeroes up to FILL_MAX ranges (u32 words) in one launch.
 constexpr uint32_t FILL_MAX = 16;
@@ -42,29 +44,27 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
                      uint32_t *tile_counter, bool keys_only, bool iota, bool write_keys,
                      uint32_t grid_cap, const SortSegs &seg = SortSegs(), uint32_t seg_tiles = 0);
 
-// Point records of the tree sort: coordinates and batch indices (id unused by
-// a first pass, which numbers the points itself; pos may be null).
-struct RecIn {
+// Input of a pass of the tree sort: the first pass reads the caller's
+// coordinate arrays (x, y, z; it numbers the points and computes their keys
+// from the tiles' quantisation), later passes the previous pass's records.
+struct PointsIn {
     const double *x, *y, *z;
-    const uint32_t *id;
+    PointsDev rec;
 };
-struct RecOut {
-    double *x, *y, *z;
-    uint32_t *id, *pos;
-};
-// One record-moving pass of the segmented tree sort over n points (segments
-// seg, seg_tiles sort tiles in all, keys of key_bits bits): the first pass
-// numbers the points itself, the last writes no keys (keys_out unused).
-void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
-                         uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
-                         int key_bits, const uint32_t *digit_off, uint32_t *lookback,
-                         uint32_t *tile_counter, bool first, bool last, const SortSegs &seg,
-                         uint32_t seg_tiles);
+
+// One pass of the segmented tree sort over n points (segments seg,
+// seg_tiles sort tiles in all), by the 8-bit digit at `shift` of keys of
+// key_bits bits; whole 32-byte records move.  pos (may be null): pos[id] =
+// output position, written by a last pass.
+void launch_sort_records(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
+                         uint32_t *pos, uint32_t n, int shift, int key_bits,
+                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
+                         bool first, const SortSegs &seg, uint32_t seg_tiles);
 
 // ---- tree (tree.hip) ----
 struct TreeOut {
-    double *xs, *ys, *zs;
-    uint32_t *ids, *pos;
+    PointsOut pts;  // the cloud in Morton order
+    uint32_t *pos;
     double4 *lf_box;
     uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid, *lf_parent, *lf_key0;
     unsigned long long *lf_zkey;
@@ -81,11 +81,12 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 // (min x, min y, max x, max y as order keys per tile).
 uint64_t bbox_key_words(uint32_t ntiles);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
-// hist: per tile SORT_SEG_HIST digit counters of every pass
+// Per tile SORT_SEG_HIST digit counters of every pass of the points' sort keys
+// (the keys themselves are not stored: the first sort pass computes them again).
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   uint32_t *keys, uint32_t *hist, int passes);
-// Leaf aggregates over the points in Morton order (T.xs, ys, zs, ids); each
-// leaf's first key is recomputed from its first point.
+                   uint32_t *hist, int passes);
+// Leaf aggregates over the points in Morton order (T.pts); each leaf's first
+// key is its first point's.
 void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T);
 // Aggregate levels: level k (1..nlev) holds the merged aggregate of every
 // aligned block of 2^k leaves of the batch, size[k] entries from entry off[k].

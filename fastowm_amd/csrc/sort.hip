@@ -17,11 +17,10 @@
 // its look-back chain, so the keys hold Morton bits only and every pass
 // permutes within a segment.
 //
-// Record form (the tree sort): every pass moves the points' x, y, z and batch
-// index with the keys, the first reading the caller's coordinate arrays and
-// the last writing the tree's (without keys: the tree recomputes the few it
-// needs), so the cloud ends in Morton order without a gather by index (random
-// 32-byte reads run at a fraction of HBM bandwidth).
+// Record form (the tree sort): every pass moves whole 32-byte point records
+// (x, y, z, batch index, key); the first reads the caller's coordinate arrays
+// and computes each point's key, so the cloud ends in Morton order without a
+// gather by index (random 32-byte reads run at a fraction of HBM bandwidth).
 #include "owm_kernels.h"
 
 namespace owm {
@@ -95,13 +94,17 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const uint32_t *__restrict_
 }
 
 // Per segment t (one workgroup each): hist[t][p][d] -> pt[t] + exclusive
-// offsets within the segment; workgroup 0 also zeroes the tile counters.
+// offsets within the segment, and tile_seg[u] = t for the segment's sort
+// tiles; workgroup 0 also zeroes the tile counters.
 __global__ __launch_bounds__(BLOCK) void k_sort_scan_seg(uint32_t *hist, const uint32_t *pt,
-                                                          uint32_t nseg, int passes,
-                                                          uint32_t *counters)
+                                                          const uint32_t *units, uint32_t nseg,
+                                                          int passes, uint32_t *counters,
+                                                          uint32_t *tile_seg)
 {
     __shared__ uint32_t tmp[WAVES];
     for (uint32_t t = blockIdx.x; t < nseg; t += gridDim.x) {
+        for (uint32_t u = units[t] + threadIdx.x; u < units[t + 1]; u += BLOCK)
+            tile_seg[u] = t;
         uint32_t *h = hist + (size_t)t * SEG_HIST;
         const uint32_t b = pt[t];
         for (int p = 0; p < passes; ++p) {
@@ -146,7 +149,7 @@ template <int TILE_T, int NW>
 __device__ __forceinline__ bool sweep_claim(SweepLds<NW> &L, uint32_t *tile_counter, uint32_t n_all,
                                             const SortSegs &seg, const uint32_t *digit_off,
                                             uint32_t &tile, uint32_t &base, uint32_t &n, bool &first,
-                                            const uint32_t *&doff)
+                                            const uint32_t *&doff, uint32_t *seg_index = nullptr)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = seg.pt ? seg.units[seg.nseg] : (n_all + TILE_T - 1) / TILE_T;
@@ -160,15 +163,19 @@ __device__ __forceinline__ bool sweep_claim(SweepLds<NW> &L, uint32_t *tile_coun
         return false;
     doff = digit_off;
     if (seg.pt) {
-        const uint32_t t = owm_find_tile(seg.units, seg.nseg, tile);
+        const uint32_t t = seg.tile_seg ? seg.tile_seg[tile] : owm_find_tile(seg.units, seg.nseg, tile);
         base = seg.pt[t] + (tile - seg.units[t]) * TILE_T;
         n = seg.pt[t + 1];
         first = tile == seg.units[t];
         doff = digit_off + (size_t)t * seg.digit_stride;
+        if (seg_index)
+            *seg_index = t;
     } else {
         base = tile * TILE_T;
         n = n_all;
         first = tile == 0;
+        if (seg_index)
+            *seg_index = 0;
     }
     return true;
 }
@@ -231,17 +238,31 @@ __device__ __forceinline__ void sweep_offsets(SweepLds<NW> &L, uint32_t tile, bo
             __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // walk back over the predecessors four flags at a time (their
+            // loads in flight together), in order: add aggregates until an
+            // inclusive prefix; at a predecessor still ranking (it holds an
+            // earlier tile, so it is resident) poll again from there.  The
+            // walk ends at the latest at the segment's first tile (inclusive),
+            // so indices below it are loaded only as spare reads.
             int32_t j = (int32_t)tile - 1;
-            for (;;) {
-                const uint32_t f = __hip_atomic_load(lookback + (size_t)j * RADIX + d, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t st = f & ~LB_CNT;
-                if (st == 0)
-                    continue;  // predecessor still ranking (it holds an earlier tile, so it is resident)
-                prefix += f & LB_CNT;
-                if (st == LB_INC)
-                    break;
-                --j;
+            for (bool done = false; !done;) {
+                uint32_t f[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    f[q] = __hip_atomic_load(lookback + (size_t)max(j - q, 0) * RADIX + d,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int q = 0;
+                for (; q < 4; ++q) {
+                    const uint32_t st = f[q] & ~LB_CNT;
+                    if (st == 0)
+                        break;
+                    prefix += f[q] & LB_CNT;
+                    if (st == LB_INC) {
+                        done = true;
+                        break;
+                    }
+                }
+                j -= q;
             }
             __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -315,180 +336,110 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
     }
 }
 
-// One tile of a record-moving pass (see k_sort_records).  FULL: the tile holds
-// TILE_T points, so no load or store needs a bound check; otherwise loads are
-// clamped to the tile's last point and the items past it take the unused tile
-// slots past cnt, so that the LDS writes need no check either.
-// K16: 16-bit keys sorted by their low byte (the first of two passes): a key is
-// its digit and the byte above it, kept from the ranking, so the keys are not
-// read a second time for the key/index staging.
-template <bool IOTA, bool LAST, bool POS, bool K16, int ITEMS_T, int NW, bool FULL>
-__device__ __forceinline__ void records_tile(SweepLds<NW> &L, double *s_buf, uint8_t *s_dig,
-                                             const uint32_t *__restrict__ keys_in, const RecIn &in,
-                                             uint32_t *__restrict__ keys_out, const RecOut &out,
-                                             uint32_t tile, uint32_t base, uint32_t cnt, bool first,
-                                             const uint32_t *doff, uint32_t *lookback, int shift)
+// One pass of the segmented tree sort (the segments are the tiles of a
+// batch).  Whole 32-byte point records move: a workgroup loads its sort
+// tile's records into registers (the first pass reads the caller's x, y, z,
+// numbers the points and computes their keys from their tile's quantisation),
+// ranks them by the pass's digit (stable: item k of lane l of wave w is tile
+// position w*64*ITEMS + k*64 + l, ranked in that order), resolves the digits'
+// output bases by decoupled look-back, and then writes the tile out STAGE
+// slots at a time: the stage's points go to LDS in slot (digit) order, then
+// consecutive lanes write consecutive 16-byte (x, y) pairs, then (z, id,
+// key) triples, so each digit's run leaves as two contiguous streams.  POS:
+// pos[id] = the point's output position.
+template <bool FIRST, bool POS, int NW, int ITEMS, int STAGE>
+__global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut out,
+                                                          uint32_t *__restrict__ pos,
+                                                          const TileDev *__restrict__ tiles,
+                                                          uint32_t key_shift, uint32_t n_all,
+                                                          int shift,
+                                                          const uint32_t *__restrict__ digit_off,
+                                                          uint32_t *lookback, uint32_t *tile_counter,
+                                                          SortSegs seg)
 {
+    constexpr int TILE_T = NW * 64 * ITEMS;
+    static_assert(TILE_T % STAGE == 0, "a sort tile is a whole number of stages");
+    __shared__ SweepLds<NW> L;
+    __shared__ uint4 s_xy[STAGE], s_zi[STAGE];  // a stage of points in slot order
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t p0 = w * (64 * ITEMS_T) + lane;  // tile position of the thread's item 0
-    unsigned long long *const s_u64 = reinterpret_cast<unsigned long long *>(s_buf);
-    // element of item k; o is p0 passed through an empty asm, so that every
-    // field computes its own addresses (otherwise the compiler keeps the 64-bit
-    // addresses of all items live from one field to the next, and spills)
-    auto fresh = [&]() {
-        uint32_t o = p0;
-        asm volatile("" : "+v"(o));
-        return o;
-    };
-    auto at = [&](uint32_t o, int k) { return base + (FULL ? o + k * 64 : min(o + k * 64, cnt - 1)); };
-    double v[ITEMS_T];
-    auto load = [&](const double *__restrict__ src) {
-        const uint32_t o = fresh();
+    const uint32_t p0 = w * (64 * ITEMS) + lane;  // tile position of the thread's item 0
+    uint4 *const oxy = reinterpret_cast<uint4 *>(out.xy);
+    uint4 *const ozi = reinterpret_cast<uint4 *>(out.zi);
+    uint32_t tile, base, n, t;
+    bool first;
+    const uint32_t *doff;
+    while (sweep_claim<TILE_T, NW>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff,
+                                   &t)) {
+        const uint32_t cnt = min((uint32_t)TILE_T, n - base);
+        double x[ITEMS], y[ITEMS], z[ITEMS];
+        uint32_t id[ITEMS], key[ITEMS];
+        // loads clamped to the tile's last point (items past it are unused)
+        if constexpr (FIRST) {
+            const TileDev &TD = tiles[t];
 #pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            v[k] = src[at(o, k)];
-    };
-    // per item its tile slot (16 bits, two per word); K16: the key's high
-    // byte (four per word)
-    uint32_t slot2[ITEMS_T / 2], hi4[K16 ? ITEMS_T / 4 : 1];
-    {
-        // keys, then (in place) [high byte << 24 |] digit << 16 | in-wave rank
-        uint32_t kr[ITEMS_T];
-        const uint32_t o = fresh();
+            for (int k = 0; k < ITEMS; ++k) {
+                const uint32_t i = base + min(p0 + k * 64, cnt - 1);
+                x[k] = in.x[i];
+                y[k] = in.y[i];
+                z[k] = in.z[i];
+                id[k] = i;
+            }
 #pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            kr[k] = keys_in[at(o, k)];
-        load(in.x);
-        sweep_rank<ITEMS_T, FULL, true, NW, K16>(L, kr, 0, cnt, shift, kr);
+            for (int k = 0; k < ITEMS; ++k)
+                key[k] = sort_key(TD, x[k], y[k], key_shift);
+        } else {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const PointRec r = owm_load_pt(in.rec, base + min(p0 + k * 64, cnt - 1));
+                x[k] = r.x;
+                y[k] = r.y;
+                z[k] = r.z;
+                id[k] = r.id;
+                key[k] = r.key;
+            }
+        }
+        uint32_t slot[ITEMS];
+        sweep_rank<ITEMS, false, false, NW>(L, key, 0, cnt, shift, slot);
         __syncthreads();
         sweep_offsets(L, tile, first, doff, lookback);
 #pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k) {
-            const uint32_t dd = (kr[k] >> 16) & 255u;
-            if constexpr (K16) {
-                const uint32_t h = kr[k] >> 24;
-                hi4[k / 4] = (k & 3) ? hi4[k / 4] | h << (8 * (k & 3)) : h;
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t d = (key[k] >> shift) & (RADIX - 1);
+            // items past the tile's end take the unused slots past cnt
+            slot[k] = p0 + k * 64 < cnt ? L.excl[d] + L.whist[w][d] + slot[k] : p0 + k * 64;
+        }
+        for (uint32_t lo = 0; lo < cnt; lo += STAGE) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const uint32_t r = slot[k] - lo;
+                if (r < (uint32_t)STAGE) {
+                    const unsigned long long bx = (unsigned long long)__double_as_longlong(x[k]);
+                    const unsigned long long by = (unsigned long long)__double_as_longlong(y[k]);
+                    const unsigned long long bz = (unsigned long long)__double_as_longlong(z[k]);
+                    s_xy[r] = make_uint4((uint32_t)bx, (uint32_t)(bx >> 32), (uint32_t)by,
+                                         (uint32_t)(by >> 32));
+                    s_zi[r] = make_uint4((uint32_t)bz, (uint32_t)(bz >> 32), id[k], key[k]);
+                }
             }
-            uint32_t sl = L.excl[dd] + L.whist[w][dd] + (kr[k] & 0xFFFFu);
-            if (!FULL)
-                sl = p0 + k * 64 < cnt ? sl : p0 + k * 64;
-            s_dig[sl] = (uint8_t)dd;
-            s_buf[sl] = v[k];
-            if (k & 1)
-                slot2[k / 2] |= sl << 16;
-            else
-                slot2[k / 2] = sl;
+            __syncthreads();
+            // first the stage's (x, y) pairs, then its (z, id, key) triples
+            const uint32_t m = min((uint32_t)STAGE, cnt - lo);
+            for (uint32_t c = tid; c < 2 * m; c += NW * 64) {
+                const bool zpart = c >= m;
+                const uint32_t r = zpart ? c - m : c;
+                const uint32_t d = (s_zi[r].w >> shift) & (RADIX - 1);
+                const uint32_t dst = L.glob[d] + lo + r;
+                if (zpart) {
+                    const uint4 v = s_zi[r];
+                    ozi[dst] = v;
+                    if (POS)
+                        pos[v.z] = dst;
+                } else {
+                    oxy[dst] = s_xy[r];
+                }
+            }
+            __syncthreads();
         }
-    }
-    __syncthreads();
-    auto slot_of = [&](int k) { return (slot2[k / 2] >> (16 * (k & 1))) & 0xFFFFu; };
-    // the staged field goes out (then a barrier); the next one, already
-    // loaded, comes in (then a barrier).  Destinations are looked up per slot
-    // (digit, then the digit's base) in a rolled loop: unrolled, the compiler
-    // hoists every lookup and address and runs out of registers.
-    auto store = [&](double *__restrict__ to) {
-#pragma unroll 2
-        for (uint32_t s = tid; s < cnt; s += NW * 64)
-            to[L.glob[s_dig[s]] + s] = s_buf[s];
-        __syncthreads();
-    };
-    auto stage = [&]() {
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            s_buf[slot_of(k)] = v[k];
-        __syncthreads();
-    };
-    load(in.y);
-    store(out.x);
-    stage();
-    load(in.z);
-    store(out.y);
-    stage();
-    if constexpr (LAST) {
-        // the last pass writes no keys: the index alone
-        const uint32_t o = fresh();
-        uint32_t id[ITEMS_T];
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            id[k] = IOTA ? base + o + k * 64 : in.id[at(o, k)];
-        store(out.z);
-        uint32_t *const s_u32 = reinterpret_cast<uint32_t *>(s_buf);
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            s_u32[slot_of(k)] = id[k];
-        __syncthreads();
-        for (uint32_t s = tid; s < cnt; s += NW * 64) {
-            const uint32_t v = s_u32[s], dst = L.glob[s_dig[s]] + s;
-            out.id[dst] = v;
-            if (POS)
-                out.pos[v] = dst;
-        }
-        __syncthreads();
-        return;
-    }
-    // key and index travel together as one 8-byte word
-    {
-        unsigned long long kv[ITEMS_T];
-        const uint32_t o = fresh();
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k) {
-            const uint32_t id = IOTA ? base + o + k * 64 : in.id[at(o, k)];
-            uint32_t key;
-            if constexpr (K16)
-                key = ((hi4[k / 4] >> (8 * (k & 3))) & 255u) << 8 | s_dig[slot_of(k)];
-            else
-                key = keys_in[at(o, k)];
-            kv[k] = ((unsigned long long)id << 32) | key;
-        }
-        store(out.z);
-#pragma unroll
-        for (int k = 0; k < ITEMS_T; ++k)
-            s_u64[slot_of(k)] = kv[k];
-        __syncthreads();
-    }
-    for (uint32_t s = tid; s < cnt; s += NW * 64) {
-        const unsigned long long kv = s_u64[s];
-        const uint32_t dst = L.glob[s_dig[s]] + s;
-        keys_out[dst] = (uint32_t)kv;
-        out.id[dst] = (uint32_t)(kv >> 32);
-    }
-    __syncthreads();
-}
-
-// One pass of the segmented tree sort that moves whole point records with the
-// keys, so that the last pass leaves the cloud in Morton order and no gather
-// by index follows.  Records are x, y, z (8 bytes each) and the point's batch
-// index.  The first pass reads the caller's coordinate arrays and numbers the
-// points itself (IOTA); the last writes the tree's arrays (and, with POS, each
-// point's sorted position at its index; LAST: no keys are written).  Ranks come
-// from the keys alone; each
-// field is then staged through one LDS buffer in digit order and written out,
-// so consecutive lanes write consecutive addresses.  The loads of a field are
-// issued before the stores of the previous one (those of x before the
-// ranking), so that their latency hides behind that work.
-template <bool IOTA, bool LAST, bool POS, bool K16, int ITEMS_T, int NW>
-__global__ __launch_bounds__(NW * 64, 8 / NW) void k_sort_records(const uint32_t *__restrict__ keys_in,
-                                                                   RecIn in, uint32_t *__restrict__ keys_out,
-                                                                   RecOut out, uint32_t n_all, int shift,
-                                                                   const uint32_t *__restrict__ digit_off,
-                                                                   uint32_t *lookback,
-                                                                   uint32_t *tile_counter, SortSegs seg)
-{
-    constexpr int TILE_T = NW * 64 * ITEMS_T;
-    __shared__ SweepLds<NW> L;
-    __shared__ double s_buf[TILE_T];
-    __shared__ uint8_t s_dig[TILE_T];  // digit of every tile slot
-    uint32_t tile, base, n;
-    bool first;
-    const uint32_t *doff;
-    while (sweep_claim<TILE_T, NW>(L, tile_counter, n_all, seg, digit_off, tile, base, n, first, doff)) {
-        if (n - base >= (uint32_t)TILE_T)
-            records_tile<IOTA, LAST, POS, K16, ITEMS_T, NW, true>(L, s_buf, s_dig, keys_in, in, keys_out, out,
-                                                       tile, base, TILE_T, first, doff, lookback, shift);
-        else
-            records_tile<IOTA, LAST, POS, K16, ITEMS_T, NW, false>(L, s_buf, s_dig, keys_in, in, keys_out, out,
-                                                        tile, base, n - base, first, doff, lookback,
-                                                        shift);
     }
 }
 
@@ -552,11 +503,12 @@ void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *count
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(BLOCK), 0, s, hist, passes, counters);
 }
 
-void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, uint32_t nseg,
-                          int passes, uint32_t *counters)
+void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, const uint32_t *units,
+                          uint32_t nseg, int passes, uint32_t *counters, uint32_t *tile_seg)
 {
     const uint32_t g = nseg < 1 ? 1 : (nseg > 4096 ? 4096 : nseg);
-    hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, hist, pt, nseg, passes, counters);
+    hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, hist, pt, units, nseg, passes,
+                       counters, tile_seg);
 }
 
 
@@ -605,57 +557,51 @@ void launch_onesweep(hipStream_t s, const uint32_t *keys_in, const uint32_t *val
                           lookback, tile_counter, keys_only, iota, wk, grid, seg);
 }
 
-// Record sort tiles: 16384 points (8 waves x 32) for sorts of >= BIG_SORT
-// points whose segments average >= 64K points, where each digit's run per tile
-// is long enough (64 points) that the scattered writes mostly fill whole cache
-// lines; 4096 (4 waves x 16) otherwise (every segment ends in a partial tile).
+// Tree-sort shapes: 8192-point sort tiles (8 waves x 16 points, two stages
+// of 4096 records in LDS) for sorts of >= BIG_SORT points whose segments
+// average >= 64K points; 4096-point tiles (8 waves x 8, one stage) otherwise,
+// so that small sorts spread over more workgroups and short segments waste
+// less of their last tile.
 static inline bool big_records(uint32_t n, uint32_t nseg)
 {
     return n >= BIG_SORT && n / (nseg ? nseg : 1u) >= 65536u;
 }
-uint32_t record_tile_keys(uint32_t n, uint32_t nseg) { return big_records(n, nseg) ? 8 * 64 * 32 : 4 * 64 * 16; }
+uint32_t record_tile_keys(uint32_t n, uint32_t nseg) { return big_records(n, nseg) ? 8192u : 4096u; }
 
-template <int IT, int NW>
-static void sort_records_shape(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
-                               uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
+template <int NW, int ITEMS, int STAGE>
+static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
+                               uint32_t *pos, uint32_t n, int shift, uint32_t key_shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                               bool first, bool last, bool k16, const SortSegs &seg, uint32_t grid)
+                               bool first, const SortSegs &seg, uint32_t grid)
 {
     const dim3 b(NW * 64);
-#define OWM_REC(IOTA, LAST, POS, K16)                                                              \
-    hipLaunchKernelGGL((k_sort_records<IOTA, LAST, POS, K16, IT, NW>), dim3(grid), b, 0, s, keys_in, \
-                       in, keys_out, out, n, shift, digit_off, lookback, tile_counter, seg)
-    if (first && last && out.pos)
-        OWM_REC(true, true, true, false);  // a single pass
-    else if (first && last)
-        OWM_REC(true, true, false, false);
-    else if (first && k16)
-        OWM_REC(true, false, false, true);
+#define OWM_REC(FIRST, POS)                                                                        \
+    hipLaunchKernelGGL((k_sort_records<FIRST, POS, NW, ITEMS, STAGE>), dim3(grid), b, 0, s, in, out, \
+                       pos, B.tiles, key_shift, n, shift, digit_off, lookback, tile_counter, seg)
+    if (first && pos)
+        OWM_REC(true, true);
     else if (first)
-        OWM_REC(true, false, false, false);
-    else if (!last)
-        OWM_REC(false, false, false, false);
-    else if (out.pos)
-        OWM_REC(false, true, true, false);
+        OWM_REC(true, false);
+    else if (pos)
+        OWM_REC(false, true);
     else
-        OWM_REC(false, true, false, false);
+        OWM_REC(false, false);
 #undef OWM_REC
 }
 
-void launch_sort_records(hipStream_t s, const uint32_t *keys_in, const RecIn &in,
-                         uint32_t *keys_out, const RecOut &out, uint32_t n, int shift,
-                         int key_bits, const uint32_t *digit_off, uint32_t *lookback,
-                         uint32_t *tile_counter, bool first, bool last, const SortSegs &seg,
-                         uint32_t seg_tiles)
+void launch_sort_records(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
+                         uint32_t *pos, uint32_t n, int shift, int key_bits,
+                         const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
+                         bool first, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
-    const bool k16 = key_bits <= 16 && shift == 0;
+    const uint32_t key_shift = 64u - (uint32_t)key_bits;
     if (big_records(n, seg.nseg))
-        sort_records_shape<32, 8>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, first, last, k16, seg, grid);
+        sort_records_shape<8, 16, 4096>(s, B, in, out, pos, n, shift, key_shift, digit_off, lookback,
+                                        tile_counter, first, seg, grid);
     else
-        sort_records_shape<16, 4>(s, keys_in, in, keys_out, out, n, shift, digit_off, lookback,
-                                  tile_counter, first, last, k16, seg, grid);
+        sort_records_shape<8, 8, 4096>(s, B, in, out, pos, n, shift, key_shift, digit_off, lookback,
+                                       tile_counter, first, seg, grid);
 }
 
 }  // namespace owm

@@ -272,9 +272,9 @@ __global__ __launch_bounds__(256) void k_lcs(BatchDev B, TreeDev T, LeafCells LC
         uint32_t key = OWM_NONE, id = OWM_NONE, cnt = 0;
         unsigned long long zk = OWM_ZKEY_NONE;
         if (in) {
-            const uint32_t p = TD.pt_off + jl * L + off;
-            const double x = T.xs[p], y = T.ys[p], z = T.zs[p];
-            id = T.ids[p];
+            const PointRec r = owm_load_pt(T.pts, TD.pt_off + jl * L + off);
+            const double x = r.x, y = r.y, z = r.z;
+            id = r.id;
             // points outside the outermost edges lie in no window
             if (x > edge_at(TD.initx, disp, hd, 0, false) &&
                 x < edge_at(TD.initx, disp, hd, TD.ncols - 1, true) &&
@@ -502,20 +502,19 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
                     if (sub >= nl_grp || li >= mraw)
                         continue;
                     const uint32_t a = sLfA[li], n = sLfB[li];
-                    const uint32_t o = seg * 64 + off;
                     if (n & S1_LEAFTAG) {
-                        if (o == 0) {
+                        if (seg * 64 + off == 0) {
                             cell[k] = n & ~S1_LEAFTAG;
                             cn[k] = T.lf_cnt[a];
                             zk[k] = T.lf_zkey[a];
                             idv[k] = T.lf_zid[a];
                         }
-                    } else if (o < n) {
-                        const uint32_t p = a + o;
-                        xv[k] = T.xs[p];
-                        yv[k] = T.ys[p];
-                        zv[k] = T.zs[p];
-                        idv[k] = T.ids[p];
+                    } else if (seg * 64 + off < n) {
+                        const PointRec r = owm_load_pt(T.pts, a + seg * 64 + off);
+                        xv[k] = r.x;
+                        yv[k] = r.y;
+                        zv[k] = r.z;
+                        idv[k] = r.id;
                         cell[k] = S1_POINT;
                     }
                 }

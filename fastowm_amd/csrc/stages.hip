@@ -157,9 +157,9 @@ __global__ void k_s3_mark(BatchDev B, TreeDev T, const uint32_t *__restrict__ ll
             x = T.X[id];
             y = T.Y[id];
         } else {
-            const uint32_t sp = T.pos[id];
-            x = T.xs[sp];
-            y = T.ys[sp];
+            const double2 xy = T.pts.xy[T.pos[id]];
+            x = xy.x;
+            y = xy.y;
         }
         if (isnan(x) || isnan(y))
             continue;
@@ -274,13 +274,14 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
                     const uint32_t l = node & ~OWM_LEAF_BIT;
                     const uint32_t s0 = T.lf_start[l], np = T.lf_npts[l];
                     for (uint32_t p = s0; p < s0 + np; ++p) {
-                        const double x = T.xs[p], y = T.ys[p];
+                        const PointRec r = owm_load_pt(T.pts, p);
+                        const double x = r.x, y = r.y;
                         if (x > b0 && x < b2 && y > b1 && y < b3) {
                             ++cnt;
-                            const double z = T.zs[p];
+                            const double z = r.z;
                             if (owm_eligible(z)) {
                                 const unsigned long long k = owm_zkey(z);
-                                const uint32_t id = T.ids[p];
+                                const uint32_t id = r.id;
                                 if (owm_less(k, id, bk, bid)) {
                                     bk = k;
                                     bid = id;
@@ -318,11 +319,11 @@ __global__ __launch_bounds__(256) void k_s3_query(BatchDev B, TreeDev T,
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     in[q] = o + lane < np[q];
-                    const uint32_t p = s0[q] + (in[q] ? o + lane : 0u);
-                    x[q] = T.xs[p];
-                    y[q] = T.ys[p];
-                    z[q] = T.zs[p];
-                    id[q] = T.ids[p];
+                    const PointRec r = owm_load_pt(T.pts, s0[q] + (in[q] ? o + lane : 0u));
+                    x[q] = r.x;
+                    y[q] = r.y;
+                    z[q] = r.z;
+                    id[q] = r.id;
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {

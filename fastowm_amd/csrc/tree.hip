@@ -196,49 +196,12 @@ __global__ void k_bbox_fold(unsigned long long *bk, uint32_t ntiles)
     }
 }
 
-__device__ __forceinline__ unsigned long long spread2(uint32_t v)
-{
-    unsigned long long x = v;
-    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
-    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
-    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
-    x = (x | (x << 2)) & 0x3333333333333333ull;
-    x = (x | (x << 1)) & 0x5555555555555555ull;
-    return x;
-}
-
-__device__ __forceinline__ uint32_t quantise(double v, double vmin, double scale)
-{
-    double q = __dmul_rn(__dsub_rn(v, vmin), scale);
-    if (!(q > 0.0))
-        return 0u;
-    if (q >= 4294967295.0)
-        return 0xFFFFFFFFu;
-    return (uint32_t)q;
-}
-
-// 64-bit 2D Morton code: 32 bits per axis over the tile's square bbox,
-// x in the even bits, y in the odd bits.
-__device__ __forceinline__ unsigned long long morton64(const TileDev &T, double x, double y)
-{
-    uint32_t qx = quantise(x, T.qminx, T.qscale);
-    uint32_t qy = quantise(y, T.qminy, T.qscale);
-    return spread2(qx) | (spread2(qy) << 1);
-}
-
-// Sort key of a point: the top bits of its Morton code (shift = 64 - key bits).
-// The tree pass recomputes a leaf's first key from its first point with this
-// same function, so the last sort pass need not write keys.
-__device__ __forceinline__ uint32_t sort_key(const TileDev &T, double x, double y, uint32_t shift)
-{
-    return (uint32_t)(morton64(T, x, y) >> shift);
-}
-
-// Sort keys (the top key_bits of each point's Morton code within its tile)
-// plus each tile's digit histograms of every radix pass.
+// Each tile's digit histograms of every radix pass of its points' sort keys
+// (the top key_bits of each point's Morton code within its tile); the first
+// sort pass computes the keys again rather than reading them.
 __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restrict__ X,
-                                                const double *__restrict__ Y,
-                                                uint32_t *__restrict__ keys, uint32_t *hist, int passes)
+                                                const double *__restrict__ Y, uint32_t *hist,
+                                                int passes)
 {
     // digit counts of the tile `cur` (uniform over the block), flushed to the
     // tile's histogram when the block moves to another tile
@@ -286,7 +249,6 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
                     const uint32_t i = base + j * TB;
                     if (i < c1) {
                         const uint32_t k = sort_key(T, xv[j], yv[j], shift);
-                        keys[i] = k;
                         for (int p = 0; p < passes; ++p)
                             atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
                     }
@@ -301,7 +263,6 @@ __global__ __launch_bounds__(TB) void k_morton(BatchDev B, const double *__restr
             const TileDev &T = B.tiles[t];
             const double x = X[i], y = Y[i];
             const uint32_t k = sort_key(T, x, y, shift);
-            keys[i] = k;
             uint32_t *g = hist + (size_t)t * SORT_SEG_HIST;
             for (int p = 0; p < passes; ++p)
                 atomicAdd(&g[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
@@ -460,7 +421,7 @@ __device__ __forceinline__ V shfl_any(V v, int src)
 // leaf's first point and stores the leaf.
 template <int LL>
 __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint32_t g0,
-                                            uint32_t lane, uint32_t shift)
+                                            uint32_t lane)
 {
     constexpr int NK = LL <= 64 ? 64 / LL : 1;  // leaves of a lane's points
     constexpr int QL = 8 / NK;                  // a lane's points per leaf
@@ -469,18 +430,19 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
 #pragma unroll 1
     for (int r = 0; r < GPTS / 64; ++r) {
         const uint32_t e0 = (g0 - TD.grp_off) * GPTS + r * 512 + seg * 64;
-        uint32_t v[8];
+        uint32_t v[8], kq[8];
         double x[8], y[8], z[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const uint32_t k = q / QL;
             const uint32_t e = e0 + q * 8 + ((sub - k) & 7);
             // loads clamped to the tile's last point (no branch per load)
-            const uint32_t p = TD.pt_off + min(e, TD.n - 1);
-            v[q] = T.ids[p];
-            x[q] = T.xs[p];
-            y[q] = T.ys[p];
-            z[q] = T.zs[p];
+            const PointRec r = owm_load_pt(T.pts, TD.pt_off + min(e, TD.n - 1));
+            v[q] = r.id;
+            x[q] = r.x;
+            y[q] = r.y;
+            z[q] = r.z;
+            kq[q] = r.key;
         }
         LeafAgg a[NK];
 #pragma unroll
@@ -497,19 +459,17 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
                 leaf_agg_step<8>(a[k]);
         }
         LeafAgg mine = a[0];
-        double fx = x[0], fy = y[0];
+        uint32_t fk = kq[0];  // the key of the leaf's first point
 #pragma unroll
         for (int k = 1; k < NK; ++k)
             if (sub == (uint32_t)k) {
                 mine = a[k];
-                fx = x[k * QL];
-                fy = y[k * QL];
+                fk = kq[k * QL];
             }
         const uint32_t jl = e0 / LL + sub;
         const bool store = LL <= 64 ? sub < (uint32_t)NK : (sub == 0 && (seg & 1) == 0);
         if (store && jl < TD.nleaf)
-            leaf_store(TD, T, sort_key(TD, fx, fy, shift), jl, TD.pt_off + jl * LL,
-                       min((uint32_t)LL, TD.n - jl * LL), mine);
+            leaf_store(TD, T, fk, jl, TD.pt_off + jl * LL, min((uint32_t)LL, TD.n - jl * LL), mine);
     }
 }
 
@@ -524,7 +484,6 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
 // the wave, so the tile lookup and the tile's fields are scalar loads.
 __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
 {
-    const uint32_t shift = 64 - B.key_bits;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (TB / 64);
@@ -534,19 +493,19 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
         const uint32_t L = TD.leaf_size;
         if (g0 + LEAF_G <= B.grp_off[t + 1] && (L == 16 || L == 32 || L == 64 || L == 128)) {
             if (L == 16)
-                leaf_rounds<16>(TD, T, g0, lane, shift);
+                leaf_rounds<16>(TD, T, g0, lane);
             else if (L == 32)
-                leaf_rounds<32>(TD, T, g0, lane, shift);
+                leaf_rounds<32>(TD, T, g0, lane);
             else if (L == 64)
-                leaf_rounds<64>(TD, T, g0, lane, shift);
+                leaf_rounds<64>(TD, T, g0, lane);
             else
-                leaf_rounds<128>(TD, T, g0, lane, shift);
+                leaf_rounds<128>(TD, T, g0, lane);
             continue;
         }
         if (L <= 64 && (L & (L - 1)) == 0 && g0 + LEAF_G <= B.grp_off[t + 1]) {
             const uint32_t nl_grp = 64 / L, sub = lane / L, off = lane & (L - 1);
             const uint32_t jl0 = (g0 - TD.grp_off) * nl_grp;  // first leaf of the groups
-            uint32_t v[LEAF_G];
+            uint32_t v[LEAF_G], kq[LEAF_G];
             double x[LEAF_G], y[LEAF_G], z[LEAF_G];
             bool have[LEAF_G];
 #pragma unroll
@@ -554,18 +513,19 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
                 const uint32_t jl = jl0 + q * nl_grp + sub, e = jl * L + off;
                 have[q] = jl < TD.nleaf && e < TD.n;
                 // loads clamped to the tile's last point (no branch per load)
-                const uint32_t p = TD.pt_off + min(e, TD.n - 1);
-                v[q] = T.ids[p];
-                x[q] = T.xs[p];
-                y[q] = T.ys[p];
-                z[q] = T.zs[p];
+                const PointRec r = owm_load_pt(T.pts, TD.pt_off + min(e, TD.n - 1));
+                v[q] = r.id;
+                x[q] = r.x;
+                y[q] = r.y;
+                z[q] = r.z;
+                kq[q] = r.key;
             }
             // lane m ends with the aggregate of leaf jl0 + m when the groups hold
             // at most 64 leaves (leaf_size >= 4); smaller leaves are stored
             // group by group from each leaf's first lane
             const bool gather = LEAF_G * nl_grp <= 64;
             LeafAgg mine = leaf_agg_empty();
-            double mine_x = 0.0, mine_y = 0.0;  // the leaf's first point
+            uint32_t mine_key = 0;  // the key of the leaf's first point
 #pragma unroll
             for (int q = 0; q < LEAF_G; ++q) {
                 LeafAgg a = leaf_agg_empty();
@@ -574,7 +534,7 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
                 leaf_agg_reduce(a, L);
                 if (!gather) {
                     const uint32_t jl = jl0 + q * nl_grp + sub;
-                    const uint32_t k0 = off == 0 && have[q] ? sort_key(TD, x[q], y[q], shift) : 0u;
+                    const uint32_t k0 = off == 0 && have[q] ? kq[q] : 0u;
                     if (off == 0 && jl < TD.nleaf)
                         leaf_store(TD, T, k0, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), a);
                     continue;
@@ -591,15 +551,12 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
                     a.bk = shfl_any(a.bk, src);
                 }
                 const int first = nl_grp > 1 ? (int)((lane & (nl_grp - 1)) * L) : 0;
-                const double fx = shfl_any(x[q], first), fy = shfl_any(y[q], first);
+                const uint32_t fk = shfl_any(kq[q], first);
                 if (take) {
                     mine = a;
-                    mine_x = fx;
-                    mine_y = fy;
+                    mine_key = fk;
                 }
             }
-            // every leaf's first key from its first point, once per wave
-            const uint32_t mine_key = sort_key(TD, mine_x, mine_y, shift);
             const uint32_t jl = jl0 + lane;
             if (gather && lane < LEAF_G * nl_grp && jl < TD.nleaf)
                 leaf_store(TD, T, mine_key, jl, TD.pt_off + jl * L, min(L, TD.n - jl * L), mine);
@@ -619,11 +576,10 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
             LeafAgg a = leaf_agg_empty();
             uint32_t k0 = 0;
             for (uint32_t k = off; k < npts; k += ls) {
-                const uint32_t pp = start + k;
-                const double px = T.xs[pp], py = T.ys[pp];
+                const PointRec r = owm_load_pt(T.pts, start + k);
                 if (k == 0)
-                    k0 = sort_key(TG, px, py, shift);
-                leaf_agg_add(a, px, py, T.zs[pp], T.ids[pp]);
+                    k0 = r.key;
+                leaf_agg_add(a, r.x, r.y, r.z, r.id);
             }
             seg_reduce(off, ls, [&](uint32_t o, bool take) {
                 uint32_t c2 = __shfl_down(a.cnt, o, 64);
@@ -881,10 +837,10 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 }
 
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
-                   uint32_t *keys, uint32_t *hist, int passes)
+                   uint32_t *hist, int passes)
 {
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, keys,
-                       hist, passes);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y, hist,
+                       passes);
 }
 
 void launch_leaf_aggs(hipStream_t s, const BatchDev &B, const TreeOut &T)
