@@ -1,17 +1,20 @@
 #!/usr/bin/env python3
 """OWM ground-seed benchmark on one MI355X.
 
-Default workload: the 20M-point 2x2 km tile (5 pts/m2, 10% of points in
-Gaussian clusters), Wsize 12, Overlap 0.8, Bsize 20, leaf 32, fp64 coordinates,
-synthetic data (fastowm_amd.synth, seed 0).  One step = the whole pipeline on
-that tile with the cloud already resident in HBM: spatial tree, stage 1,
-sort + stage 2, stage 3, merge, seeds copied back to the host.
+Headline workload (the metric's batched configuration): 512 independent tiles
+with log-normal point counts (median 400k, clipped 20k-6M, 329M points in all),
+densities 1-20 pts/m2, Wsize from {8, 12, 16}, Overlap 0.8, Bsize 20, fp64
+coordinates, synthetic data (fastowm_amd.synth, seed 0), already resident in
+HBM.  One step = the whole pipeline over the batch in one launch chain: tree,
+stage 1, sort + stage 2, stage 3, merge, seeds copied to the host.
 
-Prints ONE JSON line: Mpoints/s (higher is better), the roofline of the
-dominant kernel family (HIP-event time on the library's stream, algorithmic
-bytes), the CPU baseline (the oracle timed on this host's cores on the same
-tile) and, unless --no-batch, the 512-tile uneven batch (plain one-tile-per-
-launch-chain vs batched) on the same device.
+Prints ONE JSON line: batched Mpoints/s (higher is better; eager steps, with
+HIP events around the dominant kernel family only), the roofline of that
+family, the CPU baseline (the oracle on a bounded sample of the same batch,
+all of this host's CPU share), and beside them: the same batch replayed from
+a captured graph and run one tile per launch chain (plain), the single-tile
+configurations (20M, 120M and 1M points; the 1M one also on the CPU), the
+.xyz reader, the quadtree statistics and the serving loop.
 """
 import argparse
 import json
@@ -33,27 +36,27 @@ PHASE_KERNEL = {"stage1": "owm::k_stage1", "leaf_aggs": "owm::k_leaf_aggs",
                 "bbox": "owm::k_bbox", "agg_levels": "owm::k_agg_levels",
                 "stage1_join": "owm::k_s1_bin+k_scan_excl"}
 PHASE_BYTES = {"stage1": "24 B x N x Wsize/Displace + 4 B x windows",
-               "leaf_aggs": "28 B x N + 48 B x leaves",
-               "sort_tree_pass": "60 B x N per pass with 16-bit keys (first pass: key + x, y, z read, key + x, y, z, index written; last: key + x, y, z, index read, x, y, z, index written)",
-               "morton": "20 B x N", "bbox": "16 B x N", "agg_levels": "48 B x (leaves + level entries)",
+               "leaf_aggs": "32 B x N + 48 B x leaves",
+               "sort_tree_pass": "56 B x N for the first pass (x, y, z read; a 32-byte point written), 64 B x N for later passes (a 32-byte point read and written)",
+               "morton": "16 B x N", "bbox": "16 B x N", "agg_levels": "48 B x (leaves + level entries)",
                "karras": "24 B x leaves + 76 B x nodes", "sort_windows_pass": "12 B x windows per pass",
                "stage2_compact": "8 B x windows", "stage1_join": "72 B x leaves + 12 B x blocks"}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+TRAFFIC_BATCH = os.path.join(ROOT, "profiles", "pmc_traffic_batch_512.json")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path):
     """HBM bytes per launch of `kernel` on this workload from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by the copy-kernel factor),
     or None when no such measurement is committed."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
     k = t.get("kernels", {}).get(kernel)
     if k is None:
         return None, None
-    return int(k["bytes_per_launch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    return int(k["bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
 def log(*a):
@@ -61,84 +64,252 @@ def log(*a):
 
 
 def cpu_cores():
+    """The host cores this process may use: OMP_NUM_THREADS when set (the
+    CPU share a one-GPU box gives a process), else the affinity set."""
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
-def cpu_baseline(x, y, z, want_seeds, reps):
-    """The oracle (a C restatement of the reference TBB algorithm, OpenMP) on
-    the same tile; median of `reps` runs of tree + stages 1-3."""
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
+
+
+def oracle_tiles(jobs, workers):
+    """The oracle on many tiles at once: one tile per worker thread (the
+    oracle call releases the GIL), each single-threaded; seed arrays."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import concurrent.futures as cf
+    import pyoracle
+    saved = pyoracle.threads()
+    pyoracle.set_threads(1)
+    try:
+        with cf.ThreadPoolExecutor(workers) as ex:
+            return list(ex.map(lambda j: pyoracle.run(*j, 32)[0], jobs))
+    finally:
+        pyoracle.set_threads(saved)
+
+
+def cpu_baseline_batch(tiles, arrs, gpu_seeds, sample_points, reps):
+    """The oracle (a C restatement of the reference TBB algorithm: quadtree,
+    exact stage 1, sort + stage 2, stage 3) on the first tiles of the same
+    batch up to sample_points points, tiles in parallel on every core of this
+    host's CPU share; median of reps runs.  Its seeds must equal the GPU's."""
+    import numpy as np
+    cores = cpu_cores()
+    k, pts = 0, 0
+    while k < len(tiles) and pts < sample_points:
+        pts += tiles[k].n
+        k += 1
+    jobs = [(arrs[i][0], arrs[i][1], arrs[i][2], tiles[i].wsize, tiles[i].bsize, tiles[i].overlap)
+            for i in range(k)]
+    times, same = [], True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        got = oracle_tiles(jobs, cores)
+        times.append(time.perf_counter() - t0)
+        same = same and all(np.array_equal(a, b) for a, b in zip(got, gpu_seeds[:k]))
+    med = statistics.median(times)
+    return {"value": round(pts / med / 1e6, 2), "unit": "Mpoints/s", "cores": cores, "kind": "port",
+            "sample": f"first {k} tiles ({pts} points) of the same 512-tile batch, tiles in parallel "
+                      f"(one oracle thread each) on {cores} cores, median of {reps} runs (tree + stages 1-3)",
+            "equal_to_gpu": bool(same), "host": host_info()}
+
+
+def cpu_single(x, y, z, want, reps, name):
+    """The oracle on one whole tile with all of this host's CPU share (its
+    OpenMP threads), median of reps runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
     cores = cpu_cores()
     pyoracle.set_threads(cores)
-    times, parity = [], None
+    times, same = [], True
     for _ in range(reps):
         t0 = time.perf_counter()
         seeds, _, _ = pyoracle.run(x, y, z, 12, 20, 0.8, 32)
         times.append(time.perf_counter() - t0)
-        parity = bool(np.array_equal(seeds, want_seeds))
+        same = same and bool(np.array_equal(seeds, want))
     med = statistics.median(times)
-    return {"value": round(len(x) / med / 1e6, 3), "unit": "Mpoints/s", "cores": cores,
-            "kind": "port",
-            "sample": f"full 20M-point tile, median of {reps} runs (tree + stages 1-3), "
-                      f"{cores} OpenMP threads"}, parity
+    return {"mpts": round(len(x) / med / 1e6, 2), "cores": cores, "kind": "port",
+            "sample": f"{name}, median of {reps} runs (tree + stages 1-3), {cores} OpenMP threads",
+            "equal_to_gpu": same}
 
 
-def run_batch_bench(ctx, torch, synth, n_tiles, steps):
-    """Autotune on the batch, then plain (one run_tile launch chain per tile)
-    vs batched (all tiles in one chain, eager and replayed from a hipGraph)."""
+def phase_table(full):
+    out = {}
+    for name, p in full.items():
+        if p["calls"] == 0:
+            continue
+        out[name] = {"ms_per_step": round(p["ms"], 4), "calls_per_step": p["calls"],
+                     "gbs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1) if p["ms"] > 0 else None}
+    return out
+
+
+def roofline_of(prof, dname, steps, n_points, traffic_file):
+    """Achieved bandwidth of the dominant kernel family from the HIP events of
+    the timed steps: its algorithmic bytes per launch / its mean launch time;
+    for the tree sort also on the survey's per-point bytes (a key/index sort of
+    24 B per point and pass plus a 52 B gather)."""
+    dp = prof[dname]
+    per_launch_bytes = dp["bytes"] / dp["calls"]
+    per_launch_s = dp["ms"] * 1e-3 / dp["calls"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    kernel = PHASE_KERNEL.get(dname, dname)
+    traffic, src = measured_traffic(kernel, traffic_file)
+    r = {"bound": "hbm", "kernel": kernel, "phase": dname, "achieved": round(achieved, 1),
+         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+         "traffic": traffic, "traffic_source": src, "launch_ms": round(per_launch_s * 1e3, 4),
+         "launches_per_step": dp["calls"] // steps,
+         "algorithmic_bytes_per_launch": int(per_launch_bytes), "bytes_per_unit": PHASE_BYTES.get(dname)}
+    if dname == "sort_tree_pass":
+        passes = dp["calls"] // steps
+        survey = (24 * passes + 52) * n_points
+        step_s = dp["ms"] * 1e-3 / steps
+        r["survey_bytes_per_step"] = int(survey)
+        r["frac_survey_bytes"] = round(survey / step_s / 1e9 / HBM_PEAK_GBS, 4)
+        r["survey_bytes_rule"] = ("24 B x N per pass (a key/index sort) + 52 B x N (gather), per step "
+                                  "over the passes' summed launch time")
+    return r
+
+
+def run_headline(ctx, torch, synth, args):
+    """The 512-tile batch: tuning (untimed), warmup, one fully phase-timed
+    step, the timed eager steps (events on the dominant kernel family only),
+    graph replays, and the same tiles one launch chain each (plain)."""
     import numpy as np
-    tiles = synth.batch_tiles(n_tiles, seed=0)
+    from fastowm_amd import owm
+    tiles = synth.batch_tiles(args.batch_tiles, seed=0)
     arrs = [t.generate() for t in tiles]
     n_total = sum(t.n for t in tiles)
     dev = [torch.from_numpy(np.concatenate([a[k] for a in arrs])).to("cuda:0") for k in range(3)]
-    desc = [(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0)) for t in tiles]
+    # the tile table is built once; results are views of the context's
+    # page-locked seed buffer (copied out after the timed loops)
+    desc = owm.batch_desc([(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0))
+                           for t in tiles])
+    torch.cuda.synchronize()
+    if args.tune_table:
+        with open(args.tune_table) as f:
+            ctx.set_tune_table(json.load(f))
+        t_tune, table = 0.0, ctx.tune_table()
+    else:
+        log("autotune")
+        t0 = time.perf_counter()
+        table = ctx.autotune(*dev, desc)
+        t_tune = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        got = ctx.batch_run(*dev, desc)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.batch_run(*dev, desc)
+    full = ctx.profile_read()
+    ctx.profile(False)
+    dname = max((k for k in full if full[k]["calls"]), key=lambda k: full[k]["ms"])
+    ctx.profile(True, phases=[dname])
+    log(f"timing {args.steps} batched steps (events on {dname})")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got = ctx.batch_run(*dev, desc, copy=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    got = [a.copy() for a in got]
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ms = (t1 - t0) / args.steps * 1e3
+    roofline = roofline_of(prof, dname, args.steps, n_total, TRAFFIC_BATCH)
+    if args.save_tune_table:
+        with open(args.save_tune_table, "w") as f:
+            json.dump(table, f, indent=1)
+    res = {"tiles": len(tiles), "points": int(n_total), "ms": ms, "phases": phase_table(full),
+           "roofline": roofline,
+           "batch": {"batched_eager_mpts": round(n_total / ms * 1e-3, 1), "batched_eager_ms": round(ms, 3),
+                     "autotune_s": round(t_tune, 2),
+                     "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k",
+                                                  "summaries", "sort_bits")} for e in table]}}
+    if args.headline_only:
+        return res, got, tiles, arrs
+    log("graph replays")
+    ctx.batch_run(*dev, desc, graph=True)  # capture
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    table = ctx.autotune(*dev, desc)
-    t_tune = time.perf_counter() - t0
-    # plain: one run_tile per tile on device-resident slices
+    for _ in range(args.steps):
+        gg = ctx.batch_run(*dev, desc, graph=True, copy=False)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / args.steps
+    gg = [a.copy() for a in gg]
+    log("plain: one launch chain per tile")
     offs = np.cumsum([0] + [t.n for t in tiles])
     views = [[d[offs[i]:offs[i + 1]] for d in dev] for i in range(len(tiles))]
     for i in range(min(4, len(tiles))):
         ctx.run_tile(*views[i], tiles[i].wsize, tiles[i].bsize, tiles[i].overlap)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain = []
-    for i, t in enumerate(tiles):
-        plain.append(ctx.run_tile(*views[i], t.wsize, t.bsize, t.overlap))
+    plain = [ctx.run_tile(*views[i], t.wsize, t.bsize, t.overlap) for i, t in enumerate(tiles)]
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    ctx.batch_run(*dev, desc)
+    same = all(np.array_equal(a, b) for a, b in zip(plain, got)) and \
+        all(np.array_equal(a, b) for a, b in zip(gg, got))
+    del dev, views
+    torch.cuda.empty_cache()
+    res["batch"].update({"batched_graph_mpts": round(n_total / t_graph / 1e6, 1),
+                         "plain_mpts": round(n_total / t_plain / 1e6, 1),
+                         "batched_graph_ms": round(t_graph * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+                         "speedup_eager_vs_plain": round(t_plain * 1e3 / ms, 2),
+                         "speedup_graph_vs_plain": round(t_plain / t_graph, 2),
+                         "batched_equals_plain": same})
+    return res, got, tiles, arrs
+
+
+def run_single(ctx, torch, dev, n, steps, name, cpu=None):
+    """One resident tile: repeated calls (a repeat replays the captured chain),
+    and eager steps with events on the dominant kernel family."""
+    for _ in range(3):
+        seeds = ctx.run_tile(*dev, 12, 20, 0.8, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        eager = ctx.batch_run(*dev, desc)
+        seeds = ctx.run_tile(*dev, 12, 20, 0.8, 0)
+    torch.cuda.synchronize()
+    t_rep = (time.perf_counter() - t0) / steps
+    ctx.profile(True)
+    ctx.run_tile(*dev, 12, 20, 0.8, 0)
+    full = ctx.profile_read()
+    ctx.profile(False)
+    dname = max((k for k in full if full[k]["calls"]), key=lambda k: full[k]["ms"])
+    ctx.profile(True, phases=[dname])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        seeds = ctx.run_tile(*dev, 12, 20, 0.8, 0)
     torch.cuda.synchronize()
     t_eager = (time.perf_counter() - t0) / steps
-    ctx.batch_run(*dev, desc, graph=True)  # capture
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        got = ctx.batch_run(*dev, desc, graph=True)
-    torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
-    same = all(np.array_equal(a, b) for a, b in zip(plain, got)) and \
-        all(np.array_equal(a, b) for a, b in zip(plain, eager))
-    return {"tiles": len(tiles), "points": int(n_total),
-            "plain_mpts": round(n_total / t_plain / 1e6, 1),
-            "batched_mpts": round(n_total / t_batch / 1e6, 1),
-            "batched_eager_mpts": round(n_total / t_eager / 1e6, 1),
-            "batched_ms": round(t_batch * 1e3, 3), "batched_eager_ms": round(t_eager * 1e3, 3),
-            "plain_ms": round(t_plain * 1e3, 3),
-            "speedup": round(t_plain / t_batch, 2), "batched_equals_plain": same,
-            "autotune_s": round(t_tune, 2),
-            "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k", "summaries",
-                                         "sort_bits")} for e in table if e["ms"] > 0]}
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    out = {"workload": name, "points": n, "seeds": int(len(seeds)),
+           "repeat_ms": round(t_rep * 1e3, 3), "repeat_mpts": round(n / t_rep / 1e6, 1),
+           "eager_ms": round(t_eager * 1e3, 3), "eager_mpts": round(n / t_eager / 1e6, 1),
+           "dominant": {"phase": dname, "launch_ms": round(prof[dname]["ms"] / prof[dname]["calls"], 4),
+                        "gbs": round(prof[dname]["bytes"] / (prof[dname]["ms"] * 1e-3) / 1e9, 1)},
+           "phases": phase_table(full)}
+    return out, seeds
 
 
 def run_reader_bench(ctx, synth, x, y, z, reps=3):
@@ -282,19 +453,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 512-tile batch leg")
     ap.add_argument("--batch-tiles", type=int, default=512)
-    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--tune-table", default=None,
+                    help="load this tuning table (JSON, as owm_tune_get) instead of autotuning")
+    ap.add_argument("--save-tune-table", default=None, help="write the tuning table used (JSON)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the batched eager steps (no graph, plain or other legs: for profiling them)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
+    ap.add_argument("--cpu-sample", type=int, default=110_000_000,
+                    help="points of the batch the CPU baseline runs on")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-single", action="store_true", help="skip the 20M / 120M / 1M tiles")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-reader", action="store_true", help="skip the .xyz reader leg")
     ap.add_argument("--no-qstats", action="store_true", help="skip the quadtree statistics leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
     ap.add_argument("--serve-requests", type=int, default=160)
-    ap.add_argument("--batch-only", action="store_true",
-                    help="only the 512-tile batch leg (for profiling it)")
-    ap.add_argument("--leaf", type=int, default=0,
-                    help="leaf size (maxNumber) of the timed tile; 0 = autotuned")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -315,108 +489,65 @@ def main():
                                    "measured here", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
     from fastowm_amd import owm, synth
 
-    if args.batch_only:
-        with owm.Context(0) as bctx:
-            print(json.dumps({"batch": run_batch_bench(bctx, torch, synth, args.batch_tiles,
-                                                       max(1, args.steps // 4))}), flush=True)
-        return
-
-    tile = synth.config_c2(seed=0)
-    log("generating", tile.name)
-    x, y, z = tile.generate()
-    dx, dy, dz = (torch.from_numpy(a).to("cuda:0") for a in (x, y, z))
-    torch.cuda.synchronize()
     ctx = owm.Context(0)
-    leaf = args.leaf
-    tuned = None
-    if leaf == 0:
-        # leaf size, window block and sort width from the autotuner (untimed)
-        log("autotune")
-        tuned = ctx.autotune(dx, dy, dz, [(tile.n, dict(wsize=12, bsize=20, overlap=0.8))])
-        tuned = [e for e in tuned if e["ms"] > 0][-1]
-    for _ in range(args.warmup):
-        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, leaf)
-    torch.cuda.synchronize()
-    # one untimed step with every phase timed (two events per phase cost the
-    # stream a few microseconds each): the phase breakdown and the dominant
-    # kernel family
-    ctx.profile(True)
-    seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, leaf)
-    full = ctx.profile_read()
-    ctx.profile(False)
-    phases = {}
-    for name, p in full.items():
-        if p["calls"] == 0:
-            continue
-        phases[name] = {"ms_per_step": round(p["ms"], 4), "calls_per_step": p["calls"],
-                        "gbs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1) if p["ms"] > 0 else None}
-    dname = max((k for k in full if full[k]["calls"]), key=lambda k: full[k]["ms"])
-    # the timed steps record events around the dominant kernel family only
-    ctx.profile(True, phases=[dname])
-    log(f"timing {args.steps} steps (events on {dname})")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        seeds = ctx.run_tile(dx, dy, dz, 12, 20, 0.8, leaf)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    ms = (t1 - t0) / args.steps * 1e3
-    value = tile.n / (ms / 1e3) / 1e6
-    dp = prof[dname]
-    per_launch_bytes = dp["bytes"] / dp["calls"]
-    per_launch_s = dp["ms"] * 1e-3 / dp["calls"]
-    achieved = per_launch_bytes / per_launch_s / 1e9
-    kernel = PHASE_KERNEL.get(dname, dname)
-    if dname == "stage1" and tuned:
-        kernel = "owm::k_stage1<%d, %d, %d, %s>" % (tuned["block_x"], tuned["block_y"],
-                                                    tuned["block_k"],
-                                                    "true" if tuned["summaries"] else "false")
-    traffic, traffic_src = measured_traffic(kernel)
-    roofline = {"bound": "hbm", "kernel": kernel, "phase": dname, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "traffic_source": traffic_src,
-                "launch_ms": round(per_launch_s * 1e3, 4),
-                "algorithmic_bytes_per_launch": int(per_launch_bytes),
-                "bytes_per_unit": PHASE_BYTES.get(dname)}
-
-    line = {"metric": "Mpoints/s end-to-end OWM (tree + stages 1-3), plain GPU, 20M-point tile",
-            "value": round(value, 1), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "c2: 20M points, 2x2 km, 5 pts/m2, 10% in Gaussian clusters "
-                                   "(sigma 3 m), Wsize 12, Overlap 0.8, Bsize 20",
-                       "points": tile.n, "seeds": int(len(seeds)),
-                       "leaf_size": leaf if leaf else (tuned["leaf_size"] if tuned else None),
-                       "tuned": {k: tuned[k] for k in ("leaf_size", "block_x", "block_y", "block_k", "summaries",
-                                                       "sort_bits")} if tuned else None},
-            "roofline": roofline, "phases": phases,
+    log("generating the 512-tile batch")
+    head, batch_seeds, tiles, arrs = run_headline(ctx, torch, synth, args)
+    ms = head["ms"]
+    line = {"metric": "Mpoints/s end-to-end OWM (batched vs plain GPU vs TBB CPU); p99 tile latency",
+            "value": round(head["points"] / ms * 1e-3, 1), "unit": "Mpoints/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "batched: 512 independent tiles, log-normal point counts (median "
+                                   "400k, clipped 20k-6M), densities 1-20 pts/m2, Wsize {8,12,16}, "
+                                   "Overlap 0.8, Bsize 20, one launch chain, resident in HBM",
+                       "tiles": head["tiles"], "points": head["points"],
+                       "step": "the whole batch, eager launch chain (the graph replay is beside it)"},
+            "roofline": head["roofline"], "phases": head["phases"],
             "phases_source": "one untimed step with every phase timed by HIP events; the timed "
-                             "steps carry events around the roofline kernel only"}
-    if not args.no_batch:
-        log("batch leg")
-        line["batch"] = run_batch_bench(ctx, torch, synth, args.batch_tiles, max(1, args.steps // 4))
-    if not args.no_reader:
-        log("reader leg")
-        line["reader"], reader_text, reader_cloud = run_reader_bench(ctx, synth, x, y, z)
-    if not args.no_qstats:
-        log("quadtree statistics leg")
-        line["qtree_stats"] = run_qstats_bench(ctx, synth, dx, dy, x, y)
-    if not args.no_serve:
-        log("serve leg")
-        rates = [float(r) for r in args.serve_rates.split(",") if r]
-        line["serve"] = run_serve_bench(owm, synth, rates, args.serve_requests, 48, 3)
-    if not args.no_cpu:
-        log("cpu baseline leg")
-        cb, parity = cpu_baseline(x, y, z, seeds, args.cpu_reps)
-        line["cpu_baseline"] = cb
-        line["parity_vs_oracle"] = parity
-        if not args.no_reader:
-            line["reader"]["cpu_baseline"] = reader_cpu(reader_text, reader_cloud)
+                             "steps carry events around the roofline kernel family only",
+            "batch": head["batch"]}
+    if not args.headline_only:
+        if not args.no_cpu:
+            log("cpu baseline (oracle on a sample of the batch)")
+            line["cpu_baseline"] = cpu_baseline_batch(tiles, arrs, batch_seeds, args.cpu_sample,
+                                                      args.cpu_reps)
+        del arrs, batch_seeds
+        single = {}
+        x = y = z = dx = dy = None
+        if not args.no_single:
+            for key, cfg, steps in (("c1", synth.config_c1(0), args.steps),
+                                    ("c3", synth.config_c3(0), max(3, args.steps // 4)),
+                                    ("c2", synth.config_c2(0), args.steps)):
+                log("single tile", key)
+                x, y, z = cfg.generate()
+                dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
+                torch.cuda.synchronize()
+                single[key], seeds = run_single(ctx, torch, dev, cfg.n, steps, cfg.name)
+                if key == "c1" and not args.no_cpu:
+                    single[key]["cpu"] = cpu_single(x, y, z, seeds, 5, "the same 1M-point tile")
+                    single[key]["cpu"]["gpu_over_cpu"] = round(single[key]["repeat_mpts"] /
+                                                              single[key]["cpu"]["mpts"], 1)
+                if key != "c2":
+                    del dev
+                else:
+                    dx, dy = dev[0], dev[1]
+            line["single"] = single
+        if not args.no_reader and x is not None:
+            log("reader leg")
+            line["reader"], reader_text, reader_cloud = run_reader_bench(ctx, synth, x, y, z)
+            if not args.no_cpu:
+                line["reader"]["cpu_baseline"] = reader_cpu(reader_text, reader_cloud)
+        if not args.no_qstats and dx is not None:
+            log("quadtree statistics leg")
+            line["qtree_stats"] = run_qstats_bench(ctx, synth, dx, dy, x, y)
+        if not args.no_serve:
+            log("serve leg")
+            rates = [float(r) for r in args.serve_rates.split(",") if r]
+            line["serve"] = run_serve_bench(owm, synth, rates, args.serve_requests, 48, 3)
     ctx.close()
     print(json.dumps(line), flush=True)
 

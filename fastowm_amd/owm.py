@@ -360,8 +360,9 @@ class Tree:
             lib().owm_tree_free(self.h)
         self.h = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: module globals may already be gone at exit
+        if _finalizing():
             return  # the exit hook closed the context and its trees already
         try:
             self.free()
@@ -390,6 +391,28 @@ class Cloud:
         self.ctx._check(lib().owm_cloud_read(self.ctx.h, ctypes.byref(self.desc), x.ctypes.data,
                                              y.ctypes.data, z.ctypes.data))
         return x, y, z
+
+
+class BatchDesc:
+    """The owm_tile array of a batch (tiles as for Context.batch_run), built
+    once so that repeated runs of the same batch skip rebuilding it."""
+
+    def __init__(self, tiles):
+        self.descs = (TileDesc * len(tiles))()
+        off = 0
+        for i, (n, kw) in enumerate(tiles):
+            self.descs[i].offset = off
+            self.descs[i].n = n
+            self.descs[i].params = make_params(**kw)
+            off += n
+        self.points = off
+
+    def __len__(self):
+        return len(self.descs)
+
+
+def batch_desc(tiles):
+    return BatchDesc(tiles)
 
 
 class Context:
@@ -444,8 +467,9 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: module globals may already be gone at exit
+        if _finalizing():
             return  # closed by the exit hook; no HIP call during interpreter teardown
         try:
             self.close()
@@ -566,34 +590,36 @@ class Context:
 
     @staticmethod
     def _descs(tiles):
-        descs = (TileDesc * len(tiles))()
-        off = 0
-        for i, (n, kw) in enumerate(tiles):
-            descs[i].offset = off
-            descs[i].n = n
-            descs[i].params = make_params(**kw)
-            off += n
-        return descs, off
+        if isinstance(tiles, BatchDesc):
+            return tiles.descs, tiles.points
+        d = BatchDesc(tiles)
+        return d.descs, d.points
 
-    def batch_run(self, x, y, z, tiles, graph=False):
+    def batch_run(self, x, y, z, tiles, graph=False, copy=True):
         """tiles: list of (n, dict(wsize=, bsize=, overlap=, leaf_size=, bbox=)) in
-        cloud order.  Returns a list of seed arrays (tile-local ids).  graph=True
-        captures the device chain once per batch layout and replays it."""
+        cloud order, or a BatchDesc of such a list (built once, reused).
+        Returns a list of seed arrays (tile-local ids).  graph=True captures the
+        device chain once per batch layout and replays it.  copy=False returns
+        views of the context's page-locked seed buffer instead of copies: they
+        are valid until the next call on this context."""
         descs, off = self._descs(tiles)
-        cap = off + len(tiles) + 1
+        n_tiles = len(descs)
+        cap = off + n_tiles + 1
         out = self._seeds_out(cap)
-        so = np.zeros(len(tiles) + 1, np.uint64)
+        so = np.zeros(n_tiles + 1, np.uint64)
         flags = B_GRAPH if graph else 0
         if _is_device(x):
             st = lib().owm_batch_run_device(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), descs,
-                                            len(tiles), flags, out.ctypes.data, cap, so.ctypes.data)
+                                            n_tiles, flags, out.ctypes.data, cap, so.ctypes.data)
         else:
             (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
-            st = lib().owm_batch_run(self.h, xp, yp, zp, descs, len(tiles), flags,
+            st = lib().owm_batch_run(self.h, xp, yp, zp, descs, n_tiles, flags,
                                      out.ctypes.data, cap, so.ctypes.data)
         self._check(st)
-        # one copy out of the reusable buffer, then a view per tile
-        return np.split(out[:int(so[-1])].copy(), so[1:-1].astype(np.intp))
+        seeds = out[:int(so[-1])]
+        if copy:
+            seeds = seeds.copy()  # one copy out of the reusable buffer
+        return np.split(seeds, so[1:-1].astype(np.intp))  # a view per tile
 
     # -- autotuner ----------------------------------------------------------
     def autotune(self, x, y, z, tiles):

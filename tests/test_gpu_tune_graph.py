@@ -164,3 +164,21 @@ def test_repeated_single_tile_calls_replay_a_graph():
             say("host repeat", rep)
             assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 32, bbox=box), want)
             assert np.array_equal(ctx.run_tile(x2, y2, z2, 12, 20, 0.8, 32, bbox=box), want2)
+
+
+def test_batch_desc_and_buffer_views():
+    """A BatchDesc built once gives the same seeds as the tile list; with
+    copy=False the results are views of the context's seed buffer (valid until
+    the next call), equal to the copies."""
+    x, y, z, desc, tiles, arrs = _batch([30_000, 70_000, 5_000], seed=700)
+    with owm.Context(0) as ctx:
+        say("batch desc / views")
+        want = ctx.batch_run(x, y, z, desc)
+        bd = owm.batch_desc(desc)
+        assert len(bd) == 3 and bd.points == len(x)
+        views = ctx.batch_run(x, y, z, bd, copy=False)
+        assert _same([v.copy() for v in views], want)
+        assert _same(ctx.batch_run(x, y, z, bd, graph=True), want)
+        for t, (tx, ty, tz), g in zip(tiles, arrs, want):
+            w, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
+            assert np.array_equal(g, w)
