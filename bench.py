@@ -459,6 +459,8 @@ def main():
     ap.add_argument("--save-tune-table", default=None, help="write the tuning table used (JSON)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched eager steps (no graph, plain or other legs: for profiling them)")
+    ap.add_argument("--single-only", choices=("c1", "c2", "c3"), default=None,
+                    help="only this single-tile leg (for profiling it); prints its object")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
     ap.add_argument("--cpu-sample", type=int, default=110_000_000,
                     help="points of the batch the CPU baseline runs on")
@@ -493,6 +495,18 @@ def main():
     from fastowm_amd import owm, synth
 
     ctx = owm.Context(0)
+    if args.single_only:
+        cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c3": synth.config_c3}[args.single_only](0)
+        if args.tune_table:
+            with open(args.tune_table) as f:
+                ctx.set_tune_table(json.load(f))
+        x, y, z = cfg.generate()
+        dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
+        torch.cuda.synchronize()
+        out, _ = run_single(ctx, torch, dev, cfg.n, args.steps, cfg.name)
+        ctx.close()
+        print(json.dumps({"single": out}), flush=True)
+        return
     log("generating the 512-tile batch")
     head, batch_seeds, tiles, arrs = run_headline(ctx, torch, synth, args)
     ms = head["ms"]
