@@ -395,15 +395,25 @@ def run_qstats_bench(ctx, synth, dx, dy, x, y, reps=5):
     return out
 
 
-def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
+def run_serve_bench(owm, synth, rates, n_req, pool_size, slots, device=False):
     """Poisson request streams against the serving loop: N log-uniform
     10k-8M points, Wsize 5-20, Bsize 10-30, Overlap 0.5-0.9; one request in
-    four re-queries a resident tile (same key, new parameters)."""
+    four re-queries a resident tile (same key, new parameters).  device: the
+    pool's clouds are resident in HBM and submitted as device arrays (no
+    PCIe transfer per request); a rate of 0 submits every request at once
+    (the loop's saturated throughput)."""
     import threading
     import numpy as np
     pool = []
     for _, t in synth.serve_requests(pool_size, rate=1.0, seed=11):
-        pool.append((t, t.generate()))
+        arrs = t.generate()
+        if device:
+            import torch
+            arrs = tuple(torch.from_numpy(a).to("cuda:0") for a in arrs)
+        pool.append((t, arrs))
+    if device:
+        import torch
+        torch.cuda.synchronize()
     out = []
     with owm.Context(0) as ctx:
         srv = owm.Server(ctx, max_inflight=slots, max_batch=32)
@@ -411,7 +421,7 @@ def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
         for t, (x, y, z) in pool[:slots * 2]:
             srv.wait(srv.submit(x, y, z, t.wsize, t.bsize, t.overlap))
         for rate in rates:
-            reqs = synth.serve_requests(n_req, rate=rate, seed=int(rate))
+            reqs = synth.serve_requests(n_req, rate=rate if rate > 0 else 1e9, seed=int(rate) or 1)
             ids = [None] * n_req
             npts = [0] * n_req
             t0 = time.perf_counter()
@@ -438,11 +448,12 @@ def run_serve_bench(owm, synth, rates, n_req, pool_size, slots):
             th.join()
             wall = time.perf_counter() - t0
             lats = np.array(lats) * 1e3
-            out.append({"rate_tiles_s": rate, "requests": n_req,
+            out.append({"inputs": "device" if device else "host",
+                        "rate_tiles_s": rate if rate > 0 else "burst", "requests": n_req,
                         "p50_ms": round(float(np.percentile(lats, 50)), 2),
                         "p99_ms": round(float(np.percentile(lats, 99)), 2),
                         "mpts": round(sum(npts) / wall / 1e6, 1),
-                        "offered_mpts": round(sum(npts) / reqs[-1][0] / 1e6, 1)})
+                        "offered_mpts": round(sum(npts) / reqs[-1][0] / 1e6, 1) if rate > 0 else None})
             log("serve", out[-1])
         srv.stop()
     return out
@@ -470,6 +481,8 @@ def main():
     ap.add_argument("--no-reader", action="store_true", help="skip the .xyz reader leg")
     ap.add_argument("--no-qstats", action="store_true", help="skip the quadtree statistics leg")
     ap.add_argument("--serve-rates", default="50,500,2000")
+    ap.add_argument("--serve-device-rates", default="2000,8000,0",
+                    help="rates (tiles/s) of the device-input serving runs; 0 = all at once")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
@@ -562,6 +575,13 @@ def main():
             log("serve leg")
             rates = [float(r) for r in args.serve_rates.split(",") if r]
             line["serve"] = run_serve_bench(owm, synth, rates, args.serve_requests, 48, 3)
+            drates = [float(r) for r in args.serve_device_rates.split(",") if r]
+            line["serve"] += run_serve_bench(owm, synth, drates, args.serve_requests, 48, 3, device=True)
+            best = max(r["mpts"] for r in line["serve"])
+            line["serve_summary"] = {"best_served_mpts": best,
+                                     "vs_plain": round(best / head["batch"]["plain_mpts"], 2),
+                                     "vs_batched": round(best / line["value"], 2),
+                                     "host_input_bound": "24 B per point over PCIe"}
     ctx.close()
     print(json.dumps(line), flush=True)
 

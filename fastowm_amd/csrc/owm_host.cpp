@@ -590,8 +590,8 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     ctx->prof.end(s, pi, sort_bytes, (uint64_t)L.np * passes, (uint32_t)passes);
     pi = ctx->prof.begin(s, PH_LEAVES);
     launch_leaf_aggs(s, B, T);
-    // algorithmic bytes: x, y, z and index read per point; 48 per leaf written
-    ctx->prof.end(s, pi, 28ull * L.np + 48ull * L.nleaf, L.np);
+    // algorithmic bytes: each point's two 16-byte streams read; 48 per leaf written
+    ctx->prof.end(s, pi, 32ull * L.np + 48ull * L.nleaf, L.np);
     pi = ctx->prof.begin(s, PH_LEVELS);
     launch_agg_levels(s, T, V);
     // algorithmic bytes: 48 per leaf read, 48 per level entry written

@@ -301,6 +301,8 @@ struct owm_ctx {
     uint64_t h_tab_cap = 0;
     uint8_t *h_io = nullptr;
     uint64_t h_io_cap = 0;
+    uint8_t *h_seeds = nullptr;  // serving slots: the seeds of a group, reused
+    uint64_t h_seeds_cap = 0;
     uint8_t *pinned(uint8_t *&buf, uint64_t &cap, uint64_t need);
     owm::ServeState *serve = nullptr;
     owm::Profiler prof;  // owned; released by owm_serve_stop

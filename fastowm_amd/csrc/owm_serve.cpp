@@ -4,9 +4,10 @@
 // pool (a quarter) and `max_inflight` slots.  Each slot is an internal context
 // (own stream, own arenas) driven by its own worker thread, so the groups of
 // different slots overlap on the device.  A free worker takes the oldest
-// pending requests from one shared queue: a keyed request alone (its tree is
-// kept resident under its key, so later requests with the same key and other
-// parameters only run stages 1-3), unkeyed requests as one batch of up to
+// pending requests from one shared queue: a keyed request at the head alone
+// (its tree is kept resident under its key, so later requests with the same
+// key and other parameters only run stages 1-3), else the oldest unkeyed
+// requests (past keyed ones waiting behind the head) as one batch of up to
 // `max_batch_tiles` tiles whose estimated footprint fits the slot.  A group
 // that does not fit its slot is split; a single request that does not fit
 // fails with OWM_E_BUDGET.  Every result is computed exactly as owm_run_tile
@@ -31,6 +32,7 @@ struct Request {
     uint64_t n = 0;
     owm_params p{};
     uint64_t key = 0;
+    bool device = false;  // x, y, z are device pointers (HBM)
     clk::time_point t_submit, t_done;
     int state = 0;  // 0 pending, 1 running, 2 done
     bool cancelled = false;
@@ -62,6 +64,12 @@ struct ServeState {
 // Rough device footprint of one tile in a batch (tree, sort scratch, packed
 // cloud, upload, query scratch): used only to form groups.
 static uint64_t footprint(uint64_t n) { return 420ull * n + (8ull << 20); }
+
+// Host-input bytes a group may carry: a group's chain starts once its whole
+// upload is in, so groups of host tiles stay small enough for one slot's
+// PCIe upload to overlap the other slots' device work (device-input groups
+// grow to max_batch_tiles).
+static constexpr uint64_t HOST_GROUP_BYTES = 128ull << 20;
 
 static void finish(ServeState *S, const std::shared_ptr<Request> &r, owm_status st,
                    const char *err)
@@ -103,21 +111,34 @@ static void run_group(ServeState *S, owm_ctx *slot, std::vector<std::shared_ptr<
         tiles[i].n = g[i]->n;
         tiles[i].params = g[i]->p;
         if (g[i]->n) {
-            hipError_t e = hipMemcpyAsync(dx + off, g[i]->x, g[i]->n * 8, hipMemcpyHostToDevice, slot->stream);
+            // host inputs cross PCIe; device inputs are copied within HBM
+            const hipMemcpyKind kind = g[i]->device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            hipError_t e = hipMemcpyAsync(dx + off, g[i]->x, g[i]->n * 8, kind, slot->stream);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(dy + off, g[i]->y, g[i]->n * 8, hipMemcpyHostToDevice, slot->stream);
+                e = hipMemcpyAsync(dy + off, g[i]->y, g[i]->n * 8, kind, slot->stream);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(dz + off, g[i]->z, g[i]->n * 8, hipMemcpyHostToDevice, slot->stream);
+                e = hipMemcpyAsync(dz + off, g[i]->z, g[i]->n * 8, kind, slot->stream);
             if (e != hipSuccess)
                 st = hip_fail(slot, e, "serving upload");
         }
         off += g[i]->n;
     }
-    std::vector<uint32_t> seeds(total + g.size() + 1);
+    // seeds land in the slot's page-locked buffer (reused, never cleared:
+    // a fresh zeroed array of one slot per point would cost the host more
+    // than the device spends on the group)
+    const uint64_t cap = total + g.size() + 1;
+    uint32_t *seeds = nullptr;
+    if (st == OWM_OK) {
+        seeds = (uint32_t *)slot->pinned(slot->h_seeds, slot->h_seeds_cap, cap * 4);
+        if (!seeds) {
+            set_err(slot, "pinned host allocation failed");
+            st = OWM_E_NOMEM;
+        }
+    }
     std::vector<uint64_t> so(g.size() + 1);
     if (st == OWM_OK)
-        st = owm_batch_run_device(slot, dx, dy, dz, tiles.data(), (uint32_t)g.size(), 0,
-                                  seeds.data(), seeds.size(), so.data());
+        st = owm_batch_run_device(slot, dx, dy, dz, tiles.data(), (uint32_t)g.size(), 0, seeds,
+                                  cap, so.data());
     if (st == OWM_E_BUDGET && g.size() > 1) {
         for (auto &r : g) {
             std::vector<std::shared_ptr<Request>> one{r};
@@ -127,7 +148,7 @@ static void run_group(ServeState *S, owm_ctx *slot, std::vector<std::shared_ptr<
     }
     for (size_t i = 0; i < g.size(); ++i) {
         if (st == OWM_OK)
-            g[i]->seeds.assign(seeds.begin() + so[i], seeds.begin() + so[i + 1]);
+            g[i]->seeds.assign(seeds + so[i], seeds + so[i + 1]);
         finish(S, g[i], st, slot->err.c_str());
     }
 }
@@ -151,7 +172,7 @@ static void run_keyed(ServeState *S, owm_ctx *slot, const std::shared_ptr<Reques
     }
     owm_status st = OWM_OK;
     if (!tree) {
-        st = owm_tree_build(slot, r->x, r->y, r->z, r->n, 0, r->p.leaf_size, &tree);
+        st = owm_tree_build(slot, r->x, r->y, r->z, r->n, r->device ? 1 : 0, r->p.leaf_size, &tree);
         if (st == OWM_OK) {
             // hand the tree to the pool (or keep it private when it cannot fit)
             slot->trees_bytes -= tree->mem.bytes;
@@ -184,12 +205,19 @@ static void run_keyed(ServeState *S, owm_ctx *slot, const std::shared_ptr<Reques
             }
         }
     }
-    std::vector<uint32_t> seeds(r->n + 1);
     uint64_t ns = 0;
+    uint32_t *seeds = nullptr;
+    if (st == OWM_OK) {
+        seeds = (uint32_t *)slot->pinned(slot->h_seeds, slot->h_seeds_cap, (r->n + 1) * 4);
+        if (!seeds) {
+            set_err(slot, "pinned host allocation failed");
+            st = OWM_E_NOMEM;
+        }
+    }
     if (st == OWM_OK)
-        st = owm_tree_run(slot, tree, &r->p, seeds.data(), seeds.size(), &ns);
+        st = owm_tree_run(slot, tree, &r->p, seeds, r->n + 1, &ns);
     if (st == OWM_OK)
-        r->seeds.assign(seeds.begin(), seeds.begin() + ns);
+        r->seeds.assign(seeds, seeds + ns);
     if (tree) {
         if (cached) {
             std::lock_guard<std::mutex> g(S->mu);
@@ -214,11 +242,14 @@ static void worker(ServeState *S, owm_ctx *slot)
             S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
             if (S->stop)
                 return;
-            uint64_t bytes = 0;
-            while (!S->queue.empty()) {
-                auto r = S->queue.front();
+            // the oldest request first: a keyed one runs alone; otherwise the
+            // oldest unkeyed requests form a group, taken past any keyed ones
+            // that wait behind the head (those keep their place in the queue)
+            uint64_t bytes = 0, host_bytes = 0;
+            for (auto it = S->queue.begin(); it != S->queue.end();) {
+                auto r = *it;
                 if (r->cancelled) {
-                    S->queue.pop_front();
+                    it = S->queue.erase(it);
                     r->status = OWM_E_CANCELLED;
                     r->t_done = clk::now();
                     r->state = 2;
@@ -228,19 +259,23 @@ static void worker(ServeState *S, owm_ctx *slot)
                 if (r->key) {
                     if (group.empty()) {
                         keyed = r;
-                        S->queue.pop_front();
                         r->state = 1;
+                        S->queue.erase(it);
+                        break;
                     }
-                    break;
+                    ++it;
+                    continue;
                 }
                 uint64_t f = footprint(r->n);
-                if (!group.empty() &&
-                    (group.size() >= S->max_batch || bytes + f > S->slot_budget))
+                const uint64_t h = r->device ? 0 : 24 * r->n;
+                if (!group.empty() && (group.size() >= S->max_batch || bytes + f > S->slot_budget ||
+                                       host_bytes + h > HOST_GROUP_BYTES))
                     break;
                 group.push_back(r);
                 bytes += f;
+                host_bytes += h;
                 r->state = 1;
-                S->queue.pop_front();
+                it = S->queue.erase(it);
             }
         }
         if (keyed)
@@ -279,6 +314,8 @@ owm_ctx::~owm_ctx()
         (void)hipHostFree(h_tab);
     if (h_io)
         (void)hipHostFree(h_io);
+    if (h_seeds)
+        (void)hipHostFree(h_seeds);
     if (stream)
         (void)hipStreamDestroy(stream);
 }
@@ -326,9 +363,9 @@ owm_status owm_serve_start(owm_ctx *ctx, uint32_t max_inflight, uint32_t max_bat
     }
 }
 
-owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, const double *z,
-                            uint64_t n, const owm_params *params, uint64_t tile_key,
-                            uint64_t *req_id)
+static owm_status submit_impl(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                              uint64_t n, const owm_params *params, uint64_t tile_key,
+                              uint64_t *req_id, bool device)
 {
     if (!ctx || !params || !req_id || (n && (!x || !y || !z)))
         return OWM_E_ARG;
@@ -345,6 +382,7 @@ owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, cons
         r->n = n;
         r->p = *params;
         r->key = tile_key;
+        r->device = device;
         r->t_submit = clk::now();
         {
             std::lock_guard<std::mutex> g(S->mu);
@@ -359,6 +397,20 @@ owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, cons
         set_err(ctx, "serve submit: %s", e.what());
         return OWM_E_INTERNAL;
     }
+}
+
+owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                            uint64_t n, const owm_params *params, uint64_t tile_key,
+                            uint64_t *req_id)
+{
+    return submit_impl(ctx, x, y, z, n, params, tile_key, req_id, false);
+}
+
+owm_status owm_serve_submit_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                   const double *d_z, uint64_t n, const owm_params *params,
+                                   uint64_t tile_key, uint64_t *req_id)
+{
+    return submit_impl(ctx, d_x, d_y, d_z, n, params, tile_key, req_id, true);
 }
 
 static owm_status poll_impl(owm_ctx *ctx, uint64_t req_id, uint32_t *seeds, uint64_t cap,

@@ -224,6 +224,7 @@ def lib():
             "owm_tree_run": ([vp, vp, P(Params), dp, u64, P(u64)], ctypes.c_int),
             "owm_serve_start": ([vp, u32, u32], ctypes.c_int),
             "owm_serve_submit": ([vp, dp, dp, dp, u64, P(Params), u64, P(u64)], ctypes.c_int),
+            "owm_serve_submit_device": ([vp, dp, dp, dp, u64, P(Params), u64, P(u64)], ctypes.c_int),
             "owm_serve_poll": ([vp, u64, dp, u64, P(u64), P(ctypes.c_double)], ctypes.c_int),
             "owm_serve_wait": ([vp, u64, dp, u64, P(u64), P(ctypes.c_double)], ctypes.c_int),
             "owm_serve_cancel": ([vp, u64], ctypes.c_int),
@@ -730,9 +731,17 @@ class Server:
         self._keep = {}
 
     def submit(self, x, y, z, wsize=12, bsize=20, overlap=0.8, leaf_size=0, bbox=None, key=0):
+        """x/y/z: numpy arrays (host memory) or contiguous float64 torch
+        tensors on the GPU (owm_serve_submit_device: no PCIe transfer)."""
         p = make_params(wsize, bsize, overlap, leaf_size, bbox)
-        (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
         rid = ctypes.c_uint64()
+        if _is_device(x):
+            self.ctx._check(lib().owm_serve_submit_device(
+                self.ctx.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), len(x), ctypes.byref(p),
+                int(key), ctypes.byref(rid)))
+            self._keep[rid.value] = (x, y, z)
+            return rid.value
+        (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
         self.ctx._check(lib().owm_serve_submit(self.ctx.h, xp, yp, zp, len(xa), ctypes.byref(p),
                                                int(key), ctypes.byref(rid)))
         self._keep[rid.value] = (xa, ya, za)

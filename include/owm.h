@@ -370,6 +370,13 @@ owm_status owm_serve_start(owm_ctx *ctx, uint32_t max_inflight, uint32_t max_bat
 owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, const double *z,
                             uint64_t n, const owm_params *params, uint64_t tile_key,
                             uint64_t *req_id);
+/* The same with x/y/z already in device memory (HBM) of the context's device:
+ * a served tile then never crosses PCIe (host inputs are bounded by it at 24
+ * bytes per point).  The arrays must stay valid and unchanged until the
+ * request completes. */
+owm_status owm_serve_submit_device(owm_ctx *ctx, const double *d_x, const double *d_y,
+                                   const double *d_z, uint64_t n, const owm_params *params,
+                                   uint64_t tile_key, uint64_t *req_id);
 /* OWM_E_NOTREADY while running; on completion copies the seeds and returns
  * the request's own status.  latency_s: submit -> seeds ready. */
 owm_status owm_serve_poll(owm_ctx *ctx, uint64_t req_id, uint32_t *seeds, uint64_t cap,

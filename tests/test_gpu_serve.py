@@ -91,3 +91,33 @@ def test_poll_until_done():
         assert res is not None
         assert np.array_equal(res[0], O.run(x, y, z, 12, 20, 0.8, 32)[0])
         srv.stop()
+
+
+def test_device_resident_requests():
+    """Requests whose clouds are already in HBM (owm_serve_submit_device),
+    mixed with host requests and keyed re-queries of a device-resident tile:
+    every result equals the oracle's."""
+    torch = pytest.importorskip("torch")
+    reqs = synth.serve_requests(12, rate=1000.0, seed=21, lo=10_000, hi=600_000)
+    data = [(t, t.generate()) for _, t in reqs]
+    wants = [O.run(x, y, z, t.wsize, t.bsize, t.overlap, 32)[0] for t, (x, y, z) in data]
+    devs = [[torch.from_numpy(a).to("cuda:0") for a in arrs] for _, arrs in data]
+    kt, (kx, ky, kz) = data[0]
+    kdev = devs[0]
+    requery = [(12, 20, 0.8), (9, 17, 0.6)]
+    kwant = [O.run(kx, ky, kz, ws, bs, ov, 32)[0] for ws, bs, ov in requery]
+    with owm.Context(0) as ctx:
+        srv = owm.Server(ctx, max_inflight=2, max_batch=8)
+        say("device + host submits")
+        ids = []
+        for i, ((t, (x, y, z)), dev) in enumerate(zip(data, devs)):
+            src = dev if i % 3 else (x, y, z)
+            ids.append(srv.submit(*src, t.wsize, t.bsize, t.overlap))
+        kids = [srv.submit(*kdev, ws, bs, ov, key=5) for ws, bs, ov in requery]
+        for i, rid in enumerate(ids):
+            seeds, _ = srv.wait(rid)
+            assert np.array_equal(seeds, wants[i]), f"request {i}"
+        for w, rid in zip(kwant, kids):
+            seeds, _ = srv.wait(rid)
+            assert np.array_equal(seeds, w)
+        srv.stop()
