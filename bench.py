@@ -207,15 +207,42 @@ def run_headline(ctx, torch, synth, args):
     desc = owm.batch_desc([(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0))
                            for t in tiles])
     torch.cuda.synchronize()
+    tuning = {}
     if args.tune_table:
         with open(args.tune_table) as f:
             ctx.set_tune_table(json.load(f))
         t_tune, table = 0.0, ctx.tune_table()
+        tuning["table"] = args.tune_table
     else:
-        log("autotune")
+        # the statistics model picks the table (the headline runs with it);
+        # with --tuner both the timed-trial tuner runs too and each table's
+        # batch time is measured, alternating, for the comparison
+        log(f"autotune ({args.tuner})")
         t0 = time.perf_counter()
-        table = ctx.autotune(*dev, desc)
+        table = ctx.autotune(*dev, desc, method="stats" if args.tuner != "trials" else "trials")
         t_tune = time.perf_counter() - t0
+        tuning[("stats" if args.tuner != "trials" else "trials") + "_s"] = round(t_tune, 3)
+        if args.tuner == "both":
+            t0 = time.perf_counter()
+            trial_table = ctx.autotune(*dev, desc, method="trials")
+            tuning["trials_s"] = round(time.perf_counter() - t0, 3)
+            best = {"stats": 1e300, "trials": 1e300}
+            for _ in range(3):
+                for name, tab in (("stats", table), ("trials", trial_table)):
+                    ctx.set_tune_table(tab)
+                    ctx.batch_run(*dev, desc, copy=False)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(3):
+                        ctx.batch_run(*dev, desc, copy=False)
+                    torch.cuda.synchronize()
+                    best[name] = min(best[name], (time.perf_counter() - t0) / 3 * 1e3)
+            tuning["stats_table_ms"] = round(best["stats"], 3)
+            tuning["trials_table_ms"] = round(best["trials"], 3)
+            tuning["trials_table"] = [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y",
+                                                         "block_k", "summaries", "sort_bits")}
+                                      for e in trial_table]
+            ctx.set_tune_table(table)
     for _ in range(args.warmup):
         got = ctx.batch_run(*dev, desc)
     torch.cuda.synchronize()
@@ -242,7 +269,7 @@ def run_headline(ctx, torch, synth, args):
     res = {"tiles": len(tiles), "points": int(n_total), "ms": ms, "phases": phase_table(full),
            "roofline": roofline,
            "batch": {"batched_eager_mpts": round(n_total / ms * 1e-3, 1), "batched_eager_ms": round(ms, 3),
-                     "autotune_s": round(t_tune, 2),
+                     "autotune_s": round(t_tune, 3), "tuning": tuning,
                      "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k",
                                                   "summaries", "sort_bits")} for e in table]}}
     if args.headline_only:
@@ -468,6 +495,9 @@ def main():
     ap.add_argument("--tune-table", default=None,
                     help="load this tuning table (JSON, as owm_tune_get) instead of autotuning")
     ap.add_argument("--save-tune-table", default=None, help="write the tuning table used (JSON)")
+    ap.add_argument("--tuner", choices=("stats", "trials", "both"), default="both",
+                    help="headline table from the statistics model (stats) or timed trials; "
+                         "both: the model's table, with the trials' table timed beside it")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched eager steps (no graph, plain or other legs: for profiling them)")
     ap.add_argument("--single-only", choices=("c1", "c2", "c3"), default=None,

@@ -11,7 +11,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using namespace owm;
@@ -228,6 +230,305 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
         return OWM_E_NOMEM;
     } catch (...) {
         ctx->tuner.forced = false;
+        set_err(ctx, "internal error");
+        return OWM_E_INTERNAL;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Statistics-driven tuning.  The reference chose maxNumber and minRadius by
+// hand from its statistics tool's leaf histograms (CPP-TBB-CPU/application/
+// owm_stats.cpp, include/stats_func.hpp:6-56).  Here a cost model of the batch
+// chain reads such histograms (owm_qtree_stats, maxNumber policy) of a few
+// sample tiles per size class and picks the table without timing anything.
+//
+// Per point of a tile, in ns, for leaf size L, window block b and key width S:
+//   M_LEAF / L + M_AMP * amp0 + M_COARSE * ampc + M_PASS * S / 8 + M_BLK[b]
+// plus, once per launch, M_UNDER * max(0, BLOCKS_FULL / blocks - 1) per point.
+// Over the histogram's leaves (density rho = points / side^2, weighted by
+// their points): amp0 = mean (1 + s0/ex)(1 + s0/ey) - 1 with s0 = sqrt(L/rho)
+// the side of an L-point leaf and ex, ey = bx*D + W - D, by*D + W - D the
+// extent of a window block (W = Wsize, D = Displace), i.e. how many window
+// blocks beyond the first read a point; ampc = the same with
+// s = max(s0, tile side / 2^(S/2)) minus amp0, what a key too short to order
+// the points of one Morton cell adds; `blocks` = window blocks in the launch.
+// The constants are a least-squares fit (a free intercept per set) to the batch
+// times of 67 homogeneous tile sets x 48 configurations on MI355X
+// (fastowm_amd/tune/fit_model.py on fastowm_amd/tune/trials_mi355x.json); on
+// those sets the model's pick is 0.7 % slower than the fastest configuration
+// on average (5.8 % at most), against 38.7 % for leaf 32 / 8x8x4 / 32 bits.
+namespace {
+
+constexpr double M_LEAF = 0.470672;
+constexpr double M_AMP = 0.044566;
+constexpr double M_COARSE = 0.0180814;
+constexpr double M_PASS = 0.0192899;
+constexpr double M_BLK[4] = {-0.00100764, -0.00759359, -0.000193196, 0.00879443};
+constexpr double M_UNDER = 0.0113509;
+constexpr double BLOCKS_FULL = 1024.0;
+
+struct ModelTile {
+    double n = 0, weight = 0;
+    double W = 0, D = 0, side = 0;
+    uint32_t ncols = 0, nrows = 0;
+    std::vector<double> rho, w;
+    double sw = 0;
+    int cls = 0;
+};
+
+bool model_tile(const owm_tune_sample &s, ModelTile &m)
+{
+    if (s.n == 0 || !(s.weight >= (double)s.n) || !std::isfinite(s.weight) ||
+        (s.n_bins && !s.bins) || !(s.summary.max_radius > 0.0f))
+        return false;
+    const double *box = (s.params.flags & OWM_F_HAVE_BBOX) ? s.params.bbox : s.summary.bbox;
+    Derived d;
+    std::string why;
+    if (derive_params(s.params, s.n, box, d, why) != OWM_OK)
+        return false;
+    m.n = (double)s.n;
+    m.weight = s.weight;
+    m.W = (double)s.params.wsize;
+    m.D = d.displace;
+    m.side = std::max(box[1] - box[0], box[3] - box[2]);
+    m.ncols = d.ncols;
+    m.nrows = d.nrows;
+    m.cls = Tuner::class_of(s.n);
+    const double r0 = (double)s.summary.max_radius;
+    for (uint64_t k = 0; k < s.n_bins; ++k) {
+        const owm_leaf_bin &b = s.bins[k];
+        if (b.points == 0 || b.leaves == 0 || b.level > 62)
+            continue;
+        const double side = 2.0 * r0 / std::ldexp(1.0, (int)b.level);
+        m.rho.push_back((double)b.points / (side * side));
+        m.w.push_back((double)b.points * (double)b.leaves);
+        m.sw += m.w.back();
+    }
+    return m.sw > 0.0;
+}
+
+int block_index(const Stage1Cfg &c)
+{
+    for (int j = 0; j < 4; ++j)
+        if (BLOCKS[j].bx == c.bx && BLOCKS[j].by == c.by && BLOCKS[j].k == c.k &&
+            BLOCKS[j].lcs == c.lcs)
+            return j;
+    return 0;
+}
+
+// ns per point of one tile, launch term left out
+double tile_cost(const ModelTile &m, uint32_t leaf, int b, uint32_t sort_bits)
+{
+    const Stage1Cfg &B = BLOCKS[b];
+    const double ex = B.bx * m.D + m.W - m.D, ey = B.by * m.D + m.W - m.D;
+    const double cell = m.side / std::ldexp(1.0, (int)sort_bits / 2);
+    double a0 = 0, a = 0;
+    for (size_t i = 0; i < m.rho.size(); ++i) {
+        const double s0 = std::sqrt((double)leaf / m.rho[i]), s = std::max(s0, cell);
+        a0 += m.w[i] * ((1 + s0 / ex) * (1 + s0 / ey) - 1);
+        a += m.w[i] * ((1 + s / ex) * (1 + s / ey) - 1);
+    }
+    a0 /= m.sw;
+    a /= m.sw;
+    return M_LEAF / leaf + M_AMP * a0 + M_COARSE * (a - a0) + M_PASS * sort_bits / 8.0 + M_BLK[b];
+}
+
+double blocks_of(const ModelTile &m, int b)
+{
+    const Stage1Cfg &B = BLOCKS[b];
+    return m.weight / m.n * (double)((m.ncols + B.bx - 1) / B.bx) *
+           (double)((m.nrows + B.by - 1) / B.by);
+}
+
+struct Pick {
+    int block = 0;
+    uint32_t sort_bits = 32;
+    uint32_t leaf[6] = {0, 0, 0, 0, 0, 0};
+    double ns = 1e300;  // predicted ns for the tiles' points
+};
+
+// Window block and key width for one launch over `tiles`, and the leaf size of
+// every class among them under that choice.
+Pick pick_launch(const std::vector<const ModelTile *> &tiles)
+{
+    Pick best;
+    double pts = 0;
+    for (const ModelTile *m : tiles)
+        pts += m->weight;
+    for (int b = 0; b < 4; ++b)
+        for (uint32_t sb : SORT_BITS) {
+            double blocks = 0;
+            for (const ModelTile *m : tiles)
+                blocks += blocks_of(*m, b);
+            Pick p;
+            p.block = b;
+            p.sort_bits = sb;
+            p.ns = M_UNDER * std::max(0.0, BLOCKS_FULL / std::max(blocks, 1.0) - 1.0) * pts;
+            for (int c = 0; c < 6; ++c) {
+                double cbest = 1e300;
+                bool any = false;
+                for (uint32_t leaf : LEAVES) {
+                    double ns = 0;
+                    for (const ModelTile *m : tiles)
+                        if (m->cls == c) {
+                            ns += m->weight * tile_cost(*m, leaf, b, sb);
+                            any = true;
+                        }
+                    if (any && ns < cbest) {
+                        cbest = ns;
+                        p.leaf[c] = leaf;
+                    }
+                }
+                if (any)
+                    p.ns += cbest;
+            }
+            if (p.ns < best.ns)
+                best = p;
+        }
+    return best;
+}
+
+owm_status model_table(const owm_tune_sample *samples, uint32_t n, std::vector<Tuner::Entry> &table)
+{
+    std::vector<ModelTile> tiles(n);
+    for (uint32_t k = 0; k < n; ++k)
+        if (!model_tile(samples[k], tiles[k]))
+            return OWM_E_ARG;
+    table = Tuner().table;
+    if (n == 0)
+        return OWM_OK;
+    std::vector<const ModelTile *> all;
+    double total = 0;
+    for (const ModelTile &m : tiles) {
+        all.push_back(&m);
+        total += m.weight;
+    }
+    // the batch: leaf sizes of its classes, block and key width of its class
+    const Pick whole = pick_launch(all);
+    // a launch of one class's tiles alone: block and key width of that class
+    for (int c = 0; c < 6; ++c) {
+        std::vector<const ModelTile *> mine;
+        for (const ModelTile &m : tiles)
+            if (m.cls == c)
+                mine.push_back(&m);
+        if (mine.empty())
+            continue;
+        const Pick own = pick_launch(mine);
+        table[c].cfg = TuneCfg{whole.leaf[c], BLOCKS[own.block], own.sort_bits};
+        table[c].ms = own.ns * 1e-6;
+    }
+    const int bc = Tuner::class_of((uint64_t)std::min(total, 1.8e19));
+    table[bc].cfg.s1 = BLOCKS[whole.block];
+    table[bc].cfg.sort_bits = whole.sort_bits;
+    table[bc].ms = whole.ns * 1e-6;
+    return OWM_OK;
+}
+
+}  // namespace
+
+owm_status owm_tune_model(const owm_tune_sample *samples, uint32_t n_samples, owm_tune_entry *out,
+                          uint32_t cap, uint32_t *n_out)
+{
+    if ((n_samples && !samples) || (cap && !out))
+        return OWM_E_ARG;
+    try {
+        std::vector<Tuner::Entry> table;
+        owm_status st = model_table(samples, n_samples, table);
+        if (st != OWM_OK)
+            return st;
+        if (n_out)
+            *n_out = (uint32_t)table.size();
+        if (!out)
+            return OWM_OK;
+        if (cap < table.size())
+            return OWM_E_CAPACITY;
+        for (size_t k = 0; k < table.size(); ++k)
+            to_entry(table[k], out[k]);
+        return OWM_OK;
+    } catch (...) {
+        return OWM_E_NOMEM;
+    }
+}
+
+owm_status owm_autotune_stats(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                              int device_ptrs, const owm_tile *tiles, uint32_t n_tiles,
+                              owm_tune_entry *out, uint32_t cap, uint32_t *n_out)
+{
+    if (!ctx || !tiles || n_tiles == 0 || !x || !y || !z || (cap && !out))
+        return OWM_E_ARG;
+    try {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->tuner.forced) {
+            set_err(ctx, "autotune is running on this context");
+            return OWM_E_ARG;
+        }
+        // up to OWM_TUNE_SAMPLES tiles of each class, spread over the batch
+        std::vector<uint32_t> members[6];
+        double cls_pts[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t t = 0; t < n_tiles; ++t)
+            if (tiles[t].n > 0) {
+                const int c = Tuner::class_of(tiles[t].n);
+                members[c].push_back(t);
+                cls_pts[c] += (double)tiles[t].n;
+            }
+        std::vector<owm_tune_sample> samples;
+        std::vector<std::vector<owm_leaf_bin>> bins;
+        samples.reserve(6 * OWM_TUNE_SAMPLES);
+        bins.reserve(6 * OWM_TUNE_SAMPLES);
+        for (int c = 0; c < 6; ++c) {
+            const size_t cnt = members[c].size();
+            const size_t k = std::min<size_t>(cnt, OWM_TUNE_SAMPLES);
+            const size_t first = samples.size();
+            double got = 0;
+            for (size_t j = 0; j < k; ++j) {
+                const owm_tile &tl = tiles[members[c][(2 * j + 1) * cnt / (2 * k)]];
+                owm_qtree_params q;
+                memset(&q, 0, sizeof q);
+                q.policy = OWM_QT_MAXNUMBER;
+                q.max_number = OWM_TUNE_STATS_MAXNUMBER;
+                q.level = OWM_TUNE_STATS_LEVEL;
+                if (tl.params.flags & OWM_F_HAVE_BBOX) {
+                    q.flags = OWM_F_HAVE_BBOX;
+                    memcpy(q.bbox, tl.params.bbox, sizeof q.bbox);
+                }
+                std::vector<owm_leaf_bin> hb(1024);
+                owm_tune_sample s;
+                memset(&s, 0, sizeof s);
+                owm_status st = owm_qtree_stats(ctx, x + tl.offset, y + tl.offset, tl.n, device_ptrs,
+                                                &q, hb.data(), hb.size(), &s.summary);
+                if (st == OWM_E_CAPACITY) {
+                    hb.resize(s.summary.n_bins);
+                    st = owm_qtree_stats(ctx, x + tl.offset, y + tl.offset, tl.n, device_ptrs, &q,
+                                         hb.data(), hb.size(), &s.summary);
+                }
+                if (st == OWM_E_ARG)
+                    continue;  // more coincident points than a leaf may hold: no sample
+                if (st != OWM_OK)
+                    return st;
+                hb.resize(s.summary.n_bins);
+                bins.push_back(std::move(hb));
+                s.n = tl.n;
+                s.params = tl.params;
+                s.bins = bins.back().data();
+                s.n_bins = bins.back().size();
+                samples.push_back(s);
+                got += (double)tl.n;
+            }
+            for (size_t j = first; j < samples.size(); ++j)  // the class's share of the batch
+                samples[j].weight = (double)samples[j].n * cls_pts[c] / got;
+        }
+        std::vector<Tuner::Entry> table;
+        owm_status st = model_table(samples.data(), (uint32_t)samples.size(), table);
+        if (st != OWM_OK) {
+            set_err(ctx, "statistics of a sampled tile do not fit the tuning model");
+            return st;
+        }
+        ctx->tuner.table = table;
+        return put_table(ctx, out, cap, n_out);
+    } catch (const std::bad_alloc &) {
+        set_err(ctx, "host allocation failed");
+        return OWM_E_NOMEM;
+    } catch (...) {
         set_err(ctx, "internal error");
         return OWM_E_INTERNAL;
     }

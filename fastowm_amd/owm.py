@@ -78,6 +78,8 @@ class TuneEntry(ctypes.Structure):
 
 
 TUNE_CLASSES = 6
+# leaf histogram settings of the statistics-driven tuner (owm_autotune_stats)
+TUNE_STATS_MAXNUMBER, TUNE_STATS_LEVEL = 256, 0
 TREE_LEAF = 0x80000000
 
 
@@ -108,6 +110,12 @@ class QtreeSummary(ctypes.Structure):
     _fields_ = [("bbox", ctypes.c_double * 4), ("max_radius", ctypes.c_float),
                 ("depth", ctypes.c_uint32), ("leaves", ctypes.c_uint64),
                 ("n_bins", ctypes.c_uint64)]
+
+
+class TuneSample(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("weight", ctypes.c_double), ("params", Params),
+                ("bins", ctypes.POINTER(LeafBin)), ("n_bins", ctypes.c_uint64),
+                ("summary", QtreeSummary)]
 
 
 def qtree_params(policy=QT_MAXNUMBER, max_number=32, min_radius=0.1, level=5, bbox=None):
@@ -235,6 +243,9 @@ def lib():
             "owm_profile_read": ([vp, P(PhaseStat), u32, P(u32)], ctypes.c_int),
             "owm_autotune": ([vp, dp, dp, dp, ctypes.c_int, P(TileDesc), u32, P(TuneEntry), u32,
                               P(u32)], ctypes.c_int),
+            "owm_autotune_stats": ([vp, dp, dp, dp, ctypes.c_int, P(TileDesc), u32, P(TuneEntry),
+                                    u32, P(u32)], ctypes.c_int),
+            "owm_tune_model": ([P(TuneSample), u32, P(TuneEntry), u32, P(u32)], ctypes.c_int),
             "owm_tune_get": ([vp, P(TuneEntry), u32, P(u32)], ctypes.c_int),
             "owm_tune_set": ([vp, P(TuneEntry), u32], ctypes.c_int),
             "owm_qtree_stats": ([vp, dp, dp, u64, ctypes.c_int, P(QtreeParams), P(LeafBin),
@@ -314,6 +325,34 @@ def _dev_ptr(t):
     # the library runs on its own stream: make the producer's work visible first
     torch.cuda.current_stream(t.device).synchronize()
     return t.data_ptr()
+
+
+def tune_model(samples):
+    """The statistics model alone (owm_tune_model, host code): samples are
+    (n, weight, dict(wsize=, overlap=, bsize=, bbox=), bins, summary) with
+    bins/summary as Context.qtree_stats returns them (maxNumber policy,
+    TUNE_STATS_MAXNUMBER, level TUNE_STATS_LEVEL).  Returns the table."""
+    arr = (TuneSample * max(len(samples), 1))()
+    keep = []
+    for i, (n, weight, prm, bins, summary) in enumerate(samples):
+        b = (LeafBin * max(len(bins), 1))()
+        for k, (lv, pts, leaves) in enumerate(bins):
+            b[k].level, b[k].points, b[k].leaves = lv, pts, leaves
+        keep.append(b)
+        arr[i].n, arr[i].weight = n, weight
+        arr[i].params = make_params(prm.get("wsize", 12), prm.get("bsize", 20),
+                                    prm.get("overlap", 0.8), 0, prm.get("bbox"))
+        arr[i].bins = ctypes.cast(b, ctypes.POINTER(LeafBin))
+        arr[i].n_bins = len(bins)
+        arr[i].summary.bbox[:] = list(summary["bbox"])
+        arr[i].summary.max_radius = summary["max_radius"]
+        arr[i].summary.n_bins = len(bins)
+    out = (TuneEntry * TUNE_CLASSES)()
+    n = ctypes.c_uint32()
+    st = lib().owm_tune_model(arr, len(samples), out, TUNE_CLASSES, ctypes.byref(n))
+    if st != 0:
+        raise OWMError(st, "owm_tune_model rejected the samples")
+    return [out[i].as_dict() for i in range(n.value)]
 
 
 class Tree:
@@ -623,19 +662,21 @@ class Context:
         return np.split(seeds, so[1:-1].astype(np.intp))  # a view per tile
 
     # -- autotuner ----------------------------------------------------------
-    def autotune(self, x, y, z, tiles):
-        """Times leaf sizes, window blocks and sort widths on this batch (same
-        form as batch_run) and keeps the winners; returns the table."""
+    def autotune(self, x, y, z, tiles, method="trials"):
+        """Tunes the table for this batch (same form as batch_run) and returns
+        it.  method "trials" times leaf sizes, window blocks and sort widths
+        and keeps the winners (owm_autotune); "stats" lets the cost model pick
+        them from leaf histograms of sampled tiles (owm_autotune_stats)."""
+        fn = {"trials": lib().owm_autotune, "stats": lib().owm_autotune_stats}[method]
         descs, _ = self._descs(tiles)
         out = (TuneEntry * TUNE_CLASSES)()
         n = ctypes.c_uint32()
         if _is_device(x):
-            st = lib().owm_autotune(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), 1, descs,
-                                    len(tiles), out, TUNE_CLASSES, ctypes.byref(n))
+            st = fn(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), 1, descs, len(descs), out,
+                    TUNE_CLASSES, ctypes.byref(n))
         else:
             (xa, xp), (ya, yp), (za, zp) = _host(x), _host(y), _host(z)
-            st = lib().owm_autotune(self.h, xp, yp, zp, 0, descs, len(tiles), out, TUNE_CLASSES,
-                                    ctypes.byref(n))
+            st = fn(self.h, xp, yp, zp, 0, descs, len(descs), out, TUNE_CLASSES, ctypes.byref(n))
         self._check(st)
         return [out[i].as_dict() for i in range(n.value)]
 

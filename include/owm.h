@@ -135,7 +135,8 @@ owm_status owm_run_tile_device(owm_ctx *ctx, const double *d_x, const double *d_
  * the process, host staging lives on the call's stack, phase timing is paused,
  * and nothing is allocated (no hipMalloc/hipFree, no page-locked or host heap
  * memory, no events).  A tile whose work needs more scratch than that (the
- * work takes about 105 bytes per point, so up to about 2.5 million points) is
+ * work takes about 76 bytes per point under the default table, so up to
+ * about 3.5 million points; longer sort keys take more) is
  * refused with OWM_E_BUDGET and a message giving the bytes needed.  The scratch is shared by every call on a device and
  * keeps the last call's contents: calls that overlap on one device (from two
  * threads or two contexts) race and must be serialised by the caller. */
@@ -277,7 +278,8 @@ typedef struct owm_tune_entry {
     uint32_t block_x, block_y, block_k; /* stage-1 window block, points per lane per round */
     uint32_t summaries;  /* 1: stage 1 reads per-leaf summaries of equal window-edge classes */
     uint32_t sort_bits;  /* Morton bits of the per-tile sort key: 16, 24 or 32 (2-4 passes) */
-    double ms;           /* time of the winning trial; 0 = default, not tuned */
+    double ms;           /* time of the winning trial (owm_autotune) or the model's
+                          * predicted time (owm_autotune_stats); 0 = default */
 } owm_tune_entry;
 /* Times trial configurations on the given batch (tiles as in owm_batch_run,
  * their leaf_size ignored): leaf sizes 16..128 per tile class on the tiles of
@@ -290,8 +292,44 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
                         owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
 owm_status owm_tune_get(owm_ctx *ctx, owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
 /* Loads a table (e.g. one saved from owm_tune_get); classes not listed keep
- * the defaults. */
+ * the defaults (leaf 64, block 16x8 with 4 points per lane, no summaries,
+ * 16-bit keys: the configuration with the lowest mean time over the tile sets
+ * the statistics model below was fitted to). */
 owm_status owm_tune_set(owm_ctx *ctx, const owm_tune_entry *in, uint32_t n);
+
+/* ---- statistics-driven tuning ----------------------------------------------
+ * The reference picked maxNumber and minRadius by hand from its statistics
+ * tool's leaf histograms (CPP-TBB-CPU/application/owm_stats.cpp,
+ * include/stats_func.hpp:6-56).  Here a cost model of the batch chain, fitted
+ * to trial times on MI355X, reads such histograms (owm_qtree_stats under the
+ * maxNumber policy with the settings below) of a few tiles per size class and
+ * picks the table -- leaf size per tile class, window block and key width per
+ * class of a launch's total -- without timing anything. */
+#define OWM_TUNE_STATS_MAXNUMBER 256u /* histogram settings the model expects */
+#define OWM_TUNE_STATS_LEVEL 0u
+#define OWM_TUNE_SAMPLES 16u          /* tiles sampled per class by owm_autotune_stats */
+typedef struct owm_tune_sample {
+    uint64_t n;                /* points of the sampled tile */
+    double weight;             /* points of the batch it stands for (>= n) */
+    owm_params params;         /* its OWM parameters (wsize, overlap; bbox if flagged) */
+    const owm_leaf_bin *bins;  /* its owm_qtree_stats histogram (settings above) */
+    uint64_t n_bins;
+    owm_qtree_summary summary; /* the same call's summary (box, root radius) */
+} owm_tune_sample;
+/* The model alone, host code: the table it picks for a batch the samples
+ * describe.  Classes without samples keep the defaults; out[k].ms is the
+ * predicted time of a launch of that class's tiles (for the class of the total
+ * weight: of the whole batch).  OWM_E_ARG for a sample whose parameters or
+ * histogram the model cannot use; cap >= OWM_TUNE_CLASSES. */
+owm_status owm_tune_model(const owm_tune_sample *samples, uint32_t n_samples,
+                          owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
+/* owm_autotune's counterpart without trials: histograms of up to
+ * OWM_TUNE_SAMPLES tiles per class (spread over the batch; a tile with more
+ * coincident points than a histogram leaf may hold is skipped), then the
+ * model's table, stored in the context like owm_autotune's winners. */
+owm_status owm_autotune_stats(owm_ctx *ctx, const double *x, const double *y, const double *z,
+                              int device_ptrs, const owm_tile *tiles, uint32_t n_tiles,
+                              owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
 
 /* ---- stage-level API on a device-resident tree ---------------------------
  * Mirrors the reference's internal boundary (include/optim3_func.hpp:85-93):

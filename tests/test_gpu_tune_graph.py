@@ -182,3 +182,88 @@ def test_batch_desc_and_buffer_views():
         for t, (tx, ty, tz), g in zip(tiles, arrs, want):
             w, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
             assert np.array_equal(g, w)
+
+
+def _stats_samples(ctx, x, y, desc_list):
+    """The samples owm_autotune_stats takes: up to OWM_TUNE_SAMPLES tiles per
+    class, spread over the batch, each standing for its class's share."""
+    offs = np.cumsum([0] + [n for n, _ in desc_list])
+    members = {}
+    for t, (n, _) in enumerate(desc_list):
+        if n:
+            members.setdefault(next(i for i, m in enumerate(CLASS_MAX) if n <= m), []).append(t)
+    samples = []
+    for c, mem in sorted(members.items()):
+        k = min(len(mem), 16)
+        chosen = [mem[(2 * j + 1) * len(mem) // (2 * k)] for j in range(k)]
+        got = float(sum(desc_list[t][0] for t in chosen))
+        cls_pts = float(sum(desc_list[t][0] for t in mem))
+        for t in chosen:
+            n, prm = desc_list[t]
+            bins, sm = ctx.qtree_stats(x[offs[t]:offs[t] + n], y[offs[t]:offs[t] + n],
+                                       policy=owm.QT_MAXNUMBER, max_number=owm.TUNE_STATS_MAXNUMBER,
+                                       level=owm.TUNE_STATS_LEVEL)
+            samples.append((n, n * cls_pts / got, prm, bins, sm))
+    return samples
+
+
+CLASS_MAX = [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 2 ** 64 - 1]
+
+
+def test_stats_autotune_keeps_results_and_matches_the_model():
+    """owm_autotune_stats: the table is the model's pick from the sampled
+    tiles' histograms (as owm_tune_model gives it), host and device inputs give
+    the same table, every class present gets a predicted time, and the tuned
+    batch gives the untuned seeds, equal to the oracle's."""
+    torch = pytest.importorskip("torch")
+    sizes = [30_000, 80_000, 300_000, 1_200_000, 50_000] + [20_000 + 7_000 * i for i in range(20)]
+    x, y, z, desc, tiles, arrs = _batch(sizes, seed=410)
+    with owm.Context(0) as ctx:
+        say("untuned batch")
+        before = ctx.batch_run(x, y, z, desc)
+        say("statistics autotune, host inputs")
+        table = ctx.autotune(x, y, z, desc, method="stats")
+        assert len(table) == owm.TUNE_CLASSES
+        for e in table:
+            assert e["leaf_size"] in (16, 32, 64, 128) and e["sort_bits"] in (16, 24, 32)
+        assert all(table[c]["ms"] > 0 for c in (0, 1, 2, 3))
+        say("device inputs")
+        dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
+        assert ctx.autotune(*dev, desc, method="stats") == table
+        say("model on the same samples")
+        want = owm.tune_model(_stats_samples(ctx, x, y, desc))
+        for a, b in zip(table, want):
+            assert {k: v for k, v in a.items() if k != "ms"} == {k: v for k, v in b.items() if k != "ms"}
+            assert a["ms"] == pytest.approx(b["ms"], rel=1e-9)
+        say("tuned batch equals untuned and the oracle")
+        after = ctx.batch_run(x, y, z, desc)
+        assert _same(after, before)
+        for t, (tx, ty, tz), g in list(zip(tiles, arrs, after))[:6]:
+            w, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
+            assert np.array_equal(g, w)
+
+
+def test_stats_autotune_skips_a_tile_it_cannot_histogram():
+    """A tile with more coincident points than a histogram leaf may hold has no
+    histogram; the tuner samples the others and the batch still matches the
+    oracle."""
+    rng = np.random.default_rng(9)
+    x1 = np.full(4_000, 500_100.0) + np.repeat(rng.uniform(0, 40, 4), 1_000)
+    y1 = np.full(4_000, 4_200_100.0) + np.repeat(rng.uniform(0, 40, 4), 1_000)
+    z1 = rng.uniform(0, 5, 4_000)
+    t2 = synth.scaled(synth.config_c1(seed=11), 60_000)
+    x2, y2, z2 = t2.generate()
+    x, y, z = (np.concatenate(a) for a in ((x1, x2), (y1, y2), (z1, z2)))
+    desc = [(4_000, dict(wsize=12, bsize=20, overlap=0.8, leaf_size=0)),
+            (60_000, dict(wsize=12, bsize=20, overlap=0.8, leaf_size=0))]
+    with owm.Context(0) as ctx:
+        say("coincident tile in the stats tuner")
+        with pytest.raises(owm.OWMError):
+            ctx.qtree_stats(x1, y1, policy=owm.QT_MAXNUMBER, max_number=owm.TUNE_STATS_MAXNUMBER,
+                            level=owm.TUNE_STATS_LEVEL)
+        table = ctx.autotune(x, y, z, desc, method="stats")
+        assert table[0]["ms"] > 0
+        got = ctx.batch_run(x, y, z, desc)
+        for (tx, ty, tz), g in (((x1, y1, z1), got[0]), ((x2, y2, z2), got[1])):
+            w, _, _ = O.run(tx, ty, tz, 12, 20, 0.8, 32)
+            assert np.array_equal(g, w)

@@ -10,6 +10,8 @@ memory stays at zero bytes throughout: the no-allocation calls never grow its
 arenas."""
 import ctypes
 
+import re
+
 import numpy as np
 import pytest
 
@@ -64,13 +66,17 @@ def test_noalloc_equals_device_entry_point():
         st, _, ns = _call(L.owm_run_tile_device_noalloc, nctx, dev, len(x), p, 3)
         assert st == 5 and ns == len(got[1])  # OWM_E_CAPACITY with the size needed
         say("too large for the scratch")
-        t = synth.scaled(synth.config_c2(seed=6), 3_000_000)
+        t = synth.scaled(synth.config_c2(seed=6), 6_000_000)
         bx, by, bz = t.generate()
         bdev = [torch.from_numpy(a).to("cuda:0") for a in (bx, by, bz)]
         torch.cuda.synchronize()
         st, _, _ = _call(L.owm_run_tile_device_noalloc, nctx, bdev, len(bx), owm.make_params(12, 20, 0.8, 32),
                          len(bx) + 1)
-        assert st == 3 and "bytes of scratch" in nctx.last_error(), nctx.last_error()
+        msg = nctx.last_error()
+        assert st == 3 and "bytes of scratch" in msg, msg
+        need = int(re.search(r"needs (\d+) bytes", msg).group(1))
+        say("scratch needed", need, "bytes,", round(need / len(bx), 1), "per point")
+        assert need > 256 << 20
         assert nctx.device_bytes() == 0
         say("after the refusal")
         x, y, z, dev, p, _ = data[1]
