@@ -155,6 +155,43 @@ def cpu_single(x, y, z, want, reps, name):
             "equal_to_gpu": same}
 
 
+def cpu_window_split(x, y, z, gpu_phases, factors=(0.02, 0.05, 0.1), reps=3):
+    """The CPU side of a column split of the stage-1 windows
+    (octree_traverse_heter: nCols*factor columns to one device, the rest to
+    the other): the oracle's stage 1 (the reference's exact per-window query)
+    over the first factor*nCols columns with this host's cores, against the
+    GPU's stage 1 of the whole tile.  A split cannot finish before
+    max(GPU stage 1 x (1 - factor), CPU share); the tree's copy to the host and
+    the merge of the two halves would only add."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = cpu_cores()
+    pyoracle.set_threads(cores)
+    box = (float(x.min()), float(x.max()), float(y.min()), float(y.max()))
+    p = pyoracle.params(12, 20, 0.8, box, len(x), 32)
+    t0 = time.perf_counter()
+    tree = pyoracle.Tree(x, y, z, p)
+    t_tree = time.perf_counter() - t0
+    gpu_ms = sum(gpu_phases[k]["ms_per_step"] for k in ("stage1", "stage1_join") if k in gpu_phases)
+    rows = []
+    for f in factors:
+        cols = max(1, int(round(f * p.ncols)))
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            tree.stage1_cols(0, cols)
+            times.append(time.perf_counter() - t0)
+        cpu_ms = statistics.median(times) * 1e3
+        bound = max(gpu_ms * (1.0 - cols / p.ncols), cpu_ms)
+        rows.append({"factor": f, "cpu_columns": cols, "cpu_ms": round(cpu_ms, 3),
+                     "split_ms_at_least": round(bound, 3),
+                     "vs_gpu_alone": round(gpu_ms / bound, 4)})
+    del tree
+    return {"gpu_stage1_ms": round(gpu_ms, 4), "columns": int(p.ncols), "rows": int(p.nrows),
+            "cpu_tree_s": round(t_tree, 2), "cores": cores, "kind": "port", "splits": rows,
+            "note": "vs_gpu_alone < 1: the split is slower than the GPU alone"}
+
+
 def phase_table(full):
     out = {}
     for name, p in full.items():
@@ -588,6 +625,9 @@ def main():
                     single[key]["cpu"] = cpu_single(x, y, z, seeds, 5, "the same 1M-point tile")
                     single[key]["cpu"]["gpu_over_cpu"] = round(single[key]["repeat_mpts"] /
                                                               single[key]["cpu"]["mpts"], 1)
+                if key == "c2" and not args.no_cpu:
+                    log("window split: CPU share of the stage-1 columns")
+                    single[key]["window_split"] = cpu_window_split(x, y, z, single[key]["phases"])
                 if key != "c2":
                     del dev
                 else:

@@ -505,6 +505,32 @@ void oracle_stage1(const void *tree, const oracle_params *p, uint32_t *min_ids)
     }
 }
 
+/* stage1 over the window columns [col0, col1) only (every row), written at
+ * the same indices as oracle_stage1: the CPU share of a column split of the
+ * windows between devices (SYCL-CUDA-GPU/octree/octree_traverse.inl:119-180
+ * gives wCols = nCols*factor columns to one side, basic/traverse.h:916-1037
+ * the rest to the TBB query). */
+void oracle_stage1_cols(const void *tree, const oracle_params *p, uint32_t col0, uint32_t col1,
+                        uint32_t *min_ids)
+{
+    const otree *T = (const otree *)tree;
+    if (col1 > p->ncols)
+        col1 = p->ncols;
+    if (col0 >= col1)
+        return;
+    const int64_t w = col1 - col0, W = w * (int64_t)p->nrows;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t s = 0; s < W; ++s) {
+        uint32_t ii = col0 + (uint32_t)(s % w), jj = (uint32_t)(s / w);
+        double b[4];
+        window_box(p, ii, jj, b);
+        qres r = {0, ORACLE_NONE};
+        query(T, 0, b[0], b[1], b[2], b[3], &r);
+        min_ids[(int64_t)jj * p->ncols + ii] =
+            (r.cnt >= p->min_num_points && r.best != ORACLE_NONE) ? r.best : ORACLE_NONE;
+    }
+}
+
 static void radix_sort_u32(uint32_t *a, uint32_t n)
 {
     uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));

@@ -55,6 +55,10 @@ uint64_t oracle_tree_nodes(const void *tree);
 int oracle_tree_check(const void *tree);
 
 void oracle_stage1(const void *tree, const oracle_params *p, uint32_t *min_ids);
+/* stage1 over window columns [col0, col1) only, same indices as oracle_stage1
+ * (the CPU share of a column split of the windows). */
+void oracle_stage1_cols(const void *tree, const oracle_params *p, uint32_t col0, uint32_t col1,
+                        uint32_t *min_ids);
 uint32_t oracle_stage2(const oracle_params *p, uint32_t *min_ids, uint32_t nwin, uint32_t *llp);
 uint32_t oracle_stage3(const void *tree, const oracle_params *p, const uint32_t *llp,
                        uint32_t nllp, uint32_t *adds);

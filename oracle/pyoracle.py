@@ -58,6 +58,7 @@ def lib():
         L.oracle_tree_check.argtypes = [ctypes.c_void_p]
         L.oracle_tree_check.restype = ctypes.c_int
         L.oracle_stage1.argtypes = [ctypes.c_void_p, pp, up]
+        L.oracle_stage1_cols.argtypes = [ctypes.c_void_p, pp, ctypes.c_uint32, ctypes.c_uint32, up]
         L.oracle_stage2.argtypes = [pp, up, ctypes.c_uint32, up]
         L.oracle_stage2.restype = ctypes.c_uint32
         L.oracle_stage3.argtypes = [ctypes.c_void_p, pp, up, ctypes.c_uint32, up]
@@ -129,6 +130,14 @@ class Tree:
     def stage1(self):
         out = np.empty(self.p.ncols * self.p.nrows, np.uint32)
         lib().oracle_stage1(self.h, ctypes.byref(self.p), _up(out))
+        return out
+
+    def stage1_cols(self, col0, col1, out=None):
+        """stage1 of the windows in columns [col0, col1) (every row), at the
+        indices stage1 uses; other entries of out are left as they are."""
+        if out is None:
+            out = np.full(self.p.ncols * self.p.nrows, NONE, np.uint32)
+        lib().oracle_stage1_cols(self.h, ctypes.byref(self.p), col0, col1, _up(out))
         return out
 
     def stage3(self, llp):

@@ -82,6 +82,12 @@ static void one_cloud(const char *name, const double *x, const double *y, const 
     oracle_stage1_brute(x, y, z, n, &p, m_brute);
     CHECK(memcmp(m_tree, m_brute, sizeof(uint32_t) * nwin) == 0, "%s: stage 1 tree != brute",
           name);
+    /* the column split: three column ranges give the whole stage 1 */
+    memset(m_brute, 0xAB, sizeof(uint32_t) * nwin);
+    oracle_stage1_cols(tree, &p, 0, p.ncols / 3, m_brute);
+    oracle_stage1_cols(tree, &p, p.ncols / 3, p.ncols / 2, m_brute);
+    oracle_stage1_cols(tree, &p, p.ncols / 2, p.ncols + 5, m_brute);
+    CHECK(memcmp(m_tree, m_brute, sizeof(uint32_t) * nwin) == 0, "%s: stage 1 by columns", name);
     const uint32_t nllp = oracle_stage2(&p, m_tree, nwin, llp);
     CHECK(nllp <= nwin, "%s: stage 2 count %u > %u", name, nllp, nwin);
     uint32_t na = 0, nb = 0;

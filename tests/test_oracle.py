@@ -113,6 +113,25 @@ def test_tree_stages_match_bruteforce(n, ws, bs, ov, leaf):
     assert np.array_equal(seeds, got[0])
 
 
+@pytest.mark.parametrize("n,ws,bs,ov,leaf", CASES[:3])
+def test_stage1_by_column_ranges(n, ws, bs, ov, leaf):
+    """The column split of the windows (one side takes columns [0, c), the
+    other the rest) gives brute force's stage 1, whatever the split column."""
+    t = _tile(n, seed=n + ws + 1, cluster_frac=0.5, n_clusters=5, cluster_sigma=4.0)
+    x, y, z = t.generate()
+    box = synth.bbox(x, y)
+    p = O.params(ws, bs, ov, box, len(x), leaf)
+    want = O.stage1_brute(x, y, z, p)
+    tree = O.Tree(x, y, z, p)
+    for c in (0, 1, p.ncols // 10, p.ncols // 2, p.ncols - 1, p.ncols):
+        got = tree.stage1_cols(0, c)
+        tree.stage1_cols(c, p.ncols, got)
+        assert np.array_equal(got, want), c
+    part = tree.stage1_cols(2, 5)
+    cols = np.arange(p.ncols * p.nrows) % p.ncols
+    assert np.all(part[(cols < 2) | (cols >= 5)] == O.NONE)
+
+
 def test_tree_structure():
     t = _tile(20000, seed=1, cluster_frac=0.3, n_clusters=20)
     x, y, z = t.generate()
