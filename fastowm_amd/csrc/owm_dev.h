@@ -183,13 +183,24 @@ __device__ __forceinline__ uint32_t spread16(uint32_t v)
 // key_bits / 2 bits of each quantised coordinate (the same bits, with 32-bit
 // arithmetic).  The Morton pass (histograms) and the first sort pass (which
 // stores it in the point's record) both compute it with this function.
-__device__ __forceinline__ uint32_t sort_key(const TileDev &T, double x, double y, uint32_t shift)
+__device__ __forceinline__ uint32_t sort_key_q(double qminx, double qminy, double qscale, double x,
+                                               double y, uint32_t shift)
 {
     const uint32_t drop = 32u - (64u - shift) / 2u;  // low bits of each axis left out
-    const uint32_t hx = (uint32_t)((uint64_t)quantise(x, T.qminx, T.qscale) >> drop);
-    const uint32_t hy = (uint32_t)((uint64_t)quantise(y, T.qminy, T.qscale) >> drop);
+    const uint32_t hx = (uint32_t)((uint64_t)quantise(x, qminx, qscale) >> drop);
+    const uint32_t hy = (uint32_t)((uint64_t)quantise(y, qminy, qscale) >> drop);
     return spread16(hx) | (spread16(hy) << 1);
 }
+
+__device__ __forceinline__ uint32_t sort_key(const TileDev &T, double x, double y, uint32_t shift)
+{
+    return sort_key_q(T.qminx, T.qminy, T.qscale, x, y, shift);
+}
+
+// The quantisation frame of a tile's sort keys: its origin and scale.
+struct KeyFrame {
+    double qminx, qminy, qscale;
+};
 
 // A whole point in two 16-byte loads.
 __device__ __forceinline__ PointRec owm_load_pt(const PointsDev &P, uint32_t i)

@@ -66,13 +66,14 @@ struct ZeroList {
 
 // Regions of the pinned readback buffer for n_tiles tiles.
 struct IoLayout {
-    uint64_t ptoff, bbox, counts, seed_off, bytes;
+    uint64_t ptoff, bbox, frames, counts, seed_off, bytes;
     explicit IoLayout(uint32_t n_tiles)
     {
         const uint64_t off = align_up((n_tiles + 1) * 4ull, 256);
         ptoff = 0;
         bbox = off;
-        counts = bbox + align_up(n_tiles * 32ull, 256);
+        frames = bbox + align_up(n_tiles * 32ull, 256);
+        counts = frames + align_up(n_tiles * (uint64_t)sizeof(KeyFrame), 256);
         seed_off = counts + 256;
         bytes = seed_off + off;
     }
@@ -80,7 +81,7 @@ struct IoLayout {
 
 static const char *PHASE_NAMES[PH_COUNT] = {
     "bbox", "morton", "sort_tree_pass", "leaf_aggs", "karras", "agg_levels", "stage1",
-    "sort_windows_pass", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
+    "stage2_mark", "stage2_compact", "stage3", "merge_seeds", "stage1_join",
     "stage1_summaries", "xyz_parse", "qtree_stats"};
 
 const char *phase_name(int ph) { return PHASE_NAMES[ph]; }
@@ -303,8 +304,9 @@ int bit_length(uint64_t v)
 // Per-tile tree layout: leaves, internal nodes, Morton quantisation.
 void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
                const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
-               TreeLayout &L)
+               TreeLayout &L, const std::vector<KeyFrame> *frames)
 {
+    L.counts = nullptr;
     const uint32_t T = (uint32_t)tiles.size();
     L.ntiles = T;
     // The tree sort is segmented by tile, so the keys hold Morton bits only:
@@ -324,13 +326,19 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
         D.leaf_off = L.leaf_off[t];
         D.node_off = L.node_off[t];
         D.root = D.nleaf == 0 ? OWM_NONE : (D.nleaf == 1 ? (OWM_LEAF_BIT | D.leaf_off) : D.node_off);
-        const auto &b = qbox[t];
-        double side = std::max(b[1] - b[0], b[3] - b[2]);
-        if (!(side > 0.0) || !std::isfinite(side))
-            side = 1.0;
-        D.qminx = b[0];
-        D.qminy = b[2];
-        D.qscale = 4294967296.0 / (side * (1.0 + 1e-12));
+        if (frames) {
+            D.qminx = (*frames)[t].qminx;
+            D.qminy = (*frames)[t].qminy;
+            D.qscale = (*frames)[t].qscale;
+        } else {
+            const auto &b = qbox[t];
+            double side = std::max(b[1] - b[0], b[3] - b[2]);
+            if (!(side > 0.0) || !std::isfinite(side))
+                side = 1.0;
+            D.qminx = b[0];
+            D.qminy = b[2];
+            D.qscale = 4294967296.0 / (side * (1.0 + 1e-12));
+        }
         L.pt_off[t + 1] = L.pt_off[t] + D.n;
         L.leaf_off[t + 1] = L.leaf_off[t] + D.nleaf;
         L.node_off[t + 1] = L.node_off[t] + (D.nleaf > 0 ? D.nleaf - 1 : 0);
@@ -515,10 +523,9 @@ int run_sort(owm_ctx *ctx, int phase, hipStream_t s, SortBufs &S, uint32_t n_max
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    // algorithmic bytes per key and pass: 12 for the window minima (with the
-    // run-length pass, 44 per window over three passes), 8 for the seed merge
-    // (keys only), 16 for key/value pairs (12 in a first pass with iota values)
-    const uint64_t per = phase == PH_SORT_WIN ? 12ull : (keys_only ? 8ull : 16ull);
+    // algorithmic bytes per key and pass: 8 for keys only, 16 for key/value
+    // pairs (12 in a first pass with iota values)
+    const uint64_t per = keys_only ? 8ull : 16ull;
     ctx->prof.end(s, pi, per * n_max * passes - (iota && !keys_only ? 4ull * n_max : 0ull),
                   (uint64_t)n_max * passes, (uint32_t)passes);
     return (passes & 1) ? 1 : 0;
@@ -559,20 +566,26 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     }
     ZeroList z;
     z.add(s, guard, 64);
-    z.add(s, hist, (uint64_t)L.ntiles * SORT_SEG_HIST);
+    if (!L.counts)
+        z.add(s, hist, (uint64_t)L.ntiles * SORT_SEG_HIST);
     z.add(s, lookback, (uint64_t)passes * tiles * 256);
     z.launch(s);
-    int pi = ctx->prof.begin(s, PH_MORTON);
-    launch_morton(s, B, dx, dy, hist, passes);
-    // algorithmic bytes per point: x and y read
-    ctx->prof.end(s, pi, 16ull * L.np, L.np);
+    if (!L.counts) {
+        int pi = ctx->prof.begin(s, PH_MORTON);
+        launch_morton(s, B, dx, dy, hist, passes);
+        // algorithmic bytes per point: x and y read
+        ctx->prof.end(s, pi, 16ull * L.np, L.np);
+    }
     SortSegs seg;
     seg.pt = B.pt_off;
     seg.units = B.sort_units;
     seg.nseg = L.ntiles;
     seg.digit_stride = SORT_SEG_HIST;
     seg.tile_seg = tile_seg;
-    launch_sort_scan_seg(s, hist, seg.pt, seg.units, seg.nseg, passes, counters, tile_seg);
+    // digit counts (taken here, or in the bbox pass) -> the passes' offsets
+    launch_sort_scan_seg(s, L.counts ? L.counts : hist, hist, seg.pt, seg.units, seg.nseg, passes,
+                         counters, tile_seg);
+    int pi = 0;
     uint64_t sort_bytes = 0;
     pi = ctx->prof.begin(s, PH_SORT_TREE);
     for (int p = 0; p < passes; ++p) {
@@ -609,14 +622,16 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
 
 uint64_t query_bytes(const QueryLayout &Q, uint64_t np, uint64_t n_in)
 {
-    int bits = std::max(bit_length(np), 1);
-    int passes = (bits + 7) / 8;
-    uint64_t nseed_max = std::max<uint64_t>(Q.nwin, n_in) + Q.ncell + 1;
+    const uint64_t nwin = std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1);
+    const uint64_t nseed_max = std::max<uint64_t>(Q.nwin, n_in) + Q.ncell + 1;
+    const uint64_t nwords = (np + 31) / 32 + 1;
     uint64_t b = 0;
-    b += sort_scratch_bytes(std::max<uint64_t>(std::max<uint64_t>(Q.nwin, n_in), 1), passes, 1, 0);
-    b += sort_scratch_bytes(nseed_max, passes, 1, 0);                     // seeds sort
+    b += align_up(nwin * 4ull, 256);                                    // window minima
+    b += align_up(nseed_max * 4ull, 256) * 2;                            // LLPs + adds, seeds
+    b += align_up(nwords * 4ull, 256) * 2;                               // point bitmaps
     b += align_up(std::max<uint64_t>(Q.ncell, 1) * 4ull, 256) * 2;      // cell flag / res
-    b += align_up((nseed_max / 4096 + 2) * 4ull, 256) * 2;               // compaction look-back
+    b += align_up((nseed_max / 4096 + 2) * 4ull, 256);                   // stage-3 compaction
+    b += align_up((nwords / 4096 + 2) * 4ull, 256) * 2;                  // bitmap compactions
     b += align_up(1024, 256);                                           // counters
     b += align_up(stage1_work_words(Q.nblk) * 4ull, 256);              // stage-1 join
     b += align_up((Q.ntiles + 1) * 4ull, 256);                          // seed_off
@@ -634,42 +649,43 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
                          uint32_t any_need1, Arena &scratch, QueryOut &out, int from, int to,
                          const uint32_t *h_in, uint64_t n_in, const RunMem *mem)
 {
-    const int bits = std::max(bit_length(np), 1);
-    const int passes = (bits + 7) / 8;
     const uint64_t nwin = from == 2 ? n_in : Q.nwin;
     const uint64_t nw = std::max<uint64_t>(nwin, 1);
     const uint64_t nseed_max = std::max<uint64_t>(nwin, from == 3 ? n_in : 0) + Q.ncell + 1;
-    uint8_t *p1 = scratch.take(sort_scratch_bytes(nw, passes, 1, 0));
-    uint8_t *p2 = scratch.take(sort_scratch_bytes(nseed_max, passes, 1, 0));
+    const uint32_t nwords = (uint32_t)((np + 31) / 32);
+    uint32_t *mins = (uint32_t *)scratch.take(nw * 4ull);
+    uint32_t *seeds_a = (uint32_t *)scratch.take(nseed_max * 4ull);
+    uint32_t *seeds_b = (uint32_t *)scratch.take(nseed_max * 4ull);
+    uint32_t *seen = (uint32_t *)scratch.take((nwords + 1) * 4ull);
+    uint32_t *llpmap = (uint32_t *)scratch.take((nwords + 1) * 4ull);
     uint32_t *cell_flag = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
     uint32_t *cell_res = (uint32_t *)scratch.take(std::max<uint64_t>(Q.ncell, 1) * 4ull);
-    uint32_t *lb1 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
-    uint32_t *lb2 = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
+    uint32_t *lb_cells = (uint32_t *)scratch.take((nseed_max / 4096 + 2) * 4ull);
+    uint32_t *lb_llp = (uint32_t *)scratch.take((nwords / 4096 + 2) * 4ull);
+    uint32_t *lb_seeds = (uint32_t *)scratch.take((nwords / 4096 + 2) * 4ull);
     uint32_t *cnt = (uint32_t *)scratch.take(1024);
     uint32_t *seed_off = (uint32_t *)scratch.take((Q.ntiles + 1) * 4ull);
     uint32_t *s1work = (uint32_t *)scratch.take(stage1_work_words(Q.nblk) * 4ull);
     uint8_t *lcs_mem = Q.lcs_leaves ? scratch.take(leaf_cells_bytes(Q.lcs_leaves)) : nullptr;
-    if (!p1 || !p2 || !cell_flag || !cell_res || !lb1 || !lb2 || !cnt || !seed_off || !s1work ||
+    if (!mins || !seeds_a || !seeds_b || !seen || !llpmap || !cell_flag || !cell_res ||
+        !lb_cells || !lb_llp || !lb_seeds || !cnt || !seed_off || !s1work ||
         (Q.lcs_leaves && !lcs_mem)) {
         set_err(ctx, "scratch arena exhausted (query)");
         return OWM_E_INTERNAL;
     }
-    SortBufs W = carve_sort(p1, nw, passes);
-    SortBufs SD = carve_sort(p2, nseed_max, passes);
     out.cnt = cnt;
     out.seed_off = seed_off;
-    // counters, join work, both sorts' histograms and look-back, both
-    // compactions' look-back, stage-3 cell flags
+    // cnt = [n_llp, n_add, n_seeds, error flags, stage-2 tile counter,
+    //        stage-3 tile counter, join total, merge tile counter, ...]
     ZeroList z;
     z.add(s, cnt, 16);
     z.add(s, s1work, stage1_work_words(Q.nblk));
-    z.add(s, W.hist, 4 * 256);
-    z.add(s, W.lookback, (uint64_t)passes * sort_tiles((uint32_t)nw) * 256);
-    z.add(s, lb1, nseed_max / 4096 + 2);
+    z.add(s, seen, nwords + 1);
+    z.add(s, llpmap, nwords + 1);
+    z.add(s, lb_llp, nwords / 4096 + 2);
+    z.add(s, lb_cells, nseed_max / 4096 + 2);
+    z.add(s, lb_seeds, nwords / 4096 + 2);
     z.add(s, cell_flag, std::max<uint64_t>(Q.ncell, 1));
-    z.add(s, lb2, nseed_max / 4096 + 2);
-    z.add(s, SD.hist, 4 * 256);
-    z.add(s, SD.lookback, (uint64_t)passes * sort_tiles((uint32_t)nseed_max) * 256);
     z.launch(s);
     if (from <= 1) {
         const uint64_t list_words = mem ? mem->s1_words : ctx->s1_list.bytes / 4;
@@ -687,11 +703,11 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
             ctx->prof.end(s, pi, 28ull * np + 4ull * B.nleaf, np);  // points read once
         }
         pi = ctx->prof.begin(s, PH_STAGE1);
-        launch_stage1_bins(s, B, TV, cfg, lcs_mem ? &LC : nullptr, s1work, list, s1cap, W.k0);
+        launch_stage1_bins(s, B, TV, cfg, lcs_mem ? &LC : nullptr, s1work, list, s1cap, mins);
         // the algorithmic stage-1 traffic: every point streamed once per covering
         // window row (24 B x Wsize/Displace) and one minimum written per window
         ctx->prof.end(s, pi, Q.s1_alg_bytes + 4ull * Q.nwin, np);
-        out.min_ids = W.k0;
+        out.min_ids = mins;
     }
     if (to <= 1) {
         HIP_TRY(hipGetLastError(), "stage 1 launch");
@@ -699,30 +715,32 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
     }
     if (from <= 2) {
         if (from == 2 && nwin)
-            HIP_TRY(hipMemcpyAsync(W.k0, h_in, nwin * 4ull, hipMemcpyHostToDevice, s), "min_ids H2D");
-        int which = run_sort(ctx, PH_SORT_WIN, s, W, (uint32_t)nwin, nullptr, passes, true, false);
-        const uint32_t *sorted = which ? W.k1 : W.k0;
-        out.sorted_min_ids = sorted;
-        int pi = ctx->prof.begin(s, PH_STAGE2);
-        launch_stage2_compact(s, B, sorted, (uint32_t)nwin, any_need1, SD.k0, cnt + 0, lb1, cnt + 4);
-        ctx->prof.end(s, pi, 8ull * nwin, nwin);
+            HIP_TRY(hipMemcpyAsync(mins, h_in, nwin * 4ull, hipMemcpyHostToDevice, s), "min_ids H2D");
+        int pi = ctx->prof.begin(s, PH_S2_MARK);
+        launch_stage2_mark(s, B, mins, (uint32_t)nwin, (uint32_t)np, any_need1, seen, llpmap);
+        // per window: its minimum read, one bitmap word read and written
+        ctx->prof.end(s, pi, 12ull * nwin, nwin);
+        pi = ctx->prof.begin(s, PH_STAGE2);
+        launch_bits_compact(s, llpmap, nwords, seeds_a, cnt + 0, lb_llp, cnt + 4);
+        ctx->prof.end(s, pi, 4ull * nwords, nwords);
     } else {
         if (n_in)
-            HIP_TRY(hipMemcpyAsync(SD.k0, h_in, n_in * 4ull, hipMemcpyHostToDevice, s), "llp H2D");
+            HIP_TRY(hipMemcpyAsync(seeds_a, h_in, n_in * 4ull, hipMemcpyHostToDevice, s), "llp H2D");
         uint32_t n32 = (uint32_t)n_in;
         HIP_TRY(hipMemcpyAsync(cnt + 0, &n32, 4, hipMemcpyHostToDevice, s), "n_llp H2D");
         HIP_TRY(hipStreamSynchronize(s), "n_llp H2D");
     }
-    out.llp = SD.k0;
+    out.llp = seeds_a;
     if (to <= 2) {
         HIP_TRY(hipGetLastError(), "stage 2 launch");
         return OWM_OK;
     }
     if (Q.ncell > 0) {
         int pi = ctx->prof.begin(s, PH_STAGE3);
-        launch_stage3(s, B, TV, SD.k0, cnt + 0, (uint32_t)(nseed_max - Q.ncell), cell_flag,
+        launch_stage3(s, B, TV, seeds_a, cnt + 0, (uint32_t)(nseed_max - Q.ncell), cell_flag,
                       cell_res, cnt + 3);
-        launch_compact_valid(s, cell_res, (uint32_t)Q.ncell, SD.k0, cnt + 0, cnt + 1, lb2, cnt + 5);
+        launch_compact_valid(s, cell_res, (uint32_t)Q.ncell, seeds_a, cnt + 0, cnt + 1, lb_cells,
+                             cnt + 5);
         ctx->prof.end(s, pi, 12ull * Q.ncell, Q.ncell);
     }
     out.cell_res = cell_res;
@@ -731,11 +749,19 @@ owm_status enqueue_query(owm_ctx *ctx, hipStream_t s, const BatchDev &B, const T
         HIP_TRY(hipGetLastError(), "stage 3 launch");
         return OWM_OK;
     }
-    int which = run_sort(ctx, PH_MERGE, s, SD, (uint32_t)nseed_max, cnt + 2, passes, true, false);
-    uint32_t *merged = which ? SD.k1 : SD.k0;
-    out.seeds = merged;
-    launch_seed_ranges(s, B, merged, cnt + 2, seed_off);
-    launch_seeds_local(s, B, merged, cnt + 2, (uint32_t)nseed_max);
+    // merge: the stage-3 additions (and LLPs that came from the host) join the
+    // LLP bitmap, whose compaction lists the seeds ascending and distinct
+    int pi = ctx->prof.begin(s, PH_MERGE);
+    if (from == 3)
+        launch_bits_set(s, seeds_a, nullptr, cnt + 0, (uint32_t)nseed_max, (uint32_t)np, llpmap);
+    if (Q.ncell > 0)
+        launch_bits_set(s, seeds_a, cnt + 0, cnt + 1, (uint32_t)Q.ncell, (uint32_t)np, llpmap);
+    launch_bits_compact(s, llpmap, nwords, seeds_b, cnt + 2, lb_seeds, cnt + 7);
+    ctx->prof.end(s, pi, 4ull * nwords + 8ull * Q.ncell, nwords,
+                  1u + (from == 3 ? 1u : 0u) + (Q.ncell > 0 ? 1u : 0u));
+    out.seeds = seeds_b;
+    launch_seed_ranges(s, B, seeds_b, cnt + 2, seed_off);
+    launch_seeds_local(s, B, seeds_b, cnt + 2, (uint32_t)nseed_max);
     HIP_TRY(hipGetLastError(), "query launch");
     return OWM_OK;
 }
@@ -848,14 +874,27 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
             return OWM_E_ARG;
         }
     }
+    const int sort_bits = (int)ctx->tuner.sort_bits_for(np);
+    // With the context's own arenas the bbox pass also takes the sort keys'
+    // digit counts (over frames sampled from each tile), so the build does not
+    // read x and y again; the no-allocation path keeps its borrowed region
+    // free between the passes and takes them in the build.
+    const bool fuse = need_bbox && !mem;
+    std::vector<KeyFrame> frames;
+    uint32_t *counts = nullptr;
     if (need_bbox) {
         // tile offsets first (the boxes are placeholders until the pass below)
         for (uint32_t t = 0; t < n_tiles; ++t)
             if (!(tiles[t].params.flags & OWM_F_HAVE_BBOX))
                 P.box[t] = {0.0, 1.0, 0.0, 1.0};
-        plan_tree(32, P.td, P.n, P.leaf, P.box, P.L);
+        plan_tree(sort_bits, P.td, P.n, P.leaf, P.box, P.L);
+        const int passes = (P.L.tile_bits + P.L.key_bits + 7) / 8;
         const uint64_t bk_bytes = bbox_key_words(n_tiles) * 8;
+        const uint64_t fr_bytes = (uint64_t)n_tiles * sizeof(KeyFrame);
+        const uint64_t cnt_words = (uint64_t)n_tiles * SORT_SEG_HIST;
         uint64_t bytes = align_up(bk_bytes, 256) + align_up((n_tiles + 1) * 4ull, 256);
+        if (fuse)
+            bytes += align_up(fr_bytes, 256) + align_up(cnt_words * 4, 256);
         // the bbox pass's scratch: the context's small arena, or the start of
         // a borrowed region (released again once the boxes are on the host)
         Arena &SA = mem ? *mem->arena : ctx->small;
@@ -865,7 +904,9 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
             return st;
         unsigned long long *bk = (unsigned long long *)SA.take(bk_bytes);
         uint32_t *ptoff = (uint32_t *)SA.take((n_tiles + 1) * 4ull);
-        if (!bk || !ptoff) {
+        KeyFrame *d_frames = fuse ? (KeyFrame *)SA.take(fr_bytes) : nullptr;
+        counts = fuse ? (uint32_t *)SA.take(cnt_words * 4) : nullptr;
+        if (!bk || !ptoff || (fuse && (!d_frames || !counts))) {
             set_err(ctx, "scratch exhausted (bbox pass)");
             return OWM_E_BUDGET;
         }
@@ -883,13 +924,24 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
         B0.np = (uint32_t)np;
         B0.pt_off = ptoff;
         launch_bbox_init(s, bk, n_tiles);
+        if (fuse)
+            launch_fill_u32(s, counts, cnt_words, 0u);
         int pi = ctx->prof.begin(s, PH_BBOX);
-        if (np)
+        if (np && fuse)
+            launch_bbox_keys(s, B0, dx, dy, d_frames, bk, counts, passes, (uint32_t)P.L.key_bits);
+        else if (np)
             launch_bbox(s, B0, dx, dy, bk);
         ctx->prof.end(s, pi, 16ull * np, np);
         const unsigned long long *hk = (const unsigned long long *)(h + io.bbox);
         HIP_TRY(hipMemcpyAsync(h + io.bbox, bk, n_tiles * 32ull, hipMemcpyDeviceToHost, s), "bbox D2H");
+        if (fuse)
+            HIP_TRY(hipMemcpyAsync(h + io.frames, d_frames, fr_bytes, hipMemcpyDeviceToHost, s),
+                    "frames D2H");
         HIP_TRY(hipStreamSynchronize(s), "bbox sync");
+        if (fuse) {
+            frames.resize(n_tiles);
+            memcpy(frames.data(), h + io.frames, fr_bytes);
+        }
         ctx->prof.collect();
         auto dec = [](unsigned long long k) {
             uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
@@ -909,7 +961,8 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
         if (mem)
             SA.used = 0;
     }
-    plan_tree((int)ctx->tuner.sort_bits_for(np), P.td, P.n, P.leaf, P.box, P.L);
+    plan_tree(sort_bits, P.td, P.n, P.leaf, P.box, P.L, fuse ? &frames : nullptr);
+    P.L.counts = counts;
     return OWM_OK;
 }
 
@@ -1119,7 +1172,7 @@ static void graph_key(owm_ctx *ctx, const double *dx, const double *dy, const do
         key.insert(key.end(), b, b + n);
     };
     key.clear();
-    const void *ptrs[6] = {dx, dy, dz, ctx->arena.buf.ptr, ctx->s1_list.ptr, nullptr};
+    const void *ptrs[6] = {dx, dy, dz, ctx->arena.buf.ptr, ctx->s1_list.ptr, P.L.counts};
     put(ptrs, sizeof(ptrs));
     const uint64_t nums[6] = {ctx->arena.buf.bytes, ctx->s1_list.bytes, P.Q.s1_cap,
                               (uint64_t)P.L.key_bits, (uint64_t)P.L.tile_bits,

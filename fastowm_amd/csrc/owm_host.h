@@ -54,6 +54,9 @@ struct TreeLayout {
     uint64_t np = 0, nleaf = 0, nnode = 0, ngrp = 0;
     std::vector<uint32_t> pt_off, leaf_off, node_off, grp_off;
     std::vector<uint32_t> sort_units;  // first sort tile of every tile's segment
+    // key digit counts already taken in the bbox pass (device, the context's
+    // small arena), or null: the build takes them itself
+    const uint32_t *counts = nullptr;
 };
 
 struct QueryLayout {
@@ -67,7 +70,6 @@ struct QueryLayout {
 
 struct QueryOut {
     const uint32_t *min_ids = nullptr;         // stage-1 minima (unsorted, per window)
-    const uint32_t *sorted_min_ids = nullptr;  // after the stage-2 sort
     const uint32_t *llp = nullptr;             // LLPs (then stage-3 adds) before the merge sort
     const uint32_t *cell_res = nullptr;        // stage-3 result per cell
     const uint32_t *seeds = nullptr;           // merged, sorted
@@ -151,9 +153,11 @@ owm_status derive_params(const owm_params &P, uint64_t n, const double *bbox, De
                          std::string &why);
 void fill_query(TileDev &T, const owm_params &P, const Derived &D);
 int bit_length(uint64_t v);
+// Quantisation of tile t's keys: frames[t] when given, else the square over
+// qbox[t].
 void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uint64_t> &n,
                const std::vector<uint32_t> &leaf, const std::vector<std::array<double, 4>> &qbox,
-               TreeLayout &L);
+               TreeLayout &L, const std::vector<KeyFrame> *frames = nullptr);
 void plan_query(std::vector<TileDev> &tiles, const Stage1Cfg &cfg, QueryLayout &Q);
 void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T);
 uint64_t sort_scratch_bytes(uint64_t n, int passes, uint32_t nseg, uint64_t lb_tiles);
@@ -232,7 +236,7 @@ struct ServeState;
 
 enum Phase {
     PH_BBOX, PH_MORTON, PH_SORT_TREE, PH_LEAVES, PH_KARRAS, PH_LEVELS, PH_STAGE1,
-    PH_SORT_WIN, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_XYZ, PH_QSTATS, PH_COUNT
+    PH_S2_MARK, PH_STAGE2, PH_STAGE3, PH_MERGE, PH_S1_JOIN, PH_LCS, PH_XYZ, PH_QSTATS, PH_COUNT
 };
 
 const char *phase_name(int ph);

@@ -21,10 +21,12 @@ uint32_t sort_tiles(uint32_t n);
 // points per tile of an n-point record sort over nseg segments
 uint32_t record_tile_keys(uint32_t n, uint32_t nseg);
 constexpr uint32_t SORT_SEG_HIST = 4 * 256;  // histogram words per segment (all passes)
-// hist[t * SORT_SEG_HIST + p * 256 + d] -> segment base + exclusive offsets;
-// tile_seg[u] = t for the sort tiles u of segment t (units[t] <= u < units[t+1])
-void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, const uint32_t *units,
-                          uint32_t nseg, int passes, uint32_t *counters, uint32_t *tile_seg);
+// counts[t * SORT_SEG_HIST + p * 256 + d] -> hist[same] = segment base +
+// exclusive offsets (counts may equal hist); tile_seg[u] = t for the sort
+// tiles u of segment t (units[t] <= u < units[t+1])
+void launch_sort_scan_seg(hipStream_t s, const uint32_t *counts, uint32_t *hist, const uint32_t *pt,
+                          const uint32_t *units, uint32_t nseg, int passes, uint32_t *counters,
+                          uint32_t *tile_seg);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 // Zeroes up to FILL_MAX ranges (u32 words) in one launch.
 constexpr uint32_t FILL_MAX = 16;
@@ -83,6 +85,13 @@ uint64_t bbox_key_words(uint32_t ntiles);
 void launch_bbox_init(hipStream_t s, unsigned long long *bbox_keys, uint32_t ntiles);
 // Per tile SORT_SEG_HIST digit counters of every pass of the points' sort keys
 // (the keys themselves are not stored: the first sort pass computes them again).
+// exact bbox words (as launch_bbox) and the tiles' key histograms (as
+// launch_morton) in one pass over x and y, the keys quantised over frames
+// sampled from each tile (written to frames, ntiles entries); bk initialised
+// by launch_bbox_init, hist zeroed.
+void launch_bbox_keys(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                      KeyFrame *frames, unsigned long long *bk, uint32_t *hist, int passes,
+                      uint32_t key_bits);
 void launch_morton(hipStream_t s, const BatchDev &B, const double *x, const double *y,
                    uint32_t *hist, int passes);
 // Leaf aggregates over the points in Morton order (T.pts); each leaf's first
@@ -205,10 +214,16 @@ void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, cons
                         uint32_t cap, uint32_t *min_ids);
 uint64_t leaf_cells_bytes(uint64_t nleaf);
 LeafCells carve_leaf_cells(uint8_t *base, uint64_t nleaf);
-// sorted window minima -> LLP list (ascending) + count
-void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
-                           uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,
-                           uint32_t *tile_counter);
+// window minima -> point bitmaps: `seen` (one window) and `llp` (two, or one on
+// tiles with stage2_need == 1); both np/32 words, zeroed beforehand
+void launch_stage2_mark(hipStream_t s, const BatchDev &B, const uint32_t *min_ids, uint32_t nwin,
+                        uint32_t np, uint32_t any_need1, uint32_t *seen, uint32_t *llp);
+// ids[*d_off .. *d_off + *d_n) (n_max bounds *d_n) marked in `bits`
+void launch_bits_set(hipStream_t s, const uint32_t *ids, const uint32_t *d_off, const uint32_t *d_n,
+                     uint32_t n_max, uint32_t np, uint32_t *bits);
+// set bits of words[0..nwords) -> ascending indices + count
+void launch_bits_compact(hipStream_t s, const uint32_t *words, uint32_t nwords, uint32_t *out,
+                         uint32_t *d_count, uint32_t *lookback, uint32_t *tile_counter);
 void launch_stage3(hipStream_t s, const BatchDev &B, const TreeDev &T, const uint32_t *llp,
                    const uint32_t *n_llp, uint32_t n_llp_max, uint32_t *cell_flag,
                    uint32_t *cell_res, uint32_t *err);

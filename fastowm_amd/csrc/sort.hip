@@ -93,10 +93,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const uint32_t *__restrict_
     }
 }
 
-// Per segment t (one workgroup each): hist[t][p][d] -> pt[t] + exclusive
-// offsets within the segment, and tile_seg[u] = t for the segment's sort
-// tiles; workgroup 0 also zeroes the tile counters.
-__global__ __launch_bounds__(BLOCK) void k_sort_scan_seg(uint32_t *hist, const uint32_t *pt,
+// Per segment t (one workgroup each): counts[t][p][d] -> hist[t][p][d] =
+// pt[t] + exclusive offsets within the segment (counts may be hist), and
+// tile_seg[u] = t for the segment's sort tiles; workgroup 0 also zeroes the
+// tile counters.
+__global__ __launch_bounds__(BLOCK) void k_sort_scan_seg(const uint32_t *counts, uint32_t *hist,
+                                                          const uint32_t *pt,
                                                           const uint32_t *units, uint32_t nseg,
                                                           int passes, uint32_t *counters,
                                                           uint32_t *tile_seg)
@@ -105,10 +107,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scan_seg(uint32_t *hist, const u
     for (uint32_t t = blockIdx.x; t < nseg; t += gridDim.x) {
         for (uint32_t u = units[t] + threadIdx.x; u < units[t + 1]; u += BLOCK)
             tile_seg[u] = t;
+        const uint32_t *c = counts + (size_t)t * SEG_HIST;
         uint32_t *h = hist + (size_t)t * SEG_HIST;
         const uint32_t b = pt[t];
         for (int p = 0; p < passes; ++p) {
-            uint32_t v = h[p * RADIX + threadIdx.x];
+            uint32_t v = c[p * RADIX + threadIdx.x];
             uint32_t e = block_excl_scan(v, tmp);
             h[p * RADIX + threadIdx.x] = b + e;
             __syncthreads();
@@ -503,12 +506,13 @@ void launch_sort_scan(hipStream_t s, uint32_t *hist, int passes, uint32_t *count
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(BLOCK), 0, s, hist, passes, counters);
 }
 
-void launch_sort_scan_seg(hipStream_t s, uint32_t *hist, const uint32_t *pt, const uint32_t *units,
-                          uint32_t nseg, int passes, uint32_t *counters, uint32_t *tile_seg)
+void launch_sort_scan_seg(hipStream_t s, const uint32_t *counts, uint32_t *hist, const uint32_t *pt,
+                          const uint32_t *units, uint32_t nseg, int passes, uint32_t *counters,
+                          uint32_t *tile_seg)
 {
     const uint32_t g = nseg < 1 ? 1 : (nseg > 4096 ? 4096 : nseg);
-    hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, hist, pt, units, nseg, passes,
-                       counters, tile_seg);
+    hipLaunchKernelGGL(k_sort_scan_seg, dim3(g), dim3(BLOCK), 0, s, counts, hist, pt, units, nseg,
+                       passes, counters, tile_seg);
 }
 
 

@@ -3,7 +3,10 @@
 // OWM stages 2 and 3 on gfx950 (stage 1 is in stage1.hip).
 //
 // Stage 2 (sort + stage2, owm_optim3.cpp:158-188, optim3_func.cpp:610-625):
-// radix-sorted window minima, run-length test, ordered compaction.
+// the window minima mark a point bitmap (first and second window), and an
+// ordered compaction of the second-window bits gives the LLPs ascending, as
+// the reference's sorted run-length test does.  The seed merge is the same
+// bitmap with the stage-3 additions marked, compacted again.
 //
 // Stage 3 (stage3, optim3_func.cpp:627-662): LLPs mark their Bsize cell; one
 // wave per unmarked cell walks the LBVH for the lowest point strictly inside.
@@ -54,14 +57,40 @@ __device__ __forceinline__ uint32_t block_excl_scan_cp(uint32_t v, uint32_t *s_t
     return add + inc - v;
 }
 
-// mode 0: stage-2 run-length test over sorted window minima
-// mode 1: keep entries != NONE
-template <int MODE>
-__global__ __launch_bounds__(CP_BLOCK) void k_compact(BatchDev B, const uint32_t *__restrict__ in,
-                                                       uint32_t n, uint32_t any_need1,
-                                                       uint32_t *__restrict__ out,
-                                                       const uint32_t *d_out_base, uint32_t *d_count,
-                                                       uint32_t *lookback, uint32_t *tile_counter)
+// Decoupled look-back over compaction tiles taken in order (a tile's
+// predecessors are all held by running workgroups): publish this tile's
+// count, add the predecessors' counts back to the first inclusive prefix,
+// publish the inclusive prefix, return the exclusive one.
+__device__ __forceinline__ uint32_t tile_prefix(uint32_t *lookback, uint32_t tile, uint32_t total)
+{
+    uint32_t *flag = lookback + tile;
+    uint32_t prefix = 0;
+    if (tile == 0) {
+        __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int32_t j = (int32_t)tile - 1;
+    for (;;) {
+        uint32_t f = __hip_atomic_load(lookback + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t st = f & ~LB_CNT;
+        if (st == 0)
+            continue;
+        prefix += f & LB_CNT;
+        if (st == LB_INC)
+            break;
+        --j;
+    }
+    __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return prefix;
+}
+
+// Entries != OWM_NONE of in[0..n), in order, written from out[*d_out_base].
+__global__ __launch_bounds__(CP_BLOCK) void k_compact_valid(const uint32_t *__restrict__ in,
+                                                            uint32_t n, uint32_t *__restrict__ out,
+                                                            const uint32_t *d_out_base,
+                                                            uint32_t *d_count, uint32_t *lookback,
+                                                            uint32_t *tile_counter)
 {
     __shared__ uint32_t s_tile, s_tmp[CP_BLOCK / 64], s_prefix;
     const uint32_t ntiles = (n + CP_TILE - 1) / CP_TILE;
@@ -83,22 +112,9 @@ __global__ __launch_bounds__(CP_BLOCK) void k_compact(BatchDev B, const uint32_t
         uint32_t flags = 0, cnt = 0;
 #pragma unroll
         for (int k = 0; k < CP_ITEMS; ++k) {
-            uint32_t i = i0 + k;
-            bool f = false;
-            uint32_t v = OWM_NONE;
-            if (i < n) {
-                v = in[i];
-                if (MODE == 0) {
-                    if (v != OWM_NONE && (i == 0 || in[i - 1] != v)) {
-                        uint32_t need = 2;
-                        if (any_need1)
-                            need = B.tiles[owm_find_tile(B.pt_off, B.ntiles, v)].stage2_need;
-                        f = need == 1 || (i + 1 < n && in[i + 1] == v);
-                    }
-                } else {
-                    f = v != OWM_NONE;
-                }
-            }
+            const uint32_t i = i0 + k;
+            const uint32_t v = i < n ? in[i] : OWM_NONE;
+            const bool f = v != OWM_NONE;
             vals[k] = v;
             flags |= (f ? 1u : 0u) << k;
             cnt += f;
@@ -106,27 +122,7 @@ __global__ __launch_bounds__(CP_BLOCK) void k_compact(BatchDev B, const uint32_t
         uint32_t total;
         uint32_t excl = block_excl_scan_cp(cnt, s_tmp, total);
         if (threadIdx.x == 0) {
-            uint32_t *flag = lookback + tile;
-            uint32_t prefix = 0;
-            if (tile == 0) {
-                __hip_atomic_store(flag, LB_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(flag, LB_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                int32_t j = (int32_t)tile - 1;
-                for (;;) {
-                    uint32_t f = __hip_atomic_load(lookback + j, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                    uint32_t st = f & ~LB_CNT;
-                    if (st == 0)
-                        continue;
-                    prefix += f & LB_CNT;
-                    if (st == LB_INC)
-                        break;
-                    --j;
-                }
-                __hip_atomic_store(flag, LB_INC | (prefix + total), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
+            const uint32_t prefix = tile_prefix(lookback, tile, total);
             s_prefix = prefix;
             if (tile == ntiles - 1)
                 *d_count = prefix + total;
@@ -137,6 +133,137 @@ __global__ __launch_bounds__(CP_BLOCK) void k_compact(BatchDev B, const uint32_t
         for (int k = 0; k < CP_ITEMS; ++k)
             if ((flags >> k) & 1u)
                 out[pos++] = vals[k];
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------- stage 2 by bitmaps --
+//
+// A point is an LLP when it is the minimum of two valid windows (of one, on
+// tiles whose Overlap is 0: the reference keeps every valid minimum there).
+// Windows are listed row by row, so neighbouring windows often share their
+// minimum: only the first window of such a run acts, and a run of two or more
+// marks the point in `llp` at once.  A window alone with its minimum marks it
+// in `seen` and, when `seen` already held it (another window elsewhere),
+// in `llp`.  Compacting `llp` in bit order then lists the LLPs ascending,
+// which is what sorting the minima and keeping runs of two or more gives.
+constexpr int S2_ITEMS = 4;
+
+__global__ __launch_bounds__(256) void k_s2_mark(BatchDev B, const uint32_t *__restrict__ min_ids,
+                                                 uint32_t nwin, uint32_t np, uint32_t any_need1,
+                                                 uint32_t *__restrict__ seen,
+                                                 uint32_t *__restrict__ llp)
+{
+    const uint32_t step = gridDim.x * 256u * S2_ITEMS;
+    for (uint32_t base = blockIdx.x * 256u * S2_ITEMS + threadIdx.x; base < nwin; base += step) {
+        uint32_t v[S2_ITEMS], act[S2_ITEMS], old[S2_ITEMS];
+#pragma unroll
+        for (int k = 0; k < S2_ITEMS; ++k) {
+            const uint32_t i = base + k * 256u;
+            v[k] = i < nwin ? min_ids[i] : OWM_NONE;
+            const uint32_t prev = i > 0 && i < nwin ? min_ids[i - 1] : OWM_NONE;
+            const uint32_t next = i + 1 < nwin ? min_ids[i + 1] : OWM_NONE;
+            act[k] = 0;  // not a point, or a later window of a run
+            if (v[k] < np && prev != v[k]) {
+                uint32_t need = 2;
+                if (any_need1)
+                    need = B.tiles[owm_find_tile(B.pt_off, B.ntiles, v[k])].stage2_need;
+                act[k] = need == 1 || next == v[k] ? 1u : 2u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < S2_ITEMS; ++k)
+            old[k] = act[k] == 2 ? atomicOr(seen + (v[k] >> 5), 1u << (v[k] & 31u)) : 0u;
+#pragma unroll
+        for (int k = 0; k < S2_ITEMS; ++k) {
+            const uint32_t bit = 1u << (v[k] & 31u);
+            if (act[k] == 1 || (act[k] == 2 && (old[k] & bit)))
+                atomicOr(llp + (v[k] >> 5), bit);
+        }
+    }
+}
+
+// Points listed in ids[off .. off + n) (off, n read on the device; off may be
+// null = 0) marked in a bitmap.
+__global__ __launch_bounds__(256) void k_bits_set(const uint32_t *__restrict__ ids,
+                                                  const uint32_t *d_off, const uint32_t *d_n,
+                                                  uint32_t np, uint32_t *__restrict__ bits)
+{
+    const uint32_t off = d_off ? *d_off : 0u, n = *d_n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t v = ids[off + i];
+        if (v < np)
+            atomicOr(bits + (v >> 5), 1u << (v & 31u));
+    }
+}
+
+// The set bits of words[0 .. nwords), in order, as point indices: ordered
+// compaction with one look-back per tile of CP_BLOCK x 4 x BC_VEC words (BC_VEC
+// uint4 loads per lane).  The look-back chain is serial, so a large bitmap
+// takes large tiles (few links) and a small one small tiles (enough
+// workgroups).
+template <int BC_VEC>
+__global__ __launch_bounds__(CP_BLOCK) void k_bits_compact(const uint32_t *__restrict__ words,
+                                                           uint32_t nwords,
+                                                           uint32_t *__restrict__ out,
+                                                           uint32_t *d_count, uint32_t *lookback,
+                                                           uint32_t *tile_counter)
+{
+    constexpr uint32_t BC_TILE = CP_BLOCK * BC_VEC * 4;
+    __shared__ uint32_t s_tile, s_tmp[CP_BLOCK / 64], s_prefix;
+    const uint32_t ntiles = (nwords + BC_TILE - 1) / BC_TILE;
+    if (nwords == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            *d_count = 0;
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_tile = atomicAdd(tile_counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= ntiles)
+            break;
+        const uint32_t w0 = tile * BC_TILE + threadIdx.x * (BC_VEC * 4);
+        uint4 wv[BC_VEC];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < BC_VEC; ++k) {
+            const uint32_t w = w0 + 4 * k;
+            if (w + 4 <= nwords) {
+                wv[k] = *reinterpret_cast<const uint4 *>(words + w);
+            } else {
+                wv[k].x = w < nwords ? words[w] : 0u;
+                wv[k].y = w + 1 < nwords ? words[w + 1] : 0u;
+                wv[k].z = w + 2 < nwords ? words[w + 2] : 0u;
+                wv[k].w = 0u;
+            }
+            cnt += __popc(wv[k].x) + __popc(wv[k].y) + __popc(wv[k].z) + __popc(wv[k].w);
+        }
+        uint32_t total;
+        const uint32_t excl = block_excl_scan_cp(cnt, s_tmp, total);
+        if (threadIdx.x == 0) {
+            const uint32_t prefix = tile_prefix(lookback, tile, total);
+            s_prefix = prefix;
+            if (tile == ntiles - 1)
+                *d_count = prefix + total;
+        }
+        __syncthreads();
+        uint32_t pos = s_prefix + excl;
+        if (cnt) {
+#pragma unroll
+            for (int k = 0; k < BC_VEC; ++k) {
+                const uint32_t m4[4] = {wv[k].x, wv[k].y, wv[k].z, wv[k].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t m = m4[q];
+                    while (m) {
+                        out[pos++] = ((w0 + 4 * k + q) << 5) | (uint32_t)__builtin_ctz(m);
+                        m &= m - 1u;
+                    }
+                }
+            }
+        }
         __syncthreads();
     }
 }
@@ -399,13 +526,41 @@ static uint32_t cp_grid(uint32_t n)
     return t < 1 ? 1 : (t > 4096 ? 4096 : t);
 }
 
-// lookback ((nwin + CP_TILE - 1) / CP_TILE + 1 words) and tile_counter start at zero.
-void launch_stage2_compact(hipStream_t s, const BatchDev &B, const uint32_t *sorted, uint32_t nwin,
-                           uint32_t any_need1, uint32_t *llp, uint32_t *n_llp, uint32_t *lookback,
-                           uint32_t *tile_counter)
+// seen and llp (np/32 words each) start at zero.
+void launch_stage2_mark(hipStream_t s, const BatchDev &B, const uint32_t *min_ids, uint32_t nwin,
+                        uint32_t np, uint32_t any_need1, uint32_t *seen, uint32_t *llp)
 {
-    hipLaunchKernelGGL(k_compact<0>, dim3(cp_grid(nwin)), dim3(CP_BLOCK), 0, s, B, sorted, nwin,
-                       any_need1, llp, (const uint32_t *)nullptr, n_llp, lookback, tile_counter);
+    uint32_t g = (nwin + 256 * S2_ITEMS - 1) / (256 * S2_ITEMS);
+    g = g < 1 ? 1 : (g > 8192 ? 8192 : g);
+    hipLaunchKernelGGL(k_s2_mark, dim3(g), dim3(256), 0, s, B, min_ids, nwin, np, any_need1, seen,
+                       llp);
+}
+
+void launch_bits_set(hipStream_t s, const uint32_t *ids, const uint32_t *d_off, const uint32_t *d_n,
+                     uint32_t n_max, uint32_t np, uint32_t *bits)
+{
+    uint32_t g = (n_max + 255) / 256;
+    g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
+    hipLaunchKernelGGL(k_bits_set, dim3(g), dim3(256), 0, s, ids, d_off, d_n, np, bits);
+}
+
+// lookback (nwords / 4096 + 2 words) and tile_counter start at zero; words
+// 16-byte aligned.  Tiles of 4096 words up to 2M words, then 8192, then 16384.
+void launch_bits_compact(hipStream_t s, const uint32_t *words, uint32_t nwords, uint32_t *out,
+                         uint32_t *d_count, uint32_t *lookback, uint32_t *tile_counter)
+{
+    const uint32_t tile = nwords <= (2u << 20) ? 4096u : nwords <= (8u << 20) ? 8192u : 16384u;
+    uint32_t g = (nwords + tile - 1) / tile;
+    g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
+    if (tile == 4096u)
+        hipLaunchKernelGGL(k_bits_compact<4>, dim3(g), dim3(CP_BLOCK), 0, s, words, nwords, out,
+                           d_count, lookback, tile_counter);
+    else if (tile == 8192u)
+        hipLaunchKernelGGL(k_bits_compact<8>, dim3(g), dim3(CP_BLOCK), 0, s, words, nwords, out,
+                           d_count, lookback, tile_counter);
+    else
+        hipLaunchKernelGGL(k_bits_compact<16>, dim3(g), dim3(CP_BLOCK), 0, s, words, nwords, out,
+                           d_count, lookback, tile_counter);
 }
 
 void launch_compact_valid(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out,
@@ -413,8 +568,7 @@ void launch_compact_valid(hipStream_t s, const uint32_t *in, uint32_t n, uint32_
                           uint32_t *tile_counter)
 {
     // lookback ((n + CP_TILE - 1) / CP_TILE + 1 words) and tile_counter start at zero
-    BatchDev B{};
-    hipLaunchKernelGGL(k_compact<1>, dim3(cp_grid(n)), dim3(CP_BLOCK), 0, s, B, in, n, 0u, out,
+    hipLaunchKernelGGL(k_compact_valid, dim3(cp_grid(n)), dim3(CP_BLOCK), 0, s, in, n, out,
                        d_out_base, d_count, lookback, tile_counter);
 }
 

@@ -196,6 +196,167 @@ __global__ void k_bbox_fold(unsigned long long *bk, uint32_t ntiles)
     }
 }
 
+// ------------------------------------------------- bbox + key histograms --
+//
+// When the tiles' bboxes are not given, one pass over x and y computes both
+// the exact bbox (for the window grid) and the digit histograms of the sort
+// keys.  The keys then cannot be quantised over the exact bbox, which is known
+// only after the pass: they use a frame taken from a sample of each tile
+// (FRAME_RUNS runs of FRAME_RUN consecutive points spread over the tile;
+// every point of a tile of up to FRAME_RUNS x FRAME_RUN points).  A point
+// outside the sampled box clamps to the frame's edge cell (quantise), which
+// costs a little leaf tightness there and nothing else: the seeds do not
+// depend on the sort order.
+constexpr uint32_t FRAME_RUNS = 256, FRAME_RUN = 16;
+
+__global__ __launch_bounds__(FRAME_RUNS) void k_frame_sample(const uint32_t *__restrict__ pt_off,
+                                                             uint32_t ntiles,
+                                                             const double *__restrict__ X,
+                                                             const double *__restrict__ Y,
+                                                             KeyFrame *__restrict__ frames)
+{
+    __shared__ double red[4][FRAME_RUNS / 64];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t off = pt_off[t], n = pt_off[t + 1] - off;
+        const uint32_t i0 = off + (uint32_t)((uint64_t)threadIdx.x * n / FRAME_RUNS);
+        const uint32_t i1 = min(i0 + FRAME_RUN, off + n);
+        double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+        for (uint32_t i = i0; i < i1; ++i) {
+            const double x = X[i], y = Y[i];
+            mnx = fmin(mnx, x);
+            mxx = fmax(mxx, x);
+            mny = fmin(mny, y);
+            mxy = fmax(mxy, y);
+        }
+        const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        mnx = wave_min_d(mnx);
+        mny = wave_min_d(mny);
+        mxx = wave_max_d(mxx);
+        mxy = wave_max_d(mxy);
+        if (lane == 0) {
+            red[0][w] = mnx;
+            red[1][w] = mny;
+            red[2][w] = mxx;
+            red[3][w] = mxy;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t k = 1; k < FRAME_RUNS / 64; ++k) {
+                mnx = fmin(mnx, red[0][k]);
+                mny = fmin(mny, red[1][k]);
+                mxx = fmax(mxx, red[2][k]);
+                mxy = fmax(mxy, red[3][k]);
+            }
+            // the square frame of the sampled box (an empty or NaN sample: any
+            // frame sorts correctly)
+            if (!(mnx <= mxx))
+                mnx = mxx = 0.0;
+            if (!(mny <= mxy))
+                mny = mxy = 0.0;
+            double side = fmax(mxx - mnx, mxy - mny);
+            if (!(side > 0.0) || !isfinite(side))
+                side = 1.0;
+            frames[t] = KeyFrame{mnx, mny, 4294967296.0 / (side * (1.0 + 1e-12))};
+        }
+        __syncthreads();
+    }
+}
+
+// k_bbox and k_morton in one pass: exact per-tile bbox (replicated words,
+// folded by k_bbox_fold) and per-tile digit histograms of the keys under the
+// sampled frames.
+__global__ __launch_bounds__(TB) void k_bbox_keys(BatchDev B, const double *__restrict__ X,
+                                                   const double *__restrict__ Y,
+                                                   const KeyFrame *__restrict__ frames,
+                                                   unsigned long long *bk_all, uint32_t *hist,
+                                                   int passes, uint32_t key_bits)
+{
+    __shared__ uint32_t h[4][256];
+    __shared__ double red[4][TB / 64];
+    unsigned long long *bk = bk_all + (size_t)(blockIdx.x % BBOX_REPL) * 4 * B.ntiles;
+    for (int i = threadIdx.x; i < 4 * 256; i += TB)
+        (&h[0][0])[i] = 0;
+    uint32_t cur = OWM_NONE;
+    double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+    auto flush = [&]() {
+        bbox_block_flush(bk, cur, mnx, mny, mxx, mxy, red);  // synchronises the block
+        uint32_t *g = hist + (size_t)cur * SORT_SEG_HIST;
+        for (int i = threadIdx.x; i < passes * 256; i += TB) {
+            const uint32_t c = (&h[0][0])[i];
+            if (c) {
+                atomicAdd(&g[i], c);
+                (&h[0][0])[i] = 0;
+            }
+        }
+        __syncthreads();
+        mnx = mny = INFINITY;
+        mxx = mxy = -INFINITY;
+        cur = OWM_NONE;
+    };
+    __syncthreads();
+    const uint32_t nch = (B.np + CHUNK - 1) / CHUNK;
+    const uint32_t shift = 64 - key_bits;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t c0 = c * CHUNK, c1 = min(c0 + CHUNK, B.np);
+        uint32_t t = owm_find_tile(B.pt_off, B.ntiles, c0);
+        uint32_t next = B.pt_off[t + 1];
+        const bool single = c1 <= next;
+        if (cur != OWM_NONE && (!single || t != cur))
+            flush();
+        if (single) {
+            cur = t;
+            const KeyFrame F = frames[t];
+            for (uint32_t base = c0 + threadIdx.x; base < c1; base += 8 * TB) {
+                double xv[8], yv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t i = base + j * TB;
+                    xv[j] = i < c1 ? X[i] : NAN;
+                    yv[j] = i < c1 ? Y[i] : NAN;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    mnx = fmin(mnx, xv[j]);
+                    mxx = fmax(mxx, xv[j]);
+                    mny = fmin(mny, yv[j]);
+                    mxy = fmax(mxy, yv[j]);
+                    if (base + j * TB < c1) {
+                        const uint32_t k = sort_key_q(F.qminx, F.qminy, F.qscale, xv[j], yv[j], shift);
+                        for (int p = 0; p < passes; ++p)
+                            atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+                    }
+                }
+            }
+            continue;
+        }
+        // a chunk across tiles (rare): per-thread bbox flushes, counts straight
+        // to each tile's histogram
+        double ax = INFINITY, ay = INFINITY, bx = -INFINITY, by = -INFINITY;
+        uint32_t tt = t, nx = next;
+        for (uint32_t i = c0 + threadIdx.x; i < c1; i += TB) {
+            while (i >= nx) {
+                bbox_flush(bk, tt, ax, ay, bx, by);
+                ax = ay = INFINITY;
+                bx = by = -INFINITY;
+                nx = B.pt_off[++tt + 1];
+            }
+            const double x = X[i], y = Y[i];
+            ax = fmin(ax, x);
+            bx = fmax(bx, x);
+            ay = fmin(ay, y);
+            by = fmax(by, y);
+            const KeyFrame F = frames[tt];
+            const uint32_t k = sort_key_q(F.qminx, F.qminy, F.qscale, x, y, shift);
+            uint32_t *g = hist + (size_t)tt * SORT_SEG_HIST;
+            for (int p = 0; p < passes; ++p)
+                atomicAdd(&g[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+        }
+        bbox_flush(bk, tt, ax, ay, bx, by);
+    }
+    if (cur != OWM_NONE)
+        flush();
+}
+
 // Each tile's digit histograms of every radix pass of its points' sort keys
 // (the top key_bits of each point's Morton code within its tile); the first
 // sort pass computes the keys again rather than reading them.
@@ -832,6 +993,19 @@ void launch_bbox(hipStream_t s, const BatchDev &B, const double *x, const double
 {
     // about four chunks per block: fewer flushes, still every CU busy
     hipLaunchKernelGGL(k_bbox, dim3(grid_for(B.np, 4 * CHUNK, 2048)), dim3(TB), 0, s, B, x, y, bk);
+    hipLaunchKernelGGL(k_bbox_fold, dim3(grid_for(4ull * B.ntiles, TB, 64)), dim3(TB), 0, s, bk,
+                       B.ntiles);
+}
+
+void launch_bbox_keys(hipStream_t s, const BatchDev &B, const double *x, const double *y,
+                      KeyFrame *frames, unsigned long long *bk, uint32_t *hist, int passes,
+                      uint32_t key_bits)
+{
+    const uint32_t gf = B.ntiles < 1 ? 1 : (B.ntiles > 4096 ? 4096 : B.ntiles);
+    hipLaunchKernelGGL(k_frame_sample, dim3(gf), dim3(FRAME_RUNS), 0, s, B.pt_off, B.ntiles, x, y,
+                       frames);
+    hipLaunchKernelGGL(k_bbox_keys, dim3(grid_for(B.np, CHUNK, 2048)), dim3(TB), 0, s, B, x, y,
+                       (const KeyFrame *)frames, bk, hist, passes, key_bits);
     hipLaunchKernelGGL(k_bbox_fold, dim3(grid_for(4ull * B.ntiles, TB, 64)), dim3(TB), 0, s, bk,
                        B.ntiles);
 }

@@ -354,7 +354,11 @@ owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_i
  * sit in Morton order: a point's sort key is the top key_bits bits of the
  * 64-bit interleave of qx (even bits) and qy (odd bits), where
  * q = (uint32)((v - qmin) * qscale) in IEEE double, 0 when not > 0 (or NaN)
- * and 0xFFFFFFFF from 4294967295 up; leaf j spans tree positions
+ * and 0xFFFFFFFF from 4294967295 up.  (qmin, qscale) is the square over a
+ * sample of the cloud when the library took the bbox itself (256 runs of 16
+ * consecutive points starting at r*n/256; every point up to 4096 of them),
+ * else over the given bbox; qscale maps its side to just under 2^32.  Leaf j
+ * spans tree positions
  * [leaf_start, leaf_start + leaf_points), leaf_size of them but the last. */
 #define OWM_TREE_LEAF 0x80000000u /* child reference bit: a leaf, not an internal node */
 typedef struct owm_tree_view {

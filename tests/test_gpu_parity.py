@@ -238,8 +238,16 @@ def test_profile_phase_mask(ctx):
     assert timed == {"sort_tree_pass"}
     assert only["sort_tree_pass"]["calls"] >= 2 and only["sort_tree_pass"]["ms"] > 0
     assert only["sort_tree_pass"]["calls"] == full["sort_tree_pass"]["calls"]
-    for k in ("bbox", "morton", "leaf_aggs", "karras", "agg_levels", "stage1", "stage3"):
+    for k in ("bbox", "leaf_aggs", "karras", "agg_levels", "stage1", "stage3"):
         assert full[k]["calls"] >= 1, k
+    # the bbox pass takes the key digit counts too (no Morton pass); with the
+    # bbox given there is no bbox pass and the Morton pass takes them
+    assert full["morton"]["calls"] == 0
+    ctx.profile(True)
+    assert np.array_equal(ctx.run_tile(x, y, z, 12, 20, 0.8, 32, bbox=synth.bbox(x, y)), want)
+    given = ctx.profile_read()
+    ctx.profile(False)
+    assert given["morton"]["calls"] == 1 and given["bbox"]["calls"] == 0
 
 
 def test_errors(ctx):

@@ -5,8 +5,9 @@ brute force, in the spirit of the reference's checkNumPoints
 - Morton keys: each point's sort key is the top key_bits of the bit
   interleave of its quantised x (even bits) and y (odd bits) -- a closed form
   computed here bit by bit (basic/morton.h:6-35 is the reference's 16-bit form),
-  quantised against the exact data bbox; de-interleaving a key gives back the
-  top bits of the quantised coordinates;
+  quantised against the square over a sample of the cloud (the frame the bbox
+  pass takes its key counts with); de-interleaving a key gives back the top
+  bits of the quantised coordinates;
 - tree order: the tree's point order is numpy's stable argsort of those keys
   (the reference sorts (key, index) pairs with cub::DeviceRadixSort,
   ordered_builder.inl:460-468, whose output equals std::stable_sort), a
@@ -62,6 +63,25 @@ def deinterleave(m, bits):
     return ex, ey
 
 
+def key_frame(x, y):
+    """The keys' quantisation frame: the square over the box of 256 runs of 16
+    consecutive points starting at r*n//256 (every point when n <= 4096), NaN
+    skipped per axis (no finite sample: origin 0), its side mapped just under
+    2^32 (zero extent: side 1)."""
+    n = len(x)
+    idx = np.unique(np.concatenate([np.arange(r * n // 256, min(r * n // 256 + 16, n))
+                                    for r in range(256)]))
+    lo, hi = [], []
+    for a in (x[idx], y[idx]):
+        f = a[~np.isnan(a)]
+        lo.append(float(f.min()) if len(f) else 0.0)
+        hi.append(float(f.max()) if len(f) else 0.0)
+    side = max(hi[0] - lo[0], hi[1] - lo[1])
+    if not (side > 0 and np.isfinite(side)):
+        side = 1.0
+    return (lo[0], lo[1]), 4294967296.0 / (side * (1.0 + 1e-12))
+
+
 def _bits(a):
     return np.ascontiguousarray(a, np.float64).view(np.uint64)
 
@@ -74,12 +94,10 @@ def check_tree(t, x, y, z):
     assert np.array_equal(np.sort(ids), np.arange(n)), "ids are not a permutation"
     for a, b in ((t["xs"], x), (t["ys"], y), (t["zs"], z)):
         assert np.array_equal(_bits(a), _bits(b[ids])), "records do not follow the ids"
-    # quantisation box: the exact data bbox (NaN skipped per axis), its side
-    # mapped just under 2^32 (a point cloud of zero extent takes side 1)
-    assert t["qmin"] == (np.nanmin(x), np.nanmin(y))
-    side = max(np.nanmax(x) - np.nanmin(x), np.nanmax(y) - np.nanmin(y))
-    side = side if side > 0 else 1.0
-    assert side * t["qscale"] < 2.0 ** 32 and side * t["qscale"] > 2.0 ** 32 * (1 - 1e-9)
+    # quantisation frame: the square over the sampled box, exactly
+    qmin, qscale = key_frame(x, y)
+    assert t["qmin"] == qmin and t["qscale"] == qscale, "quantisation frame"
+    assert np.nanmin(x) <= qmin[0] and np.nanmin(y) <= qmin[1]
     # keys by the closed form; order = stable argsort
     qx, qy = quantise(x, t["qmin"][0], t["qscale"]), quantise(y, t["qmin"][1], t["qscale"])
     keys = interleave(qx, qy) >> np.uint64(64 - kb)
