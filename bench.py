@@ -39,7 +39,7 @@ PHASE_BYTES = {"stage1": "24 B x N x Wsize/Displace + 4 B x windows",
                "leaf_aggs": "32 B x N + 48 B x leaves",
                "sort_tree_pass": "56 B x N for the first pass (x, y, z read; a 32-byte point written), 64 B x N for later passes (a 32-byte point read and written)",
                "morton": "16 B x N", "bbox": "16 B x N", "agg_levels": "48 B x (leaves + level entries)",
-               "karras": "24 B x leaves + 76 B x nodes", "stage2_mark": "12 B x windows",
+               "karras": "24 B x leaves + 56 B x nodes", "stage2_mark": "12 B x windows",
                "stage2_compact": "4 B x N / 32 (the LLP bitmap read)",
                "merge_seeds": "4 B x N / 32 + 8 B x stage-3 cells",
                "stage1_join": "72 B x leaves + 12 B x blocks"}

@@ -387,11 +387,11 @@ uint64_t tree_bytes(const TreeLayout &L)
     b += align_up(L.np * 16ull, 256) * 2;                 // pts: xy, (z, id, key)
     b += align_up(L.np * 4ull, 256);                      // pos
     b += align_up(L.nleaf * 32ull, 256);                  // lf_box
-    b += align_up(L.nleaf * 4ull, 256) * 6;               // cnt start npts zid parent key0
+    b += align_up(L.nleaf * 4ull, 256) * 5;               // cnt start npts zid key0
     b += align_up(L.nleaf * 8ull, 256);                   // lf_zkey
     b += align_up(std::max<uint64_t>(L.nnode, 1) * 32ull, 256);  // nd_box
-    b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 3;  // cnt zid parent
-    b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 3;  // zkey child range
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 4ull, 256) * 2;  // cnt zid
+    b += align_up(std::max<uint64_t>(L.nnode, 1) * 8ull, 256) * 2;  // zkey child
     return b;
 }
 
@@ -412,16 +412,13 @@ void carve_tree(uint8_t *base, const TreeLayout &L, TreeOut &T)
     T.lf_start = (uint32_t *)take(L.nleaf * 4ull);
     T.lf_npts = (uint32_t *)take(L.nleaf * 4ull);
     T.lf_zid = (uint32_t *)take(L.nleaf * 4ull);
-    T.lf_parent = (uint32_t *)take(L.nleaf * 4ull);
     T.lf_key0 = (uint32_t *)take(L.nleaf * 4ull);
     T.lf_zkey = (unsigned long long *)take(L.nleaf * 8ull);
     T.nd_box = (double4 *)take(nn * 32ull);
     T.nd_cnt = (uint32_t *)take(nn * 4ull);
     T.nd_zid = (uint32_t *)take(nn * 4ull);
-    T.nd_parent = (uint32_t *)take(nn * 4ull);
     T.nd_zkey = (unsigned long long *)take(nn * 8ull);
     T.nd_child = (uint2 *)take(nn * 8ull);
-    T.nd_range = (uint2 *)take(nn * 8ull);
 }
 
 TreeDev tree_view(const TreeOut &T)
@@ -611,9 +608,10 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
     ctx->prof.end(s, pi, 48ull * (L.nleaf + lv_entries), L.nleaf);
     pi = ctx->prof.begin(s, PH_KARRAS);
     launch_karras(s, B, T, V);
-    // per node: the neighbours' first keys, a range/children/parent record and
-    // an aggregate written (28 + 48 B), the blocks of its range read
-    ctx->prof.end(s, pi, 24ull * L.nleaf + 76ull * L.nnode, L.nnode);
+    // per leaf: the first keys the split searches read (24 B); per node: its
+    // children and aggregate written (8 + 48 B).  The aggregate-level blocks
+    // that tile a node's range (at most two per level) are read on top.
+    ctx->prof.end(s, pi, 24ull * L.nleaf + 56ull * L.nnode, L.nnode);
     HIP_TRY(hipGetLastError(), "tree build launch");
     return OWM_OK;
 }

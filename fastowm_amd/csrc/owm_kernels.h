@@ -68,12 +68,12 @@ struct TreeOut {
     PointsOut pts;  // the cloud in Morton order
     uint32_t *pos;
     double4 *lf_box;
-    uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid, *lf_parent, *lf_key0;
+    uint32_t *lf_cnt, *lf_start, *lf_npts, *lf_zid, *lf_key0;
     unsigned long long *lf_zkey;
     double4 *nd_box;
-    uint32_t *nd_cnt, *nd_zid, *nd_parent;
+    uint32_t *nd_cnt, *nd_zid;
     unsigned long long *nd_zkey;
-    uint2 *nd_child, *nd_range;  // children; leaf range [lo, hi] (tile-local leaf indices)
+    uint2 *nd_child;  // children (OWM_LEAF_BIT | leaf, or node)
 };
 
 // bbox per tile as order-preserving u64 keys [minx, miny, maxx(inv), maxy(inv)]

@@ -555,7 +555,6 @@ __device__ __forceinline__ void leaf_store(const TileDev &TD, TreeOut &T, uint32
     T.lf_zkey[leaf] = a.bk;
     T.lf_zid[leaf] = a.bid;
     T.lf_key0[leaf] = key0;
-    T.lf_parent[leaf] = OWM_NONE;
 }
 
 constexpr int LEAF_G = 8;  // leaf groups a wave reduces at once
@@ -933,25 +932,10 @@ __global__ __launch_bounds__(TB) void k_karras(BatchDev B, TreeOut T, AggLevels 
         }
         int gamma = i + s * d + (d < 0 ? -1 : 0);
         int lo = min(i, j), hi = max(i, j);
-        uint32_t left, right;
-        if (lo == gamma) {
-            left = OWM_LEAF_BIT | (TD.leaf_off + gamma);
-            T.lf_parent[TD.leaf_off + gamma] = g;
-        } else {
-            left = TD.node_off + gamma;
-            T.nd_parent[TD.node_off + gamma] = g;
-        }
-        if (hi == gamma + 1) {
-            right = OWM_LEAF_BIT | (TD.leaf_off + gamma + 1);
-            T.lf_parent[TD.leaf_off + gamma + 1] = g;
-        } else {
-            right = TD.node_off + gamma + 1;
-            T.nd_parent[TD.node_off + gamma + 1] = g;
-        }
+        const uint32_t left = lo == gamma ? OWM_LEAF_BIT | (TD.leaf_off + gamma) : TD.node_off + gamma;
+        const uint32_t right = hi == gamma + 1 ? OWM_LEAF_BIT | (TD.leaf_off + gamma + 1)
+                                               : TD.node_off + gamma + 1;
         T.nd_child[g] = make_uint2(left, right);
-        T.nd_range[g] = make_uint2((uint32_t)lo, (uint32_t)hi);
-        if (i == 0)
-            T.nd_parent[g] = OWM_NONE;
         // blocks of [leaf_off + lo, leaf_off + hi] level by level (half-open)
         LeafAgg acc = leaf_agg_empty();
         uint32_t ea = TD.leaf_off + (uint32_t)lo, ee = TD.leaf_off + (uint32_t)hi + 1;
