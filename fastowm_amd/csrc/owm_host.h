@@ -102,7 +102,7 @@ struct Tuner {
     {
         table.clear();
         for (uint64_t m : CLASS_MAX)
-            table.push_back({m, {64, {16, 8, 4, 0}, 16}, 0.0});
+            table.push_back({m, {64, {16, 16, 4, 0}, 16}, 0.0});
     }
     static int class_of(uint64_t n)
     {

@@ -21,7 +21,9 @@ using namespace owm;
 namespace {
 
 constexpr uint32_t LEAVES[] = {16, 32, 64, 128};
-constexpr Stage1Cfg BLOCKS[] = {{8, 8, 4, 0}, {16, 8, 4, 0}, {8, 8, 2, 0}, {16, 8, 4, 1}};
+constexpr Stage1Cfg BLOCKS[] = {{8, 8, 4, 0}, {16, 8, 4, 0}, {8, 8, 2, 0},
+                                {16, 8, 4, 1}, {16, 16, 4, 0}, {32, 16, 4, 0}};
+constexpr int NBLOCKS = (int)(sizeof(BLOCKS) / sizeof(BLOCKS[0]));
 constexpr uint32_t SORT_BITS[] = {32, 24, 16};
 constexpr uint64_t SUB_POINTS = 8ull << 20;  // points per leaf-size trial batch
 
@@ -244,28 +246,38 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
 //
 // Per point of a tile, in ns, for leaf size L, window block b and key width S:
 //   M_LEAF / L + M_AMP * amp0 + M_COARSE * ampc + M_PASS * S / 8 + M_BLK[b]
-// plus, once per launch, M_UNDER * max(0, BLOCKS_FULL / blocks - 1) per point.
-// Over the histogram's leaves (density rho = points / side^2, weighted by
-// their points): amp0 = mean (1 + s0/ex)(1 + s0/ey) - 1 with s0 = sqrt(L/rho)
-// the side of an L-point leaf and ex, ey = bx*D + W - D, by*D + W - D the
-// extent of a window block (W = Wsize, D = Displace), i.e. how many window
-// blocks beyond the first read a point; ampc = the same with
-// s = max(s0, tile side / 2^(S/2)) minus amp0, what a key too short to order
-// the points of one Morton cell adds; `blocks` = window blocks in the launch.
-// The constants are a least-squares fit (a free intercept per set) to the batch
-// times of 67 homogeneous tile sets x 48 configurations on MI355X
+//   + M_READ * rf
+// plus, once per launch, (M_QUANT * q + M_UNDER * u) per point.  Over the
+// histogram's leaves (density rho = points / side^2, weighted by their
+// points): amp0 = mean (1 + s0/ex)(1 + s0/ey) - 1 with s0 = sqrt(L/rho) the
+// side of an L-point leaf and ex, ey = bx*D + W - D, by*D + W - D the extent
+// of a window block (W = Wsize, D = Displace), i.e. how many window blocks
+// beyond the first read a point of a leaf that straddles their edges;
+// ampc = the same with s = max(s0, tile side / 2^(S/2)) minus amp0, what a key
+// too short to order the points of one Morton cell adds; rf = the points the
+// tile's window blocks read over its points, from its window grid (blocks at
+// the grid's right and top edges hold fewer windows but the same margins).
+// With `slots` = workgroups of the shape the chip holds at once and `blocks` =
+// window blocks in the launch: q = ceil(blocks / slots) * slots / blocks - 1
+// (the idle share of the last round of workgroups), u = max(0, slots / blocks
+// - 1) (a launch that does not fill the chip).  The constants are a
+// least-squares fit (a free intercept per set) to the batch times of 67
+// homogeneous tile sets x 72 configurations on MI355X
 // (fastowm_amd/tune/fit_model.py on fastowm_amd/tune/trials_mi355x.json); on
-// those sets the model's pick is 0.7 % slower than the fastest configuration
-// on average (5.8 % at most), against 38.7 % for leaf 32 / 8x8x4 / 32 bits.
+// those sets the model's pick is 1.35 % slower than the fastest configuration
+// on average (5.9 % at most), against 46 % for leaf 32 / 8x8x4 / 32 bits.
 namespace {
 
-constexpr double M_LEAF = 0.470672;
-constexpr double M_AMP = 0.044566;
-constexpr double M_COARSE = 0.0180814;
-constexpr double M_PASS = 0.0192899;
-constexpr double M_BLK[4] = {-0.00100764, -0.00759359, -0.000193196, 0.00879443};
-constexpr double M_UNDER = 0.0113509;
-constexpr double BLOCKS_FULL = 1024.0;
+constexpr double M_LEAF = 0.398855;
+constexpr double M_AMP = 0.0439984;
+constexpr double M_COARSE = 0.0198026;
+constexpr double M_PASS = 0.0194433;
+constexpr double M_BLK[NBLOCKS] = {-0.0059079, -0.00613674, -0.00501937, 0.0111246, -0.000501001, 0.00644038};
+constexpr double M_READ = 0.0178601;
+constexpr double M_QUANT = 0.00498984;
+constexpr double M_UNDER = 0.00395825;
+constexpr double CUS = 256.0;                                     // compute units of the part
+constexpr double WG_PER_CU[NBLOCKS] = {4.0, 4.0, 4.0, 3.0, 2.0, 1.0};  // workgroups of each shape
 
 struct ModelTile {
     double n = 0, weight = 0;
@@ -307,15 +319,6 @@ bool model_tile(const owm_tune_sample &s, ModelTile &m)
     return m.sw > 0.0;
 }
 
-int block_index(const Stage1Cfg &c)
-{
-    for (int j = 0; j < 4; ++j)
-        if (BLOCKS[j].bx == c.bx && BLOCKS[j].by == c.by && BLOCKS[j].k == c.k &&
-            BLOCKS[j].lcs == c.lcs)
-            return j;
-    return 0;
-}
-
 // ns per point of one tile, launch term left out
 double tile_cost(const ModelTile &m, uint32_t leaf, int b, uint32_t sort_bits)
 {
@@ -330,7 +333,11 @@ double tile_cost(const ModelTile &m, uint32_t leaf, int b, uint32_t sort_bits)
     }
     a0 /= m.sw;
     a /= m.sw;
-    return M_LEAF / leaf + M_AMP * a0 + M_COARSE * (a - a0) + M_PASS * sort_bits / 8.0 + M_BLK[b];
+    const double nbx = (double)((m.ncols + B.bx - 1) / B.bx), nby = (double)((m.nrows + B.by - 1) / B.by);
+    const double rf = (m.ncols * m.D + nbx * (m.W - m.D)) * (m.nrows * m.D + nby * (m.W - m.D)) /
+                      ((m.ncols * m.D + m.W - m.D) * (m.nrows * m.D + m.W - m.D));
+    return M_LEAF / leaf + M_AMP * a0 + M_COARSE * (a - a0) + M_PASS * sort_bits / 8.0 + M_BLK[b] +
+           M_READ * rf;
 }
 
 double blocks_of(const ModelTile &m, int b)
@@ -355,7 +362,7 @@ Pick pick_launch(const std::vector<const ModelTile *> &tiles)
     double pts = 0;
     for (const ModelTile *m : tiles)
         pts += m->weight;
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < NBLOCKS; ++b)
         for (uint32_t sb : SORT_BITS) {
             double blocks = 0;
             for (const ModelTile *m : tiles)
@@ -363,7 +370,10 @@ Pick pick_launch(const std::vector<const ModelTile *> &tiles)
             Pick p;
             p.block = b;
             p.sort_bits = sb;
-            p.ns = M_UNDER * std::max(0.0, BLOCKS_FULL / std::max(blocks, 1.0) - 1.0) * pts;
+            const double slots = CUS * WG_PER_CU[b];
+            blocks = std::max(blocks, 1.0);
+            const double q = std::ceil(blocks / slots) * slots / blocks - 1.0;
+            p.ns = (M_QUANT * q + M_UNDER * std::max(0.0, slots / blocks - 1.0)) * pts;
             for (int c = 0; c < 6; ++c) {
                 double cbest = 1e300;
                 bool any = false;

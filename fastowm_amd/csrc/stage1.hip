@@ -326,14 +326,14 @@ __global__ __launch_bounds__(256) void k_lcs(BatchDev B, TreeDev T, LeafCells LC
     }
 }
 
-template <int BX, int BY>
+template <int BX, int BY, int NT>
 struct S1Shape {
     static constexpr int NXC = 4 * BX + 1;
     static constexpr int NYC = 4 * BY + 1;
     static constexpr int NCELL = NXC * NYC;
     // the x-pass results (class row x window column) reuse the cell arrays
     static constexpr int M1 = NYC * BX;
-    static constexpr int M1_PER_THREAD = (M1 + S1_THREADS - 1) / S1_THREADS;
+    static constexpr int M1_PER_THREAD = (M1 + NT - 1) / NT;
     static_assert(M1 <= NCELL, "x-pass results fit the cell arrays");
 };
 
@@ -341,14 +341,19 @@ constexpr uint32_t S1_ENT_LEAVES = 64 / LCS_SLOTS;  // summary leaves per unit o
 
 // USE_LCS compiles the leaf-summary items in (their registers cost occupancy
 // when no tile uses them).
-template <int BX, int BY, int S1_K, bool USE_LCS>
-__global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
+// NT threads per workgroup (256 for blocks up to 16x8; larger blocks take
+// more waves to hide the latency of one workgroup per LDS-sized share of a
+// CU).  The second bound is waves per SIMD: 4 keeps every variant within 128
+// VGPRs.
+template <int BX, int BY, int S1_K, bool USE_LCS, int NT = S1_THREADS>
+__global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4))) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
                                                         const uint32_t *__restrict__ blk_start,
                                                         const uint32_t *__restrict__ list,
                                                         uint32_t cap,
                                                         uint32_t *__restrict__ min_ids)
 {
-    using S = S1Shape<BX, BY>;
+    using S = S1Shape<BX, BY, NT>;
+    constexpr uint32_t NWV = NT / 64;
     constexpr int NXC = S::NXC;
     __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
     __shared__ uint32_t sGLx[BX], sGRx[BX], sGLy[BY], sGRy[BY];  // global classes of the edges
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
                 sGRy[tid - 64] = gclass(sRy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
             }
         }
-        for (int c = tid; c < S::NCELL; c += S1_THREADS) {
+        for (int c = tid; c < S::NCELL; c += NT) {
             sCnt[c] = 0;
             sZ[c] = OWM_ZKEY_NONE;
             sId[c] = OWM_NONE;
@@ -465,7 +470,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
             const uint32_t mraw = sNraw, ment = sNent;
             const uint32_t neu = (ment + S1_ENT_LEAVES - 1) / S1_ENT_LEAVES;
             const uint32_t nunits = neu + (mraw + nl_grp - 1) / nl_grp * nseg;
-            const uint32_t rounds = (nunits + 4 * S1_K - 1) / (4 * S1_K);
+            const uint32_t rounds = (nunits + NWV * S1_K - 1) / (NWV * S1_K);
 
             for (uint32_t r = 0; r < rounds; ++r) {
                 uint32_t cell[S1_K], idv[S1_K];
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
 #pragma unroll
                 for (int k = 0; k < S1_K; ++k) {
                     cell[k] = OWM_NONE;
-                    const uint32_t u = (r * S1_K + k) * 4 + wave;
+                    const uint32_t u = (r * S1_K + k) * NWV + wave;
                     if (u >= nunits)
                         continue;
                     if (USE_LCS && u < neu) {
@@ -578,7 +583,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
         unsigned long long xz[S::M1_PER_THREAD];
 #pragma unroll
         for (int t = 0; t < S::M1_PER_THREAD; ++t) {
-            const uint32_t q = tid + t * S1_THREADS;
+            const uint32_t q = tid + t * NT;
             xc[t] = 0;
             xi[t] = OWM_NONE;
             xz[t] = OWM_ZKEY_NONE;
@@ -598,7 +603,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < S::M1_PER_THREAD; ++t) {
-            const uint32_t q = tid + t * S1_THREADS;
+            const uint32_t q = tid + t * NT;
             if (q >= nyc * ni)
                 continue;
             const uint32_t rr = q / ni, w = q - rr * ni;
@@ -608,7 +613,7 @@ __global__ __launch_bounds__(S1_THREADS, USE_LCS ? 3 : (S1_K > 4 ? 2 : 4)) void 
         }
         __syncthreads();
         // ... then y-pass per window
-        for (uint32_t q = tid; q < ni * nj; q += S1_THREADS) {
+        for (uint32_t q = tid; q < ni * nj; q += NT) {
             uint32_t v = q / ni, w = q - v * ni;
             uint32_t cnt = 0, id = OWM_NONE;
             unsigned long long z = OWM_ZKEY_NONE;
@@ -721,12 +726,12 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t *__restri
 
 uint64_t stage1_work_words(uint64_t nblk) { return 2 * (nblk + 1) + (nblk / SC_TILE + 2) + 8; }
 
-template <int BX, int BY, int K, bool LCS>
+template <int BX, int BY, int K, bool LCS, int NT = S1_THREADS>
 static void stage1_shape(hipStream_t s, const BatchDev &B, const TreeDev &T, const LeafCells &LC,
                          const uint32_t *start, const uint32_t *list, uint32_t cap, uint32_t *min_ids)
 {
     uint32_t grid = B.nblk < 1 ? 1 : (B.nblk > 65536 ? 65536 : B.nblk);
-    hipLaunchKernelGGL((k_stage1<BX, BY, K, LCS>), dim3(grid), dim3(S1_THREADS), 0, s, B, T, LC,
+    hipLaunchKernelGGL((k_stage1<BX, BY, K, LCS, NT>), dim3(grid), dim3(NT), 0, s, B, T, LC,
                        start, list, cap, min_ids);
 }
 
@@ -738,7 +743,8 @@ bool stage1_shape_ok(const Stage1Cfg &c)
         return c.k == 4 || (!c.lcs && (c.k == 2 || c.k == 8));
     if (c.bx == 16 && c.by == 8)
         return c.k == 4;
-    return !c.lcs && c.k == 4 && ((c.bx == 16 && c.by == 16) || (c.bx == 32 && c.by == 8));
+    return !c.lcs && c.k == 4 &&
+           ((c.bx == 16 && c.by == 16) || (c.bx == 32 && c.by == 8) || (c.bx == 32 && c.by == 16));
 }
 
 static uint64_t al256(uint64_t v) { return (v + 255) / 256 * 256; }
@@ -814,9 +820,11 @@ void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, cons
         else
             stage1_shape<16, 8, 4, false>(s, B, T, LC, start, list, cap, min_ids);
     } else if (cfg.bx == 16 && cfg.by == 16) {
-        stage1_shape<16, 16, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+        stage1_shape<16, 16, 4, false, 512>(s, B, T, LC, start, list, cap, min_ids);
     } else if (cfg.bx == 32 && cfg.by == 8) {
-        stage1_shape<32, 8, 4, false>(s, B, T, LC, start, list, cap, min_ids);
+        stage1_shape<32, 8, 4, false, 512>(s, B, T, LC, start, list, cap, min_ids);
+    } else if (cfg.bx == 32 && cfg.by == 16) {
+        stage1_shape<32, 16, 4, false, 1024>(s, B, T, LC, start, list, cap, min_ids);
     } else if (cfg.k == 2) {
         stage1_shape<8, 8, 2, false>(s, B, T, LC, start, list, cap, min_ids);
     } else if (cfg.k == 8) {

@@ -5,7 +5,7 @@ Model, per point of a tile, in ns, for leaf size L, window block b = (bx, by,
 k, summaries) and sort key width S:
 
     c_leaf / L + c_amp * amp0 + c_coarse * ampc + c_pass * S / 8
-      + c_blk[b] + c_under * max(0, BLOCKS_FULL / blocks - 1)
+      + c_blk[b] + c_read * rf + c_quant * q + c_under * u
 
 over the tile's leaf histogram (statsmaxnum leaves: density rho = points /
 side^2 of each leaf, weighted by its points):
@@ -16,7 +16,15 @@ side^2 of each leaf, weighted by its points):
   - ampc: the same with s = max(s0, tile side / 2^(S/2)) minus amp0: what a
     key too short to order points inside one Morton cell adds (a leaf then
     spans the whole cell);
-  - blocks: window blocks in the launch (few blocks leave compute units idle).
+  - rf: the points a tile's window blocks read over its points, from its window
+    grid: blocks of bx x by windows (fewer at the grid's right and top edges)
+    each read an extent of ni*D + W - D by nj*D + W - D;
+  - with `slots` = workgroups the chip holds at once for the block shape (256
+    compute units x 4, 4, 4, 3, 2, 1 workgroups of the shapes below) and
+    `blocks` = window blocks in the launch: q = ceil(blocks / slots) * slots /
+    blocks - 1, the idle share of the last round of workgroups, and
+    u = max(0, slots / blocks - 1), the idle share of a launch that does not
+    fill the chip.
 The first term counts leaves (leaf records, Karras nodes, aggregate levels),
 the fourth the sort passes.  The fit is least squares on the times of each set
 about that set's mean (a free intercept per set), so only differences between
@@ -33,10 +41,11 @@ import sys
 import numpy as np
 
 LEAVES = (16, 32, 64, 128)
-BLOCKS = ((8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1))
+BLOCKS = ((8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1), (16, 16, 4, 0), (32, 16, 4, 0))
 SORTS = (32, 24, 16)
-BLOCKS_FULL = 1024.0  # window blocks below which stage 1 leaves compute units idle
-N_COEF = 4 + len(BLOCKS) + 1
+CU = 256
+SLOTS = dict(zip(BLOCKS, (4, 4, 4, 3, 2, 1)))  # workgroups per compute unit
+N_COEF = 4 + len(BLOCKS) + 3
 
 
 def density(sample):
@@ -66,9 +75,15 @@ def features(sample, wsize, leaf, block, sort_bits, blocks_in_launch, rho=None, 
     sw = w.sum()
     amp0 = float((w * ((1 + s0 / ex) * (1 + s0 / ey) - 1)).sum() / sw)
     ampc = float((w * ((1 + s / ex) * (1 + s / ey) - 1)).sum() / sw) - amp0
+    nc, nr = sample["ncols"], sample["nrows"]
+    nbx, nby = math.ceil(nc / bx), math.ceil(nr / by)
+    rf = (nc * D + nbx * (W - D)) * (nr * D + nby * (W - D)) / ((nc * D + W - D) * (nr * D + W - D))
+    slots = CU * SLOTS[tuple(block)]
+    q = math.ceil(blocks_in_launch / slots) * slots / blocks_in_launch - 1.0
+    u = max(0.0, slots / blocks_in_launch - 1.0)
     f = [1.0 / leaf, amp0, ampc, sort_bits / 8.0]
     f += [1.0 if b == block else 0.0 for b in BLOCKS]
-    f.append(max(0.0, BLOCKS_FULL / blocks_in_launch - 1.0))
+    f += [rf, q, u]
     return np.array(f)
 
 
@@ -121,7 +136,9 @@ def cpp_block(coef):
     names = ["LEAF", "AMP", "COARSE", "PASS"]
     lines = ["constexpr double M_%s = %.6g;" % (n, c) for n, c in zip(names, coef[:4])]
     blk = ", ".join("%.6g" % coef[4 + j] for j in range(len(BLOCKS)))
-    lines.append("constexpr double M_BLK[4] = {%s};" % blk)
+    lines.append("constexpr double M_BLK[%d] = {%s};" % (len(BLOCKS), blk))
+    lines.append("constexpr double M_READ = %.6g;" % coef[-3])
+    lines.append("constexpr double M_QUANT = %.6g;" % coef[-2])
     lines.append("constexpr double M_UNDER = %.6g;" % coef[-1])
     return "\n".join(lines)
 

@@ -340,6 +340,22 @@ def _near_edge_cloud(n, seed, side, origin):
     return x, y, z
 
 
+@pytest.mark.parametrize("block", [(16, 16, 4), (32, 8, 4), (32, 16, 4)])
+@pytest.mark.parametrize("ws,bs,ov,leaf", [(10, 20, 0.9, 16), (7, 11, 0.5, 8), (12, 20, 0.8, 64)])
+def test_points_on_edges_large_blocks(ctx, restore_table, block, ws, bs, ov, leaf):
+    """The larger window blocks (512- and 1024-thread workgroups) give the
+    brute-force window scan on a lattice whose points sit on window edges."""
+    _stage1_table(ctx, "off", block)
+    say("lattice, large block", block, ws, bs, ov, leaf)
+    x, y, z = _lattice_cloud(60_000, seed=ws * 31 + leaf)
+    box = synth.bbox(x, y)
+    p = O.params(ws, bs, ov, box, len(x), leaf)
+    tree = ctx.parallel_qtree(x, y, z, leaf_size=leaf)
+    gm, _ = ctx.stage1(tree, ws, ov, bs)
+    assert np.array_equal(gm, O.stage1_brute(x, y, z, p))
+    tree.free()
+
+
 @pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
 def test_points_ulps_from_edges(ctx, restore_table, origin):
     """Stage 1 classifies by arithmetic away from edges and by exact compares

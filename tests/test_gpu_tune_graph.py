@@ -77,7 +77,7 @@ def test_autotune_keeps_results_and_table_round_trips():
             assert e["leaf_size"] in (16, 32, 64, 128)
             assert e["sort_bits"] in (16, 24, 32)
             assert (e["block_x"], e["block_y"], e["block_k"], e["summaries"]) in (
-                (8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1))
+                (8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1), (16, 16, 4, 0), (32, 16, 4, 0))
         # the classes that held tiles were timed
         assert all(table[c]["ms"] > 0 for c in (0, 1, 2, 3))
         say("tuned batch equals untuned")
@@ -95,7 +95,9 @@ def test_autotune_keeps_results_and_table_round_trips():
 
 @pytest.mark.parametrize("sort_bits,leaf,block", [(24, 16, (8, 8, 2, 0)), (24, 128, (16, 8, 4, 0)),
                                                   (32, 64, (16, 8, 4, 0)), (24, 32, (16, 8, 4, 1)),
-                                                  (32, 16, (8, 8, 4, 1)), (16, 32, (16, 8, 4, 0))])
+                                                  (32, 16, (8, 8, 4, 1)), (16, 32, (16, 8, 4, 0)),
+                                                  (16, 64, (16, 16, 4, 0)), (16, 128, (32, 8, 4, 0)),
+                                                  (16, 64, (32, 16, 4, 0)), (24, 16, (32, 16, 4, 0))])
 def test_forced_table_entries_match_oracle(sort_bits, leaf, block):
     say("table entry", sort_bits, leaf, block)
     t = synth.scaled(synth.config_c2(seed=5), 700_000)
