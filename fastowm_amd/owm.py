@@ -635,13 +635,15 @@ class Context:
         d = BatchDesc(tiles)
         return d.descs, d.points
 
-    def batch_run(self, x, y, z, tiles, graph=False, copy=True):
+    def batch_run(self, x, y, z, tiles, graph=False, copy=True, split=True):
         """tiles: list of (n, dict(wsize=, bsize=, overlap=, leaf_size=, bbox=)) in
         cloud order, or a BatchDesc of such a list (built once, reused).
         Returns a list of seed arrays (tile-local ids).  graph=True captures the
         device chain once per batch layout and replays it.  copy=False returns
         views of the context's page-locked seed buffer instead of copies: they
-        are valid until the next call on this context."""
+        are valid until the next call on this context.  split=False returns
+        the C ABI's form instead of the list: all seeds in one array and the
+        n_tiles + 1 offsets of each tile's range in it (no per-tile objects)."""
         descs, off = self._descs(tiles)
         n_tiles = len(descs)
         cap = off + n_tiles + 1
@@ -659,6 +661,8 @@ class Context:
         seeds = out[:int(so[-1])]
         if copy:
             seeds = seeds.copy()  # one copy out of the reusable buffer
+        if not split:
+            return seeds, so
         return np.split(seeds, so[1:-1].astype(np.intp))  # a view per tile
 
     # -- autotuner ----------------------------------------------------------

@@ -356,8 +356,8 @@ owm_status owm_tree_info(const owm_tree *tree, uint64_t *n_leaves, uint64_t *n_i
  * 64-bit interleave of qx (even bits) and qy (odd bits), where
  * q = (uint32)((v - qmin) * qscale) in IEEE double, 0 when not > 0 (or NaN)
  * and 0xFFFFFFFF from 4294967295 up.  (qmin, qscale) is the square over a
- * sample of the cloud when the library took the bbox itself (256 runs of 16
- * consecutive points starting at r*n/256; every point up to 4096 of them),
+ * sample of the cloud when the library took the bbox itself (128 runs of 16
+ * consecutive points starting at r*n/128; every point up to 2048 of them),
  * else over the given bbox; qscale maps its side to just under 2^32.  Leaf j
  * spans tree positions
  * [leaf_start, leaf_start + leaf_points), leaf_size of them but the last. */

@@ -64,13 +64,13 @@ def deinterleave(m, bits):
 
 
 def key_frame(x, y):
-    """The keys' quantisation frame: the square over the box of 256 runs of 16
-    consecutive points starting at r*n//256 (every point when n <= 4096), NaN
+    """The keys' quantisation frame: the square over the box of 128 runs of 16
+    consecutive points starting at r*n//128 (every point when n <= 2048), NaN
     skipped per axis (no finite sample: origin 0), its side mapped just under
     2^32 (zero extent: side 1)."""
     n = len(x)
-    idx = np.unique(np.concatenate([np.arange(r * n // 256, min(r * n // 256 + 16, n))
-                                    for r in range(256)]))
+    idx = np.unique(np.concatenate([np.arange(r * n // 128, min(r * n // 128 + 16, n))
+                                    for r in range(128)]))
     lo, hi = [], []
     for a in (x[idx], y[idx]):
         f = a[~np.isnan(a)]

@@ -171,7 +171,8 @@ def test_repeated_single_tile_calls_replay_a_graph():
 def test_batch_desc_and_buffer_views():
     """A BatchDesc built once gives the same seeds as the tile list; with
     copy=False the results are views of the context's seed buffer (valid until
-    the next call), equal to the copies."""
+    the next call), equal to the copies; split=False gives the flat array and
+    per-tile offsets."""
     x, y, z, desc, tiles, arrs = _batch([30_000, 70_000, 5_000], seed=700)
     with owm.Context(0) as ctx:
         say("batch desc / views")
@@ -180,6 +181,9 @@ def test_batch_desc_and_buffer_views():
         assert len(bd) == 3 and bd.points == len(x)
         views = ctx.batch_run(x, y, z, bd, copy=False)
         assert _same([v.copy() for v in views], want)
+        flat, offs = ctx.batch_run(x, y, z, bd, split=False)
+        assert len(offs) == 4 and offs[0] == 0 and offs[-1] == len(flat)
+        assert _same([flat[offs[i]:offs[i + 1]] for i in range(3)], want)
         assert _same(ctx.batch_run(x, y, z, bd, graph=True), want)
         for t, (tx, ty, tz), g in zip(tiles, arrs, want):
             w, _, _ = O.run(tx, ty, tz, t.wsize, t.bsize, t.overlap, 32)
