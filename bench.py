@@ -290,8 +290,9 @@ def run_headline(ctx, torch, synth, args):
     full = ctx.profile_read()
     ctx.profile(False)
     dname = max((k for k in full if full[k]["calls"]), key=lambda k: full[k]["ms"])
-    ctx.profile(True, phases=[dname])
-    log(f"timing {args.steps} batched steps (events on {dname})")
+    ev_phases = sorted({dname, "sort_tree_pass"})
+    ctx.profile(True, phases=ev_phases)
+    log(f"timing {args.steps} batched steps (events on {', '.join(ev_phases)})")
     # a step ends with the seeds in host memory as the C ABI returns them (one
     # array, per-tile offsets); the per-tile arrays are made after the clock
     t0 = time.perf_counter()
@@ -304,11 +305,12 @@ def run_headline(ctx, torch, synth, args):
     ctx.profile(False)
     ms = (t1 - t0) / args.steps * 1e3
     roofline = roofline_of(prof, dname, args.steps, n_total, TRAFFIC_BATCH)
+    roofline_sort = roofline_of(prof, "sort_tree_pass", args.steps, n_total, TRAFFIC_BATCH)
     if args.save_tune_table:
         with open(args.save_tune_table, "w") as f:
             json.dump(table, f, indent=1)
     res = {"tiles": len(tiles), "points": int(n_total), "ms": ms, "phases": phase_table(full),
-           "roofline": roofline,
+           "roofline": roofline, "roofline_sort": roofline_sort,
            "batch": {"batched_eager_mpts": round(n_total / ms * 1e-3, 1), "batched_eager_ms": round(ms, 3),
                      "autotune_s": round(t_tune, 3), "tuning": tuning,
                      "tuned": [{k: e[k] for k in ("max_n", "leaf_size", "block_x", "block_y", "block_k",
@@ -604,9 +606,11 @@ def main():
                                    "Overlap 0.8, Bsize 20, one launch chain, resident in HBM",
                        "tiles": head["tiles"], "points": head["points"],
                        "step": "the whole batch, eager launch chain (the graph replay is beside it)"},
-            "roofline": head["roofline"], "phases": head["phases"],
+            "roofline": head["roofline"], "roofline_sort": head["roofline_sort"],
+            "phases": head["phases"],
             "phases_source": "one untimed step with every phase timed by HIP events; the timed "
-                             "steps carry events around the roofline kernel family only",
+                             "steps carry events around the roofline kernel family and the "
+                             "tree sort only",
             "batch": head["batch"]}
     if not args.headline_only:
         if not args.no_cpu:

@@ -829,9 +829,9 @@ uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested)
     return ctx->tuner.leaf_for(n);
 }
 
-Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total)
+Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t mean_tile)
 {
-    Stage1Cfg c = ctx->tuner.stage1_for(np_total);
+    Stage1Cfg c = ctx->tuner.stage1_for(mean_tile);
     // "bx,by,k,lcs" forces one stage-1 configuration (for experiments)
     if (const char *ov = getenv("OWM_S1_SHAPE")) {
         Stage1Cfg f{};
@@ -872,7 +872,7 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
             return OWM_E_ARG;
         }
     }
-    const int sort_bits = (int)ctx->tuner.sort_bits_for(np);
+    const int sort_bits = (int)ctx->tuner.sort_bits_for(np / std::max<uint32_t>(n_tiles, 1));
     // With the context's own arenas the bbox pass also takes the sort keys'
     // digit counts (over frames sampled from each tile), so the build does not
     // read x and y again; the no-allocation path keeps its borrowed region
@@ -987,7 +987,7 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
         s1_alg += 24.0 * (double)P.td[t].n * ((double)pp.wsize / D.displace);
     }
     P.Q.s1_alg_bytes = (uint64_t)s1_alg;
-    P.cfg = pick_stage1(ctx, np_total);
+    P.cfg = pick_stage1(ctx, np_total / std::max<uint32_t>(n_tiles, 1));
     P.Q.ntiles = n_tiles;
     plan_query(P.td, P.cfg, P.Q);
     // expected size of the leaf/block join: a leaf of side ~sqrt(L/density)

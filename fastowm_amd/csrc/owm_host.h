@@ -79,7 +79,9 @@ struct QueryOut {
 
 // Size-class tuning table.  The leaf size is looked up by a tile's point
 // count; the stage-1 window block and the sort key width (Morton bits per
-// tile, 16, 24 or 32: two to four 8-bit passes) by the batch's total.
+// tile, 8, 16, 24 or 32: one to four 8-bit passes) by a launch's mean tile
+// size (points / tiles), so that a batch of many small tiles and one large
+// tile of the same total keep apart.
 struct TuneCfg {
     uint32_t leaf;
     Stage1Cfg s1;
@@ -209,7 +211,7 @@ owm_status ensure_s1_list(owm_ctx *ctx, uint64_t cap);
 // message; a stage-1 list overflow alone sets *s1_needed and returns OWM_OK.
 owm_status check_guards(owm_ctx *ctx, const uint32_t *hc, uint64_t *s1_needed);
 uint32_t pick_leaf(owm_ctx *ctx, uint64_t n, uint32_t requested);
-Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t np_total);
+Stage1Cfg pick_stage1(owm_ctx *ctx, uint64_t mean_tile);
 void set_err(owm_ctx *ctx, const char *fmt, ...);
 owm_status hip_fail(owm_ctx *ctx, hipError_t e, const char *what);
 // Returns the failure status (with `ctx` in scope) when a HIP call fails.

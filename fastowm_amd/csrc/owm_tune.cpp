@@ -1,9 +1,9 @@
 // Size-class autotuner: times trial configurations of the whole batch chain
 // and keeps the fastest per size class.  Knobs: leaf size (16..128, per tile
 // class), stage-1 window block (bx x by windows, k points per lane per round,
-// with or without leaf cell summaries) and the sort key width (16, 24 or 32
-// Morton bits: two to four 8-bit radix passes), the last two per class of the
-// batch total.  This is the GPU counterpart of the reference's tuning of its tree
+// with or without leaf cell summaries) and the sort key width (8, 16, 24 or
+// 32 Morton bits: one to four 8-bit radix passes), the last two per class of
+// the batch's mean tile size.  This is the GPU counterpart of the reference's tuning of its tree
 // granularity by sweeps over maxNumber and TBB chunk sizes (CPP-TBB-CPU
 // scripts and Results/*.csv); there the sweep is a script, here it is a
 // library call whose table the batch and serving paths use.
@@ -24,7 +24,7 @@ constexpr uint32_t LEAVES[] = {16, 32, 64, 128};
 constexpr Stage1Cfg BLOCKS[] = {{8, 8, 4, 0}, {16, 8, 4, 0}, {8, 8, 2, 0},
                                 {16, 8, 4, 1}, {16, 16, 4, 0}, {32, 16, 4, 0}};
 constexpr int NBLOCKS = (int)(sizeof(BLOCKS) / sizeof(BLOCKS[0]));
-constexpr uint32_t SORT_BITS[] = {32, 24, 16};
+constexpr uint32_t SORT_BITS[] = {32, 24, 16, 8};
 constexpr uint64_t SUB_POINTS = 8ull << 20;  // points per leaf-size trial batch
 
 void to_entry(const Tuner::Entry &e, owm_tune_entry &o)
@@ -106,7 +106,7 @@ owm_status owm_tune_set(owm_ctx *ctx, const owm_tune_entry *in, uint32_t n)
         const owm_tune_entry &e = in[k];
         TuneCfg c{e.leaf_size, {e.block_x, e.block_y, e.block_k, e.summaries}, e.sort_bits};
         if (c.leaf == 0 || c.leaf > 1024 || !stage1_shape_ok(c.s1) ||
-            (c.sort_bits != 16 && c.sort_bits != 24 && c.sort_bits != 32)) {
+            (c.sort_bits != 8 && c.sort_bits != 16 && c.sort_bits != 24 && c.sort_bits != 32)) {
             set_err(ctx, "tuning entry %u: unsupported configuration", k);
             return OWM_E_ARG;
         }
@@ -205,7 +205,7 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
         // Window block and sort width for the class of the whole batch, with
         // the tuned leaf sizes.
         ctx->tuner.table = table;
-        const int bcls = Tuner::class_of(np);
+        const int bcls = Tuner::class_of(np / n_tiles);  // the launch's mean tile
         double best = 1e300;
         TuneCfg win = table[bcls].cfg;
         for (const Stage1Cfg &s1 : BLOCKS)
@@ -262,20 +262,21 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
 // (the idle share of the last round of workgroups), u = max(0, slots / blocks
 // - 1) (a launch that does not fill the chip).  The constants are a
 // least-squares fit (a free intercept per set) to the batch times of 67
-// homogeneous tile sets x 72 configurations on MI355X
+// homogeneous tile sets x 96 configurations on MI355X
 // (fastowm_amd/tune/fit_model.py on fastowm_amd/tune/trials_mi355x.json); on
-// those sets the model's pick is 1.35 % slower than the fastest configuration
-// on average (5.9 % at most), against 46 % for leaf 32 / 8x8x4 / 32 bits.
+// those sets the model's pick is 2.4 % slower than the fastest configuration
+// on average (25 % at most, a set of 2M-point tiles at Overlap 0.5 whose
+// one-pass keys it overrates), against 60 % for leaf 32 / 8x8x4 / 32 bits.
 namespace {
 
-constexpr double M_LEAF = 0.398855;
-constexpr double M_AMP = 0.0439984;
-constexpr double M_COARSE = 0.0198026;
-constexpr double M_PASS = 0.0194433;
-constexpr double M_BLK[NBLOCKS] = {-0.0059079, -0.00613674, -0.00501937, 0.0111246, -0.000501001, 0.00644038};
-constexpr double M_READ = 0.0178601;
-constexpr double M_QUANT = 0.00498984;
-constexpr double M_UNDER = 0.00395825;
+constexpr double M_LEAF = 0.544718;
+constexpr double M_AMP = 0.049079;
+constexpr double M_COARSE = 0.011036;
+constexpr double M_PASS = 0.0142663;
+constexpr double M_BLK[NBLOCKS] = {-0.00612738, -0.0080286, -0.00462846, 0.0145554, -0.00237545, 0.00660445};
+constexpr double M_READ = 0.0215696;
+constexpr double M_QUANT = 0.00443822;
+constexpr double M_UNDER = 0.00704832;
 constexpr double CUS = 256.0;                                     // compute units of the part
 constexpr double WG_PER_CU[NBLOCKS] = {4.0, 4.0, 4.0, 3.0, 2.0, 1.0};  // workgroups of each shape
 
@@ -427,7 +428,10 @@ owm_status model_table(const owm_tune_sample *samples, uint32_t n, std::vector<T
         table[c].cfg = TuneCfg{whole.leaf[c], BLOCKS[own.block], own.sort_bits};
         table[c].ms = own.ns * 1e-6;
     }
-    const int bc = Tuner::class_of((uint64_t)std::min(total, 1.8e19));
+    double ntiles = 0;  // tiles the samples stand for
+    for (const ModelTile &m : tiles)
+        ntiles += m.weight / m.n;
+    const int bc = Tuner::class_of((uint64_t)std::min(total / std::max(ntiles, 1.0), 1.8e19));
     table[bc].cfg.s1 = BLOCKS[whole.block];
     table[bc].cfg.sort_bits = whole.sort_bits;
     table[bc].ms = whole.ns * 1e-6;

@@ -26,7 +26,7 @@ from fastowm_amd import owm, synth
 
 LEAVES = (16, 32, 64, 128)
 BLOCKS = ((8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1), (16, 16, 4, 0), (32, 16, 4, 0))
-SORTS = (32, 24, 16)
+SORTS = (32, 24, 16, 8)
 SET_POINTS = 8 << 20  # points per homogeneous set of small tiles
 
 

@@ -42,7 +42,7 @@ import numpy as np
 
 LEAVES = (16, 32, 64, 128)
 BLOCKS = ((8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1), (16, 16, 4, 0), (32, 16, 4, 0))
-SORTS = (32, 24, 16)
+SORTS = (32, 24, 16, 8)
 CU = 256
 SLOTS = dict(zip(BLOCKS, (4, 4, 4, 3, 2, 1)))  # workgroups per compute unit
 N_COEF = 4 + len(BLOCKS) + 3

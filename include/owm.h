@@ -270,14 +270,15 @@ owm_status owm_batch_run_device(owm_ctx *ctx, const double *d_x, const double *d
  * tuned by sweep scripts over maxNumber).  Point-count classes: up to 2^16,
  * 2^18, 2^20, 2^22, 2^24 and beyond.  A tile whose params.leaf_size is 0
  * takes the leaf size of its own class; the stage-1 window block and the sort
- * key width come from the class of the batch's total point count. */
+ * key width come from the class of the launch's mean tile size (its points
+ * over its tiles). */
 #define OWM_TUNE_CLASSES 6u
 typedef struct owm_tune_entry {
     uint64_t max_n;      /* class bound: point counts up to max_n */
     uint32_t leaf_size;  /* points per leaf (maxNumber) */
     uint32_t block_x, block_y, block_k; /* stage-1 window block, points per lane per round */
     uint32_t summaries;  /* 1: stage 1 reads per-leaf summaries of equal window-edge classes */
-    uint32_t sort_bits;  /* Morton bits of the per-tile sort key: 16, 24 or 32 (2-4 passes) */
+    uint32_t sort_bits;  /* Morton bits of the per-tile sort key: 8, 16, 24 or 32 (1-4 passes) */
     double ms;           /* time of the winning trial (owm_autotune) or the model's
                           * predicted time (owm_autotune_stats); 0 = default */
 } owm_tune_entry;

@@ -75,7 +75,7 @@ def test_autotune_keeps_results_and_table_round_trips():
         assert [e["max_n"] for e in table] == [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 2 ** 64 - 1]
         for e in table:
             assert e["leaf_size"] in (16, 32, 64, 128)
-            assert e["sort_bits"] in (16, 24, 32)
+            assert e["sort_bits"] in (8, 16, 24, 32)
             assert (e["block_x"], e["block_y"], e["block_k"], e["summaries"]) in (
                 (8, 8, 4, 0), (16, 8, 4, 0), (8, 8, 2, 0), (16, 8, 4, 1), (16, 16, 4, 0), (32, 16, 4, 0))
         # the classes that held tiles were timed
@@ -231,7 +231,7 @@ def test_stats_autotune_keeps_results_and_matches_the_model():
         table = ctx.autotune(x, y, z, desc, method="stats")
         assert len(table) == owm.TUNE_CLASSES
         for e in table:
-            assert e["leaf_size"] in (16, 32, 64, 128) and e["sort_bits"] in (16, 24, 32)
+            assert e["leaf_size"] in (16, 32, 64, 128) and e["sort_bits"] in (8, 16, 24, 32)
         assert all(table[c]["ms"] > 0 for c in (0, 1, 2, 3))
         say("device inputs")
         dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
