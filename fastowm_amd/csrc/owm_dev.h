@@ -1,7 +1,7 @@
 // Device-side data layout and helpers shared by the OWM kernels and the host
 // orchestration.  Layout in HBM (all tiles of a batch concatenated):
 //   points    x, y, z            fp64 SoA, caller order, tile t = [pt_off, pt_off+n)
-//   sorted    xy (double2) + PointZ {z, id, key} (16 B each), Morton order
+//   sorted    xy (double2) + PointZ {z, id, aux} (16 B each), Morton order
 //   leaves    box (minx,miny,maxx,maxy) f64x4, cnt, start, npts, zkey (u64), zid,
 //             parent, key0 (Morton key of the first point)
 //   internal  box f64x4, cnt, zkey, zid, child (u32x2, bit 31 = leaf), parent, flag
@@ -14,9 +14,12 @@
 #define OWM_LEAF_BIT 0x80000000u
 #define OWM_ZKEY_NONE 0xFFFFFFFFFFFFFFFFull
 
-// One point of the sorted cloud: coordinates, batch index and sort key (the
-// top key_bits of its Morton code within its tile).  Stored as two 16-byte
-// streams, (x, y) and PointZ (z, id, key): every access is one 16-byte load
+// One point of the sorted cloud: coordinates, batch index and one more word
+// (aux).  Between sort passes aux is the point's sort key (the top key_bits
+// of its Morton code within its tile); in a tree's final order it is the
+// key, or, when the tile's gcls is set, the point's global window-edge
+// classes for the query planned with the tree (owm_gstamp).  Stored as two
+// 16-byte streams, (x, y) and PointZ (z, id, aux): every access is one 16-byte load
 // or store per lane, coalesced when lanes take consecutive points, and the
 // tree sort's digit runs are written at about the speed of a contiguous copy
 // (four SoA streams of 8- and 4-byte fields written the same way are about a
@@ -24,11 +27,11 @@
 // fetched span per load).
 struct PointRec {
     double x, y, z;
-    uint32_t id, key;
+    uint32_t id, aux;
 };
 struct __align__(16) PointZ {
     double z;
-    uint32_t id, key;
+    uint32_t id, aux;
 };
 struct PointsDev {
     const double2 *xy;
@@ -49,7 +52,11 @@ struct TileDev {
     uint32_t node_off, leaf_size;
     uint32_t root;      // child reference of the root (OWM_LEAF_BIT | leaf) or internal
     uint32_t grp_off;   // first leaf group (64/leaf_size leaves per wave) of the tile
-    uint32_t pad_t;
+    // 1: the final sort pass writes each point's global window-edge classes
+    // (for this tile's query fields) into aux and stage 1 reads them instead
+    // of x and y; only set when the tree is built in the same call as the
+    // query and the classes fit 16 bits per axis
+    uint32_t gcls;
     double qminx, qminy, qscale;  // quantisation: q = (x - qminx) * qscale in [0, 2^32)
     // query (stages 1-3)
     uint32_t ncols, nrows, ncolsg, nrowsg;
@@ -60,6 +67,11 @@ struct TileDev {
     uint32_t lcs;  // stage 1 reads leaf cell summaries (else every leaf point by point)
     double displace, initx, inity, initxg, inityg;
     double inv_disp;  // 1 / displace (estimates only; exact compares follow)
+    // global window classes, per axis (x, y): the first left edge and the
+    // right edges' offset in steps ((R_0 - L_0) / displace, rounded); the
+    // tolerance 2^-46 M / displace for the larger axis bound
+    // M = max(|init|, |first left edge|, |last right edge|) + half side
+    double ge0[2], gc[2], gtol;
 };
 
 // Offsets of every per-tile range, one array per kind, for binary searches.
@@ -144,6 +156,87 @@ __device__ __forceinline__ double owm_wcenter(double init, double disp, uint32_t
     return __dadd_rn(init, __dmul_rn((double)(int)i, disp));
 }
 
+// Global window-edge classes.  Over the multiset of all left edges L_i and
+// right edges R_i (i < n) of a tile's windows along one axis,
+// G(x) = #{e < x} + #{e <= x}.  For every edge e of the multiset,
+// e < x <=> G(e) < G(x) and e <= x <=> G(e) <= G(x), so a window block's
+// class of x (the same count over the block's own edges, a subset) and
+// "strictly inside the block" are functions of G(x) alone.  G(NaN) = 0.
+__device__ __forceinline__ double owm_edge_at(double init, double disp, double hd, int i, bool right)
+{
+    const double c = owm_wcenter(init, disp, (uint32_t)i);
+    return right ? __dadd_rn(c, hd) : __dsub_rn(c, hd);
+}
+
+// 2 #{edges of one side < x} + [x is such an edge]; the edges of one side
+// are strictly increasing in the window index.  The estimate from the step
+// is exact but for rounding near an edge, so two exact edge compares confirm
+// it and a walk fixes it otherwise.
+__device__ __forceinline__ uint32_t owm_gside(double x, double init, double disp, double inv,
+                                              double hd, bool right, uint32_t n)
+{
+    const double f = (x - (right ? init + hd : init - hd)) * inv;
+    int k = f >= 0.0 ? (f < (double)n ? (int)f + 1 : (int)n) : 0;
+    const double elo = owm_edge_at(init, disp, hd, k > 0 ? k - 1 : 0, right);
+    double ehi = owm_edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
+    if (!((k == 0 || elo < x) && (k == (int)n || !(ehi < x)))) {
+        while (k > 0 && !(owm_edge_at(init, disp, hd, k - 1, right) < x))
+            --k;
+        while (k < (int)n && owm_edge_at(init, disp, hd, k, right) < x)
+            ++k;
+        ehi = owm_edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
+    }
+    return 2u * (uint32_t)k + ((k < (int)n && ehi == x) ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t owm_gclass(double x, double init, double disp, double inv,
+                                               double hd, uint32_t n)
+{
+    return owm_gside(x, init, disp, inv, hd, false, n) + owm_gside(x, init, disp, inv, hd, true, n);
+}
+
+// One axis of owm_gclass by arithmetic, when that is exact.  The edges of one
+// side are E_i = fl(fl(init + fl(i*disp)) +- hd), so with M bounding |init|
+// and every edge, E_i = E_0 + i*disp + e_i with |e_i| <= 2^-51 M, and for
+// |x| <= M, fl(fl(x - E_0) * inv) is within 3 * 2^-52 M / disp of
+// (x - E_0) / disp; the right side's estimate f_L - c (c = (R_0 - L_0) * inv,
+// rounded) adds at most 2^-50 M / disp.  When an estimate is farther than
+// tol = 2^-46 M / disp from every integer (four times the sum of the bounds),
+// x lies strictly between E_floor(f) and E_floor(f)+1 (or beyond the first or
+// last edge), so that side contributes 2 clamp(floor(f) + 1, 0, n).  For
+// |x| > M, x lies beyond every edge, at least half a window step beyond the
+// outermost (M exceeds it by the half side), so the estimate clamps to the
+// exact 0 or n.  Returns false where the estimate is too close to an integer
+// (and for NaN).  e0 = L_0, c and tol are the tile's constants.
+__device__ __forceinline__ bool owm_gaxis_fast(double x, double e0, double c, double tol, double inv,
+                                               uint32_t n, uint32_t &g)
+{
+    const double fl = (x - e0) * inv, fr = fl - c;
+    const double il = floor(fl), ir = floor(fr);
+    const double dl = fl - il, dr = fr - ir;
+    const double nn = (double)n;
+    g = 2u * ((uint32_t)fmin(fmax(il + 1.0, 0.0), nn) + (uint32_t)fmin(fmax(ir + 1.0, 0.0), nn));
+    return fmin(dl, dr) > tol && fmax(dl, dr) < 1.0 - tol;
+}
+
+// Most windows per axis for which a tile's classes are stamped (G <= 4n).
+#define OWM_GCLS_MAX_WINDOWS 16383u
+
+// (gy << 16 | gx) of a point for a tile's query fields (at most
+// OWM_GCLS_MAX_WINDOWS windows per axis, so that G <= 4n <= 0xFFFC fits 16
+// bits), or OWM_GCLS_NONE where the arithmetic of owm_gaxis_fast is not exact
+// (the reader then classifies the point by its coordinates).  The tile's g*
+// fields hold the constants per axis (set with its query fields).
+#define OWM_GCLS_NONE 0xFFFFFFFFu
+template <class TD>
+__device__ __forceinline__ uint32_t owm_gstamp(const TD &T, double x, double y)
+{
+    uint32_t gx, gy;
+    const bool ok = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx) &
+                    owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
+    return ok ? gy << 16 | gx : OWM_GCLS_NONE;
+}
+
 // Stage-3 cell centre: initX + ii*Bsize with the integer product.
 __device__ __forceinline__ double owm_gcenter(double init, uint32_t bsize, uint32_t i)
 {
@@ -212,7 +305,7 @@ __device__ __forceinline__ PointRec owm_load_pt(const PointsDev &P, uint32_t i)
     r.y = xy.y;
     r.z = __hiloint2double((int)b.y, (int)b.x);
     r.id = b.z;
-    r.key = b.w;
+    r.aux = b.w;
     return r;
 }
 

@@ -287,6 +287,36 @@ void fill_query(TileDev &T, const owm_params &P, const Derived &D)
     T.inity = D.inity;
     T.initxg = D.initxg;
     T.inityg = D.inityg;
+    T.gcls = 0;
+    // the constants of the global window classes (the device's edge formula:
+    // centre init + i*displace, then -+ the half side, each step rounded)
+    const double hd = (double)(float)D.h;
+    auto edge = [&](double init, uint32_t i, bool right) {
+        const double c = init + (double)(int)i * D.displace;
+        return right ? c + hd : c - hd;
+    };
+    const double init[2] = {D.initx, D.inity};
+    const uint32_t n[2] = {D.ncols, D.nrows};
+    double m = 0.0;
+    for (int a = 0; a < 2; ++a) {
+        T.ge0[a] = edge(init[a], 0, false);
+        T.gc[a] = (edge(init[a], 0, true) - T.ge0[a]) * T.inv_disp;
+        const double last = n[a] ? edge(init[a], n[a] - 1, true) : 0.0;
+        m = std::max(m, std::max(std::fabs(init[a]), std::max(std::fabs(T.ge0[a]), std::fabs(last))) + hd);
+    }
+    T.gtol = std::ldexp(m, -46) * T.inv_disp;
+}
+
+// For a tree built in the same chain as its query: the last sort pass stamps
+// every point's global window classes wherever they fit 16 bits per axis.
+static void stamp_classes(Prepared &P)
+{
+    P.L.stamp = false;
+    for (TileDev &T : P.td) {
+        T.gcls = T.ncols != 0 && T.nrows != 0 && T.ncols <= OWM_GCLS_MAX_WINDOWS &&
+                 T.nrows <= OWM_GCLS_MAX_WINDOWS;
+        P.L.stamp = P.L.stamp || T.gcls;
+    }
 }
 
 int bit_length(uint64_t v)
@@ -312,6 +342,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
     // The tree sort is segmented by tile, so the keys hold Morton bits only:
     // sort_bits of them (16, 24 or 32: two to four 8-bit passes).
     L.tile_bits = 0;
+    L.stamp = false;  // set by the callers that plan the query in the same chain
     L.key_bits = std::max(std::min(sort_bits, 32) & ~1, 4);
     L.pt_off.assign(T + 1, 0);
     L.leaf_off.assign(T + 1, 0);
@@ -592,7 +623,8 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         const PointsOut out = last ? T.pts : rec[p & 1];
         launch_sort_records(s, B, in, out, last ? T.pos : nullptr, (uint32_t)L.np, 8 * p,
                             L.tile_bits + L.key_bits, hist + 256 * p,
-                            lookback + (uint64_t)p * tiles * 256, counters + p, p == 0, seg, tiles);
+                            lookback + (uint64_t)p * tiles * 256, counters + p, p == 0,
+                            last && L.stamp, seg, tiles);
         // algorithmic bytes per point and pass: a 32-byte record written, and
         // read (the first pass reads x, y and z, and numbers the points)
         sort_bytes += (p == 0 ? 56ull : 64ull) * L.np + (last && T.pos ? 4ull * L.np : 0ull);
@@ -1286,6 +1318,7 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     st = prepare_query(ctx, &tiles[0].params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
+    stamp_classes(P);
     bool graph = (flags & OWM_B_GRAPH) != 0;
     const bool auto_graph = (flags & B_AUTO_GRAPH) != 0 && !graph;
     for (int attempt = 0;; ++attempt) {
@@ -1374,6 +1407,7 @@ owm_status run_tile_fixed(owm_ctx *ctx, const double *dx, const double *dy, cons
     st = prepare_query(ctx, &tile.params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
+    stamp_classes(P);
     for (int attempt = 0;; ++attempt) {
         const uint64_t list_bytes = align_up(std::max<uint64_t>(P.Q.s1_cap, 1) * 4, 256);
         *need = 256 + table_bytes(1) + tree_bytes(P.L) + build_scratch_bytes(P.L) +

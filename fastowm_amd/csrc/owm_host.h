@@ -57,6 +57,8 @@ struct TreeLayout {
     // key digit counts already taken in the bbox pass (device, the context's
     // small arena), or null: the build takes them itself
     const uint32_t *counts = nullptr;
+    // the last pass stamps window classes (some tile's gcls is set)
+    bool stamp = false;
 };
 
 struct QueryLayout {

@@ -348,9 +348,14 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // output bases by decoupled look-back, and then writes the tile out STAGE
 // slots at a time: the stage's points go to LDS in slot (digit) order, then
 // consecutive lanes write consecutive 16-byte (x, y) pairs, then (z, id,
-// key) triples, so each digit's run leaves as two contiguous streams.  POS:
-// pos[id] = the point's output position.
-template <bool FIRST, bool POS, int NW, int ITEMS, int STAGE>
+// aux) triples, so each digit's run leaves as two contiguous streams.
+// MODE REC_POS: pos[id] = the point's output position.  MODE REC_STAMP (the
+// last pass of a tree built with its query): in tiles with gcls set, aux
+// leaves as the point's global window-edge classes (owm_gstamp, computed once
+// the keys are ranked) instead of its key, and the digits of the staged
+// points are kept beside them.
+constexpr int REC_PLAIN = 0, REC_POS = 1, REC_STAMP = 2;
+template <bool FIRST, int MODE, int NW, int ITEMS, int STAGE>
 __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut out,
                                                           uint32_t *__restrict__ pos,
                                                           const TileDev *__restrict__ tiles,
@@ -361,9 +366,12 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                                                           SortSegs seg)
 {
     constexpr int TILE_T = NW * 64 * ITEMS;
+    constexpr bool POS = MODE == REC_POS, STAMP = MODE == REC_STAMP;
     static_assert(TILE_T % STAGE == 0, "a sort tile is a whole number of stages");
+    static_assert(TILE_T <= 65536, "a slot and its digit share a word");
     __shared__ SweepLds<NW> L;
     __shared__ uint4 s_xy[STAGE], s_zi[STAGE];  // a stage of points in slot order
+    __shared__ uint8_t s_dig[STAMP ? STAGE : 1];  // their digits (REC_STAMP)
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t p0 = w * (64 * ITEMS) + lane;  // tile position of the thread's item 0
     uint4 *const oxy = reinterpret_cast<uint4 *>(out.xy);
@@ -384,8 +392,7 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                 const uint32_t i = base + min(p0 + k * 64, cnt - 1);
                 x[k] = in.x[i];
                 y[k] = in.y[i];
-                z[k] = in.z[i];
-                id[k] = i;
+                z[k] = in.z[i];  // the point's index i is recomputed when staged
             }
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k)
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                 y[k] = r.y;
                 z[k] = r.z;
                 id[k] = r.id;
-                key[k] = r.key;
+                key[k] = r.aux;
             }
         }
         uint32_t slot[ITEMS];
@@ -410,27 +417,45 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
             const uint32_t d = (key[k] >> shift) & (RADIX - 1);
             // items past the tile's end take the unused slots past cnt
             slot[k] = p0 + k * 64 < cnt ? L.excl[d] + L.whist[w][d] + slot[k] : p0 + k * 64;
+            if (STAMP)
+                slot[k] |= d << 16;
+        }
+        if constexpr (STAMP) {
+            // the keys are spent: the classes take their place
+            const TileDev &TS = tiles[__builtin_amdgcn_readfirstlane(t)];
+            if (TS.gcls) {
+                // one item at a time (scheduling barriers): sixteen
+                // interleaved computations would spill registers
+#pragma unroll
+                for (int k = 0; k < ITEMS; ++k) {
+                    key[k] = owm_gstamp(TS, x[k], y[k]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
         }
         for (uint32_t lo = 0; lo < cnt; lo += STAGE) {
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
-                const uint32_t r = slot[k] - lo;
+                const uint32_t r = (STAMP ? slot[k] & 0xFFFFu : slot[k]) - lo;
                 if (r < (uint32_t)STAGE) {
                     const unsigned long long bx = (unsigned long long)__double_as_longlong(x[k]);
                     const unsigned long long by = (unsigned long long)__double_as_longlong(y[k]);
                     const unsigned long long bz = (unsigned long long)__double_as_longlong(z[k]);
                     s_xy[r] = make_uint4((uint32_t)bx, (uint32_t)(bx >> 32), (uint32_t)by,
                                          (uint32_t)(by >> 32));
-                    s_zi[r] = make_uint4((uint32_t)bz, (uint32_t)(bz >> 32), id[k], key[k]);
+                    const uint32_t pid = FIRST ? base + min(p0 + k * 64, cnt - 1) : id[k];
+                    s_zi[r] = make_uint4((uint32_t)bz, (uint32_t)(bz >> 32), pid, key[k]);
+                    if (STAMP)
+                        s_dig[r] = (uint8_t)(slot[k] >> 16);
                 }
             }
             __syncthreads();
-            // first the stage's (x, y) pairs, then its (z, id, key) triples
+            // first the stage's (x, y) pairs, then its (z, id, aux) triples
             const uint32_t m = min((uint32_t)STAGE, cnt - lo);
             for (uint32_t c = tid; c < 2 * m; c += NW * 64) {
                 const bool zpart = c >= m;
                 const uint32_t r = zpart ? c - m : c;
-                const uint32_t d = (s_zi[r].w >> shift) & (RADIX - 1);
+                const uint32_t d = STAMP ? (uint32_t)s_dig[r] : (s_zi[r].w >> shift) & (RADIX - 1);
                 const uint32_t dst = L.glob[d] + lo + r;
                 if (zpart) {
                     const uint4 v = s_zi[r];
@@ -576,36 +601,44 @@ template <int NW, int ITEMS, int STAGE>
 static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
                                uint32_t *pos, uint32_t n, int shift, uint32_t key_shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                               bool first, const SortSegs &seg, uint32_t grid)
+                               bool first, bool stamp, const SortSegs &seg, uint32_t grid)
 {
     const dim3 b(NW * 64);
-#define OWM_REC(FIRST, POS)                                                                        \
-    hipLaunchKernelGGL((k_sort_records<FIRST, POS, NW, ITEMS, STAGE>), dim3(grid), b, 0, s, in, out, \
+    const int mode = pos ? REC_POS : (stamp ? REC_STAMP : REC_PLAIN);
+#define OWM_REC(FIRST, MODE)                                                                        \
+    hipLaunchKernelGGL((k_sort_records<FIRST, MODE, NW, ITEMS, STAGE>), dim3(grid), b, 0, s, in, out, \
                        pos, B.tiles, key_shift, n, shift, digit_off, lookback, tile_counter, seg)
-    if (first && pos)
-        OWM_REC(true, true);
-    else if (first)
-        OWM_REC(true, false);
-    else if (pos)
-        OWM_REC(false, true);
-    else
-        OWM_REC(false, false);
+    if (first) {
+        if (mode == REC_POS)
+            OWM_REC(true, REC_POS);
+        else if (mode == REC_STAMP)
+            OWM_REC(true, REC_STAMP);
+        else
+            OWM_REC(true, REC_PLAIN);
+    } else {
+        if (mode == REC_POS)
+            OWM_REC(false, REC_POS);
+        else if (mode == REC_STAMP)
+            OWM_REC(false, REC_STAMP);
+        else
+            OWM_REC(false, REC_PLAIN);
+    }
 #undef OWM_REC
 }
 
 void launch_sort_records(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
                          uint32_t *pos, uint32_t n, int shift, int key_bits,
                          const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool first, const SortSegs &seg, uint32_t seg_tiles)
+                         bool first, bool stamp, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
     const uint32_t key_shift = 64u - (uint32_t)key_bits;
     if (big_records(n, seg.nseg))
         sort_records_shape<8, 16, 4096>(s, B, in, out, pos, n, shift, key_shift, digit_off, lookback,
-                                        tile_counter, first, seg, grid);
+                                        tile_counter, first, stamp, seg, grid);
     else
         sort_records_shape<8, 8, 4096>(s, B, in, out, pos, n, shift, key_shift, digit_off, lookback,
-                                       tile_counter, first, seg, grid);
+                                       tile_counter, first, stamp, seg, grid);
 }
 
 }  // namespace owm

@@ -26,6 +26,15 @@ constexpr int S1_THREADS = 256;
 constexpr int S1_CHUNK = S1_THREADS;  // leaves whose metadata is staged at once
 constexpr uint32_t S1_LEAFTAG = 0x80000000u;
 constexpr uint32_t S1_POINT = 0xFFFFFFFEu;  // item is a point, classified after the loads
+constexpr uint32_t S1_GPOINT = 0xFFFFFFFDu; // item is a point with stamped classes
+// Entries of a block's class tables (global class -> the block's class, per
+// axis); blocks whose class range is longer (overlaps above ~0.99) read
+// coordinates instead of stamped classes.
+constexpr uint32_t S1_GT = 512;
+template <bool B>
+struct S1Tag {
+    static constexpr bool value = B;
+};
 
 // #{k : E[k] < x} and #{k : E[k] <= x} over a strictly increasing edge list
 // E[0..n) spaced ~1/inv apart: estimate, then correct with exact compares.
@@ -195,50 +204,12 @@ __global__ __launch_bounds__(256) void k_s1_bin(BatchDev B, TreeDev T, uint32_t 
 
 // ---------------------------------------------------- leaf cell summaries --
 //
-// Global edge classes: over the multiset of all left edges L_i and right
-// edges R_i of a tile's windows (one axis), G(x) = #{e < x} + #{e <= x}.
-// Points with equal G lie in exactly the same windows along that axis, and
-// for every edge e of the multiset, e < x <=> G(e) < G(x) and e <= x <=>
-// G(e) <= G(x).  A window block's local class of x (the same count over the
-// block's own edges, a subset) is therefore a function of G(x) alone, and so
-// is "strictly inside the block" (G(BL) < G(x) < G(BR)).  A leaf's points
-// with equal (Gx, Gy) can thus be binned as one aggregate (count, lowest
-// (z, index)) by every block; k_lcs builds one summary per run of equal
-// classes in the leaf's Morton order (up to LCS_SLOTS), and stage 1 reads
-// them instead of the points (leaves with more runs stay point by point).
-
-__device__ __forceinline__ double edge_at(double init, double disp, double hd, int i, bool right)
-{
-    const double c = owm_wcenter(init, disp, (uint32_t)i);
-    return right ? __dadd_rn(c, hd) : __dsub_rn(c, hd);
-}
-
-// 2 #{edges of one side < x} + [x is such an edge]; the edges of one side
-// are strictly increasing in the window index.  The estimate from the step
-// is exact but for rounding near an edge, so two exact edge compares confirm
-// it and a walk fixes it otherwise.
-__device__ __forceinline__ uint32_t gside(double x, double init, double disp, double inv, double hd,
-                                          bool right, uint32_t n)
-{
-    const double f = (x - (right ? init + hd : init - hd)) * inv;
-    int k = f >= 0.0 ? (f < (double)n ? (int)f + 1 : (int)n) : 0;
-    const double elo = edge_at(init, disp, hd, k > 0 ? k - 1 : 0, right);
-    double ehi = edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
-    if (!((k == 0 || elo < x) && (k == (int)n || !(ehi < x)))) {
-        while (k > 0 && !(edge_at(init, disp, hd, k - 1, right) < x))
-            --k;
-        while (k < (int)n && edge_at(init, disp, hd, k, right) < x)
-            ++k;
-        ehi = edge_at(init, disp, hd, k < (int)n ? k : (int)n - 1, right);
-    }
-    return 2u * (uint32_t)k + ((k < (int)n && ehi == x) ? 1u : 0u);
-}
-
-__device__ __forceinline__ uint32_t gclass(double x, double init, double disp, double inv, double hd,
-                                           uint32_t n)
-{
-    return gside(x, init, disp, inv, hd, false, n) + gside(x, init, disp, inv, hd, true, n);
-}
+// Global edge classes (owm_gclass): points with equal (Gx, Gy) lie in exactly
+// the same windows, so a leaf's points with equal classes can be binned as
+// one aggregate (count, lowest (z, index)) by every block; k_lcs builds one
+// summary per run of equal classes in the leaf's Morton order (up to
+// LCS_SLOTS), and stage 1 reads them instead of the points (leaves with more
+// runs stay point by point).
 
 // #{a[k] < g} + #{a[k] <= g} over a strictly increasing array of n classes
 __device__ __forceinline__ uint32_t icnt2(const uint32_t *a, uint32_t n, uint32_t g)
@@ -276,12 +247,12 @@ __global__ __launch_bounds__(256) void k_lcs(BatchDev B, TreeDev T, LeafCells LC
             const double x = r.x, y = r.y, z = r.z;
             id = r.id;
             // points outside the outermost edges lie in no window
-            if (x > edge_at(TD.initx, disp, hd, 0, false) &&
-                x < edge_at(TD.initx, disp, hd, TD.ncols - 1, true) &&
-                y > edge_at(TD.inity, disp, hd, 0, false) &&
-                y < edge_at(TD.inity, disp, hd, TD.nrows - 1, true)) {
-                const uint32_t gx = gclass(x, TD.initx, disp, inv, hd, TD.ncols);
-                const uint32_t gy = gclass(y, TD.inity, disp, inv, hd, TD.nrows);
+            if (x > owm_edge_at(TD.initx, disp, hd, 0, false) &&
+                x < owm_edge_at(TD.initx, disp, hd, TD.ncols - 1, true) &&
+                y > owm_edge_at(TD.inity, disp, hd, 0, false) &&
+                y < owm_edge_at(TD.inity, disp, hd, TD.nrows - 1, true)) {
+                const uint32_t gx = owm_gclass(x, TD.initx, disp, inv, hd, TD.ncols);
+                const uint32_t gy = owm_gclass(y, TD.inity, disp, inv, hd, TD.nrows);
                 key = (gy << 16) | gx;
                 cnt = 1;
                 zk = owm_eligible(z) ? owm_zkey(z) : OWM_ZKEY_NONE;
@@ -358,6 +329,8 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
     __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
     __shared__ uint32_t sGLx[BX], sGRx[BX], sGLy[BY], sGRy[BY];  // global classes of the edges
     __shared__ uint32_t sCxl[BX], sCxh[BX], sCyl[BY], sCyh[BY];
+    // the block's class of global class GBL + 1 + i (x) and GBB + 1 + i (y)
+    __shared__ uint8_t sTx[S1_GT], sTy[S1_GT];
     __shared__ uint32_t sCnt[S::NCELL];
     __shared__ unsigned long long sZ[S::NCELL];
     __shared__ uint32_t sId[S::NCELL];
@@ -381,23 +354,25 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
         const double hd = (double)(float)TD.h;
         const double inv = 1.0 / TD.displace;
         const bool lcs = USE_LCS && TD.lcs != 0;
+        // the tree's records carry each point's global classes for these windows
+        const bool gcls = TD.gcls != 0;
 
         if (tid < ni) {
             double c = owm_wcenter(TD.initx, TD.displace, i0 + tid);
             sLx[tid] = __dsub_rn(c, hd);
             sRx[tid] = __dadd_rn(c, hd);
-            if (lcs) {
-                sGLx[tid] = gclass(sLx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
-                sGRx[tid] = gclass(sRx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
+            if (lcs || gcls) {
+                sGLx[tid] = owm_gclass(sLx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
+                sGRx[tid] = owm_gclass(sRx[tid], TD.initx, TD.displace, TD.inv_disp, hd, TD.ncols);
             }
         }
         if (tid >= 64 && tid < 64 + nj) {
             double c = owm_wcenter(TD.inity, TD.displace, j0 + tid - 64);
             sLy[tid - 64] = __dsub_rn(c, hd);
             sRy[tid - 64] = __dadd_rn(c, hd);
-            if (lcs) {
-                sGLy[tid - 64] = gclass(sLy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
-                sGRy[tid - 64] = gclass(sRy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
+            if (lcs || gcls) {
+                sGLy[tid - 64] = owm_gclass(sLy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
+                sGRy[tid - 64] = owm_gclass(sRy[tid - 64], TD.inity, TD.displace, TD.inv_disp, hd, TD.nrows);
             }
         }
         for (int c = tid; c < S::NCELL; c += NT) {
@@ -420,6 +395,16 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
         const AxisEdges AX = axis_edges(sLx, sRx, ni, TD.initx, hd, inv);
         const AxisEdges AY = axis_edges(sLy, sRy, nj, TD.inity, hd, inv);
         const uint32_t GBL = sGLx[0], GBR = sGRx[ni - 1], GBB = sGLy[0], GBT = sGRy[nj - 1];
+        // points are read by their stamped classes when the block's class
+        // tables (global class strictly inside the block -> block class) fit;
+        // the tables are complete after the barrier that opens every leaf chunk
+        const bool gc = gcls && GBR - GBL - 1 <= S1_GT && GBT - GBB - 1 <= S1_GT;
+        if (gc) {
+            for (uint32_t i = tid; i < GBR - GBL - 1; i += NT)
+                sTx[i] = (uint8_t)(icnt2(sGLx, ni, GBL + 1 + i) + icnt2(sGRx, ni, GBL + 1 + i));
+            for (uint32_t i = tid; i < GBT - GBB - 1; i += NT)
+                sTy[i] = (uint8_t)(icnt2(sGLy, nj, GBB + 1 + i) + icnt2(sGRy, nj, GBB + 1 + i));
+        }
 
         const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
         const uint32_t nl = s1 - s0;
@@ -430,6 +415,12 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
         const uint32_t sub = lane / ls, off = lane - sub * ls;
         const uint32_t esub = lane / LCS_SLOTS, eslot = lane - esub * LCS_SLOTS;
 
+        // the leaf chunks, compiled once for stamped classes and once for
+        // coordinates, so that each keeps its own registers
+        auto chunks = [&](auto gc_tag) {
+        constexpr bool GC = decltype(gc_tag)::value;
+        // a stamped point takes one load and fewer registers: twice the items
+        constexpr int KK = GC ? 2 * S1_K : S1_K;
         for (uint32_t ch = 0; ch < nl; ch += S1_CHUNK) {
             const uint32_t m = min((uint32_t)S1_CHUNK, nl - ch);
             // the class ranges (first chunk) and the previous chunk's rounds
@@ -470,19 +461,125 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
             const uint32_t mraw = sNraw, ment = sNent;
             const uint32_t neu = (ment + S1_ENT_LEAVES - 1) / S1_ENT_LEAVES;
             const uint32_t nunits = neu + (mraw + nl_grp - 1) / nl_grp * nseg;
-            const uint32_t rounds = (nunits + NWV * S1_K - 1) / (NWV * S1_K);
+            const uint32_t rounds = (nunits + NWV * KK - 1) / (NWV * KK);
 
+            if constexpr (GC) {
             for (uint32_t r = 0; r < rounds; ++r) {
-                uint32_t cell[S1_K], idv[S1_K];
-                unsigned long long zk[S1_K];
+                // points: z (low, high word), index, classes; aggregates: the
+                // z-key (low, high word), index, count.  After binning every
+                // item holds its z-key and index.
+                uint32_t cell[KK];
+                uint4 pr[KK];
                 uint32_t dropped = 0;  // bit k: item k lowered its cell's minimum
-                double xv[S1_K], yv[S1_K], zv[S1_K];
-                uint32_t cn[S1_K];
                 // issue every load of the round before using any of them
 #pragma unroll
-                for (int k = 0; k < S1_K; ++k) {
+                for (int k = 0; k < KK; ++k) {
                     cell[k] = OWM_NONE;
-                    const uint32_t u = (r * S1_K + k) * NWV + wave;
+                    const uint32_t u = (r * KK + k) * NWV + wave;
+                    if (u >= nunits)
+                        continue;
+                    if (USE_LCS && u < neu) {
+                        const uint32_t li = u * S1_ENT_LEAVES + esub;
+                        if (li >= ment)
+                            continue;
+                        const uint32_t v = sEnt[li];
+                        if (eslot >= (v & 31u))
+                            continue;
+                        const size_t e = (size_t)(v >> 5) * LCS_SLOTS + eslot;
+                        const unsigned long long kc = LC.kc[e], z = LC.zk[e];
+                        pr[k] = make_uint4((uint32_t)z, (uint32_t)(z >> 32), LC.id[e], (uint32_t)(kc >> 32));
+                        cell[k] = S1_LEAFTAG | (uint32_t)kc;  // classes, mapped below
+                        continue;
+                    }
+                    const uint32_t uu = u - neu;
+                    const uint32_t g = uu / nseg, seg = uu - g * nseg;
+                    const uint32_t li = g * nl_grp + sub;
+                    if (sub >= nl_grp || li >= mraw)
+                        continue;
+                    const uint32_t a = sLfA[li], n = sLfB[li];
+                    if (n & S1_LEAFTAG) {
+                        if (seg * 64 + off == 0) {
+                            const unsigned long long z = T.lf_zkey[a];
+                            pr[k] = make_uint4((uint32_t)z, (uint32_t)(z >> 32), T.lf_zid[a], T.lf_cnt[a]);
+                            cell[k] = n & ~S1_LEAFTAG;
+                        }
+                    } else if (seg * 64 + off < n) {
+                        // z, index and classes in one 16-byte load
+                        pr[k] = *reinterpret_cast<const uint4 *>(T.pts.zi + a + seg * 64 + off);
+                        cell[k] = S1_GPOINT;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < KK; ++k) {
+                    if (cell[k] == OWM_NONE)
+                        continue;
+                    uint32_t cn = pr[k].w;
+                    unsigned long long zk = (unsigned long long)pr[k].y << 32 | pr[k].x;
+                    if (cell[k] == S1_GPOINT) {
+                        if (__builtin_expect(pr[k].w == OWM_GCLS_NONE, 0)) {
+                            // not stamped (near an edge, or far out): by the coordinates
+                            const uint32_t uu = (r * KK + k) * NWV + wave - neu;
+                            const uint32_t g = uu / nseg, seg = uu - g * nseg;
+                            const double2 p = T.pts.xy[sLfA[g * nl_grp + sub] + seg * 64 + off];
+                            if (!(p.x > BL && p.x < BR && p.y > BB && p.y < BT)) {
+                                cell[k] = OWM_NONE;
+                                continue;
+                            }
+                            cell[k] = cls_f(AY, p.y) * NXC + cls_f(AX, p.x);
+                        } else {
+                            const uint32_t gx = pr[k].w & 0xFFFFu, gy = pr[k].w >> 16;
+                            if (!(gx > GBL && gx < GBR && gy > GBB && gy < GBT)) {
+                                cell[k] = OWM_NONE;
+                                continue;
+                            }
+                            cell[k] = (uint32_t)sTy[gy - GBB - 1] * NXC + sTx[gx - GBL - 1];
+                        }
+                        cn = 1;
+                        const double z = __hiloint2double((int)pr[k].y, (int)pr[k].x);
+                        zk = owm_eligible(z) ? owm_zkey(z) : OWM_ZKEY_NONE;
+                    } else if (USE_LCS && (cell[k] & S1_LEAFTAG)) {
+                        const uint32_t gx = cell[k] & 0xFFFFu, gy = (cell[k] >> 16) & 0x7FFFu;
+                        if (!(gx > GBL && gx < GBR && gy > GBB && gy < GBT)) {
+                            cell[k] = OWM_NONE;
+                            continue;
+                        }
+                        cell[k] = (icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy)) * NXC +
+                                  icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
+                    }
+                    const uint32_t c = cell[k];
+                    atomicAdd(&sCnt[c], cn);
+                    if (zk != OWM_ZKEY_NONE)
+                        dropped |= (atomicMin(&sZ[c], zk) > zk ? 1u : 0u) << k;
+                    else
+                        cell[k] = OWM_NONE;  // counted, never a minimum
+                    pr[k].x = (uint32_t)zk;
+                    pr[k].y = (uint32_t)(zk >> 32);
+                }
+                __syncthreads();
+                // the index phases of the coordinate path (below)
+#pragma unroll
+                for (int k = 0; k < KK; ++k)
+                    if (cell[k] != OWM_NONE && sZ[cell[k]] == ((unsigned long long)pr[k].y << 32 | pr[k].x) &&
+                        ((dropped >> k) & 1u))
+                        sId[cell[k]] = OWM_NONE;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < KK; ++k)
+                    if (cell[k] != OWM_NONE && sZ[cell[k]] == ((unsigned long long)pr[k].y << 32 | pr[k].x))
+                        atomicMin(&sId[cell[k]], pr[k].z);
+            }
+            } else {
+            for (uint32_t r = 0; r < rounds; ++r) {
+                uint32_t cell[KK], idv[KK];
+                unsigned long long zk[KK];
+                uint32_t dropped = 0;  // bit k: item k lowered its cell's minimum
+                double xv[KK], yv[KK], zv[KK];
+                uint32_t cn[KK];
+                // issue every load of the round before using any of them
+#pragma unroll
+                for (int k = 0; k < KK; ++k) {
+                    cell[k] = OWM_NONE;
+                    const uint32_t u = (r * KK + k) * NWV + wave;
                     if (u >= nunits)
                         continue;
                     if (USE_LCS && u < neu) {
@@ -524,7 +621,7 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < S1_K; ++k) {
+                for (int k = 0; k < KK; ++k) {
                     if (cell[k] == OWM_NONE)
                         continue;
                     if (cell[k] == S1_POINT) {
@@ -558,7 +655,7 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                 // one before its own update saw the drop (lanes that only tie
                 // an earlier minimum found it equal and keep the index).
 #pragma unroll
-                for (int k = 0; k < S1_K; ++k)
+                for (int k = 0; k < KK; ++k)
                     if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k] && ((dropped >> k) & 1u))
                         sId[cell[k]] = OWM_NONE;
                 __syncthreads();
@@ -567,11 +664,17 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                 // next round's reset discards anyway; the next round's resets
                 // follow a barrier that every update of this round precedes.
 #pragma unroll
-                for (int k = 0; k < S1_K; ++k)
+                for (int k = 0; k < KK; ++k)
                     if (cell[k] != OWM_NONE && sZ[cell[k]] == zk[k])
                         atomicMin(&sId[cell[k]], idv[k]);
             }
+            }
         }
+        };
+        if (gc)
+            chunks(S1Tag<true>{});
+        else
+            chunks(S1Tag<false>{});
         // the class ranges of a block without leaves, and the last round's
         // index updates, are complete before the window passes
         __syncthreads();

@@ -581,7 +581,7 @@ __device__ __forceinline__ V shfl_any(V v, int src)
 // leaf's first point and stores the leaf.
 template <int LL>
 __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint32_t g0,
-                                            uint32_t lane)
+                                            uint32_t lane, uint32_t shift)
 {
     constexpr int NK = LL <= 64 ? 64 / LL : 1;  // leaves of a lane's points
     constexpr int QL = 8 / NK;                  // a lane's points per leaf
@@ -590,7 +590,7 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
 #pragma unroll 1
     for (int r = 0; r < GPTS / 64; ++r) {
         const uint32_t e0 = (g0 - TD.grp_off) * GPTS + r * 512 + seg * 64;
-        uint32_t v[8], kq[8];
+        uint32_t v[8];
         double x[8], y[8], z[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -602,7 +602,6 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
             x[q] = r.x;
             y[q] = r.y;
             z[q] = r.z;
-            kq[q] = r.key;
         }
         LeafAgg a[NK];
 #pragma unroll
@@ -619,13 +618,16 @@ __device__ __forceinline__ void leaf_rounds(const TileDev &TD, TreeOut &T, uint3
                 leaf_agg_step<8>(a[k]);
         }
         LeafAgg mine = a[0];
-        uint32_t fk = kq[0];  // the key of the leaf's first point
+        double fx = x[0], fy = y[0];  // the leaf's first point
 #pragma unroll
         for (int k = 1; k < NK; ++k)
             if (sub == (uint32_t)k) {
                 mine = a[k];
-                fk = kq[k * QL];
+                fx = x[k * QL];
+                fy = y[k * QL];
             }
+        // its sort key (the records' aux word may hold window classes)
+        const uint32_t fk = sort_key(TD, fx, fy, shift);
         const uint32_t jl = e0 / LL + sub;
         const bool store = LL <= 64 ? sub < (uint32_t)NK : (sub == 0 && (seg & 1) == 0);
         if (store && jl < TD.nleaf)
@@ -647,19 +649,20 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (TB / 64);
+    const uint32_t shift = 64u - B.key_bits;
     for (uint32_t g0 = (blockIdx.x * (TB / 64) + wv) * LEAF_G; g0 < B.ngrp; g0 += nw * LEAF_G) {
         const uint32_t t = owm_find_tile(B.grp_off, B.ntiles, g0);
         const TileDev &TD = B.tiles[t];
         const uint32_t L = TD.leaf_size;
         if (g0 + LEAF_G <= B.grp_off[t + 1] && (L == 16 || L == 32 || L == 64 || L == 128)) {
             if (L == 16)
-                leaf_rounds<16>(TD, T, g0, lane);
+                leaf_rounds<16>(TD, T, g0, lane, shift);
             else if (L == 32)
-                leaf_rounds<32>(TD, T, g0, lane);
+                leaf_rounds<32>(TD, T, g0, lane, shift);
             else if (L == 64)
-                leaf_rounds<64>(TD, T, g0, lane);
+                leaf_rounds<64>(TD, T, g0, lane, shift);
             else
-                leaf_rounds<128>(TD, T, g0, lane);
+                leaf_rounds<128>(TD, T, g0, lane, shift);
             continue;
         }
         if (L <= 64 && (L & (L - 1)) == 0 && g0 + LEAF_G <= B.grp_off[t + 1]) {
@@ -678,7 +681,7 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
                 x[q] = r.x;
                 y[q] = r.y;
                 z[q] = r.z;
-                kq[q] = r.key;
+                kq[q] = sort_key(TD, r.x, r.y, shift);
             }
             // lane m ends with the aggregate of leaf jl0 + m when the groups hold
             // at most 64 leaves (leaf_size >= 4); smaller leaves are stored
@@ -738,7 +741,7 @@ __global__ __launch_bounds__(TB) void k_leaf_aggs(BatchDev B, TreeOut T)
             for (uint32_t k = off; k < npts; k += ls) {
                 const PointRec r = owm_load_pt(T.pts, start + k);
                 if (k == 0)
-                    k0 = r.key;
+                    k0 = sort_key(TG, r.x, r.y, shift);
                 leaf_agg_add(a, r.x, r.y, r.z, r.id);
             }
             seg_reduce(off, ls, [&](uint32_t o, bool take) {

@@ -356,6 +356,49 @@ def test_points_on_edges_large_blocks(ctx, restore_table, block, ws, bs, ov, lea
     tree.free()
 
 
+@pytest.mark.parametrize("block", [(16, 16, 4), (32, 16, 4)])
+@pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
+def test_stamped_classes_near_edges(ctx, restore_table, block, origin):
+    """Whole runs build the tree and plan the query in one chain, so the tree
+    sort stamps every point's global window classes and stage 1 bins by them;
+    points a few ulps from an edge are not stamped and are binned by their
+    coordinates.  Seeds against the oracle, with the larger window blocks."""
+    _stage1_table(ctx, "off", block)
+    for ws, bs, ov, leaf in ((10, 20, 0.9, 16), (7, 11, 0.5, 64), (12, 20, 0.7, 32)):
+        say("stamped, near edges", block, origin, ws, bs, ov, leaf)
+        x, y, z = _near_edge_cloud(30_000, seed=ws * 7 + leaf, side=200, origin=origin)
+        want, _, _ = O.run(x, y, z, ws, bs, ov, leaf)
+        assert np.array_equal(ctx.run_tile(x, y, z, ws, bs, ov, leaf), want)
+
+
+def test_stamped_classes_high_overlap_and_wide_grid(ctx, restore_table):
+    """Blocks whose class range exceeds the class tables (overlap 0.995 with
+    32x16 blocks) read coordinates; a tile with more than 16383 window columns
+    is not stamped at all.  Both in one batch with an ordinary tile."""
+    _stage1_table(ctx, "off", (32, 16, 4))
+    rng = np.random.default_rng(5)
+    clouds = []
+    # 2 m windows at overlap 0.995: a step of 1 cm, 200 steps a window
+    clouds.append(((rng.random(20_000) * 10.0, rng.random(20_000) * 10.0,
+                    np.round(rng.normal(50, 2, 20_000), 2)), (2, 10, 0.995)))
+    # Wsize 2 at overlap 0.9 over 3.5 km: 17500 window columns
+    n = 30_000
+    clouds.append(((rng.random(n) * 3500.0, rng.random(n) * 6.0, np.round(rng.normal(50, 2, n), 2)),
+                   (2, 10, 0.9)))
+    x, y, z = _tile(40_000, seed=9).generate()
+    clouds.append(((x, y, z), (12, 20, 0.8)))
+    wants = []
+    for (cx, cy, cz), (ws, bs, ov) in clouds:
+        say("stamp fallback tile", len(cx), ws, bs, ov)
+        wants.append(O.run(cx, cy, cz, ws, bs, ov, 32)[0])
+        assert np.array_equal(ctx.run_tile(cx, cy, cz, ws, bs, ov, 32), wants[-1])
+    desc = [(len(c[0][0]), dict(wsize=c[1][0], bsize=c[1][1], overlap=c[1][2], leaf_size=32))
+            for c in clouds]
+    got = ctx.batch_run(*(np.concatenate([c[0][k] for c in clouds]) for k in range(3)), desc)
+    for i, (g, w) in enumerate(zip(got, wants)):
+        assert np.array_equal(g, w), f"tile {i}"
+
+
 @pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
 def test_points_ulps_from_edges(ctx, restore_table, origin):
     """Stage 1 classifies by arithmetic away from edges and by exact compares
