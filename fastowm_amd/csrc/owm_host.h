@@ -102,11 +102,14 @@ struct Tuner {
     bool forced = false;  // a tuning trial overrides the table
     TuneCfg force{};
     Tuner() { reset(); }
+    // defaults: tiles of up to 2^20 points leaf 128 and 8-bit keys (one sort
+    // pass), larger ones leaf 64 and 16-bit keys; 16x16 window blocks
     void reset()
     {
         table.clear();
         for (uint64_t m : CLASS_MAX)
-            table.push_back({m, {64, {16, 16, 4, 0}, 16}, 0.0});
+            table.push_back({m, m <= (1ull << 20) ? TuneCfg{128, {16, 16, 4, 0}, 8}
+                                                  : TuneCfg{64, {16, 16, 4, 0}, 16}, 0.0});
     }
     static int class_of(uint64_t n)
     {

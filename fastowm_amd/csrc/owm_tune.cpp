@@ -264,19 +264,19 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
 // least-squares fit (a free intercept per set) to the batch times of 67
 // homogeneous tile sets x 96 configurations on MI355X
 // (fastowm_amd/tune/fit_model.py on fastowm_amd/tune/trials_mi355x.json); on
-// those sets the model's pick is 2.4 % slower than the fastest configuration
-// on average (25 % at most, a set of 2M-point tiles at Overlap 0.5 whose
-// one-pass keys it overrates), against 60 % for leaf 32 / 8x8x4 / 32 bits.
+// those sets the model's pick is 2.2 % slower than the fastest configuration
+// on average (16 % at most; 2.5 % and 17 % with each set left out of the
+// fit), against 64 % for leaf 32 / 8x8x4 / 32 bits.
 namespace {
 
-constexpr double M_LEAF = 0.544718;
-constexpr double M_AMP = 0.049079;
-constexpr double M_COARSE = 0.011036;
-constexpr double M_PASS = 0.0142663;
-constexpr double M_BLK[NBLOCKS] = {-0.00612738, -0.0080286, -0.00462846, 0.0145554, -0.00237545, 0.00660445};
-constexpr double M_READ = 0.0215696;
-constexpr double M_QUANT = 0.00443822;
-constexpr double M_UNDER = 0.00704832;
+constexpr double M_LEAF = 0.581932;
+constexpr double M_AMP = 0.0504456;
+constexpr double M_COARSE = 0.00798269;
+constexpr double M_PASS = 0.0149227;
+constexpr double M_BLK[NBLOCKS] = {-0.00602212, -0.00863262, -0.00479202, 0.0158341, -0.00237524, 0.00598786};
+constexpr double M_READ = 0.0196671;
+constexpr double M_QUANT = 0.00418061;
+constexpr double M_UNDER = 0.00662172;
 constexpr double CUS = 256.0;                                     // compute units of the part
 constexpr double WG_PER_CU[NBLOCKS] = {4.0, 4.0, 4.0, 3.0, 2.0, 1.0};  // workgroups of each shape
 

@@ -293,10 +293,11 @@ owm_status owm_autotune(owm_ctx *ctx, const double *x, const double *y, const do
                         owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
 owm_status owm_tune_get(owm_ctx *ctx, owm_tune_entry *out, uint32_t cap, uint32_t *n_out);
 /* Loads a table (e.g. one saved from owm_tune_get); classes not listed keep
- * the defaults (leaf 64, block 16x16 with 4 points per lane, no summaries,
- * 16-bit keys: among the configurations with the lowest mean time over the
- * tile sets the statistics model below was fitted to, the one with the lowest
- * worst case). */
+ * the defaults: block 16x16 with 4 points per lane, no summaries; up to 2^20
+ * points leaf 128 and 8-bit keys, beyond leaf 64 and 16-bit keys (over the
+ * tile sets the statistics model below was fitted to, 4 % slower than each
+ * set's fastest configuration on average, against 15 % for the best single
+ * configuration for all classes). */
 owm_status owm_tune_set(owm_ctx *ctx, const owm_tune_entry *in, uint32_t n);
 
 /* ---- statistics-driven tuning ----------------------------------------------

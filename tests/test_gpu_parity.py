@@ -236,7 +236,9 @@ def test_profile_phase_mask(ctx):
     assert np.array_equal(got, want)
     timed = {k for k, v in only.items() if v["calls"]}
     assert timed == {"sort_tree_pass"}
-    assert only["sort_tree_pass"]["calls"] >= 2 and only["sort_tree_pass"]["ms"] > 0
+    entry = next(e for e in ctx.tune_table() if len(x) <= e["max_n"])
+    passes = entry["sort_bits"] // 8
+    assert only["sort_tree_pass"]["calls"] == passes and only["sort_tree_pass"]["ms"] > 0
     assert only["sort_tree_pass"]["calls"] == full["sort_tree_pass"]["calls"]
     for k in ("bbox", "leaf_aggs", "karras", "agg_levels", "stage1", "stage3"):
         assert full[k]["calls"] >= 1, k

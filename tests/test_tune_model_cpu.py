@@ -85,7 +85,7 @@ def test_rejects_unusable_samples(recs):
         owm.tune_model([(n, w, dict(prm, overlap=1.0), bins, sm)])  # parameters derive rejects
     default = owm.tune_model([])
     assert [(e["leaf_size"], e["block_x"], e["block_y"], e["block_k"], e["summaries"], e["sort_bits"])
-            for e in default] == [(64, 16, 16, 4, 0, 16)] * 6
+            for e in default] == [(128, 16, 16, 4, 0, 8)] * 3 + [(64, 16, 16, 4, 0, 16)] * 3
     assert all(e["ms"] == 0 for e in default)
 
 
