@@ -338,13 +338,18 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
     // leaf chunk: point leaves (first sorted position, and point count or
     // S1_LEAFTAG | class cell when binned whole from the leaf aggregate), and
     // summary leaves (leaf << 5 | number of summaries)
-    __shared__ uint32_t sLfA[S1_CHUNK], sLfB[S1_CHUNK];
+    __shared__ uint2 sLf[S1_CHUNK];
+    // the aggregate of a leaf binned whole: z-key (low, high word), index, count
+    // (loaded while the chunk is staged, so that the rounds' point loads are
+    // the only global loads in flight)
+    __shared__ uint4 sAg[S1_CHUNK];
     __shared__ uint32_t sEnt[USE_LCS ? S1_CHUNK : 1];
     unsigned long long *m1z = sZ;
     uint32_t *m1cnt = sCnt;
     uint32_t *m1id = sId;
 
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: the per-item unit arithmetic stays scalar
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     for (uint32_t blk = blockIdx.x; blk < B.nblk; blk += gridDim.x) {
         const TileDev &TD = B.tiles[owm_find_tile(B.blk_off, B.ntiles, blk)];
@@ -412,6 +417,9 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
         const uint32_t nl_grp = L <= 64 ? 64 / L : 1;
         const uint32_t ls = L <= 64 ? L : 64;
         const uint32_t nseg = L <= 64 ? 1 : (L + 63) / 64;
+        // units per leaf group as a shift where it is a power of two (it is
+        // for leaf sizes up to 128): the per-item division is scalar work
+        const uint32_t nseg_sh = (nseg & (nseg - 1)) == 0 ? (uint32_t)__builtin_ctz(nseg) : 32u;
         const uint32_t sub = lane / ls, off = lane - sub * ls;
         const uint32_t esub = lane / LCS_SLOTS, eslot = lane - esub * LCS_SLOTS;
 
@@ -452,8 +460,11 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                     }
                     if (n) {
                         const uint32_t k = atomicAdd(&sNraw, 1u);
-                        sLfA[k] = a;
-                        sLfB[k] = n;
+                        sLf[k] = make_uint2(a, n);
+                        if (n & S1_LEAFTAG) {
+                            const unsigned long long z = T.lf_zkey[leaf];
+                            sAg[k] = make_uint4((uint32_t)z, (uint32_t)(z >> 32), T.lf_zid[leaf], T.lf_cnt[leaf]);
+                        }
                     }
                 }
             }
@@ -492,15 +503,15 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                         continue;
                     }
                     const uint32_t uu = u - neu;
-                    const uint32_t g = uu / nseg, seg = uu - g * nseg;
+                    const uint32_t g = nseg_sh < 32 ? uu >> nseg_sh : uu / nseg, seg = uu - g * nseg;
                     const uint32_t li = g * nl_grp + sub;
                     if (sub >= nl_grp || li >= mraw)
                         continue;
-                    const uint32_t a = sLfA[li], n = sLfB[li];
+                    const uint2 lf = sLf[li];
+                    const uint32_t a = lf.x, n = lf.y;
                     if (n & S1_LEAFTAG) {
                         if (seg * 64 + off == 0) {
-                            const unsigned long long z = T.lf_zkey[a];
-                            pr[k] = make_uint4((uint32_t)z, (uint32_t)(z >> 32), T.lf_zid[a], T.lf_cnt[a]);
+                            pr[k] = sAg[li];
                             cell[k] = n & ~S1_LEAFTAG;
                         }
                     } else if (seg * 64 + off < n) {
@@ -519,8 +530,8 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                         if (__builtin_expect(pr[k].w == OWM_GCLS_NONE, 0)) {
                             // not stamped (near an edge, or far out): by the coordinates
                             const uint32_t uu = (r * KK + k) * NWV + wave - neu;
-                            const uint32_t g = uu / nseg, seg = uu - g * nseg;
-                            const double2 p = T.pts.xy[sLfA[g * nl_grp + sub] + seg * 64 + off];
+                            const uint32_t g = nseg_sh < 32 ? uu >> nseg_sh : uu / nseg, seg = uu - g * nseg;
+                            const double2 p = T.pts.xy[sLf[g * nl_grp + sub].x + seg * 64 + off];
                             if (!(p.x > BL && p.x < BR && p.y > BB && p.y < BT)) {
                                 cell[k] = OWM_NONE;
                                 continue;
@@ -599,17 +610,19 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                         continue;
                     }
                     const uint32_t uu = u - neu;
-                    const uint32_t g = uu / nseg, seg = uu - g * nseg;
+                    const uint32_t g = nseg_sh < 32 ? uu >> nseg_sh : uu / nseg, seg = uu - g * nseg;
                     const uint32_t li = g * nl_grp + sub;
                     if (sub >= nl_grp || li >= mraw)
                         continue;
-                    const uint32_t a = sLfA[li], n = sLfB[li];
+                    const uint2 lf = sLf[li];
+                    const uint32_t a = lf.x, n = lf.y;
                     if (n & S1_LEAFTAG) {
                         if (seg * 64 + off == 0) {
+                            const uint4 ag = sAg[li];
                             cell[k] = n & ~S1_LEAFTAG;
-                            cn[k] = T.lf_cnt[a];
-                            zk[k] = T.lf_zkey[a];
-                            idv[k] = T.lf_zid[a];
+                            cn[k] = ag.w;
+                            zk[k] = (unsigned long long)ag.y << 32 | ag.x;
+                            idv[k] = ag.z;
                         }
                     } else if (seg * 64 + off < n) {
                         const PointRec r = owm_load_pt(T.pts, a + seg * 64 + off);
