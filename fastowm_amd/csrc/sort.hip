@@ -347,13 +347,12 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // position w*64*ITEMS + k*64 + l, ranked in that order), resolves the digits'
 // output bases by decoupled look-back, and then writes the tile out STAGE
 // slots at a time: the stage's points go to LDS in slot (digit) order, then
-// consecutive lanes write consecutive 16-byte (x, y) pairs, then (z, id,
+// consecutive lanes write consecutive slots' 16-byte (x, y) pairs and (z, id,
 // aux) triples, so each digit's run leaves as two contiguous streams.
 // MODE REC_POS: pos[id] = the point's output position.  MODE REC_STAMP (the
 // last pass of a tree built with its query): in tiles with gcls set, aux
-// leaves as the point's global window-edge classes (owm_gstamp, computed once
-// the keys are ranked) instead of its key, and the digits of the staged
-// points are kept beside them.
+// leaves as the point's global window-edge classes (owm_gstamp, computed from
+// the staged coordinates as the slot is written) instead of its key.
 constexpr int REC_PLAIN = 0, REC_POS = 1, REC_STAMP = 2;
 template <bool FIRST, int MODE, int NW, int ITEMS, int STAGE>
 __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut out,
@@ -368,10 +367,8 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
     constexpr int TILE_T = NW * 64 * ITEMS;
     constexpr bool POS = MODE == REC_POS, STAMP = MODE == REC_STAMP;
     static_assert(TILE_T % STAGE == 0, "a sort tile is a whole number of stages");
-    static_assert(TILE_T <= 65536, "a slot and its digit share a word");
     __shared__ SweepLds<NW> L;
     __shared__ uint4 s_xy[STAGE], s_zi[STAGE];  // a stage of points in slot order
-    __shared__ uint8_t s_dig[STAMP ? STAGE : 1];  // their digits (REC_STAMP)
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t p0 = w * (64 * ITEMS) + lane;  // tile position of the thread's item 0
     uint4 *const oxy = reinterpret_cast<uint4 *>(out.xy);
@@ -417,26 +414,14 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
             const uint32_t d = (key[k] >> shift) & (RADIX - 1);
             // items past the tile's end take the unused slots past cnt
             slot[k] = p0 + k * 64 < cnt ? L.excl[d] + L.whist[w][d] + slot[k] : p0 + k * 64;
-            if (STAMP)
-                slot[k] |= d << 16;
         }
-        if constexpr (STAMP) {
-            // the keys are spent: the classes take their place
-            const TileDev &TS = tiles[__builtin_amdgcn_readfirstlane(t)];
-            if (TS.gcls) {
-                // one item at a time (scheduling barriers): sixteen
-                // interleaved computations would spill registers
-#pragma unroll
-                for (int k = 0; k < ITEMS; ++k) {
-                    key[k] = owm_gstamp(TS, x[k], y[k]);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-        }
+        // the last pass of a tree built with its query stamps classes
+        const TileDev &TS = tiles[__builtin_amdgcn_readfirstlane(t)];
+        const bool gstamp = STAMP && TS.gcls;
         for (uint32_t lo = 0; lo < cnt; lo += STAGE) {
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
-                const uint32_t r = (STAMP ? slot[k] & 0xFFFFu : slot[k]) - lo;
+                const uint32_t r = slot[k] - lo;
                 if (r < (uint32_t)STAGE) {
                     const unsigned long long bx = (unsigned long long)__double_as_longlong(x[k]);
                     const unsigned long long by = (unsigned long long)__double_as_longlong(y[k]);
@@ -445,26 +430,24 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                                          (uint32_t)(by >> 32));
                     const uint32_t pid = FIRST ? base + min(p0 + k * 64, cnt - 1) : id[k];
                     s_zi[r] = make_uint4((uint32_t)bz, (uint32_t)(bz >> 32), pid, key[k]);
-                    if (STAMP)
-                        s_dig[r] = (uint8_t)(slot[k] >> 16);
                 }
             }
             __syncthreads();
-            // first the stage's (x, y) pairs, then its (z, id, aux) triples
+            // each slot's (x, y) pair and (z, id, aux) triple: consecutive
+            // lanes write consecutive 16-byte elements of both streams
             const uint32_t m = min((uint32_t)STAGE, cnt - lo);
-            for (uint32_t c = tid; c < 2 * m; c += NW * 64) {
-                const bool zpart = c >= m;
-                const uint32_t r = zpart ? c - m : c;
-                const uint32_t d = STAMP ? (uint32_t)s_dig[r] : (s_zi[r].w >> shift) & (RADIX - 1);
+            for (uint32_t r = tid; r < m; r += NW * 64) {
+                uint4 v = s_zi[r];
+                const uint32_t d = (v.w >> shift) & (RADIX - 1);
                 const uint32_t dst = L.glob[d] + lo + r;
-                if (zpart) {
-                    const uint4 v = s_zi[r];
-                    ozi[dst] = v;
-                    if (POS)
-                        pos[v.z] = dst;
-                } else {
-                    oxy[dst] = s_xy[r];
-                }
+                const uint4 p = s_xy[r];
+                oxy[dst] = p;
+                if (gstamp)
+                    v.w = owm_gstamp(TS, __hiloint2double((int)p.y, (int)p.x),
+                                     __hiloint2double((int)p.w, (int)p.z));
+                ozi[dst] = v;
+                if (POS)
+                    pos[v.z] = dst;
             }
             __syncthreads();
         }
