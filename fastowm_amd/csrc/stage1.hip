@@ -351,13 +351,21 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
     // the wave index is uniform: the per-item unit arithmetic stays scalar
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
+    __shared__ uint32_t sTile;
     for (uint32_t blk = blockIdx.x; blk < B.nblk; blk += gridDim.x) {
-        const TileDev &TD = B.tiles[owm_find_tile(B.blk_off, B.ntiles, blk)];
+        // the block's tile: every thread tests tile boundaries in parallel
+        // (one coalesced load each) instead of one dependent binary search;
+        // empty tiles never match, exactly one tile does
+        for (uint32_t t = tid; t < B.ntiles; t += NT)
+            if (B.blk_off[t] <= blk && blk < B.blk_off[t + 1])
+                sTile = t;
+        __syncthreads();
+        const TileDev &TD = B.tiles[__builtin_amdgcn_readfirstlane(sTile)];
         const uint32_t b = blk - TD.blk_off;
         const uint32_t i0 = (b % TD.nbx) * BX, j0 = (b / TD.nbx) * BY;
         const uint32_t ni = min((uint32_t)BX, TD.ncols - i0), nj = min((uint32_t)BY, TD.nrows - j0);
         const double hd = (double)(float)TD.h;
-        const double inv = 1.0 / TD.displace;
+        const double inv = TD.inv_disp;
         const bool lcs = USE_LCS && TD.lcs != 0;
         // the tree's records carry each point's global classes for these windows
         const bool gcls = TD.gcls != 0;
@@ -536,14 +544,16 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                                 cell[k] = OWM_NONE;
                                 continue;
                             }
-                            cell[k] = cls_f(AY, p.y) * NXC + cls_f(AX, p.x);
+                            const uint32_t cx = cls_f(AX, p.x), cy = cls_f(AY, p.y);
+                            cell[k] = cy * NXC + cx;
                         } else {
                             const uint32_t gx = pr[k].w & 0xFFFFu, gy = pr[k].w >> 16;
                             if (!(gx > GBL && gx < GBR && gy > GBB && gy < GBT)) {
                                 cell[k] = OWM_NONE;
                                 continue;
                             }
-                            cell[k] = (uint32_t)sTy[gy - GBB - 1] * NXC + sTx[gx - GBL - 1];
+                            const uint32_t tx = sTx[gx - GBL - 1], ty = sTy[gy - GBB - 1];
+                            cell[k] = ty * NXC + tx;
                         }
                         cn = 1;
                         const double z = __hiloint2double((int)pr[k].y, (int)pr[k].x);
@@ -554,8 +564,9 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                             cell[k] = OWM_NONE;
                             continue;
                         }
-                        cell[k] = (icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy)) * NXC +
-                                  icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
+                        const uint32_t cx = icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
+                        const uint32_t cy = icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy);
+                        cell[k] = cy * NXC + cx;
                     }
                     const uint32_t c = cell[k];
                     atomicAdd(&sCnt[c], cn);
@@ -643,7 +654,8 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                             cell[k] = OWM_NONE;
                             continue;
                         }
-                        cell[k] = cls_f(AY, y) * NXC + cls_f(AX, x);
+                        const uint32_t cx = cls_f(AX, x), cy = cls_f(AY, y);
+                        cell[k] = cy * NXC + cx;
                         cn[k] = 1;
                         zk[k] = owm_eligible(zv[k]) ? owm_zkey(zv[k]) : OWM_ZKEY_NONE;
                     } else if (USE_LCS && (cell[k] & S1_LEAFTAG)) {
@@ -652,8 +664,9 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
                             cell[k] = OWM_NONE;
                             continue;
                         }
-                        cell[k] = (icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy)) * NXC +
-                                  icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
+                        const uint32_t cx = icnt2(sGLx, ni, gx) + icnt2(sGRx, ni, gx);
+                        const uint32_t cy = icnt2(sGLy, nj, gy) + icnt2(sGRy, nj, gy);
+                        cell[k] = cy * NXC + cx;
                     }
                     const uint32_t c = cell[k];
                     atomicAdd(&sCnt[c], cn[k]);
@@ -694,7 +707,20 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
 
         // separable reduction: x-pass per (class row, window column), held in
         // registers until every cell has been read, then stored over the cells ...
-        const uint32_t nyc = 4 * nj + 1;
+        // Odd classes hold only points lying exactly on an edge.  Window
+        // class ranges start and end on even classes, so with every odd class
+        // column (row) empty the reductions step over odd columns (rows).
+        int ox = 0, oy = 0;
+        for (int c = tid; c < S::NCELL; c += NT) {
+            const int cy = c / NXC, cx = c - cy * NXC;
+            if ((cx | cy) & 1) {
+                const bool used = sCnt[c] != 0;
+                ox |= (cx & 1) && used;
+                oy |= (cy & 1) && used;
+            }
+        }
+        const uint32_t sx = __syncthreads_or(ox) ? 1u : 2u, sy = __syncthreads_or(oy) ? 1u : 2u;
+        const uint32_t nyc = 4 * nj + 1, nyr = sy == 1u ? nyc : 2 * nj + 1;
         uint32_t xc[S::M1_PER_THREAD], xi[S::M1_PER_THREAD];
         unsigned long long xz[S::M1_PER_THREAD];
 #pragma unroll
@@ -703,11 +729,11 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
             xc[t] = 0;
             xi[t] = OWM_NONE;
             xz[t] = OWM_ZKEY_NONE;
-            if (q >= nyc * ni)
+            if (q >= nyr * ni)
                 continue;
-            const uint32_t rr = q / ni, w = q - rr * ni;
+            const uint32_t rr = q / ni * sy, w = q % ni;
             const uint32_t chi = min(sCxh[w], 4 * ni);
-            for (uint32_t c = sCxl[w]; c <= chi; ++c) {
+            for (uint32_t c = sCxl[w]; c <= chi; c += sx) {
                 const uint32_t cc = rr * NXC + c;
                 xc[t] += sCnt[cc];
                 if (owm_less(sZ[cc], sId[cc], xz[t], xi[t])) {
@@ -720,9 +746,9 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
 #pragma unroll
         for (int t = 0; t < S::M1_PER_THREAD; ++t) {
             const uint32_t q = tid + t * NT;
-            if (q >= nyc * ni)
+            if (q >= nyr * ni)
                 continue;
-            const uint32_t rr = q / ni, w = q - rr * ni;
+            const uint32_t rr = q / ni * sy, w = q % ni;
             m1cnt[rr * BX + w] = xc[t];
             m1id[rr * BX + w] = xi[t];
             m1z[rr * BX + w] = xz[t];
@@ -734,7 +760,7 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4
             uint32_t cnt = 0, id = OWM_NONE;
             unsigned long long z = OWM_ZKEY_NONE;
             const uint32_t rhi = min(sCyh[v], 4 * nj);
-            for (uint32_t rr = sCyl[v]; rr <= rhi; ++rr) {
+            for (uint32_t rr = sCyl[v]; rr <= rhi; rr += sy) {
                 uint32_t k = rr * BX + w;
                 cnt += m1cnt[k];
                 if (owm_less(m1z[k], m1id[k], z, id)) {
