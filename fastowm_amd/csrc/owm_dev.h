@@ -232,9 +232,9 @@ template <class TD>
 __device__ __forceinline__ uint32_t owm_gstamp(const TD &T, double x, double y)
 {
     uint32_t gx, gy;
-    const bool ok = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx) &
-                    owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
-    return ok ? gy << 16 | gx : OWM_GCLS_NONE;
+    const bool okx = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx);
+    const bool oky = owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
+    return okx && oky ? gy << 16 | gx : OWM_GCLS_NONE;
 }
 
 // Stage-3 cell centre: initX + ii*Bsize with the integer product.

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_lcs(BatchDev B, TreeDev T, LeafCells LC
         const uint32_t prev = __shfl_up(key, 1, 64);
         const bool head = in && (off == 0 || prev != key);
         const unsigned long long heads = __ballot(head);
-        const uint32_t start = 63 - __clzll(heads & ((2ull << lane) - 1ull) | 1ull);
+        const uint32_t start = 63 - __clzll((heads & ((2ull << lane) - 1ull)) | 1ull);
 #pragma unroll
         for (uint32_t o = 1; o < 64; o <<= 1) {
             const uint32_t c2 = __shfl_up(cnt, o, 64);
