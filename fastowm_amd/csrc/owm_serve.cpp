@@ -83,15 +83,39 @@ static void finish(ServeState *S, const std::shared_ptr<Request> &r, owm_status 
     S->cv_done.notify_all();
 }
 
-// Runs unkeyed requests as one batch on a slot; splits on OWM_E_BUDGET.
+// Runs unkeyed requests as one batch on a slot; splits on OWM_E_BUDGET and
+// OWM_E_ARG.
 static void run_group(ServeState *S, owm_ctx *slot, std::vector<std::shared_ptr<Request>> &g)
 {
     uint64_t total = 0;
     for (auto &r : g)
         total += r->n;
     const uint64_t b = (total * 8 + 255) / 256 * 256;
+    // device requests that are consecutive slices of one cloud (tiles of a
+    // resident survey submitted in order) already form the batch's input:
+    // the chain reads them in place
+    const double *x0 = nullptr, *y0 = nullptr, *z0 = nullptr;  // the first slice
+    const double *ix = nullptr, *iy = nullptr, *iz = nullptr;  // where the next must start
+    bool inplace = true;
+    for (auto &r : g) {
+        if (r->n == 0)
+            continue;
+        if (!r->device || (ix && (r->x != ix || r->y != iy || r->z != iz))) {
+            inplace = false;
+            break;
+        }
+        if (!x0) {
+            x0 = r->x;
+            y0 = r->y;
+            z0 = r->z;
+        }
+        ix = r->x + r->n;
+        iy = r->y + r->n;
+        iz = r->z + r->n;
+    }
+    inplace = inplace && x0 != nullptr;
     owm_status st = OWM_OK;
-    if (slot->upload.bytes < 3 * b) {
+    if (!inplace && slot->upload.bytes < 3 * b) {
         if (slot->held_bytes() - slot->upload.bytes + 3 * b > slot->budget) {
             set_err(slot, "group input needs %llu bytes over the slot budget",
                     (unsigned long long)(3 * b));
@@ -105,19 +129,20 @@ static void run_group(ServeState *S, owm_ctx *slot, std::vector<std::shared_ptr<
     }
     std::vector<owm_tile> tiles(g.size());
     uint64_t off = 0;
-    double *dx = (double *)slot->upload.ptr, *dy = dx + b / 8, *dz = dy + b / 8;
+    double *ux = (double *)slot->upload.ptr, *uy = ux + b / 8, *uz = uy + b / 8;
+    const double *dx = inplace ? x0 : ux, *dy = inplace ? y0 : uy, *dz = inplace ? z0 : uz;
     for (size_t i = 0; i < g.size() && st == OWM_OK; ++i) {
         tiles[i].offset = off;
         tiles[i].n = g[i]->n;
         tiles[i].params = g[i]->p;
-        if (g[i]->n) {
+        if (g[i]->n && !inplace) {
             // host inputs cross PCIe; device inputs are copied within HBM
             const hipMemcpyKind kind = g[i]->device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-            hipError_t e = hipMemcpyAsync(dx + off, g[i]->x, g[i]->n * 8, kind, slot->stream);
+            hipError_t e = hipMemcpyAsync(ux + off, g[i]->x, g[i]->n * 8, kind, slot->stream);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(dy + off, g[i]->y, g[i]->n * 8, kind, slot->stream);
+                e = hipMemcpyAsync(uy + off, g[i]->y, g[i]->n * 8, kind, slot->stream);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(dz + off, g[i]->z, g[i]->n * 8, kind, slot->stream);
+                e = hipMemcpyAsync(uz + off, g[i]->z, g[i]->n * 8, kind, slot->stream);
             if (e != hipSuccess)
                 st = hip_fail(slot, e, "serving upload");
         }
@@ -139,7 +164,10 @@ static void run_group(ServeState *S, owm_ctx *slot, std::vector<std::shared_ptr<
     if (st == OWM_OK)
         st = owm_batch_run_device(slot, dx, dy, dz, tiles.data(), (uint32_t)g.size(), 0, seeds,
                                   cap, so.data());
-    if (st == OWM_E_BUDGET && g.size() > 1) {
+    // a group over the slot's budget, or holding a request the library
+    // rejects (an empty or all-NaN tile, bad parameters), runs its requests
+    // one at a time, so that each gets the result it would get alone
+    if ((st == OWM_E_BUDGET || st == OWM_E_ARG) && g.size() > 1) {
         for (auto &r : g) {
             std::vector<std::shared_ptr<Request>> one{r};
             run_group(S, slot, one);

@@ -121,3 +121,42 @@ def test_device_resident_requests():
             seeds, _ = srv.wait(rid)
             assert np.array_equal(seeds, w)
         srv.stop()
+
+
+def test_device_slices_read_in_place():
+    """Tiles of one resident cloud submitted as consecutive device slices
+    (groups of them are read in place, without a copy), then the same slices
+    out of order and with an empty slice between them (copied groups): every
+    result equals the oracle's, and the empty request fails alone, as it does
+    when run by itself, without failing the requests batched with it."""
+    torch = pytest.importorskip("torch")
+    reqs = synth.serve_requests(10, rate=1000.0, seed=33, lo=10_000, hi=400_000)
+    data = [(t, t.generate()) for _, t in reqs]
+    wants = [O.run(x, y, z, t.wsize, t.bsize, t.overlap, 32)[0] for t, (x, y, z) in data]
+    dev = [torch.from_numpy(np.concatenate([a[k] for _, a in data])).to("cuda:0") for k in range(3)]
+    offs = np.concatenate([[0], np.cumsum([t.n for t, _ in data])])
+    views = [[d[offs[i]:offs[i + 1]] for d in dev] for i in range(len(data))]
+    empty = [d[offs[3]:offs[3]] for d in dev]
+    orders = [list(range(len(data))), [1, 0, 2, 4, 3, "e", 5, 6, 9, 8, 7]]
+    with owm.Context(0) as ctx:
+        srv = owm.Server(ctx, max_inflight=1, max_batch=4)
+        for order in orders:
+            say("slices in order", order)
+            ids = []
+            for i in order:
+                if i == "e":
+                    ids.append((i, srv.submit(*empty, 12, 20, 0.8)))
+                else:
+                    t = data[i][0]
+                    ids.append((i, srv.submit(*views[i], t.wsize, t.bsize, t.overlap)))
+            for i, rid in ids:
+                if i == "e":
+                    with pytest.raises(owm.OWMError):
+                        srv.wait(rid)
+                else:
+                    seeds, _ = srv.wait(rid)
+                    assert np.array_equal(seeds, wants[i]), f"tile {i}"
+        srv.stop()
+        say("the empty slice alone")
+        with pytest.raises(owm.OWMError):
+            ctx.run_tile(*empty, 12, 20, 0.8)
