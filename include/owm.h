@@ -417,8 +417,10 @@ owm_status owm_serve_submit(owm_ctx *ctx, const double *x, const double *y, cons
                             uint64_t *req_id);
 /* The same with x/y/z already in device memory (HBM) of the context's device:
  * a served tile then never crosses PCIe (host inputs are bounded by it at 24
- * bytes per point).  The arrays must stay valid and unchanged until the
- * request completes. */
+ * bytes per point).  Requests that are consecutive slices of one cloud
+ * (tiles of a resident survey submitted in order) are batched in place;
+ * others are copied within HBM into their group's buffer.  The arrays must
+ * stay valid and unchanged until the request completes. */
 owm_status owm_serve_submit_device(owm_ctx *ctx, const double *d_x, const double *d_y,
                                    const double *d_z, uint64_t n, const owm_params *params,
                                    uint64_t tile_key, uint64_t *req_id);
