@@ -326,7 +326,7 @@ owm_status owm_qtree_stats(owm_ctx *ctx, const double *x, const double *y, uint6
         for (const auto &c : cls) {
             leaves += c.leaves;
             settled += (uint64_t)c.points * c.leaves;
-            if (c.level > sl)  // routed once per depth below the sorted depths
+            if (c.level > (uint32_t)sl)  // routed once per depth below the sorted depths
                 routed += (uint64_t)c.points * c.leaves * (c.level - sl);
         }
         if (settled != n) {

@@ -314,10 +314,11 @@ constexpr uint32_t S1_ENT_LEAVES = 64 / LCS_SLOTS;  // summary leaves per unit o
 // when no tile uses them).
 // NT threads per workgroup (256 for blocks up to 16x8; larger blocks take
 // more waves to hide the latency of one workgroup per LDS-sized share of a
-// CU).  The second bound is waves per SIMD: 4 keeps every variant within 128
-// VGPRs.
+// CU).  The second bound is waves per SIMD: 4 keeps the 8x8 and the 512- and
+// 1024-thread variants within 128 VGPRs; 16x8 blocks take 3 (their LDS admits
+// three workgroups per CU), leaf summaries 3 and eight items per lane 2.
 template <int BX, int BY, int S1_K, bool USE_LCS, int NT = S1_THREADS>
-__global__ __launch_bounds__(NT, NT > 256 ? 4 : (USE_LCS ? 3 : (S1_K > 4 ? 2 : 4))) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
+__global__ __launch_bounds__(NT, NT > 256 ? 4 : ((USE_LCS || BX * BY > 64) ? 3 : (S1_K > 4 ? 2 : 4))) void k_stage1(BatchDev B, TreeDev T, LeafCells LC,
                                                         const uint32_t *__restrict__ blk_start,
                                                         const uint32_t *__restrict__ list,
                                                         uint32_t cap,
