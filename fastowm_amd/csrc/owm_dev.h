@@ -57,6 +57,11 @@ struct TileDev {
     // of x and y; only set when the tree is built in the same call as the
     // query and the classes fit 16 bits per axis
     uint32_t gcls;
+    // m when the window side is m window steps (every right edge R_i within
+    // rounding of the left edge L_{i+m}) and stage 1 runs on window columns:
+    // the records then carry each point's column per axis instead of its
+    // classes (owm_gstamp); 0 otherwise
+    uint32_t pairs;
     double qminx, qminy, qscale;  // quantisation: q = (x - qminx) * qscale in [0, 2^32)
     // query (stages 1-3)
     uint32_t ncols, nrows, ncolsg, nrowsg;
@@ -219,6 +224,26 @@ __device__ __forceinline__ bool owm_gaxis_fast(double x, double e0, double c, do
     return fmin(dl, dr) > tol && fmax(dl, dr) < 1.0 - tol;
 }
 
+// The window column of x when a tile's window side is m steps: the left edges
+// cut the axis into columns [L_c, L_{c+1}), and when #{R <= x} = #{L <= x} - m
+// (every point but those within rounding of a right edge and the left edge it
+// pairs with) the windows holding x are exactly c - m + 1 .. c.  With both
+// estimates of owm_gaxis_fast exact, clamp(floor(fl) + 1, 0, n) = #{L <= x}
+// and clamp(floor(fr) + 1, 0, n) = #{R <= x}, so floor(fr) = floor(fl) - m
+// gives the pairing and floor(fl) the column; the column is clamped to
+// [-1, n + m - 1] (a column beyond either end lies in no window, as does
+// every column past those) and stored + 1.  Returns false where the
+// estimates are not exact, the pairing fails or x is NaN.
+__device__ __forceinline__ bool owm_paxis_fast(double x, double e0, double c, double tol, double inv,
+                                               uint32_t n, uint32_t m, uint32_t &v)
+{
+    const double fl = (x - e0) * inv, fr = fl - c;
+    const double il = floor(fl), ir = floor(fr);
+    const double dl = fl - il, dr = fr - ir;
+    v = (uint32_t)(fmin(fmax(il, -1.0), (double)(n + m - 1)) + 1.0);
+    return fmin(dl, dr) > tol && fmax(dl, dr) < 1.0 - tol && ir == il - (double)m;
+}
+
 // Most windows per axis for which a tile's classes are stamped (G <= 4n).
 #define OWM_GCLS_MAX_WINDOWS 16383u
 
@@ -226,14 +251,22 @@ __device__ __forceinline__ bool owm_gaxis_fast(double x, double e0, double c, do
 // OWM_GCLS_MAX_WINDOWS windows per axis, so that G <= 4n <= 0xFFFC fits 16
 // bits), or OWM_GCLS_NONE where the arithmetic of owm_gaxis_fast is not exact
 // (the reader then classifies the point by its coordinates).  The tile's g*
-// fields hold the constants per axis (set with its query fields).
+// fields hold the constants per axis (set with its query fields).  A tile
+// with pairs = m stamps its window columns + 1 instead (owm_paxis_fast; at
+// most n + m <= 16399 per axis).
 #define OWM_GCLS_NONE 0xFFFFFFFFu
 template <class TD>
 __device__ __forceinline__ uint32_t owm_gstamp(const TD &T, double x, double y)
 {
     uint32_t gx, gy;
-    const bool okx = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx);
-    const bool oky = owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
+    bool okx, oky;
+    if (T.pairs) {
+        okx = owm_paxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, T.pairs, gx);
+        oky = owm_paxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, T.pairs, gy);
+    } else {
+        okx = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx);
+        oky = owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
+    }
     return okx && oky ? gy << 16 | gx : OWM_GCLS_NONE;
 }
 

@@ -288,6 +288,7 @@ void fill_query(TileDev &T, const owm_params &P, const Derived &D)
     T.initxg = D.initxg;
     T.inityg = D.inityg;
     T.gcls = 0;
+    T.pairs = 0;
     // the constants of the global window classes (the device's edge formula:
     // centre init + i*displace, then -+ the half side, each step rounded)
     const double hd = (double)(float)D.h;
@@ -309,14 +310,41 @@ void fill_query(TileDev &T, const owm_params &P, const Derived &D)
 
 // For a tree built in the same chain as its query: the last sort pass stamps
 // every point's global window classes wherever they fit 16 bits per axis.
-static void stamp_classes(Prepared &P)
+static uint64_t join_estimate(const Prepared &P);
+static void join_floor(owm_ctx *ctx, Prepared &P);
+
+// A tree built in the same chain as its query stamps every point's window
+// classes, or, when every tile's window side is m <= S1_PAIRS_MAX window
+// steps, its window columns: stage 1 then runs on 32x32-window blocks of
+// column cells (k_stage1_pairs), re-planned here.  OWM_NO_PAIRS=1 keeps the
+// classes (a switch for comparisons).
+static void stamp_classes(owm_ctx *ctx, Prepared &P)
 {
     P.L.stamp = false;
+    bool pairs = !getenv("OWM_NO_PAIRS");
+    bool any = false;
     for (TileDev &T : P.td) {
         T.gcls = T.ncols != 0 && T.nrows != 0 && T.ncols <= OWM_GCLS_MAX_WINDOWS &&
                  T.nrows <= OWM_GCLS_MAX_WINDOWS;
         P.L.stamp = P.L.stamp || T.gcls;
+        if (T.ncols == 0 || T.nrows == 0)
+            continue;  // no windows, no blocks
+        any = true;
+        const double m = std::round(T.gc[0]);
+        pairs = pairs && T.gcls && m >= 1.0 && m <= (double)S1_PAIRS_MAX &&
+                std::fabs(T.gc[0] - m) < 1e-6 && std::fabs(T.gc[1] - m) < 1e-6;
     }
+    if (!pairs || !any)
+        return;
+    for (TileDev &T : P.td) {
+        T.pairs = T.gcls ? (uint32_t)std::round(T.gc[0]) : 0u;
+        T.lcs = 0;
+    }
+    P.cfg = Stage1Cfg{32, 32, 4, 0, 1};
+    P.Q.lcs_leaves = 0;
+    plan_query(P.td, P.cfg, P.Q);
+    P.Q.s1_cap = join_estimate(P);
+    join_floor(ctx, P);
 }
 
 int bit_length(uint64_t v)
@@ -997,6 +1025,35 @@ owm_status prepare_tree(owm_ctx *ctx, const double *dx, const double *dy, const 
 }
 
 // Query fields of every tile (derived parameters) and the query layout.
+// Expected size of the leaf/block join: a leaf of side ~sqrt(L/density)
+// meets about (side + 2h + 2D) / (block side) + 1 blocks per axis.
+static uint64_t join_estimate(const Prepared &P)
+{
+    double est = 4096.0;
+    for (const TileDev &D : P.td) {
+        if (D.nleaf == 0 || D.ncols == 0)
+            continue;
+        const double dens =
+            std::max(D.n / std::max(1e-9, (double)D.ncols * D.nrows * D.displace * D.displace), 1e-9);
+        const double side = std::sqrt(D.leaf_size / dens);
+        const double ext = side + 2.0 * D.h + 2.0 * D.displace;
+        const double fx = ext / (P.cfg.bx * D.displace) + 1.0, fy = ext / (P.cfg.by * D.displace) + 1.0;
+        est += 1.5 * D.nleaf * fx * fy + 64;
+    }
+    return (uint64_t)std::min(est, 4.0e9);
+}
+
+// A join that overflowed before sets the floor (and keeps captured chains
+// valid); OWM_S1_LIST_CAP overrides the first guess (used to exercise the
+// re-run).
+static void join_floor(owm_ctx *ctx, Prepared &P)
+{
+    if (ctx->s1_need_max)
+        P.Q.s1_cap = std::max<uint64_t>(P.Q.s1_cap, ctx->s1_need_max + ctx->s1_need_max / 8 + 1024);
+    if (const char *ov = getenv("OWM_S1_LIST_CAP"))
+        P.Q.s1_cap = std::max<uint64_t>(1, strtoull(ov, nullptr, 10));
+}
+
 owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride_bytes,
                          const std::vector<std::array<double, 4>> &box, uint64_t np_total,
                          Prepared &P)
@@ -1022,21 +1079,7 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
     P.cfg = pick_stage1(ctx, np_total / std::max<uint32_t>(n_tiles, 1));
     P.Q.ntiles = n_tiles;
     plan_query(P.td, P.cfg, P.Q);
-    // expected size of the leaf/block join: a leaf of side ~sqrt(L/density)
-    // meets about (side + 2h + 2D) / (block side) + 1 blocks per axis
-    double est = 4096.0;
-    for (uint32_t t = 0; t < n_tiles; ++t) {
-        const TileDev &D = P.td[t];
-        if (D.nleaf == 0 || D.ncols == 0)
-            continue;
-        const double dens = std::max(P.td[t].n / std::max(1e-9, (double)D.ncols * D.nrows *
-                                                                  D.displace * D.displace), 1e-9);
-        const double side = std::sqrt(D.leaf_size / dens);
-        const double ext = side + 2.0 * D.h + 2.0 * D.displace;
-        const double fx = ext / (P.cfg.bx * D.displace) + 1.0, fy = ext / (P.cfg.by * D.displace) + 1.0;
-        est += 1.5 * D.nleaf * fx * fy + 64;
-    }
-    P.Q.s1_cap = (uint64_t)std::min(est, 4.0e9);
+    P.Q.s1_cap = join_estimate(P);
     // Leaf cell summaries pay off when a leaf spans few window-edge classes:
     // estimate the classes a leaf of side sqrt(L / density) meets (edges every
     // D, or every D/2 when left and right edges do not coincide).
@@ -1059,12 +1102,7 @@ owm_status prepare_query(owm_ctx *ctx, const owm_params *params, uint32_t stride
             P.Q.lcs_leaves = P.L.nleaf;
         }
     }
-    // a join that overflowed before sets the floor (and keeps captured chains valid)
-    if (ctx->s1_need_max)
-        P.Q.s1_cap = std::max<uint64_t>(P.Q.s1_cap, ctx->s1_need_max + ctx->s1_need_max / 8 + 1024);
-    // OWM_S1_LIST_CAP overrides the first guess (used to exercise the re-run)
-    if (const char *ov = getenv("OWM_S1_LIST_CAP"))
-        P.Q.s1_cap = std::max<uint64_t>(1, strtoull(ov, nullptr, 10));
+    join_floor(ctx, P);
     if (P.Q.nwin >= (1ull << 31) || P.Q.ncell >= (1ull << 31)) {
         set_err(ctx, "too many windows or cells in one batch");
         return OWM_E_ARG;
@@ -1318,7 +1356,7 @@ owm_status run_batch(owm_ctx *ctx, const double *dx, const double *dy, const dou
     st = prepare_query(ctx, &tiles[0].params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
-    stamp_classes(P);
+    stamp_classes(ctx, P);
     bool graph = (flags & OWM_B_GRAPH) != 0;
     const bool auto_graph = (flags & B_AUTO_GRAPH) != 0 && !graph;
     for (int attempt = 0;; ++attempt) {
@@ -1407,7 +1445,7 @@ owm_status run_tile_fixed(owm_ctx *ctx, const double *dx, const double *dy, cons
     st = prepare_query(ctx, &tile.params, sizeof(owm_tile), P.box, P.L.np, P);
     if (st != OWM_OK)
         return st;
-    stamp_classes(P);
+    stamp_classes(ctx, P);
     for (int attempt = 0;; ++attempt) {
         const uint64_t list_bytes = align_up(std::max<uint64_t>(P.Q.s1_cap, 1) * 4, 256);
         *need = 256 + table_bytes(1) + tree_bytes(P.L) + build_scratch_bytes(P.L) +

@@ -195,7 +195,10 @@ struct Stage1Cfg {
     uint32_t bx, by;  // window block shape
     uint32_t k;       // points per lane per binning round
     uint32_t lcs;     // 1: leaf cell summaries where a tile's leaves are small enough
+    uint32_t pairs;   // 1: window columns (every tile's edges pair up; records stamped so)
 };
+// Largest window side in window steps for stage 1 on window columns.
+constexpr uint32_t S1_PAIRS_MAX = 7;
 bool stage1_shape_ok(const Stage1Cfg &c);
 uint64_t stage1_work_words(uint64_t nblk);
 // Stage 1 in three launches.  The join lists, per window block, the leaves

@@ -776,6 +776,328 @@ __global__ __launch_bounds__(NT, NT > 256 ? 4 : ((USE_LCS || BX * BY > 64) ? 3 :
     }
 }
 
+// ------------------------------------------------ window columns (pairs) --
+//
+// When a tile's window side is m window steps, every right edge R_i lies
+// within rounding of the left edge L_{i+m}, and the left edges cut each axis
+// into columns [L_c, L_{c+1}) whose points lie in windows c - m + 1 .. c.
+// The tree's records then carry every point's column per axis
+// (owm_paxis_fast), and a block of BX x BY windows needs only
+// (BX + m - 1) x (BY + m - 1) column cells where the class cells of k_stage1
+// take (4 BX + 1) x (4 BY + 1): blocks four times larger fit the LDS, so
+// fewer points are read again for block margins and for leaves that straddle
+// blocks, and the per-block work is spread over four times the windows.  A
+// window's minimum is that of its m x m column cells.  The rest -- points
+// within rounding of an edge or between a right edge and the left edge it
+// pairs with, which carry no column -- take their windows from exact compares
+// with the block's edges and update per-window minima directly, P1_FB at a
+// time, with the same three phases (count and minimum z-key, index reset
+// where the minimum dropped, index minimum) as the cells.
+constexpr uint32_t P1_MMAX = S1_PAIRS_MAX;  // (m + 1)^2 windows of an exception fit 64 bits
+constexpr int P1_FB = 256;        // exceptions taken per pass
+constexpr int P1_KK = 8;          // stamped points per lane per round
+constexpr uint32_t P1_EXC = 0xFFFFFFFCu;  // item is an exception (its windows in pr.w)
+
+template <int BX, int BY, int NT>
+__global__ __launch_bounds__(NT, 4) void k_stage1_pairs(BatchDev B, TreeDev T,
+                                                        const uint32_t *__restrict__ blk_start,
+                                                        const uint32_t *__restrict__ list,
+                                                        uint32_t cap, uint32_t *__restrict__ min_ids)
+{
+    constexpr int NXC = BX + (int)P1_MMAX - 1, NYC = BY + (int)P1_MMAX - 1;
+    constexpr int NCELL = NXC * NYC;
+    constexpr int NWIN = BX * BY;
+    constexpr uint32_t NWV = NT / 64;
+    constexpr int M1 = NYC * BX;
+    constexpr int M1_PER_THREAD = (M1 + NT - 1) / NT;
+    static_assert(M1 <= NCELL, "x-pass results fit the cell arrays");
+    static_assert(NT >= 64 + BY && BX <= 64, "one thread per block edge");
+    static_assert((P1_MMAX + 1) * (P1_MMAX + 1) <= 64, "an exception's windows fit a 64-bit mask");
+    __shared__ double sLx[BX], sRx[BX], sLy[BY], sRy[BY];
+    __shared__ uint32_t sCnt[NCELL];
+    __shared__ unsigned long long sZ[NCELL];
+    __shared__ uint32_t sId[NCELL];
+    __shared__ uint32_t wCnt[NWIN];  // the exceptions' per-window counts and minima
+    __shared__ unsigned long long wZ[NWIN];
+    __shared__ uint32_t wId[NWIN];
+    __shared__ uint4 sFb[P1_FB];                // z-key (low, high word), index, windows
+    __shared__ unsigned long long sFd[P1_FB];   // windows whose minimum the entry lowered
+    __shared__ uint32_t sNfb, sNraw;
+    __shared__ uint2 sLf[S1_CHUNK];             // leaf: first sorted position, points
+    __shared__ uint32_t sTile;
+    unsigned long long *m1z = sZ;
+    uint32_t *m1cnt = sCnt;
+    uint32_t *m1id = sId;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    for (uint32_t blk = blockIdx.x; blk < B.nblk; blk += gridDim.x) {
+        for (uint32_t t = tid; t < B.ntiles; t += NT)
+            if (B.blk_off[t] <= blk && blk < B.blk_off[t + 1])
+                sTile = t;
+        __syncthreads();
+        const TileDev &TD = B.tiles[__builtin_amdgcn_readfirstlane(sTile)];
+        const uint32_t b = blk - TD.blk_off;
+        const uint32_t i0 = (b % TD.nbx) * BX, j0 = (b / TD.nbx) * BY;
+        const uint32_t ni = min((uint32_t)BX, TD.ncols - i0), nj = min((uint32_t)BY, TD.nrows - j0);
+        const uint32_t m = TD.pairs;
+        const int ncx = (int)(ni + m - 1), ncy = (int)(nj + m - 1);
+        const double hd = (double)(float)TD.h;
+        const double inv = TD.inv_disp;
+        if (tid < ni) {
+            const double c = owm_wcenter(TD.initx, TD.displace, i0 + tid);
+            sLx[tid] = __dsub_rn(c, hd);
+            sRx[tid] = __dadd_rn(c, hd);
+        }
+        if (tid >= 64 && tid < 64 + nj) {
+            const double c = owm_wcenter(TD.inity, TD.displace, j0 + tid - 64);
+            sLy[tid - 64] = __dsub_rn(c, hd);
+            sRy[tid - 64] = __dadd_rn(c, hd);
+        }
+        for (int c = tid; c < NCELL; c += NT) {
+            sCnt[c] = 0;
+            sZ[c] = OWM_ZKEY_NONE;
+            sId[c] = OWM_NONE;
+        }
+        for (int w = tid; w < NWIN; w += NT) {
+            wCnt[w] = 0;
+            wZ[w] = OWM_ZKEY_NONE;
+            wId[w] = OWM_NONE;
+        }
+        if (tid == 0)
+            sNfb = 0;
+        __syncthreads();
+        const double BL = sLx[0], BR = sRx[ni - 1], BB = sLy[0], BT = sRy[nj - 1];
+
+        const uint32_t s0 = min(blk_start[blk], cap), s1 = min(blk_start[blk + 1], cap);
+        const uint32_t nl = s1 - s0;
+        const uint32_t L = TD.leaf_size;
+        const uint32_t nl_grp = L <= 64 ? 64 / L : 1;
+        const uint32_t ls = L <= 64 ? L : 64;
+        const uint32_t nseg = L <= 64 ? 1 : (L + 63) / 64;
+        const uint32_t nseg_sh = (nseg & (nseg - 1)) == 0 ? (uint32_t)__builtin_ctz(nseg) : 32u;
+        const uint32_t sub = lane / ls, off = lane - sub * ls;
+
+        for (uint32_t ch = 0; ch < nl; ch += S1_CHUNK) {
+            const uint32_t mc = min((uint32_t)S1_CHUNK, nl - ch);
+            // the previous chunk's rounds are complete before its leaves go
+            if (tid == 0)
+                sNraw = 0;
+            __syncthreads();
+            if (tid < mc) {
+                const uint32_t leaf = list[s0 + ch + tid];
+                const uint32_t np = T.lf_npts[leaf];
+                if (np)
+                    sLf[atomicAdd(&sNraw, 1u)] = make_uint2(T.lf_start[leaf], np);
+            }
+            __syncthreads();
+            const uint32_t mraw = sNraw;
+            const uint32_t nunits = (mraw + nl_grp - 1) / nl_grp * nseg;
+            const uint32_t rounds = (nunits + NWV * P1_KK - 1) / (NWV * P1_KK);
+            for (uint32_t r = 0; r < rounds; ++r) {
+                // z (low, high word), index, columns; after binning the z-key
+                // (low, high word), index and, for an exception, its windows
+                uint32_t cell[P1_KK];
+                uint4 pr[P1_KK];
+                uint32_t dropped = 0;  // bit k: item k lowered its cell's minimum
+                uint32_t pend = 0;     // bit k: item k is an exception not yet taken
+#pragma unroll
+                for (int k = 0; k < P1_KK; ++k) {
+                    cell[k] = OWM_NONE;
+                    const uint32_t u = (r * P1_KK + k) * NWV + wave;
+                    if (u >= nunits)
+                        continue;
+                    const uint32_t g = nseg_sh < 32 ? u >> nseg_sh : u / nseg, seg = u - g * nseg;
+                    const uint32_t li = g * nl_grp + sub;
+                    if (sub >= nl_grp || li >= mraw)
+                        continue;
+                    const uint2 lf = sLf[li];
+                    if (seg * 64 + off < lf.y) {
+                        pr[k] = *reinterpret_cast<const uint4 *>(T.pts.zi + lf.x + seg * 64 + off);
+                        cell[k] = S1_GPOINT;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < P1_KK; ++k) {
+                    if (cell[k] == OWM_NONE)
+                        continue;
+                    const double z = __hiloint2double((int)pr[k].y, (int)pr[k].x);
+                    const unsigned long long zk = owm_eligible(z) ? owm_zkey(z) : OWM_ZKEY_NONE;
+                    if (__builtin_expect(pr[k].w == OWM_GCLS_NONE, 0)) {
+                        // no column: the windows by exact compares with the edges
+                        const uint32_t u = (r * P1_KK + k) * NWV + wave;
+                        const uint32_t g = nseg_sh < 32 ? u >> nseg_sh : u / nseg, seg = u - g * nseg;
+                        const double2 p = T.pts.xy[sLf[g * nl_grp + sub].x + seg * 64 + off];
+                        if (!(p.x > BL && p.x < BR && p.y > BB && p.y < BT)) {
+                            cell[k] = OWM_NONE;
+                            continue;
+                        }
+                        const uint32_t wx0 = cnt_le(sRx, ni, p.x, inv), wx1 = cnt_lt(sLx, ni, p.x, inv);
+                        const uint32_t wy0 = cnt_le(sRy, nj, p.y, inv), wy1 = cnt_lt(sLy, nj, p.y, inv);
+                        if (wx0 >= wx1 || wy0 >= wy1) {
+                            cell[k] = OWM_NONE;
+                            continue;
+                        }
+                        pr[k] = make_uint4((uint32_t)zk, (uint32_t)(zk >> 32), pr[k].z,
+                                           wx0 | wx1 << 8 | wy0 << 16 | wy1 << 24);
+                        cell[k] = P1_EXC;
+                        pend |= 1u << k;
+                        continue;
+                    }
+                    const int cx = (int)(pr[k].w & 0xFFFFu) - 1 - (int)i0;
+                    const int cy = (int)(pr[k].w >> 16) - 1 - (int)j0;
+                    if (cx < 0 || cx >= ncx || cy < 0 || cy >= ncy) {
+                        cell[k] = OWM_NONE;
+                        continue;
+                    }
+                    const uint32_t c = (uint32_t)(cy * NXC + cx);
+                    atomicAdd(&sCnt[c], 1u);
+                    if (zk != OWM_ZKEY_NONE) {
+                        dropped |= (atomicMin(&sZ[c], zk) > zk ? 1u : 0u) << k;
+                        cell[k] = c;
+                    } else {
+                        cell[k] = OWM_NONE;  // counted, never a minimum
+                    }
+                    pr[k].x = (uint32_t)zk;
+                    pr[k].y = (uint32_t)(zk >> 32);
+                }
+                __syncthreads();
+                // the index phases of k_stage1
+#pragma unroll
+                for (int k = 0; k < P1_KK; ++k)
+                    if (cell[k] < (uint32_t)NCELL &&
+                        sZ[cell[k]] == ((unsigned long long)pr[k].y << 32 | pr[k].x) &&
+                        ((dropped >> k) & 1u))
+                        sId[cell[k]] = OWM_NONE;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < P1_KK; ++k)
+                    if (cell[k] < (uint32_t)NCELL &&
+                        sZ[cell[k]] == ((unsigned long long)pr[k].y << 32 | pr[k].x))
+                        atomicMin(&sId[cell[k]], pr[k].z);
+                // exceptions, P1_FB at a time (the block-wide test is the
+                // barrier that orders each pass after the previous one)
+                while (__syncthreads_or(pend != 0)) {
+#pragma unroll
+                    for (int k = 0; k < P1_KK; ++k)
+                        if ((pend >> k) & 1u) {
+                            const uint32_t e = atomicAdd(&sNfb, 1u);
+                            if (e < (uint32_t)P1_FB) {
+                                sFb[e] = pr[k];
+                                pend &= ~(1u << k);
+                            }
+                        }
+                    __syncthreads();
+                    const uint32_t nf = min(sNfb, (uint32_t)P1_FB);
+                    for (uint32_t e = tid; e < nf; e += NT) {
+                        const uint4 f = sFb[e];
+                        const unsigned long long zk = (unsigned long long)f.y << 32 | f.x;
+                        const uint32_t x0 = f.w & 255u, x1 = (f.w >> 8) & 255u;
+                        const uint32_t y0 = (f.w >> 16) & 255u, y1 = f.w >> 24;
+                        unsigned long long low = 0;
+                        uint32_t bit = 0;
+                        for (uint32_t v = y0; v < y1; ++v)
+                            for (uint32_t w = x0; w < x1; ++w, ++bit) {
+                                const uint32_t q = v * BX + w;
+                                atomicAdd(&wCnt[q], 1u);
+                                if (zk != OWM_ZKEY_NONE && atomicMin(&wZ[q], zk) > zk)
+                                    low |= 1ull << bit;
+                            }
+                        sFd[e] = low;
+                    }
+                    __syncthreads();
+                    for (uint32_t e = tid; e < nf; e += NT) {
+                        const uint4 f = sFb[e];
+                        const unsigned long long zk = (unsigned long long)f.y << 32 | f.x;
+                        const unsigned long long low = sFd[e];
+                        const uint32_t x0 = f.w & 255u, x1 = (f.w >> 8) & 255u;
+                        const uint32_t y0 = (f.w >> 16) & 255u, y1 = f.w >> 24;
+                        uint32_t bit = 0;
+                        for (uint32_t v = y0; v < y1; ++v)
+                            for (uint32_t w = x0; w < x1; ++w, ++bit) {
+                                const uint32_t q = v * BX + w;
+                                if (((low >> bit) & 1ull) && wZ[q] == zk)
+                                    wId[q] = OWM_NONE;
+                            }
+                    }
+                    __syncthreads();
+                    for (uint32_t e = tid; e < nf; e += NT) {
+                        const uint4 f = sFb[e];
+                        const unsigned long long zk = (unsigned long long)f.y << 32 | f.x;
+                        if (zk == OWM_ZKEY_NONE)
+                            continue;
+                        const uint32_t x0 = f.w & 255u, x1 = (f.w >> 8) & 255u;
+                        const uint32_t y0 = (f.w >> 16) & 255u, y1 = f.w >> 24;
+                        for (uint32_t v = y0; v < y1; ++v)
+                            for (uint32_t w = x0; w < x1; ++w) {
+                                const uint32_t q = v * BX + w;
+                                if (wZ[q] == zk)
+                                    atomicMin(&wId[q], f.z);
+                            }
+                    }
+                    __syncthreads();
+                    if (tid == 0)
+                        sNfb = 0;
+                }
+            }
+        }
+        // every round's index updates are complete before the window passes
+        __syncthreads();
+
+        // x-pass per (column row, window column), held in registers until
+        // every cell has been read, then stored over the cells ...
+        uint32_t xc[M1_PER_THREAD], xi[M1_PER_THREAD];
+        unsigned long long xz[M1_PER_THREAD];
+#pragma unroll
+        for (int t = 0; t < M1_PER_THREAD; ++t) {
+            const uint32_t q = tid + t * NT;
+            xc[t] = 0;
+            xi[t] = OWM_NONE;
+            xz[t] = OWM_ZKEY_NONE;
+            if (q >= (uint32_t)ncy * ni)
+                continue;
+            const uint32_t rr = q / ni, w = q - rr * ni;
+            for (uint32_t c = w; c < w + m; ++c) {
+                const uint32_t cc = rr * NXC + c;
+                xc[t] += sCnt[cc];
+                if (owm_less(sZ[cc], sId[cc], xz[t], xi[t])) {
+                    xz[t] = sZ[cc];
+                    xi[t] = sId[cc];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < M1_PER_THREAD; ++t) {
+            const uint32_t q = tid + t * NT;
+            if (q >= (uint32_t)ncy * ni)
+                continue;
+            const uint32_t rr = q / ni, w = q - rr * ni;
+            m1cnt[rr * BX + w] = xc[t];
+            m1id[rr * BX + w] = xi[t];
+            m1z[rr * BX + w] = xz[t];
+        }
+        __syncthreads();
+        // ... then the y-pass per window, with the window's exceptions
+        for (uint32_t q = tid; q < ni * nj; q += NT) {
+            const uint32_t v = q / ni, w = q - v * ni;
+            const uint32_t qw = v * BX + w;
+            uint32_t cnt = wCnt[qw], id = wId[qw];
+            unsigned long long z = wZ[qw];
+            for (uint32_t rr = v; rr < v + m; ++rr) {
+                const uint32_t k = rr * BX + w;
+                cnt += m1cnt[k];
+                if (owm_less(m1z[k], m1id[k], z, id)) {
+                    z = m1z[k];
+                    id = m1id[k];
+                }
+            }
+            const bool valid = cnt >= TD.min_num_points && z != OWM_ZKEY_NONE;
+            min_ids[TD.win_off + (j0 + v) * TD.ncols + (i0 + w)] = valid ? id : OWM_NONE;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------ exclusive scan ----
 
 constexpr int SC_BLOCK = 256;
@@ -957,6 +1279,13 @@ void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, cons
         LC = *lc;
     const uint32_t *start = work + (B.nblk + 1);
     const bool use = lc != nullptr;
+    if (cfg.pairs) {
+        // window columns: every tile's edges pair up and its records carry columns
+        uint32_t grid = B.nblk > 65536 ? 65536 : B.nblk;
+        hipLaunchKernelGGL((k_stage1_pairs<32, 32, 512>), dim3(grid), dim3(512), 0, s, B, T, start,
+                           list, cap, min_ids);
+        return;
+    }
     if (cfg.bx == 16 && cfg.by == 8) {
         if (use)
             stage1_shape<16, 8, 4, true>(s, B, T, LC, start, list, cap, min_ids);

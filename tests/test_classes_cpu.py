@@ -118,3 +118,56 @@ def test_points_on_edges_are_left_to_coordinates():
     b = brute(np.concatenate([L, R]), L, R)
     coincide = np.isin(np.concatenate([L, R]), np.intersect1d(L, R))
     assert (b[~coincide] % 2 == 1).all()
+
+
+def fast_columns(x, L, R, inv, mm, m):
+    """owm_paxis_fast: (column + 1, claimed) when the window side is m steps."""
+    e0 = L[0]
+    c = (R[0] - L[0]) * inv
+    tol = np.ldexp(mm, -46) * inv
+    n = len(L)
+    with np.errstate(invalid="ignore", over="ignore"):
+        fl = (x - e0) * inv
+        fr = fl - c
+        il, ir = np.floor(fl), np.floor(fr)
+        dl, dr = fl - il, fr - ir
+        # fmax(NaN, -1) is -1 on the device
+        v = np.nan_to_num(np.clip(il, -1, n + m - 1), nan=-1.0).astype(np.int64) + 1
+        ok = (np.minimum(dl, dr) > tol) & (np.maximum(dl, dr) < 1.0 - tol) & (ir == il - m)
+    return v, ok
+
+
+@pytest.mark.parametrize("ws,ov,origin,side", CASES)
+def test_window_columns_equal_brute_force_where_claimed(ws, ov, origin, side):
+    """Where the column stamp is claimed, the windows c - m + 1 .. c of its
+    column c are exactly the windows whose open interval (L_i, R_i) holds x;
+    with a window side of a whole number of steps nearly every point is
+    claimed, and no point on an edge or NaN ever is."""
+    rng = np.random.default_rng(int(ws * 7 + ov * 1000))
+    box = (origin, origin + side, origin - side / 3, origin + side * 2 / 3)
+    p = O.params(ws, 20, ov, box, 100_000, 32)
+    hd = float(np.float32(p.h))
+    inv = 1.0 / p.displace
+    for init, n, lo, hi in ((p.initx, p.ncols, box[0], box[1]), (p.inity, p.nrows, box[2], box[3])):
+        L, R = edges(init, p.displace, hd, n)
+        mm = max(abs(init), abs(L[0]), abs(R[-1])) + hd
+        m = int(round((R[0] - L[0]) * inv))
+        x = rng.uniform(lo - 2 * ws, hi + 2 * ws, 200_000)
+        e = np.concatenate([L, R])
+        x = np.concatenate([x, nudged(np.repeat(e, 6), rng, 3), e,
+                            [lo - 1e6, hi + 1e6, -1e300, 1e300, np.inf, -np.inf, np.nan]])
+        v, ok = fast_columns(x, L, R, inv, mm, m)
+        col = v - 1
+        # windows holding x by brute force: #{R <= x} .. #{L < x} - 1
+        w0 = np.searchsorted(R, x, side="right")
+        w1 = np.searchsorted(L, x, side="left")
+        c0 = np.clip(col - m + 1, 0, n)
+        c1 = np.clip(col + 1, 0, n)
+        empty = w0 >= w1
+        same = np.where(empty, c0 >= c1, (c0 == w0) & (c1 == w1))
+        assert same[ok].all(), np.flatnonzero(ok & ~same)[:5]
+        assert not ok[np.isnan(x)].any()
+        assert not ok[np.isin(x, e)].any()
+        assert (v[ok] >= 0).all() and (v[ok] <= n + m).all()
+        if abs((R[0] - L[0]) * inv - m) < 1e-6:
+            assert ok[:200_000].mean() > 0.999
