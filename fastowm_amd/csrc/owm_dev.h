@@ -60,7 +60,7 @@ struct TileDev {
     // m when the window side is m window steps (every right edge R_i within
     // rounding of the left edge L_{i+m}) and stage 1 runs on window columns:
     // the records then carry each point's column per axis instead of its
-    // classes (owm_gstamp); 0 otherwise
+    // classes (owm_cstamp); 0 otherwise
     uint32_t pairs;
     double qminx, qminy, qscale;  // quantisation: q = (x - qminx) * qscale in [0, 2^32)
     // query (stages 1-3)
@@ -251,23 +251,26 @@ __device__ __forceinline__ bool owm_paxis_fast(double x, double e0, double c, do
 // OWM_GCLS_MAX_WINDOWS windows per axis, so that G <= 4n <= 0xFFFC fits 16
 // bits), or OWM_GCLS_NONE where the arithmetic of owm_gaxis_fast is not exact
 // (the reader then classifies the point by its coordinates).  The tile's g*
-// fields hold the constants per axis (set with its query fields).  A tile
-// with pairs = m stamps its window columns + 1 instead (owm_paxis_fast; at
-// most n + m <= 16399 per axis).
+// fields hold the constants per axis (set with its query fields).
 #define OWM_GCLS_NONE 0xFFFFFFFFu
 template <class TD>
 __device__ __forceinline__ uint32_t owm_gstamp(const TD &T, double x, double y)
 {
     uint32_t gx, gy;
-    bool okx, oky;
-    if (T.pairs) {
-        okx = owm_paxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, T.pairs, gx);
-        oky = owm_paxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, T.pairs, gy);
-    } else {
-        okx = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx);
-        oky = owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
-    }
+    const bool okx = owm_gaxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, gx);
+    const bool oky = owm_gaxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, gy);
     return okx && oky ? gy << 16 | gx : OWM_GCLS_NONE;
+}
+
+// The same with a tile's window columns + 1 (pairs = m; owm_paxis_fast, at
+// most n + m <= 16390 per axis), or OWM_GCLS_NONE.
+template <class TD>
+__device__ __forceinline__ uint32_t owm_cstamp(const TD &T, double x, double y)
+{
+    uint32_t cx, cy;
+    const bool okx = owm_paxis_fast(x, T.ge0[0], T.gc[0], T.gtol, T.inv_disp, T.ncols, T.pairs, cx);
+    const bool oky = owm_paxis_fast(y, T.ge0[1], T.gc[1], T.gtol, T.inv_disp, T.nrows, T.pairs, cy);
+    return okx && oky ? cy << 16 | cx : OWM_GCLS_NONE;
 }
 
 // Stage-3 cell centre: initX + ii*Bsize with the integer product.

@@ -320,13 +320,14 @@ static void join_floor(owm_ctx *ctx, Prepared &P);
 // classes (a switch for comparisons).
 static void stamp_classes(owm_ctx *ctx, Prepared &P)
 {
-    P.L.stamp = false;
+    P.L.stamp = 0;
     bool pairs = !getenv("OWM_NO_PAIRS");
     bool any = false;
     for (TileDev &T : P.td) {
         T.gcls = T.ncols != 0 && T.nrows != 0 && T.ncols <= OWM_GCLS_MAX_WINDOWS &&
                  T.nrows <= OWM_GCLS_MAX_WINDOWS;
-        P.L.stamp = P.L.stamp || T.gcls;
+        if (T.gcls)
+            P.L.stamp = 1;
         if (T.ncols == 0 || T.nrows == 0)
             continue;  // no windows, no blocks
         any = true;
@@ -340,6 +341,7 @@ static void stamp_classes(owm_ctx *ctx, Prepared &P)
         T.pairs = T.gcls ? (uint32_t)std::round(T.gc[0]) : 0u;
         T.lcs = 0;
     }
+    P.L.stamp = 2;
     P.cfg = Stage1Cfg{32, 32, 4, 0, 1};
     P.Q.lcs_leaves = 0;
     plan_query(P.td, P.cfg, P.Q);
@@ -370,7 +372,7 @@ void plan_tree(int sort_bits, std::vector<TileDev> &tiles, const std::vector<uin
     // The tree sort is segmented by tile, so the keys hold Morton bits only:
     // sort_bits of them (16, 24 or 32: two to four 8-bit passes).
     L.tile_bits = 0;
-    L.stamp = false;  // set by the callers that plan the query in the same chain
+    L.stamp = 0;  // set by the callers that plan the query in the same chain
     L.key_bits = std::max(std::min(sort_bits, 32) & ~1, 4);
     L.pt_off.assign(T + 1, 0);
     L.leaf_off.assign(T + 1, 0);
@@ -652,7 +654,7 @@ owm_status build_tree(owm_ctx *ctx, hipStream_t s, const TreeLayout &L,
         launch_sort_records(s, B, in, out, last ? T.pos : nullptr, (uint32_t)L.np, 8 * p,
                             L.tile_bits + L.key_bits, hist + 256 * p,
                             lookback + (uint64_t)p * tiles * 256, counters + p, p == 0,
-                            last && L.stamp, seg, tiles);
+                            last ? L.stamp : 0, seg, tiles);
         // algorithmic bytes per point and pass: a 32-byte record written, and
         // read (the first pass reads x, y and z, and numbers the points)
         sort_bytes += (p == 0 ? 56ull : 64ull) * L.np + (last && T.pos ? 4ull * L.np : 0ull);

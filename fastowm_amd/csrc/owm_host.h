@@ -57,8 +57,9 @@ struct TreeLayout {
     // key digit counts already taken in the bbox pass (device, the context's
     // small arena), or null: the build takes them itself
     const uint32_t *counts = nullptr;
-    // the last pass stamps window classes (some tile's gcls is set)
-    bool stamp = false;
+    // the last pass stamps window classes (1: some tile's gcls is set) or
+    // window columns (2: every tile's pairs is set)
+    int stamp = 0;
 };
 
 struct QueryLayout {

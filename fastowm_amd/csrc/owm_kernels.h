@@ -61,7 +61,7 @@ struct PointsIn {
 void launch_sort_records(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
                          uint32_t *pos, uint32_t n, int shift, int key_bits,
                          const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool first, bool stamp, const SortSegs &seg, uint32_t seg_tiles);
+                         bool first, int stamp, const SortSegs &seg, uint32_t seg_tiles);
 
 // ---- tree (tree.hip) ----
 struct TreeOut {

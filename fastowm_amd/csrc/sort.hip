@@ -352,8 +352,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t *__restrict__
 // MODE REC_POS: pos[id] = the point's output position.  MODE REC_STAMP (the
 // last pass of a tree built with its query): in tiles with gcls set, aux
 // leaves as the point's global window-edge classes (owm_gstamp, computed from
-// the staged coordinates as the slot is written) instead of its key.
-constexpr int REC_PLAIN = 0, REC_POS = 1, REC_STAMP = 2;
+// the staged coordinates as the slot is written) instead of its key; MODE
+// REC_COLS: its window columns (owm_cstamp, tiles whose window side is a
+// whole number of steps).
+constexpr int REC_PLAIN = 0, REC_POS = 1, REC_STAMP = 2, REC_COLS = 3;
 template <bool FIRST, int MODE, int NW, int ITEMS, int STAGE>
 __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut out,
                                                           uint32_t *__restrict__ pos,
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                                                           SortSegs seg)
 {
     constexpr int TILE_T = NW * 64 * ITEMS;
-    constexpr bool POS = MODE == REC_POS, STAMP = MODE == REC_STAMP;
+    constexpr bool POS = MODE == REC_POS, STAMP = MODE == REC_STAMP || MODE == REC_COLS;
     static_assert(TILE_T % STAGE == 0, "a sort tile is a whole number of stages");
     __shared__ SweepLds<NW> L;
     __shared__ uint4 s_xy[STAGE], s_zi[STAGE];  // a stage of points in slot order
@@ -442,9 +444,11 @@ __global__ __launch_bounds__(NW * 64) void k_sort_records(PointsIn in, PointsOut
                 const uint32_t dst = L.glob[d] + lo + r;
                 const uint4 p = s_xy[r];
                 oxy[dst] = p;
-                if (gstamp)
-                    v.w = owm_gstamp(TS, __hiloint2double((int)p.y, (int)p.x),
-                                     __hiloint2double((int)p.w, (int)p.z));
+                if (gstamp) {
+                    const double px = __hiloint2double((int)p.y, (int)p.x);
+                    const double py = __hiloint2double((int)p.w, (int)p.z);
+                    v.w = MODE == REC_COLS ? owm_cstamp(TS, px, py) : owm_gstamp(TS, px, py);
+                }
                 ozi[dst] = v;
                 if (POS)
                     pos[v.z] = dst;
@@ -584,10 +588,10 @@ template <int NW, int ITEMS, int STAGE>
 static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
                                uint32_t *pos, uint32_t n, int shift, uint32_t key_shift,
                                const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                               bool first, bool stamp, const SortSegs &seg, uint32_t grid)
+                               bool first, int stamp, const SortSegs &seg, uint32_t grid)
 {
     const dim3 b(NW * 64);
-    const int mode = pos ? REC_POS : (stamp ? REC_STAMP : REC_PLAIN);
+    const int mode = pos ? REC_POS : (stamp == 2 ? REC_COLS : (stamp ? REC_STAMP : REC_PLAIN));
 #define OWM_REC(FIRST, MODE)                                                                        \
     hipLaunchKernelGGL((k_sort_records<FIRST, MODE, NW, ITEMS, STAGE>), dim3(grid), b, 0, s, in, out, \
                        pos, B.tiles, key_shift, n, shift, digit_off, lookback, tile_counter, seg)
@@ -596,6 +600,8 @@ static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn 
             OWM_REC(true, REC_POS);
         else if (mode == REC_STAMP)
             OWM_REC(true, REC_STAMP);
+        else if (mode == REC_COLS)
+            OWM_REC(true, REC_COLS);
         else
             OWM_REC(true, REC_PLAIN);
     } else {
@@ -603,6 +609,8 @@ static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn 
             OWM_REC(false, REC_POS);
         else if (mode == REC_STAMP)
             OWM_REC(false, REC_STAMP);
+        else if (mode == REC_COLS)
+            OWM_REC(false, REC_COLS);
         else
             OWM_REC(false, REC_PLAIN);
     }
@@ -612,7 +620,7 @@ static void sort_records_shape(hipStream_t s, const BatchDev &B, const PointsIn 
 void launch_sort_records(hipStream_t s, const BatchDev &B, const PointsIn &in, const PointsOut &out,
                          uint32_t *pos, uint32_t n, int shift, int key_bits,
                          const uint32_t *digit_off, uint32_t *lookback, uint32_t *tile_counter,
-                         bool first, bool stamp, const SortSegs &seg, uint32_t seg_tiles)
+                         bool first, int stamp, const SortSegs &seg, uint32_t seg_tiles)
 {
     const uint32_t grid = seg_tiles < 1 ? 1 : seg_tiles;
     const uint32_t key_shift = 64u - (uint32_t)key_bits;

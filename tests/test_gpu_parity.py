@@ -402,6 +402,35 @@ def test_stamped_classes_high_overlap_and_wide_grid(ctx, restore_table):
 
 
 @pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
+def test_window_columns_near_edges(ctx, origin):
+    """A window side of a whole number of steps (m = 2, 4, 5 and 7 here) runs
+    stage 1 on window columns when the tree is built with its query; points on
+    an edge (the integer lattice, where these grids put integer edges) or a few
+    ulps from one carry no column and take their windows by exact compares,
+    many per block (more than one pass of exceptions per round).  Seeds
+    against the oracle, tile by tile and as one batch."""
+    cfgs = [(8, 20, 0.5, 16), (10, 20, 0.75, 32), (12, 20, 0.8, 64), (14, 20, 1.0 - 1.0 / 7.0, 32)]
+    clouds, wants = [], []
+    for i, (ws, bs, ov, leaf) in enumerate(cfgs):
+        if i % 2:
+            x, y, z = _lattice_cloud(30_000, seed=41 + i)
+            x, y = x + origin[0], y + origin[1]
+        else:
+            x, y, z = _near_edge_cloud(30_000, seed=43 + i, side=200, origin=origin)
+        say("window columns", origin, ws, bs, ov, leaf)
+        want, _, _ = O.run(x, y, z, ws, bs, ov, leaf)
+        assert np.array_equal(ctx.run_tile(x, y, z, ws, bs, ov, leaf), want)
+        clouds.append((x, y, z))
+        wants.append(want)
+    say("window columns, one batch", origin)
+    desc = [(len(c[0]), dict(wsize=ws, bsize=bs, overlap=ov, leaf_size=leaf))
+            for c, (ws, bs, ov, leaf) in zip(clouds, cfgs)]
+    got = ctx.batch_run(*(np.concatenate([c[k] for c in clouds]) for k in range(3)), desc)
+    for i, (g, w) in enumerate(zip(got, wants)):
+        assert np.array_equal(g, w), f"tile {i}"
+
+
+@pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
 def test_points_ulps_from_edges(ctx, restore_table, origin):
     """Stage 1 classifies by arithmetic away from edges and by exact compares
     near them; points a few ulps either side of an edge, at small and at
