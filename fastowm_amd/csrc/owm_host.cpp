@@ -316,7 +316,8 @@ static void join_floor(owm_ctx *ctx, Prepared &P);
 // A tree built in the same chain as its query stamps every point's window
 // classes, or, when every tile's window side is m <= S1_PAIRS_MAX window
 // steps, its window columns: stage 1 then runs on 32x32-window blocks of
-// column cells (k_stage1_pairs), re-planned here.  OWM_NO_PAIRS=1 keeps the
+// column cells (16x16 when there are too few large blocks to fill the device;
+// k_stage1_pairs), re-planned here.  OWM_NO_PAIRS=1 keeps the
 // classes (a switch for comparisons).
 static void stamp_classes(owm_ctx *ctx, Prepared &P)
 {
@@ -342,9 +343,15 @@ static void stamp_classes(owm_ctx *ctx, Prepared &P)
         T.lcs = 0;
     }
     P.L.stamp = 2;
+    // 32x32-window blocks where there are enough of them to occupy every
+    // workgroup slot (two 512-thread workgroups per CU), else 16x16
     P.cfg = Stage1Cfg{32, 32, 4, 0, 1};
     P.Q.lcs_leaves = 0;
     plan_query(P.td, P.cfg, P.Q);
+    if (P.Q.nblk < S1_PAIRS_BIG_BLOCKS) {
+        P.cfg = Stage1Cfg{16, 16, 4, 0, 1};
+        plan_query(P.td, P.cfg, P.Q);
+    }
     P.Q.s1_cap = join_estimate(P);
     join_floor(ctx, P);
 }

@@ -197,8 +197,11 @@ struct Stage1Cfg {
     uint32_t lcs;     // 1: leaf cell summaries where a tile's leaves are small enough
     uint32_t pairs;   // 1: window columns (every tile's edges pair up; records stamped so)
 };
-// Largest window side in window steps for stage 1 on window columns.
+// Largest window side in window steps for stage 1 on window columns, and the
+// fewest 32x32-window blocks for which it uses them (one per workgroup slot
+// of the device, two per CU; else 16x16).
 constexpr uint32_t S1_PAIRS_MAX = 7;
+constexpr uint64_t S1_PAIRS_BIG_BLOCKS = 512;
 bool stage1_shape_ok(const Stage1Cfg &c);
 uint64_t stage1_work_words(uint64_t nblk);
 // Stage 1 in three launches.  The join lists, per window block, the leaves

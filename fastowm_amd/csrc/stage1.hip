@@ -1282,8 +1282,12 @@ void launch_stage1_bins(hipStream_t s, const BatchDev &B, const TreeDev &T, cons
     if (cfg.pairs) {
         // window columns: every tile's edges pair up and its records carry columns
         uint32_t grid = B.nblk > 65536 ? 65536 : B.nblk;
-        hipLaunchKernelGGL((k_stage1_pairs<32, 32, 512>), dim3(grid), dim3(512), 0, s, B, T, start,
-                           list, cap, min_ids);
+        if (cfg.bx == 32)
+            hipLaunchKernelGGL((k_stage1_pairs<32, 32, 512>), dim3(grid), dim3(512), 0, s, B, T,
+                               start, list, cap, min_ids);
+        else
+            hipLaunchKernelGGL((k_stage1_pairs<16, 16, 512>), dim3(grid), dim3(512), 0, s, B, T,
+                               start, list, cap, min_ids);
         return;
     }
     if (cfg.bx == 16 && cfg.by == 8) {
