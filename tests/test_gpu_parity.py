@@ -430,6 +430,22 @@ def test_window_columns_near_edges(ctx, origin):
         assert np.array_equal(g, w), f"tile {i}"
 
 
+def test_window_columns_large_blocks_on_edges(ctx):
+    """A launch with more 32x32-window blocks than workgroup slots runs the
+    32x32 column blocks: one tile of 1000 x 1000 windows (Wsize 8, Overlap
+    0.5, integer edges every 4 m) whose integer-lattice points lie on an edge
+    along x or y about half the time, so nearly every round of every block
+    takes several passes of exceptions.  Seeds against the oracle."""
+    rng = np.random.default_rng(77)
+    n = 1_000_000
+    x = rng.integers(0, 4001, n).astype(np.float64)
+    y = rng.integers(0, 4001, n).astype(np.float64)
+    z = np.round(rng.normal(50.0, 3.0, n), 1)
+    say("large column blocks, edge lattice", n)
+    want, _, _ = O.run(x, y, z, 8, 20, 0.5, 32)
+    assert np.array_equal(ctx.run_tile(x, y, z, 8, 20, 0.5, 32), want)
+
+
 @pytest.mark.parametrize("origin", [(0.0, 0.0), (612_345.0, 4_512_345.0)])
 def test_points_ulps_from_edges(ctx, restore_table, origin):
     """Stage 1 classifies by arithmetic away from edges and by exact compares
