@@ -276,7 +276,10 @@ owm_status owm_batch_run_device(owm_ctx *ctx, const double *d_x, const double *d
 typedef struct owm_tune_entry {
     uint64_t max_n;      /* class bound: point counts up to max_n */
     uint32_t leaf_size;  /* points per leaf (maxNumber) */
-    uint32_t block_x, block_y, block_k; /* stage-1 window block, points per lane per round */
+    /* stage-1 window block, points per lane per round (whole runs whose every
+     * tile has a window side of at most 7 whole window steps use 32x32 or
+     * 16x16 blocks of window columns instead) */
+    uint32_t block_x, block_y, block_k;
     uint32_t summaries;  /* 1: stage 1 reads per-leaf summaries of equal window-edge classes */
     uint32_t sort_bits;  /* Morton bits of the per-tile sort key: 8, 16, 24 or 32 (1-4 passes) */
     double ms;           /* time of the winning trial (owm_autotune) or the model's
