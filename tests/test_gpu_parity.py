@@ -430,6 +430,24 @@ def test_window_columns_near_edges(ctx, origin):
         assert np.array_equal(g, w), f"tile {i}"
 
 
+def test_class_cells_and_window_columns_agree(ctx, monkeypatch):
+    """The same batch (every tile's window side 5 steps) through stage 1 on
+    window columns and, with OWM_NO_PAIRS=1, on class cells: both give the
+    oracle's seeds for every tile."""
+    tiles = synth.batch_tiles(6, seed=12, median=300_000, lo=100_000, hi=900_000)
+    clouds = [t.generate() for t in tiles]
+    wants = [O.run(*c, t.wsize, t.bsize, t.overlap, 32)[0] for c, t in zip(clouds, tiles)]
+    desc = [(t.n, dict(wsize=t.wsize, bsize=t.bsize, overlap=t.overlap, leaf_size=0)) for t in tiles]
+    xyz = [np.concatenate([c[k] for c in clouds]) for k in range(3)]
+    for mode in ("columns", "classes"):
+        if mode == "classes":
+            monkeypatch.setenv("OWM_NO_PAIRS", "1")
+        say("batch of 6 through", mode)
+        got = ctx.batch_run(*xyz, desc)
+        for i, (g, w) in enumerate(zip(got, wants)):
+            assert np.array_equal(g, w), f"{mode}: tile {i}"
+
+
 def test_window_columns_large_blocks_on_edges(ctx):
     """A launch with more 32x32-window blocks than workgroup slots runs the
     32x32 column blocks: one tile of 1000 x 1000 windows (Wsize 8, Overlap
